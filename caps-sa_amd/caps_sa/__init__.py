@@ -1,0 +1,533 @@
+"""Python mirror of the caps-sa 0.7.0 public API over the MI355X C ABI.
+
+Names, argument meaning and error behaviour follow the Rust crate
+(src/lib.rs:28-45): ``build_in_memory*`` (sample_sort.rs:97-206),
+``(try_)build_in_memory_sample_sort*`` and ``(try_)build_ext_mem_for_positions*``
+(ext_mem.rs:370-871), ``Opts`` / ``ExtMemOpts``, ``PlainText`` /
+``SegmentedText`` / ``BoundaryRank`` (limits.rs), ``PackedPrefixSeedPolicy``
+(pack.rs:37-69).  Every build runs on the GPU through
+``libcaps_sa_hip.so``; there is no CPU fallback, so a missing library or a
+missing GPU raises.
+
+Text is a numpy array of ``uint8`` or ``uint16`` (``bytes`` is accepted as
+``uint8``).  The index type is chosen with ``index_dtype`` (``numpy.uint32`` or
+``numpy.uint64``), the counterpart of the crate's ``I: Index`` generic.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import os
+from dataclasses import dataclass, field
+from typing import Callable, Optional, Sequence
+
+import numpy as np
+
+_PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_DIR, "libcaps_sa_hip.so")
+
+CAPS_OK = 0
+CAPS_E_INVALID = 1
+CAPS_E_DEVICE = 2
+CAPS_E_EMIT = 3
+CAPS_E_UNSUPPORTED = 4
+CAPS_E_NOMEM = 5
+
+MAX_CONTEXT = (1 << 64) - 1  # usize::MAX
+
+
+class _Limits(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_uint32),
+        ("order", ctypes.c_uint32),
+        ("ends", ctypes.POINTER(ctypes.c_uint64)),
+        ("n_ends", ctypes.c_uint64),
+    ]
+
+
+class _Opts(ctypes.Structure):
+    _fields_ = [
+        ("max_context", ctypes.c_uint64),
+        ("subproblem_count", ctypes.c_uint64),
+        ("router", ctypes.c_uint32),
+        ("key_geometry", ctypes.c_uint32),
+        ("gpu_buckets", ctypes.c_uint32),
+        ("tile_keys", ctypes.c_uint32),
+        ("want_lcp", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        (name, ctypes.c_double)
+        for name in (
+            "ms_total", "ms_alphabet", "ms_sample", "ms_classify", "ms_scatter", "ms_tile_sort", "ms_merge",
+            "ms_detect", "ms_refine", "ms_giant",
+        )
+    ] + [
+        (name, ctypes.c_uint64)
+        for name in (
+            "buckets", "tiles", "oversized", "pairs", "small_groups", "mid_groups", "giant_groups",
+            "router_err_sum", "router_err_max", "router_fallbacks", "router_count",
+        )
+    ] + [("key_bits", ctypes.c_uint32), ("key_k", ctypes.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+EMIT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64)
+
+_lib = None
+
+# Every entry point of include/caps_sa.h with its ctypes signature.
+_SIGNATURES = {
+    "caps_opts_default": (None, [ctypes.POINTER(_Opts)]),
+    "caps_last_error": (ctypes.c_char_p, []),
+    "caps_last_emit_code": (ctypes.c_int, []),
+    "caps_device_count": (ctypes.c_int, []),
+    "caps_last_stats": (ctypes.c_int, [ctypes.POINTER(Stats)]),
+    "caps_build_in_memory": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                             ctypes.POINTER(_Limits), ctypes.POINTER(_Opts), ctypes.c_void_p,
+                                             ctypes.c_void_p]),
+    "caps_build_in_memory_for_positions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                                                           ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                                                           ctypes.POINTER(_Limits), ctypes.POINTER(_Opts),
+                                                           ctypes.c_void_p, ctypes.c_void_p]),
+    "caps_build_in_memory_sample_sort": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                                                         ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(_Limits),
+                                                         ctypes.POINTER(_Opts), EMIT_FN, ctypes.c_void_p]),
+    "caps_build_ext_mem_for_positions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                                                         ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(_Limits),
+                                                         ctypes.POINTER(_Opts), EMIT_FN, ctypes.c_void_p]),
+    "caps_build_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p,
+                                          ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(_Limits),
+                                          ctypes.POINTER(_Opts), ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p]),
+    "caps_lcp_pairs_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p,
+                                              ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p]),
+    "caps_suffix_cmp_pairs_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                                                     ctypes.POINTER(_Limits), ctypes.c_void_p, ctypes.c_uint64,
+                                                     ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p]),
+    "caps_seed_keys_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                                              ctypes.POINTER(_Limits), ctypes.c_uint32, ctypes.c_uint32,
+                                              ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
+                                              ctypes.POINTER(ctypes.c_uint32), ctypes.c_void_p]),
+    "caps_merge_lcp": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(_Limits),
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p,
+                                       ctypes.c_void_p]),
+    "caps_cascade_merge": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                                           ctypes.POINTER(_Limits), ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
+                                           ctypes.c_void_p, ctypes.c_void_p]),
+}
+
+
+class CapsError(RuntimeError):
+    """A nonzero return code of the C ABI."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"caps error {code}: {message}")
+        self.code = code
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded HIP library; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != CAPS_OK:
+        raise CapsError(rc, lib().caps_last_error().decode())
+
+
+def last_stats() -> dict:
+    st = Stats()
+    _check(lib().caps_last_stats(ctypes.byref(st)))
+    return st.as_dict()
+
+
+# ---------------------------------------------------------------------------
+# Limit providers (limits.rs)
+# ---------------------------------------------------------------------------
+class BoundaryRank(enum.Enum):
+    """limits.rs:113-123."""
+
+    SHORTER_FIRST = 0
+    LONGER_FIRST = 1
+
+
+class LimitProvider:
+    """Base of the providers the GPU build has a device descriptor for.
+
+    A user-defined ``boundary_order`` cannot run on the GPU; only the default
+    order (shorter first) and the STAR convention are available."""
+
+    kind = 0
+    order = 0
+    ends: Optional[np.ndarray] = None
+
+    def boundary_rank(self) -> Optional[BoundaryRank]:
+        return BoundaryRank.LONGER_FIRST if self.order == 1 else BoundaryRank.SHORTER_FIRST
+
+    def _c(self) -> _Limits:
+        lim = _Limits()
+        lim.kind = self.kind
+        lim.order = self.order
+        if self.ends is not None and len(self.ends):
+            self._keep = np.ascontiguousarray(self.ends, dtype=np.uint64)
+            lim.ends = self._keep.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+            lim.n_ends = len(self._keep)
+        return lim
+
+
+class PlainText(LimitProvider):
+    """limits.rs:130-156: lim_at(p) = n - p."""
+
+    def __init__(self, n: int):
+        self.n = n
+
+    def lim_at(self, p: int) -> int:
+        return self.n - p
+
+
+class SegmentedText(LimitProvider):
+    """limits.rs:182-336 with the default (shorter first) boundary order."""
+
+    kind = 1
+
+    def __init__(self, text_len: int, ends: Sequence[int]):
+        ends = np.asarray(ends, dtype=np.uint64)
+        if len(ends) and np.any(ends[1:] <= ends[:-1]):
+            raise ValueError("SegmentedText::from_ends: ends must be strictly increasing")
+        if len(ends) == 0 and text_len != 0:
+            raise ValueError(f"SegmentedText::from_ends: empty ends but text_len ({text_len}) != 0")
+        if len(ends) and int(ends[-1]) != text_len:
+            raise ValueError(f"SegmentedText::from_ends: last end ({int(ends[-1])}) != text_len ({text_len})")
+        self.n = text_len
+        self.ends = ends
+
+    @classmethod
+    def from_lengths(cls, text_len: int, lengths: Sequence[int]) -> "SegmentedText":
+        ends = np.cumsum(np.asarray(lengths, dtype=np.uint64), dtype=np.uint64)
+        total = int(ends[-1]) if len(ends) else 0
+        if total != text_len:
+            raise ValueError(
+                f"SegmentedText::from_lengths: per-segment lengths sum to {total} but text_len is {text_len}")
+        return cls(text_len, ends)
+
+    @classmethod
+    def from_ends(cls, text_len: int, ends: Sequence[int]) -> "SegmentedText":
+        return cls(text_len, ends)
+
+    def n_segments(self) -> int:
+        return len(self.ends)
+
+    def lim_at(self, p: int) -> int:
+        i = int(np.searchsorted(self.ends, p, side="right"))
+        return int(self.ends[i]) - p if i < len(self.ends) else self.n - p
+
+
+class StarSegmented(SegmentedText):
+    """STAR's provider: lim_b.cmp(&lim_a).then(p_a.cmp(&p_b)),
+    BoundaryRank::LongerFirst (sample_sort.rs:998-1015, pack.rs:329-351)."""
+
+    order = 1
+
+
+# ---------------------------------------------------------------------------
+# Options
+# ---------------------------------------------------------------------------
+class PackedPrefixSeedPolicy(enum.Enum):
+    """pack.rs:37-69.  On the GPU every build seeds with packed keys; the
+    policy picks the key geometry (Disabled/DenseAlphabetOnly: dense fields;
+    Remap: the reference geometry where eligible).  Output never changes."""
+
+    DISABLED = 0
+    DENSE_ALPHABET_ONLY = 1
+    REMAP = 2
+
+
+class Router(enum.IntEnum):
+    BSEARCH = 0
+    PWL = 1
+    MLP = 2
+
+
+@dataclass
+class Opts:
+    """sample_sort.rs:71-85."""
+
+    max_context: int = MAX_CONTEXT
+
+
+@dataclass
+class ExtMemOpts:
+    """ext_mem.rs:58-239 (the fields with a GPU meaning) plus GPU knobs."""
+
+    max_context: int = MAX_CONTEXT
+    subproblem_count: int = 0
+    packed_prefix_seed: PackedPrefixSeedPolicy = PackedPrefixSeedPolicy.DISABLED
+    router: Router = Router.BSEARCH
+    gpu_buckets: int = 0
+    tile_keys: int = 0
+
+    @classmethod
+    def from_env(cls) -> "ExtMemOpts":
+        """ext_mem.rs:154-196, for the variables with a GPU meaning."""
+        o = cls()
+        v = os.environ.get("CAPS_SA_SUBPROBLEMS")
+        if v and v.isdigit():
+            o.subproblem_count = int(v)
+        v = os.environ.get("CAPS_SA_MAX_CONTEXT")
+        if v and v.isdigit():
+            o.max_context = int(v)
+        if os.environ.get("CAPS_SA_PACKED_PREFIX_REMAP_BYTES", "").isdigit():
+            o.packed_prefix_seed = PackedPrefixSeedPolicy.REMAP
+        elif os.environ.get("CAPS_SA_PACKED_PREFIX_SEED", "") in ("1", "true", "yes", "on"):
+            o.packed_prefix_seed = PackedPrefixSeedPolicy.DENSE_ALPHABET_ONLY
+        return o
+
+
+def _c_opts(max_context=MAX_CONTEXT, subproblem_count=0, router=0, key_geometry=0, gpu_buckets=0, tile_keys=0,
+            want_lcp=0) -> _Opts:
+    o = _Opts()
+    lib().caps_opts_default(ctypes.byref(o))
+    o.max_context = min(int(max_context), MAX_CONTEXT)
+    o.subproblem_count = int(subproblem_count)
+    o.router = int(router)
+    o.key_geometry = int(key_geometry)
+    o.gpu_buckets = int(gpu_buckets)
+    o.tile_keys = int(tile_keys)
+    o.want_lcp = int(want_lcp)
+    return o
+
+
+def _ext_c_opts(opts: Optional[ExtMemOpts]) -> _Opts:
+    opts = opts or ExtMemOpts()
+    geom = 1 if opts.packed_prefix_seed == PackedPrefixSeedPolicy.REMAP else 0
+    return _c_opts(opts.max_context, opts.subproblem_count, int(opts.router), geom, opts.gpu_buckets,
+                   opts.tile_keys)
+
+
+def _as_text(text) -> np.ndarray:
+    if isinstance(text, (bytes, bytearray, memoryview)):
+        return np.frombuffer(bytes(text), dtype=np.uint8)
+    arr = np.ascontiguousarray(text)
+    if arr.dtype not in (np.uint8, np.uint16):
+        raise CapsError(CAPS_E_UNSUPPORTED, f"symbol type {arr.dtype} is outside the GPU build (u8, u16)")
+    return arr
+
+
+def _limits(lp: Optional[LimitProvider], n: int) -> _Limits:
+    return (lp or PlainText(n))._c()
+
+
+# ---------------------------------------------------------------------------
+# In-memory builders (sample_sort.rs:97-206)
+# ---------------------------------------------------------------------------
+def build_in_memory_for_positions_with(text, positions, lp: Optional[LimitProvider] = None,
+                                       opts: Optional[Opts] = None, index_dtype=np.uint32,
+                                       return_lcp: bool = False, router: int = 0, gpu_buckets: int = 0,
+                                       key_geometry: int = 0):
+    """sample_sort.rs:168-206.  Returns the sorted positions (and the LCP
+    array, merge_sort's postcondition, when ``return_lcp``)."""
+    t = _as_text(text)
+    n = len(t)
+    idt = np.dtype(index_dtype)
+    pos = np.ascontiguousarray(positions, dtype=idt)
+    out = np.empty(len(pos), dtype=idt)
+    lcp = np.empty(len(pos), dtype=idt) if return_lcp else None
+    lim = _limits(lp, n)
+    o = _c_opts((opts or Opts()).max_context, router=router, gpu_buckets=gpu_buckets, key_geometry=key_geometry,
+                want_lcp=int(return_lcp))
+    rc = lib().caps_build_in_memory_for_positions(
+        t.ctypes.data, n, t.dtype.itemsize, pos.ctypes.data, len(pos), idt.itemsize, ctypes.byref(lim),
+        ctypes.byref(o), out.ctypes.data, lcp.ctypes.data if lcp is not None else None)
+    _check(rc)
+    return (out, lcp) if return_lcp else out
+
+
+def build_in_memory_with(text, lp: Optional[LimitProvider] = None, opts: Optional[Opts] = None,
+                         index_dtype=np.uint32, return_lcp: bool = False, router: int = 0, gpu_buckets: int = 0,
+                         key_geometry: int = 0):
+    """sample_sort.rs:118-127."""
+    t = _as_text(text)
+    n = len(t)
+    idt = np.dtype(index_dtype)
+    out = np.empty(n, dtype=idt)
+    lcp = np.empty(n, dtype=idt) if return_lcp else None
+    lim = _limits(lp, n)
+    o = _c_opts((opts or Opts()).max_context, router=router, gpu_buckets=gpu_buckets, key_geometry=key_geometry,
+                want_lcp=int(return_lcp))
+    rc = lib().caps_build_in_memory(t.ctypes.data, n, t.dtype.itemsize, idt.itemsize, ctypes.byref(lim),
+                                    ctypes.byref(o), out.ctypes.data,
+                                    lcp.ctypes.data if lcp is not None else None)
+    _check(rc)
+    return (out, lcp) if return_lcp else out
+
+
+def build_in_memory(text, index_dtype=np.uint32):
+    """sample_sort.rs:97-103."""
+    return build_in_memory_with(text, None, None, index_dtype)
+
+
+def build_in_memory_with_opts(text, opts: Opts, index_dtype=np.uint32):
+    """sample_sort.rs:106-112."""
+    return build_in_memory_with(text, None, opts, index_dtype)
+
+
+def build_in_memory_for_positions(text, positions, index_dtype=np.uint32):
+    """sample_sort.rs:143-149."""
+    return build_in_memory_for_positions_with(text, positions, None, None, index_dtype)
+
+
+def build_in_memory_for_positions_with_opts(text, positions, opts: Opts, index_dtype=np.uint32):
+    """sample_sort.rs:152-165."""
+    return build_in_memory_for_positions_with(text, positions, None, opts, index_dtype)
+
+
+# ---------------------------------------------------------------------------
+# Streaming builders (ext_mem.rs)
+# ---------------------------------------------------------------------------
+class BuildError(Exception):
+    """ext_mem.rs:257-277: Io (device failure) or Emit (callback error)."""
+
+    def __init__(self, kind: str, inner):
+        super().__init__(f"{kind}: {inner}")
+        self.kind = kind
+        self.inner = inner
+
+
+def _stream(entry, text, positions, lp, opts, emit):
+    t = _as_text(text)
+    n = len(t)
+    lim = _limits(lp, n)
+    o = _ext_c_opts(opts)
+    err_box = []
+
+    def _cb(_ctx, ptr, count):
+        try:
+            arr = np.ctypeslib.as_array(ptr, shape=(count,))
+            for v in arr.tolist():
+                emit(v)
+            return 0
+        except Exception as exc:  # the callback's error aborts the build
+            err_box.append(exc)
+            return 1
+
+    cb = EMIT_FN(_cb)
+    if positions is None:
+        pos_ptr, n_pos = None, n
+    else:
+        pos = np.ascontiguousarray(positions, dtype=np.uint64)
+        pos_ptr, n_pos = pos.ctypes.data, len(pos)
+    rc = entry(t.ctypes.data, n, t.dtype.itemsize, pos_ptr, n_pos, ctypes.byref(lim), ctypes.byref(o), cb, None)
+    if rc == CAPS_E_EMIT:
+        raise BuildError("Emit", err_box[0] if err_box else lib().caps_last_emit_code())
+    if rc != CAPS_OK:
+        raise BuildError("Io", CapsError(rc, lib().caps_last_error().decode()))
+
+
+def try_build_in_memory_sample_sort_with(text, lp, opts: Optional[ExtMemOpts], emit: Callable[[int], None]):
+    """ext_mem.rs:755-783."""
+    _stream(lib().caps_build_in_memory_sample_sort, text, None, lp, opts, emit)
+
+
+def try_build_in_memory_sample_sort(text, opts: Optional[ExtMemOpts], emit):
+    """ext_mem.rs:726-736."""
+    try_build_in_memory_sample_sort_with(text, None, opts, emit)
+
+
+def build_in_memory_sample_sort(text, opts: Optional[ExtMemOpts], emit):
+    """ext_mem.rs:717-723: errors surface as OSError (io::Error)."""
+    try:
+        try_build_in_memory_sample_sort(text, opts, emit)
+    except BuildError as e:
+        raise OSError(str(e)) from e
+
+
+def build_in_memory_sample_sort_with(text, lp, opts, emit):
+    """ext_mem.rs:740-752."""
+    try:
+        try_build_in_memory_sample_sort_with(text, lp, opts, emit)
+    except BuildError as e:
+        raise OSError(str(e)) from e
+
+
+def try_build_in_memory_sample_sort_for_positions_with(text, positions, lp, opts, emit):
+    """ext_mem.rs:842-871."""
+    _stream(lib().caps_build_in_memory_sample_sort, text, positions, lp, opts, emit)
+
+
+def build_in_memory_sample_sort_for_positions(text, positions, opts, emit):
+    """ext_mem.rs:787-800."""
+    try:
+        try_build_in_memory_sample_sort_for_positions_with(text, positions, None, opts, emit)
+    except BuildError as e:
+        raise OSError(str(e)) from e
+
+
+def try_build_ext_mem_for_positions_with(text, positions, lp, opts, emit):
+    """ext_mem.rs:417-449 (records stay in HBM; no disk spill)."""
+    _stream(lib().caps_build_ext_mem_for_positions, text, positions, lp, opts, emit)
+
+
+def build_ext_mem_for_positions(text, positions, opts, emit):
+    """ext_mem.rs:370-381."""
+    try:
+        try_build_ext_mem_for_positions_with(text, positions, None, opts, emit)
+    except BuildError as e:
+        raise OSError(str(e)) from e
+
+
+def build_ext_mem_for_positions_with(text, positions, lp, opts, emit):
+    """ext_mem.rs:399-414."""
+    try:
+        try_build_ext_mem_for_positions_with(text, positions, lp, opts, emit)
+    except BuildError as e:
+        raise OSError(str(e)) from e
+
+
+# ---------------------------------------------------------------------------
+# Kernel-level entry points (for parity tests)
+# ---------------------------------------------------------------------------
+def merge_lcp(text, x, lcp_x, y, lcp_y, lp=None, max_context=MAX_CONTEXT):
+    """sample_sort.rs:505-522 on the GPU (merge-path LCP merge)."""
+    t = _as_text(text)
+    x, lx, y, ly = (np.ascontiguousarray(a, dtype=np.uint64) for a in (x, lcp_x, y, lcp_y))
+    z = np.empty(len(x) + len(y), dtype=np.uint64)
+    lz = np.empty_like(z)
+    lim = _limits(lp, len(t))
+    _check(lib().caps_merge_lcp(t.ctypes.data, len(t), t.dtype.itemsize, ctypes.byref(lim), x.ctypes.data,
+                                lx.ctypes.data, len(x), y.ctypes.data, ly.ctypes.data, len(y),
+                                min(max_context, MAX_CONTEXT), z.ctypes.data, lz.ctypes.data))
+    return z, lz
+
+
+def cascade_merge(text, sa, lcp, boundaries, lp=None, max_context=MAX_CONTEXT):
+    """ext_mem.rs:2144-2243 on the GPU."""
+    t = _as_text(text)
+    sa, lcp, b = (np.ascontiguousarray(a, dtype=np.uint64) for a in (sa, lcp, boundaries))
+    out = np.empty_like(sa)
+    lout = np.empty_like(sa)
+    lim = _limits(lp, len(t))
+    _check(lib().caps_cascade_merge(t.ctypes.data, len(t), t.dtype.itemsize, ctypes.byref(lim), sa.ctypes.data,
+                                    lcp.ctypes.data, len(sa), b.ctypes.data, len(b),
+                                    min(max_context, MAX_CONTEXT), out.ctypes.data, lout.ctypes.data))
+    return out, lout
+
+
+__all__ = [name for name in dir() if not name.startswith("_")]

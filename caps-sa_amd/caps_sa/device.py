@@ -1,0 +1,81 @@
+"""Device-pointer entry points of the C ABI over torch tensors.
+
+torch is used for device memory and streams only.  u32/u64 index arrays
+live in int32/int64 tensors (same bits); u16 text lives in an int16 tensor.
+Calls run on torch's current stream.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import (CAPS_OK, MAX_CONTEXT, CapsError, LimitProvider, PlainText, _c_opts, _check, lib)
+
+_IDX = {4: torch.int32, 8: torch.int64}
+
+
+def _stream_ptr() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def text_to_device(text: np.ndarray, device="cuda:0") -> torch.Tensor:
+    text = np.ascontiguousarray(text)
+    if text.dtype == np.uint8:
+        return torch.from_numpy(text).to(device)
+    if text.dtype == np.uint16:
+        return torch.from_numpy(text.view(np.int16)).to(device)
+    raise CapsError(4, f"symbol type {text.dtype} is outside the GPU build")
+
+
+def build_device(d_text: torch.Tensor, n: int, sym_bytes: int, d_positions=None, n_pos=None, idx_bytes=4,
+                 lp: LimitProvider = None, max_context=MAX_CONTEXT, router=0, gpu_buckets=0, key_geometry=0,
+                 d_out=None, d_lcp=None, want_lcp=False):
+    """caps_build_device: all buffers already in HBM.  Returns (sa, lcp)."""
+    n_pos = n if d_positions is None else (n_pos if n_pos is not None else d_positions.numel())
+    if d_out is None:
+        d_out = torch.empty(n_pos, dtype=_IDX[idx_bytes], device=d_text.device)
+    if want_lcp and d_lcp is None:
+        d_lcp = torch.empty(n_pos, dtype=_IDX[idx_bytes], device=d_text.device)
+    lim = (lp or PlainText(n))._c()
+    o = _c_opts(max_context, router=router, gpu_buckets=gpu_buckets, key_geometry=key_geometry,
+                want_lcp=int(d_lcp is not None))
+    rc = lib().caps_build_device(d_text.data_ptr(), n, sym_bytes,
+                                 d_positions.data_ptr() if d_positions is not None else None, n_pos, idx_bytes,
+                                 ctypes.byref(lim), ctypes.byref(o), d_out.data_ptr(),
+                                 d_lcp.data_ptr() if d_lcp is not None else None, _stream_ptr())
+    _check(rc)
+    return d_out, d_lcp
+
+
+def lcp_pairs(d_text: torch.Tensor, n: int, sym_bytes: int, pairs: np.ndarray, max_context=MAX_CONTEXT):
+    d_pairs = torch.from_numpy(np.ascontiguousarray(pairs, dtype=np.uint64).view(np.int64)).to(d_text.device)
+    count = len(pairs) // 2
+    out = torch.empty(max(count, 1), dtype=torch.int64, device=d_text.device)
+    _check(lib().caps_lcp_pairs_device(d_text.data_ptr(), n, sym_bytes, d_pairs.data_ptr(), count,
+                                       min(max_context, MAX_CONTEXT), out.data_ptr(), _stream_ptr()))
+    return out.cpu().numpy().view(np.uint64)[:count]
+
+
+def suffix_cmp_pairs(d_text: torch.Tensor, n: int, sym_bytes: int, pairs: np.ndarray, lp=None,
+                     max_context=MAX_CONTEXT):
+    d_pairs = torch.from_numpy(np.ascontiguousarray(pairs, dtype=np.uint64).view(np.int64)).to(d_text.device)
+    count = len(pairs) // 2
+    out = torch.empty(max(count, 1), dtype=torch.int32, device=d_text.device)
+    lim = (lp or PlainText(n))._c()
+    _check(lib().caps_suffix_cmp_pairs_device(d_text.data_ptr(), n, sym_bytes, ctypes.byref(lim),
+                                              d_pairs.data_ptr(), count, min(max_context, MAX_CONTEXT),
+                                              out.data_ptr(), _stream_ptr()))
+    return out.cpu().numpy()[:count]
+
+
+def seed_keys(d_text: torch.Tensor, n: int, sym_bytes: int, positions: np.ndarray, lp=None, geometry=1, rank=0):
+    d_pos = torch.from_numpy(np.ascontiguousarray(positions, dtype=np.uint64).view(np.int64)).to(d_text.device)
+    count = len(positions)
+    keys = torch.empty(max(count, 1), dtype=torch.int64, device=d_text.device)
+    geo = (ctypes.c_uint32 * 2)()
+    lim = (lp or PlainText(n))._c()
+    _check(lib().caps_seed_keys_device(d_text.data_ptr(), n, sym_bytes, ctypes.byref(lim), geometry, rank,
+                                       d_pos.data_ptr(), count, keys.data_ptr(), geo, _stream_ptr()))
+    return keys.cpu().numpy().view(np.uint64)[:count], (geo[0], geo[1])

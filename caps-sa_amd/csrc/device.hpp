@@ -1,0 +1,174 @@
+// Device-side primitives shared by every kernel of the suffix-array path:
+// the limit provider, packed prefix keys, the suffix comparator and the
+// 64-lane wavefront LCP scan.  gfx950 only (wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace caps {
+
+constexpr uint64_t kMaxCtx = ~0ull;
+
+// Limit provider descriptor.  PlainText when ends == nullptr
+// (limits.rs:130-156); SegmentedText otherwise (limits.rs:182-336), with the
+// coarse block directory of limits.rs:194-236 when dir != nullptr.
+struct DevLimits {
+    uint64_t n;
+    const uint64_t *ends;
+    uint64_t n_ends;
+    const uint32_t *dir;
+    uint32_t block_shift;
+    uint32_t order;  // 0: shorter suffix first; 1: STAR (longer first, then position)
+};
+
+// limits.rs:277-303.
+__device__ __forceinline__ uint64_t lim_at(const DevLimits &L, uint64_t p) {
+    if (L.ends == nullptr) return L.n - p;
+    uint64_t lo = 0, hi = L.n_ends;
+    if (L.dir != nullptr && p < L.n) {
+        uint64_t b = p >> L.block_shift;
+        lo = L.dir[b];
+        hi = L.dir[b + 1];
+        if (hi < lo + 1) hi = lo + 1;
+        if (hi > L.n_ends) hi = L.n_ends;
+    }
+    while (lo < hi) {
+        uint64_t mid = lo + ((hi - lo) >> 1);
+        if (L.ends[mid] <= p) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < L.n_ends ? L.ends[lo] - p : L.n - p;
+}
+
+// Comparison window of a suffix: its limit, clipped by max_context.
+__device__ __forceinline__ uint64_t cap_at(const DevLimits &L, uint64_t p, uint64_t max_ctx) {
+    uint64_t l = lim_at(L, p);
+    return l < max_ctx ? l : max_ctx;
+}
+
+__device__ __forceinline__ int cmp64(uint64_t a, uint64_t b) { return (a > b) - (a < b); }
+
+// limits.rs:66-75 (default) and the STAR convention
+// lim_b.cmp(&lim_a).then(p_a.cmp(&p_b)) (sample_sort.rs:1006-1014).  Under
+// the default convention, suffixes the provider calls Equal are ordered by
+// ascending position (a canonical order; the reference leaves it unspecified).
+__device__ __forceinline__ int boundary_then_pos(const DevLimits &L, uint64_t pa, uint64_t la, uint64_t pb,
+                                                 uint64_t lb) {
+    int c = L.order == 1 ? cmp64(lb, la) : cmp64(la, lb);
+    return c ? c : cmp64(pa, pb);
+}
+
+// Packed prefix key geometry.  Fields are `bits` wide, `k` per key, MSB
+// first, left-aligned by `shift` = 64 - k*bits.  A suffix contributes
+// min(k, remaining window) codes (+bias); the rest of the key is `pad`.
+//   reference geometry (pack.rs:187-203): bias 1 / pad 0 (shorter first) or
+//     bias 0 / pad alphabet (longer first), strictly order-preserving;
+//   dense geometry: bias 0, pad 0 (shorter first) or all-ones (longer first);
+//     every code is a symbol, so keys are weakly order-preserving: a strict
+//     key inequality still implies the same suffix inequality, and equal keys
+//     are settled by refinement.
+struct KeyGeom {
+    uint32_t bits, k, bias, pad, shift;
+};
+
+template <typename S>
+struct CodeMap;
+template <>
+struct CodeMap<uint8_t> {
+    const uint8_t *code;  // nullptr = identity
+    __device__ __forceinline__ uint32_t operator()(uint8_t c) const { return code ? code[c] : c; }
+};
+template <>
+struct CodeMap<uint16_t> {
+    const uint16_t *code;  // nullptr = identity
+    __device__ __forceinline__ uint32_t operator()(uint16_t c) const { return code ? code[c] : c; }
+};
+
+// Key of the `k` symbols starting at depth d of suffix p, whose comparison
+// window is `cap` symbols (limit and max_context already applied).
+template <typename S>
+__device__ __forceinline__ uint64_t key_at(const S *__restrict__ t, CodeMap<S> cm, const KeyGeom &g, uint64_t p,
+                                           uint64_t d, uint64_t cap) {
+    uint64_t rem = cap > d ? cap - d : 0;
+    uint32_t take = rem < g.k ? (uint32_t)rem : g.k;
+    uint64_t key = 0;
+    const S *s = t + p + d;
+    for (uint32_t i = 0; i < g.k; ++i) {
+        uint64_t f = i < take ? (uint64_t)(cm(s[i]) + g.bias) : (uint64_t)g.pad;
+        key = (key << g.bits) | f;
+    }
+    return key << g.shift;
+}
+
+// Symbols two keys share, from the first differing field (pack.rs:207-209).
+__device__ __forceinline__ uint32_t shared_fields(const KeyGeom &g, uint64_t a, uint64_t b) {
+    uint64_t x = a ^ b;
+    uint32_t lz = x ? (uint32_t)__builtin_clzll(x) : 64u;
+    uint32_t f = lz / g.bits;
+    return f < g.k ? f : g.k;
+}
+
+// Scalar LCP of text[p+start..] and text[q+start..], stopping at `cap`.
+template <typename S>
+__device__ __forceinline__ uint64_t thread_lcp(const S *__restrict__ t, uint64_t p, uint64_t q, uint64_t start,
+                                               uint64_t cap) {
+    uint64_t o = start;
+    while (o < cap && t[p + o] == t[q + o]) ++o;
+    return o;
+}
+
+// 64-lane wavefront LCP (the SIMD LCP of lcp.rs:292-360 as a wave-wide
+// compare): every lane compares one symbol per step, a ballot collects the
+// mismatch mask and ffs picks the first one.  Four steps (256 symbols) per
+// iteration.  All 64 lanes must call it with the same arguments.
+template <typename S>
+__device__ __forceinline__ uint64_t wave_lcp(const S *__restrict__ t, uint64_t p, uint64_t q, uint64_t start,
+                                             uint64_t cap) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint64_t off = start;
+    while (off < cap) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            uint64_t o = off + (uint64_t)(j * 64) + lane;
+            bool mism = o < cap ? (t[p + o] != t[q + o]) : true;
+            uint64_t mask = __ballot(mism);
+            if (mask) {
+                uint64_t r = off + (uint64_t)(j * 64) + (uint64_t)(__ffsll((unsigned long long)mask) - 1);
+                return r < cap ? r : cap;
+            }
+        }
+        off += 256;
+    }
+    return cap < start ? start : cap;
+}
+
+// Full order of two suffixes known to agree on their first `known` symbols
+// (clipped to the window): lcp.rs:184-201 followed by the position tie.
+// Returns <0, 0 (same position) or >0, and the LCP (window-capped) in *lcp.
+template <typename S>
+__device__ __forceinline__ int thread_cmp(const S *__restrict__ t, const DevLimits &L, uint64_t max_ctx, uint64_t a,
+                                          uint64_t b, uint64_t known, uint64_t *lcp) {
+    uint64_t la = lim_at(L, a), lb = lim_at(L, b);
+    uint64_t ca = la < max_ctx ? la : max_ctx, cb = lb < max_ctx ? lb : max_ctx;
+    uint64_t cap = ca < cb ? ca : cb;
+    uint64_t s = known < cap ? known : cap;
+    uint64_t c = thread_lcp(t, a, b, s, cap);
+    *lcp = c;
+    if (c < cap) return t[a + c] < t[b + c] ? -1 : 1;
+    return boundary_then_pos(L, a, la, b, lb);
+}
+
+template <typename S>
+__device__ __forceinline__ int wave_cmp(const S *__restrict__ t, const DevLimits &L, uint64_t max_ctx, uint64_t a,
+                                        uint64_t b, uint64_t known, uint64_t *lcp) {
+    uint64_t la = lim_at(L, a), lb = lim_at(L, b);
+    uint64_t ca = la < max_ctx ? la : max_ctx, cb = lb < max_ctx ? lb : max_ctx;
+    uint64_t cap = ca < cb ? ca : cb;
+    uint64_t s = known < cap ? known : cap;
+    uint64_t c = wave_lcp(t, a, b, s, cap);
+    *lcp = c;
+    if (c < cap) return t[a + c] < t[b + c] ? -1 : 1;
+    return boundary_then_pos(L, a, la, b, lb);
+}
+
+}  // namespace caps

@@ -1,0 +1,850 @@
+// Host side of the MI355X suffix-array constructor: device buffers, the
+// phase pipeline, and the C ABI declared in include/caps_sa.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/caps_sa.h"
+#include "engine.hpp"
+#include "kernels.hpp"
+
+namespace caps {
+
+thread_local std::string g_err;
+thread_local int g_emit_code = 0;
+thread_local caps_stats_t g_stats;
+std::mutex g_mu;  // one build at a time per process (the device context is serialized)
+
+struct CapsError : std::runtime_error {
+    int code;
+    CapsError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                                         \
+    do {                                                                                                  \
+        hipError_t e_ = (x);                                                                              \
+        if (e_ != hipSuccess)                                                                             \
+            throw CapsError(e_ == hipErrorOutOfMemory ? CAPS_E_NOMEM : CAPS_E_DEVICE,                     \
+                            std::string(#x) + ": " + hipGetErrorString(e_));                              \
+    } while (0)
+
+#define LAUNCHCHK() HIPCHK(hipGetLastError())
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+void *DevBuf::get(size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    if (bytes > cap) {
+        release();
+        HIPCHK(hipMalloc(&p, bytes));
+        cap = bytes;
+    }
+    return p;
+}
+
+Workspace &workspace() {
+    static Workspace *w = new Workspace();
+    return *w;
+}
+
+static inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+static unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 65536u * 4) {
+    uint64_t g = ceil_div(n ? n : 1, block);
+    return (unsigned)std::min<uint64_t>(g, cap);
+}
+
+// ---------------------------------------------------------------------------
+// Limits: upload ends + the reference's coarse directory (limits.rs:194-236).
+// ---------------------------------------------------------------------------
+DevLimits upload_limits(Workspace &W, const caps_limits_t *lim, uint64_t n, hipStream_t st) {
+    DevLimits L{};
+    L.n = n;
+    L.order = lim ? lim->order : CAPS_ORDER_SHORTER_FIRST;
+    if (L.order > 1) throw CapsError(CAPS_E_INVALID, "unknown boundary order");
+    if (!lim || lim->kind == CAPS_LIMITS_PLAIN) return L;
+    if (lim->kind != CAPS_LIMITS_SEGMENTED) throw CapsError(CAPS_E_INVALID, "unknown limits kind");
+    const uint64_t *ends = lim->ends;
+    uint64_t ne = lim->n_ends;
+    // SegmentedText::from_ends checks (limits.rs:261-281)
+    for (uint64_t i = 1; i < ne; ++i)
+        if (!(ends[i - 1] < ends[i])) throw CapsError(CAPS_E_INVALID, "segment ends must be strictly increasing");
+    if (ne == 0 ? n != 0 : ends[ne - 1] != n) throw CapsError(CAPS_E_INVALID, "last segment end must equal n");
+    if (ne == 0) return L;
+    W.h_ends.assign(ends, ends + ne);
+    uint64_t *d_ends = (uint64_t *)W.ends.get(ne * 8);
+    HIPCHK(hipMemcpyAsync(d_ends, W.h_ends.data(), ne * 8, hipMemcpyHostToDevice, st));
+    L.ends = d_ends;
+    L.n_ends = ne;
+    if (ne >= 256 && ne <= 0xFFFFFFFFull && n > 0) {
+        uint64_t target = std::min<uint64_t>(std::max<uint64_t>(ne * 2, 1), 2000000);
+        uint64_t minb = ceil_div(n, target);
+        uint64_t bs = 1;
+        while (bs < minb) bs <<= 1;
+        uint32_t shift = (uint32_t)__builtin_ctzll(bs);
+        uint64_t nb = ceil_div(n, bs);
+        W.h_dir.resize(nb + 1);
+        uint64_t e = 0;
+        for (uint64_t b = 0; b <= nb; ++b) {
+            uint64_t s = std::min(b * bs, n);
+            while (e < ne && ends[e] <= s) ++e;
+            W.h_dir[b] = (uint32_t)e;
+        }
+        uint32_t *d_dir = (uint32_t *)W.dir.get((nb + 1) * 4);
+        HIPCHK(hipMemcpyAsync(d_dir, W.h_dir.data(), (nb + 1) * 4, hipMemcpyHostToDevice, st));
+        L.dir = d_dir;
+        L.block_shift = shift;
+    }
+    return L;
+}
+
+// ---------------------------------------------------------------------------
+// Key geometry from the alphabet (pack.rs:103-176 for the reference form).
+// ---------------------------------------------------------------------------
+template <typename S>
+Geometry compute_geometry(Workspace &W, const S *d_text, uint64_t n, uint32_t mode, uint32_t order, hipStream_t st) {
+    constexpr uint32_t WORDS = sizeof(S) == 1 ? 8 : 2048;
+    uint32_t *bm = (uint32_t *)W.bitmap.get(WORDS * 4);
+    HIPCHK(hipMemsetAsync(bm, 0, WORDS * 4, st));
+    if (n) {
+        k_alphabet<S><<<grid_for(n, 256, 2048), 256, 0, st>>>(d_text, n, bm);
+        LAUNCHCHK();
+    }
+    std::vector<uint32_t> h(WORDS);
+    HIPCHK(hipMemcpyAsync(h.data(), bm, WORDS * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    uint32_t distinct = 0, maxv = 0;
+    bool identity = true;
+    std::vector<uint32_t> code(WORDS * 32, 0);
+    for (uint32_t v = 0; v < WORDS * 32; ++v)
+        if (h[v >> 5] >> (v & 31) & 1) {
+            code[v] = distinct;
+            identity &= (distinct == v);
+            ++distinct;
+            maxv = v;
+        }
+    Geometry G{};
+    G.distinct = distinct;
+    bool longer_first = order == CAPS_ORDER_STAR;
+    bool ref = false;
+    if (mode == CAPS_KEY_REFERENCE && sizeof(S) == 1) {
+        uint32_t bits = distinct <= 1 ? 1 : distinct <= 3 ? 2 : distinct <= 15 ? 4 : 8;
+        if ((uint64_t)distinct < (1ull << bits)) {  // has_sentinel (pack.rs:174-176)
+            ref = true;
+            G.g.bits = bits;
+            G.g.k = 64 / bits;
+            G.g.shift = 0;
+            G.g.bias = longer_first ? 0 : 1;
+            G.g.pad = longer_first ? distinct : 0;
+            G.remap = !identity;
+        }
+    }
+    if (!ref) {
+        uint32_t bits = 1;
+        while ((1ull << bits) < distinct) ++bits;
+        G.g.bits = bits;
+        G.g.k = 64 / bits;
+        G.g.shift = 64 - G.g.k * bits;
+        G.g.bias = 0;
+        G.g.pad = longer_first ? (uint32_t)((1ull << bits) - 1) : 0;
+        G.remap = (uint64_t)maxv >= (1ull << bits);
+    }
+    G.reference = ref;
+    if (G.remap) {
+        if (sizeof(S) == 1) {
+            W.h_code8.resize(256);
+            for (int v = 0; v < 256; ++v) W.h_code8[v] = (uint8_t)code[v];
+            void *d = W.code.get(256);
+            HIPCHK(hipMemcpyAsync(d, W.h_code8.data(), 256, hipMemcpyHostToDevice, st));
+            G.code = d;
+        } else {
+            W.h_code16.resize(65536);
+            for (int v = 0; v < 65536; ++v) W.h_code16[v] = (uint16_t)code[v];
+            void *d = W.code.get(65536 * 2);
+            HIPCHK(hipMemcpyAsync(d, W.h_code16.data(), 65536 * 2, hipMemcpyHostToDevice, st));
+            G.code = d;
+        }
+    }
+    return G;
+}
+
+template <typename S>
+static CodeMap<S> codemap(const Geometry &G) {
+    CodeMap<S> cm;
+    cm.code = (const decltype(cm.code))G.code;
+    return cm;
+}
+
+// ---------------------------------------------------------------------------
+// Timing
+// ---------------------------------------------------------------------------
+struct PhaseTimer {
+    hipStream_t st;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> tag;
+    explicit PhaseTimer(hipStream_t s) : st(s) { mark(-1); }
+    void mark(int t) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipEventRecord(e, st));
+        ev.push_back(e);
+        tag.push_back(t);
+    }
+    ~PhaseTimer() {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+    void collect(caps_stats_t &s) {
+        double *slot[] = {&s.ms_alphabet, &s.ms_sample, &s.ms_classify, &s.ms_scatter, &s.ms_tile_sort,
+                          &s.ms_merge,    &s.ms_detect, &s.ms_refine,   &s.ms_giant};
+        for (size_t i = 1; i < ev.size(); ++i) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, ev[i - 1], ev[i]));
+            if (tag[i] >= 0 && tag[i] < 9) *slot[tag[i]] += ms;
+        }
+        float tot = 0;
+        HIPCHK(hipEventElapsedTime(&tot, ev.front(), ev.back()));
+        s.ms_total = tot;
+    }
+};
+enum { PH_ALPHA, PH_SAMPLE, PH_CLASSIFY, PH_SCATTER, PH_TILE, PH_MERGE, PH_DETECT, PH_REFINE, PH_GIANT };
+
+// Sort `count` keys (with u32 payload) in place: LDS tiles + key merges.
+template <int T>
+static void sort_keys_single_range(Workspace &W, uint64_t *keys, uint32_t *ids, uint64_t *keys2, uint32_t *ids2,
+                                   uint64_t count, hipStream_t st, uint64_t **out_keys, uint32_t **out_ids) {
+    uint64_t nt = ceil_div(count, T);
+    W.h_tiles.resize(nt);
+    for (uint64_t i = 0; i < nt; ++i) {
+        W.h_tiles[i].start = i * T;
+        W.h_tiles[i].len = (uint32_t)std::min<uint64_t>(T, count - i * T);
+        W.h_tiles[i].pad = 0;
+    }
+    Tile *dt = (Tile *)W.stiles.get(nt * sizeof(Tile));
+    HIPCHK(hipMemcpyAsync(dt, W.h_tiles.data(), nt * sizeof(Tile), hipMemcpyHostToDevice, st));
+    k_tile_sort<uint32_t, T, 256><<<(unsigned)nt, 256, 0, st>>>(keys, ids, dt);
+    LAUNCHCHK();
+    Range r{0, count};
+    W.h_range = r;
+    Range *dr = (Range *)W.srange.get(sizeof(Range));
+    HIPCHK(hipMemcpyAsync(dr, &W.h_range, sizeof(Range), hipMemcpyHostToDevice, st));
+    uint64_t *ks = keys, *kd = keys2;
+    uint32_t *is = ids, *id = ids2;
+    for (uint64_t run = T; run < count; run *= 2) {
+        dim3 grid((unsigned)ceil_div(ceil_div(count, 8), 256), 1);
+        k_key_merge<uint32_t><<<grid, 256, 0, st>>>(dr, run, ks, is, kd, id);
+        LAUNCHCHK();
+        std::swap(ks, kd);
+        std::swap(is, id);
+    }
+    *out_keys = ks;
+    *out_ids = is;
+}
+
+// ---------------------------------------------------------------------------
+// The build.
+// ---------------------------------------------------------------------------
+template <typename S, typename I>
+void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, const caps_limits_t *lim,
+                const caps_opts_t &o, I *d_out, I *d_lcp_out, hipStream_t st, caps_stats_t &stats) {
+    Workspace &W = workspace();
+    memset(&stats, 0, sizeof(stats));
+    PhaseTimer tm(st);
+    const uint64_t max_ctx = o.max_context;
+    DevLimits L = upload_limits(W, lim, n, st);
+    if (n_pos == 0) {
+        tm.mark(PH_ALPHA);
+        HIPCHK(hipStreamSynchronize(st));
+        tm.collect(stats);
+        return;
+    }
+    if (d_pos && (const void *)d_pos == (const void *)d_out) {
+        I *cp = (I *)W.pos_copy.get(n_pos * sizeof(I));
+        HIPCHK(hipMemcpyAsync(cp, d_pos, n_pos * sizeof(I), hipMemcpyDeviceToDevice, st));
+        d_pos = cp;
+    }
+    const bool want_lcp = d_lcp_out != nullptr;
+    if (n_pos == 1) {
+        if (d_pos) HIPCHK(hipMemcpyAsync(d_out, d_pos, sizeof(I), hipMemcpyDeviceToDevice, st));
+        else HIPCHK(hipMemsetAsync(d_out, 0, sizeof(I), st));
+        if (want_lcp) HIPCHK(hipMemsetAsync(d_lcp_out, 0, sizeof(I), st));
+        tm.mark(PH_ALPHA);
+        HIPCHK(hipStreamSynchronize(st));
+        tm.collect(stats);
+        return;
+    }
+
+    // ---- alphabet and key geometry ----
+    Geometry G = compute_geometry<S>(W, d_text, n, o.key_geometry, L.order, st);
+    const KeyGeom g = G.g;
+    const CodeMap<S> cm = codemap<S>(G);
+    stats.key_bits = g.bits;
+    stats.key_k = g.k;
+    tm.mark(PH_ALPHA);
+
+    // ---- buckets and splitters ----
+    constexpr uint32_t T = 4096;
+    uint64_t target = T / 2;
+    uint64_t p = o.gpu_buckets ? o.gpu_buckets : ceil_div(n_pos, target);
+    p = std::max<uint64_t>(1, std::min<uint64_t>(p, std::min<uint64_t>(n_pos, 1u << 22)));
+    const uint64_t oversample = 16;
+    uint64_t s = p > 1 ? std::min<uint64_t>(n_pos, p * oversample) : 0;
+    uint64_t *spl = (uint64_t *)W.spl.get(std::max<uint64_t>(p, 1) * 8);
+    if (p > 1) {
+        uint64_t *sk = (uint64_t *)W.skeys.get(s * 8), *sk2 = (uint64_t *)W.skeys2.get(s * 8);
+        uint32_t *si = (uint32_t *)W.sids.get(s * 4), *si2 = (uint32_t *)W.sids2.get(s * 4);
+        k_sample_keys<S, I><<<grid_for(s, 256, 1u << 30), 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, s,
+                                                                         sk, si);
+        LAUNCHCHK();
+        uint64_t *sorted;
+        uint32_t *sorted_ids;
+        sort_keys_single_range<T>(W, sk, si, sk2, si2, s, st, &sorted, &sorted_ids);
+        k_pick_splitters<<<(unsigned)ceil_div(p, 256), 256, 0, st>>>(sorted, s, (uint32_t)p, spl);
+        LAUNCHCHK();
+    }
+    Splitters SP{};
+    SP.spl = spl;
+    SP.n_spl = (uint32_t)(p - 1);
+    SP.stride = (uint32_t)std::max<uint64_t>(1, ceil_div(p - 1, kCoarseMax));
+    SP.n_coarse = (uint32_t)ceil_div(p - 1, SP.stride);
+    stats.buckets = p;
+    tm.mark(PH_SAMPLE);
+
+    // ---- classify + histogram ----
+    uint32_t *bid = (uint32_t *)W.bid.get(n_pos * 4);
+    uint32_t *hist = (uint32_t *)W.hist.get(p * 4);
+    uint64_t *off = (uint64_t *)W.off.get((p + 1) * 8);
+    unsigned long long *cursor = (unsigned long long *)W.cursor.get(p * 8);
+    HIPCHK(hipMemsetAsync(hist, 0, p * 4, st));
+    Router R{};
+    unsigned grid = grid_for(n_pos, 256, 8192);
+    k_classify<S, I, false><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, bid, hist);
+    LAUNCHCHK();
+    k_scan_u32<<<1, 1024, 0, st>>>(hist, p, off);
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(cursor, off, p * 8, hipMemcpyDeviceToDevice, st));
+    tm.mark(PH_CLASSIFY);
+
+    // ---- scatter ----
+    uint64_t *keys = (uint64_t *)W.keys.get(n_pos * 8);
+    k_scatter<S, I><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, bid, cursor, keys, d_out);
+    LAUNCHCHK();
+    tm.mark(PH_SCATTER);
+
+    // ---- tiles ----
+    uint32_t *tcnt = (uint32_t *)W.tcnt.get(p * 4);
+    uint64_t *toff = (uint64_t *)W.toff.get((p + 1) * 8);
+    uint64_t max_tiles = ceil_div(n_pos, T) + p + 1;
+    Tile *tiles = (Tile *)W.tiles.get(max_tiles * sizeof(Tile));
+    Range *big = (Range *)W.big.get(p * sizeof(Range));
+    unsigned int *nbig = (unsigned int *)W.counts.get(64);
+    HIPCHK(hipMemsetAsync(nbig, 0, 64, st));
+    k_tile_counts<<<(unsigned)ceil_div(p, 256), 256, 0, st>>>(off, (uint32_t)p, T, tcnt);
+    LAUNCHCHK();
+    k_scan_u32<<<1, 1024, 0, st>>>(tcnt, p, toff);
+    LAUNCHCHK();
+    k_tile_emit<<<(unsigned)ceil_div(p, 256), 256, 0, st>>>(off, (uint32_t)p, T, toff, tiles, big, nbig);
+    LAUNCHCHK();
+    uint64_t n_tiles = 0;
+    unsigned int n_big = 0;
+    HIPCHK(hipMemcpyAsync(&n_tiles, toff + p, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&n_big, nbig, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    stats.tiles = n_tiles;
+    stats.oversized = n_big;
+    if (n_tiles) {
+        k_tile_sort<I, T, 256><<<(unsigned)n_tiles, 256, 0, st>>>(keys, d_out, tiles);
+        LAUNCHCHK();
+    }
+    tm.mark(PH_TILE);
+
+    // ---- oversized buckets: merge the sorted tiles by key ----
+    if (n_big) {
+        std::vector<Range> hb(n_big);
+        HIPCHK(hipMemcpyAsync(hb.data(), big, n_big * sizeof(Range), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        uint64_t maxlen = 0;
+        for (auto &r : hb) maxlen = std::max(maxlen, r.len);
+        uint64_t *k2 = (uint64_t *)W.keys2.get(n_pos * 8);
+        I *i2 = (I *)W.idx2.get(n_pos * sizeof(I));
+        uint64_t *ks = keys, *kd = k2;
+        I *is = d_out, *id = i2;
+        int levels = 0;
+        for (uint64_t run = T; run < maxlen; run *= 2, ++levels) {
+            for (unsigned y0 = 0; y0 < n_big; y0 += 65535) {
+                dim3 gr((unsigned)ceil_div(ceil_div(maxlen, 8), 256), std::min(65535u, n_big - y0));
+                k_key_merge<I><<<gr, 256, 0, st>>>(big + y0, run, ks, is, kd, id);
+                LAUNCHCHK();
+            }
+            std::swap(ks, kd);
+            std::swap(is, id);
+        }
+        if (levels & 1) {
+            for (unsigned y0 = 0; y0 < n_big; y0 += 65535) {
+                dim3 gr((unsigned)std::min<uint64_t>(ceil_div(maxlen, 256), 4096), std::min(65535u, n_big - y0));
+                k_copy_ranges<I><<<gr, 256, 0, st>>>(big + y0, k2, i2, keys, d_out);
+                LAUNCHCHK();
+            }
+        }
+    }
+    tm.mark(PH_MERGE);
+
+    // ---- groups of equal keys ----
+    I *lcp = want_lcp ? d_lcp_out : nullptr;
+    GroupLists gl{};
+    gl.pairs = (uint64_t *)W.pairs.get((n_pos / 2 + 1) * 8);
+    gl.small = (Group *)W.small.get((n_pos / 3 + 1) * sizeof(Group));
+    gl.mid = (Group *)W.mid.get((n_pos / (kSmallMax + 1) + 1) * sizeof(Group));
+    gl.giant = (Group *)W.giant.get((n_pos / (kMidMax + 1) + 1) * sizeof(Group));
+    gl.counts = nbig + 4;
+    HIPCHK(hipMemsetAsync(gl.counts, 0, 16, st));
+    k_detect<I><<<grid_for(n_pos, 256, 16384), 256, 0, st>>>(keys, n_pos, gl);
+    LAUNCHCHK();
+    unsigned int cnt[4];
+    HIPCHK(hipMemcpyAsync(cnt, gl.counts, 16, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    stats.pairs = cnt[0];
+    stats.small_groups = cnt[1];
+    stats.mid_groups = cnt[2];
+    stats.giant_groups = cnt[3];
+    tm.mark(PH_DETECT);
+
+    // ---- refinement ----
+    const uint64_t depth0 = g.k;
+    if (cnt[0]) {
+        k_pairs<S, I><<<(unsigned)ceil_div(cnt[0], 4), 256, 0, st>>>(d_text, L, max_ctx, gl.pairs, cnt[0], depth0,
+                                                                      d_out, lcp);
+        LAUNCHCHK();
+    }
+    if (cnt[1]) {
+        k_groups<S, I, 64, kSmallMax><<<cnt[1], 64, 0, st>>>(d_text, cm, g, L, max_ctx, gl.small, depth0, d_out, lcp,
+                                                              0, 0);
+        LAUNCHCHK();
+    }
+    if (cnt[2]) {
+        k_groups<S, I, 256, kMidMax><<<cnt[2], 256, 0, st>>>(d_text, cm, g, L, max_ctx, gl.mid, depth0, d_out, lcp,
+                                                              0, 0);
+        LAUNCHCHK();
+    }
+    tm.mark(PH_REFINE);
+
+    // ---- giant groups: chunked refinement + merge-path LCP merges ----
+    if (cnt[3]) {
+        std::vector<Group> hg(cnt[3]);
+        HIPCHK(hipMemcpyAsync(hg.data(), gl.giant, cnt[3] * sizeof(Group), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        I *lbuf = lcp ? lcp : (I *)W.lcp.get(n_pos * sizeof(I));
+        I *i2 = (I *)W.idx2.get(n_pos * sizeof(I));
+        I *l2 = (I *)W.lcp2.get(n_pos * sizeof(I));
+        for (unsigned gi = 0; gi < cnt[3]; ++gi) {
+            const Group gr = hg[gi];
+            unsigned nch = (unsigned)ceil_div(gr.size, kMidMax);
+            k_groups<S, I, 256, kMidMax><<<dim3(1, nch), 256, 0, st>>>(d_text, cm, g, L, max_ctx, gl.giant + gi,
+                                                                         depth0, d_out, lbuf, 1, kMidMax);
+            LAUNCHCHK();
+            I *is = d_out, *ls = lbuf, *id = i2, *ld = l2;
+            int levels = 0;
+            for (uint64_t run = kMidMax; run < gr.size; run *= 2, ++levels) {
+                W.h_jobs.clear();
+                for (uint64_t b = 0; b < gr.size; b += 2 * run) {
+                    MergeJob jb;
+                    jb.start = gr.start + b;
+                    jb.nx = std::min<uint64_t>(run, gr.size - b);
+                    jb.ny = std::min<uint64_t>(run, gr.size - b - jb.nx);
+                    W.h_jobs.push_back(jb);
+                }
+                MergeJob *dj = (MergeJob *)W.jobs.get(W.h_jobs.size() * sizeof(MergeJob));
+                HIPCHK(hipMemcpyAsync(dj, W.h_jobs.data(), W.h_jobs.size() * sizeof(MergeJob), hipMemcpyHostToDevice,
+                                      st));
+                constexpr int MW = 32;
+                dim3 gr2((unsigned)ceil_div(ceil_div(2 * run, MW), 128), (unsigned)W.h_jobs.size());
+                k_lcp_merge<S, I, MW><<<gr2, 128, 0, st>>>(d_text, L, max_ctx, depth0, dj, is, ls, id, ld);
+                LAUNCHCHK();
+                HIPCHK(hipStreamSynchronize(st));  // h_jobs is reused by the next level
+                std::swap(is, id);
+                std::swap(ls, ld);
+            }
+            if (levels & 1) {
+                k_copy_idx_lcp<I><<<grid_for(gr.size, 256, 4096), 256, 0, st>>>(gr.start, gr.size, i2, l2, d_out,
+                                                                                 lbuf);
+                LAUNCHCHK();
+            }
+        }
+    }
+    if (lcp) {
+        k_boundary_lcp<I><<<grid_for(n_pos, 256, 16384), 256, 0, st>>>(keys, d_out, n_pos, L, max_ctx, g, lcp);
+        LAUNCHCHK();
+    }
+    tm.mark(PH_GIANT);
+    HIPCHK(hipStreamSynchronize(st));
+    tm.collect(stats);
+}
+
+template void build_impl<uint8_t, uint32_t>(const uint8_t *, uint64_t, const uint32_t *, uint64_t,
+                                            const caps_limits_t *, const caps_opts_t &, uint32_t *, uint32_t *,
+                                            hipStream_t, caps_stats_t &);
+
+// ---------------------------------------------------------------------------
+// Small device helpers used by the C ABI.
+// ---------------------------------------------------------------------------
+struct HostText {
+    const void *d;
+    uint64_t n;
+    uint32_t w;
+};
+
+static const void *upload_text(Workspace &W, const void *text, uint64_t n, uint32_t w, hipStream_t st) {
+    void *d = W.text.get(n * w + 16);
+    if (n) HIPCHK(hipMemcpyAsync(d, text, n * w, hipMemcpyHostToDevice, st));
+    return d;
+}
+
+template <typename S, typename I>
+static void build_host(const void *text, uint64_t n, const void *positions, uint64_t n_pos,
+                       const caps_limits_t *lim, const caps_opts_t &o, void *out_sa, void *out_lcp) {
+    Workspace &W = workspace();
+    hipStream_t st = nullptr;
+    const S *dt = (const S *)upload_text(W, text, n, sizeof(S), st);
+    const I *dp = nullptr;
+    if (positions) {
+        I *d = (I *)W.hpos.get(n_pos * sizeof(I) + 8);
+        if (n_pos) HIPCHK(hipMemcpyAsync(d, positions, n_pos * sizeof(I), hipMemcpyHostToDevice, st));
+        dp = d;
+    }
+    I *dout = (I *)W.out.get(n_pos * sizeof(I) + 8);
+    I *dl = out_lcp ? (I *)W.outlcp.get(n_pos * sizeof(I) + 8) : nullptr;
+    build_impl<S, I>(dt, n, dp, n_pos, lim, o, dout, dl, st, g_stats);
+    if (n_pos) {
+        HIPCHK(hipMemcpyAsync(out_sa, dout, n_pos * sizeof(I), hipMemcpyDeviceToHost, st));
+        if (out_lcp) HIPCHK(hipMemcpyAsync(out_lcp, dl, n_pos * sizeof(I), hipMemcpyDeviceToHost, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+}
+
+template <typename S, typename I>
+static void dispatch_positions(const void *text, uint64_t n, const void *positions, uint64_t n_pos,
+                               const caps_limits_t *lim, const caps_opts_t &o, void *out_sa, void *out_lcp) {
+    build_host<S, I>(text, n, positions, n_pos, lim, o, out_sa, out_lcp);
+}
+
+static void check_common(const void *text, uint64_t n, uint32_t sym_bytes, const caps_limits_t *lim) {
+    if (!text && n) throw CapsError(CAPS_E_INVALID, "text is NULL");
+    if (sym_bytes != 1 && sym_bytes != 2)
+        throw CapsError(CAPS_E_UNSUPPORTED, "the GPU build covers 1- and 2-byte symbols only");
+    if (lim && lim->kind == CAPS_LIMITS_SEGMENTED && lim->n_ends && !lim->ends)
+        throw CapsError(CAPS_E_INVALID, "segment ends pointer is NULL");
+}
+
+template <typename F>
+static int guarded(F &&f) {
+    try {
+        std::lock_guard<std::mutex> lk(g_mu);
+        f();
+        return CAPS_OK;
+    } catch (const CapsError &e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_err = "host allocation failed";
+        return CAPS_E_NOMEM;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return CAPS_E_DEVICE;
+    }
+}
+
+static caps_opts_t opts_or_default(const caps_opts_t *o) {
+    caps_opts_t d;
+    caps_opts_default(&d);
+    return o ? *o : d;
+}
+
+template <typename S, typename I>
+static void run_device(const void *dt, uint64_t n, const void *dp, uint64_t n_pos, const caps_limits_t *lim,
+                       const caps_opts_t &o, void *dout, void *dl, hipStream_t st) {
+    build_impl<S, I>((const S *)dt, n, (const I *)dp, n_pos, lim, o, (I *)dout, (I *)dl, st, g_stats);
+}
+
+}  // namespace caps
+
+using namespace caps;
+
+extern "C" {
+
+void caps_opts_default(caps_opts_t *o) {
+    memset(o, 0, sizeof(*o));
+    o->max_context = ~0ull;
+}
+
+const char *caps_last_error(void) { return g_err.c_str(); }
+
+int caps_last_emit_code(void) { return g_emit_code; }
+
+int caps_device_count(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+int caps_last_stats(caps_stats_t *out) {
+    if (!out) return CAPS_E_INVALID;
+    *out = g_stats;
+    return CAPS_OK;
+}
+
+int caps_build_in_memory_for_positions(const void *text, uint64_t n, uint32_t sym_bytes, const void *positions,
+                                       uint64_t n_pos, uint32_t idx_bytes, const caps_limits_t *limits,
+                                       const caps_opts_t *opts, void *out_sa, void *out_lcp) {
+    return guarded([&] {
+        check_common(text, n, sym_bytes, limits);
+        if (!out_sa && n_pos) throw CapsError(CAPS_E_INVALID, "out_sa is NULL");
+        if (idx_bytes != 4 && idx_bytes != 8) throw CapsError(CAPS_E_UNSUPPORTED, "index width must be 4 or 8");
+        if (idx_bytes == 4 && n > (1ull << 32)) throw CapsError(CAPS_E_INVALID, "u32 index cannot address text");
+        caps_opts_t o = opts_or_default(opts);
+        if (sym_bytes == 1 && idx_bytes == 4) build_host<uint8_t, uint32_t>(text, n, positions, n_pos, limits, o, out_sa, out_lcp);
+        else if (sym_bytes == 1) build_host<uint8_t, uint64_t>(text, n, positions, n_pos, limits, o, out_sa, out_lcp);
+        else if (idx_bytes == 4) build_host<uint16_t, uint32_t>(text, n, positions, n_pos, limits, o, out_sa, out_lcp);
+        else build_host<uint16_t, uint64_t>(text, n, positions, n_pos, limits, o, out_sa, out_lcp);
+    });
+}
+
+int caps_build_in_memory(const void *text, uint64_t n, uint32_t sym_bytes, uint32_t idx_bytes,
+                         const caps_limits_t *limits, const caps_opts_t *opts, void *out_sa, void *out_lcp) {
+    return caps_build_in_memory_for_positions(text, n, sym_bytes, nullptr, n, idx_bytes, limits, opts, out_sa,
+                                              out_lcp);
+}
+
+static int stream_build(const void *text, uint64_t n, uint32_t sym_bytes, const uint64_t *positions, uint64_t n_pos,
+                        const caps_limits_t *limits, const caps_opts_t *opts, caps_emit_fn emit, void *ctx) {
+    g_emit_code = 0;
+    std::vector<uint64_t> sa64;
+    int rc = guarded([&] {
+        check_common(text, n, sym_bytes, limits);
+        if (!emit) throw CapsError(CAPS_E_INVALID, "emit callback is NULL");
+        caps_opts_t o = opts_or_default(opts);
+        sa64.resize(n_pos);
+        // index width follows the reference: u32 iff text.len() <= u32::MAX + 1
+        bool narrow = n <= (1ull << 32);
+        if (narrow) {
+            std::vector<uint32_t> pos32, out32(n_pos);
+            if (positions) {
+                pos32.resize(n_pos);
+                for (uint64_t i = 0; i < n_pos; ++i) {
+                    if (positions[i] >= n) throw CapsError(CAPS_E_INVALID, "position out of range");
+                    pos32[i] = (uint32_t)positions[i];
+                }
+            }
+            if (sym_bytes == 1)
+                build_host<uint8_t, uint32_t>(text, n, positions ? pos32.data() : nullptr, n_pos, limits, o,
+                                              out32.data(), nullptr);
+            else
+                build_host<uint16_t, uint32_t>(text, n, positions ? pos32.data() : nullptr, n_pos, limits, o,
+                                               out32.data(), nullptr);
+            for (uint64_t i = 0; i < n_pos; ++i) sa64[i] = out32[i];
+        } else {
+            if (positions)
+                for (uint64_t i = 0; i < n_pos; ++i)
+                    if (positions[i] >= n) throw CapsError(CAPS_E_INVALID, "position out of range");
+            if (sym_bytes == 1)
+                build_host<uint8_t, uint64_t>(text, n, positions, n_pos, limits, o, sa64.data(), nullptr);
+            else
+                build_host<uint16_t, uint64_t>(text, n, positions, n_pos, limits, o, sa64.data(), nullptr);
+        }
+    });
+    if (rc != CAPS_OK) return rc;
+    // emit in order, in batches, on the caller's thread (ext_mem.rs:1933-1937)
+    const uint64_t B = 1 << 20;
+    for (uint64_t i = 0; i < n_pos; i += B) {
+        int r = emit(ctx, sa64.data() + i, std::min(B, n_pos - i));
+        if (r != 0) {
+            g_emit_code = r;
+            g_err = "emit callback aborted the build";
+            return CAPS_E_EMIT;
+        }
+    }
+    return CAPS_OK;
+}
+
+int caps_build_in_memory_sample_sort(const void *text, uint64_t n, uint32_t sym_bytes, const uint64_t *positions,
+                                     uint64_t n_pos, const caps_limits_t *limits, const caps_opts_t *opts,
+                                     caps_emit_fn emit, void *ctx) {
+    return stream_build(text, n, sym_bytes, positions, positions ? n_pos : n, limits, opts, emit, ctx);
+}
+
+int caps_build_ext_mem_for_positions(const void *text, uint64_t n, uint32_t sym_bytes, const uint64_t *positions,
+                                     uint64_t n_pos, const caps_limits_t *limits, const caps_opts_t *opts,
+                                     caps_emit_fn emit, void *ctx) {
+    if (!positions && n_pos) {
+        g_err = "positions is NULL";
+        return CAPS_E_INVALID;
+    }
+    return stream_build(text, n, sym_bytes, positions, n_pos, limits, opts, emit, ctx);
+}
+
+int caps_build_device(const void *d_text, uint64_t n, uint32_t sym_bytes, const void *d_positions, uint64_t n_pos,
+                      uint32_t idx_bytes, const caps_limits_t *limits, const caps_opts_t *opts, void *d_out_sa,
+                      void *d_out_lcp, void *stream) {
+    return guarded([&] {
+        check_common(d_text, n, sym_bytes, limits);
+        if (idx_bytes != 4 && idx_bytes != 8) throw CapsError(CAPS_E_UNSUPPORTED, "index width must be 4 or 8");
+        if (idx_bytes == 4 && n > (1ull << 32)) throw CapsError(CAPS_E_INVALID, "u32 index cannot address text");
+        if (!d_positions) n_pos = n;
+        caps_opts_t o = opts_or_default(opts);
+        hipStream_t st = (hipStream_t)stream;
+        if (sym_bytes == 1 && idx_bytes == 4) run_device<uint8_t, uint32_t>(d_text, n, d_positions, n_pos, limits, o, d_out_sa, d_out_lcp, st);
+        else if (sym_bytes == 1) run_device<uint8_t, uint64_t>(d_text, n, d_positions, n_pos, limits, o, d_out_sa, d_out_lcp, st);
+        else if (idx_bytes == 4) run_device<uint16_t, uint32_t>(d_text, n, d_positions, n_pos, limits, o, d_out_sa, d_out_lcp, st);
+        else run_device<uint16_t, uint64_t>(d_text, n, d_positions, n_pos, limits, o, d_out_sa, d_out_lcp, st);
+    });
+}
+
+int caps_lcp_pairs_device(const void *d_text, uint64_t n, uint32_t sym_bytes, const uint64_t *d_pairs,
+                          uint64_t count, uint64_t max_ctx, uint64_t *d_out, void *stream) {
+    return guarded([&] {
+        check_common(d_text, n, sym_bytes, nullptr);
+        hipStream_t st = (hipStream_t)stream;
+        if (!count) return;
+        unsigned gr = (unsigned)ceil_div(count, 4);
+        if (sym_bytes == 1) k_lcp_pairs<uint8_t><<<gr, 256, 0, st>>>((const uint8_t *)d_text, n, d_pairs, count, max_ctx, d_out);
+        else k_lcp_pairs<uint16_t><<<gr, 256, 0, st>>>((const uint16_t *)d_text, n, d_pairs, count, max_ctx, d_out);
+        LAUNCHCHK();
+        HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
+int caps_suffix_cmp_pairs_device(const void *d_text, uint64_t n, uint32_t sym_bytes, const caps_limits_t *limits,
+                                 const uint64_t *d_pairs, uint64_t count, uint64_t max_ctx, int32_t *d_out,
+                                 void *stream) {
+    return guarded([&] {
+        check_common(d_text, n, sym_bytes, limits);
+        hipStream_t st = (hipStream_t)stream;
+        DevLimits L = upload_limits(workspace(), limits, n, st);
+        if (!count) return;
+        unsigned gr = (unsigned)ceil_div(count, 4);
+        if (sym_bytes == 1) k_cmp_pairs<uint8_t><<<gr, 256, 0, st>>>((const uint8_t *)d_text, L, d_pairs, count, max_ctx, d_out);
+        else k_cmp_pairs<uint16_t><<<gr, 256, 0, st>>>((const uint16_t *)d_text, L, d_pairs, count, max_ctx, d_out);
+        LAUNCHCHK();
+        HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
+int caps_seed_keys_device(const void *d_text, uint64_t n, uint32_t sym_bytes, const caps_limits_t *limits,
+                          uint32_t key_geometry, uint32_t rank, const uint64_t *d_pos, uint64_t count,
+                          uint64_t *d_keys, uint32_t *geometry_out, void *stream) {
+    return guarded([&] {
+        check_common(d_text, n, sym_bytes, limits);
+        hipStream_t st = (hipStream_t)stream;
+        Workspace &W = workspace();
+        DevLimits L = upload_limits(W, limits, n, st);
+        uint32_t order = rank ? CAPS_ORDER_STAR : CAPS_ORDER_SHORTER_FIRST;
+        unsigned gr = (unsigned)ceil_div(count ? count : 1, 256);
+        if (sym_bytes == 1) {
+            Geometry G = compute_geometry<uint8_t>(W, (const uint8_t *)d_text, n, key_geometry, order, st);
+            if (key_geometry == CAPS_KEY_REFERENCE && !G.reference)
+                throw CapsError(CAPS_E_UNSUPPORTED, "no sentinel code fits the alphabet (pack.rs:174-176)");
+            if (geometry_out) geometry_out[0] = G.g.bits, geometry_out[1] = G.g.k;
+            if (count) k_seed_keys<uint8_t><<<gr, 256, 0, st>>>((const uint8_t *)d_text, codemap<uint8_t>(G), G.g, L, d_pos, count, d_keys);
+        } else {
+            if (key_geometry == CAPS_KEY_REFERENCE)
+                throw CapsError(CAPS_E_UNSUPPORTED, "the reference key geometry is u8-only (pack.rs:223-228)");
+            Geometry G = compute_geometry<uint16_t>(W, (const uint16_t *)d_text, n, key_geometry, order, st);
+            if (geometry_out) geometry_out[0] = G.g.bits, geometry_out[1] = G.g.k;
+            if (count) k_seed_keys<uint16_t><<<gr, 256, 0, st>>>((const uint16_t *)d_text, codemap<uint16_t>(G), G.g, L, d_pos, count, d_keys);
+        }
+        LAUNCHCHK();
+        HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
+}  // extern "C"
+
+namespace caps {
+template <typename S>
+static void merge_jobs_host(const void *text, uint64_t n, const caps_limits_t *lim, const uint64_t *sa,
+                            const uint64_t *lcp, uint64_t n_sa, std::vector<uint64_t> runs, uint64_t max_ctx,
+                            uint64_t *out_sa, uint64_t *out_lcp) {
+    Workspace &W = workspace();
+    hipStream_t st = nullptr;
+    const S *dt = (const S *)upload_text(W, text, n, sizeof(S), st);
+    DevLimits L = upload_limits(W, lim, n, st);
+    uint64_t *a_sa = (uint64_t *)W.m_a.get(n_sa * 8 + 8), *a_l = (uint64_t *)W.m_al.get(n_sa * 8 + 8);
+    uint64_t *b_sa = (uint64_t *)W.m_b.get(n_sa * 8 + 8), *b_l = (uint64_t *)W.m_bl.get(n_sa * 8 + 8);
+    HIPCHK(hipMemcpyAsync(a_sa, sa, n_sa * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(a_l, lcp, n_sa * 8, hipMemcpyHostToDevice, st));
+    uint64_t *s_sa = a_sa, *s_l = a_l, *d_sa = b_sa, *d_l = b_l;
+    while (runs.size() > 1) {
+        W.h_jobs.clear();
+        std::vector<uint64_t> next;
+        uint64_t off = 0, maxlen = 0;
+        for (size_t i = 0; i < runs.size(); i += 2) {
+            MergeJob jb;
+            jb.start = off;
+            jb.nx = runs[i];
+            jb.ny = i + 1 < runs.size() ? runs[i + 1] : 0;
+            W.h_jobs.push_back(jb);
+            next.push_back(jb.nx + jb.ny);
+            off += jb.nx + jb.ny;
+            maxlen = std::max(maxlen, jb.nx + jb.ny);
+        }
+        MergeJob *dj = (MergeJob *)W.jobs.get(W.h_jobs.size() * sizeof(MergeJob));
+        HIPCHK(hipMemcpyAsync(dj, W.h_jobs.data(), W.h_jobs.size() * sizeof(MergeJob), hipMemcpyHostToDevice, st));
+        constexpr int MW = 32;
+        dim3 gr((unsigned)ceil_div(ceil_div(maxlen, MW), 128), (unsigned)W.h_jobs.size());
+        k_lcp_merge<S, uint64_t, MW><<<gr, 128, 0, st>>>(dt, L, max_ctx, 0, dj, s_sa, s_l, d_sa, d_l);
+        LAUNCHCHK();
+        HIPCHK(hipStreamSynchronize(st));
+        std::swap(s_sa, d_sa);
+        std::swap(s_l, d_l);
+        runs.swap(next);
+    }
+    HIPCHK(hipMemcpyAsync(out_sa, s_sa, n_sa * 8, hipMemcpyDeviceToHost, st));
+    if (out_lcp) HIPCHK(hipMemcpyAsync(out_lcp, s_l, n_sa * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+}
+}  // namespace caps
+
+extern "C" {
+
+int caps_merge_lcp(const void *text, uint64_t n, uint32_t sym_bytes, const caps_limits_t *limits, const uint64_t *x,
+                   const uint64_t *lcp_x, uint64_t nx, const uint64_t *y, const uint64_t *lcp_y, uint64_t ny,
+                   uint64_t max_ctx, uint64_t *z, uint64_t *lcp_z) {
+    return guarded([&] {
+        check_common(text, n, sym_bytes, limits);
+        std::vector<uint64_t> sa(nx + ny), lc(nx + ny);
+        std::copy(x, x + nx, sa.begin());
+        std::copy(y, y + ny, sa.begin() + nx);
+        std::copy(lcp_x, lcp_x + nx, lc.begin());
+        std::copy(lcp_y, lcp_y + ny, lc.begin() + nx);
+        std::vector<uint64_t> runs;
+        runs.push_back(nx);
+        runs.push_back(ny);
+        if (nx + ny == 0) return;
+        if (sym_bytes == 1) merge_jobs_host<uint8_t>(text, n, limits, sa.data(), lc.data(), nx + ny, runs, max_ctx, z, lcp_z);
+        else merge_jobs_host<uint16_t>(text, n, limits, sa.data(), lc.data(), nx + ny, runs, max_ctx, z, lcp_z);
+    });
+}
+
+int caps_cascade_merge(const void *text, uint64_t n, uint32_t sym_bytes, const caps_limits_t *limits,
+                       const uint64_t *sa, const uint64_t *lcp, uint64_t n_sa, const uint64_t *boundaries,
+                       uint64_t n_boundaries, uint64_t max_ctx, uint64_t *out_sa, uint64_t *out_lcp) {
+    return guarded([&] {
+        check_common(text, n, sym_bytes, limits);
+        if (n_sa == 0) return;
+        std::vector<uint64_t> runs;  // ext_mem.rs:2215-2221: non-empty sub-subarrays
+        for (uint64_t i = 0; i + 1 < n_boundaries; ++i)
+            if (boundaries[i + 1] > boundaries[i]) runs.push_back(boundaries[i + 1] - boundaries[i]);
+        if (sym_bytes == 1) merge_jobs_host<uint8_t>(text, n, limits, sa, lcp, n_sa, runs, max_ctx, out_sa, out_lcp);
+        else merge_jobs_host<uint16_t>(text, n, limits, sa, lcp, n_sa, runs, max_ctx, out_sa, out_lcp);
+    });
+}
+
+}  // extern "C"

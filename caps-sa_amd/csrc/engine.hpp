@@ -1,0 +1,58 @@
+// Host-side state of the suffix-array build: grow-only device buffers kept
+// across calls so repeated builds do not reallocate.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace caps {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    void *get(size_t bytes);
+    void release();
+};
+
+struct Geometry {
+    KeyGeom g;
+    const void *code = nullptr;  // device code map or nullptr (identity)
+    bool remap = false;
+    bool reference = false;
+    uint32_t distinct = 0;
+};
+
+struct Workspace {
+    // inputs staged from host pointers
+    DevBuf text, hpos, out, outlcp, pos_copy;
+    // limits
+    DevBuf ends, dir;
+    std::vector<uint64_t> h_ends;
+    std::vector<uint32_t> h_dir;
+    // key geometry
+    DevBuf bitmap, code;
+    std::vector<uint8_t> h_code8;
+    std::vector<uint16_t> h_code16;
+    // sample and splitters
+    DevBuf skeys, skeys2, sids, sids2, spl, stiles, srange;
+    std::vector<Tile> h_tiles;
+    Range h_range;
+    // classification / scatter
+    DevBuf bid, hist, off, cursor, keys, keys2, idx2;
+    // tiles
+    DevBuf tcnt, toff, tiles, big, counts;
+    // refinement
+    DevBuf pairs, small, mid, giant, lcp, lcp2, first, jobs;
+    std::vector<MergeJob> h_jobs;
+    // standalone merges
+    DevBuf m_a, m_al, m_b, m_bl;
+    // router
+    DevBuf r_a, r_b, r_lo, r_hi, r_stats, r_x, r_y, r_w;
+};
+
+Workspace &workspace();
+
+}  // namespace caps

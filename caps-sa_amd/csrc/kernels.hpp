@@ -1,0 +1,839 @@
+// HIP kernels of the suffix-array build (gfx950, wave64).
+//
+// Pipeline (one text, any position subset):
+//   alphabet scan -> sample keys -> splitters -> classify+histogram ->
+//   scan -> scatter (key, idx) into buckets -> LDS tile sort by key ->
+//   key merges for oversized buckets -> group detection (equal keys) ->
+//   pair refinement (wave LCP) / group refinement (rounds of k-symbol keys) /
+//   giant groups (chunked refinement + merge-path LCP merge).
+#pragma once
+#include "device.hpp"
+
+namespace caps {
+
+// ---------------------------------------------------------------------------
+// Alphabet scan (pack.rs:105-122): presence bitmap of the symbol values.
+// ---------------------------------------------------------------------------
+template <typename S>
+__global__ __launch_bounds__(256) void k_alphabet(const S *__restrict__ t, uint64_t n, uint32_t *__restrict__ bitmap) {
+    constexpr uint32_t WORDS = sizeof(S) == 1 ? 8 : 2048;
+    __shared__ uint32_t bm[WORDS];
+    for (uint32_t w = threadIdx.x; w < WORDS; w += blockDim.x) bm[w] = 0;
+    __syncthreads();
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        uint32_t c = t[i];
+        uint32_t bit = 1u << (c & 31);
+        if (!(bm[c >> 5] & bit)) atomicOr(&bm[c >> 5], bit);
+    }
+    __syncthreads();
+    for (uint32_t w = threadIdx.x; w < WORDS; w += blockDim.x)
+        if (bm[w]) atomicOr(&bitmap[w], bm[w]);
+}
+
+// ---------------------------------------------------------------------------
+// Keys of evenly spaced sample positions (ext_mem.rs:1291-1305 midpoints).
+// ---------------------------------------------------------------------------
+template <typename S, typename I>
+__global__ __launch_bounds__(256) void k_sample_keys(const S *__restrict__ t, CodeMap<S> cm, KeyGeom g, DevLimits L,
+                                                     uint64_t max_ctx, const I *__restrict__ pos, uint64_t n_pos,
+                                                     uint64_t s, uint64_t *__restrict__ keys, uint32_t *__restrict__ ids) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s) return;
+    uint64_t j = ((2 * i + 1) * n_pos) / (2 * s);
+    uint64_t p = pos ? (uint64_t)pos[j] : j;
+    keys[i] = key_at(t, cm, g, p, 0, cap_at(L, p, max_ctx));
+    ids[i] = (uint32_t)i;
+}
+
+// splitter j-1 = sorted_sample[(j * s) / p], j = 1..p-1 (ext_mem.rs:1341).
+__global__ void k_pick_splitters(const uint64_t *__restrict__ sorted, uint64_t s, uint32_t p,
+                                 uint64_t *__restrict__ spl) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (j >= p) return;
+    spl[j - 1] = sorted[((uint64_t)j * s) / p];
+}
+
+// ---------------------------------------------------------------------------
+// Bucket classification.  Bucket of a key = number of splitters <= key, so
+// equal keys always share a bucket.  The splitter table is searched with an
+// LDS-resident coarse table (every COARSE-th splitter) followed by a short
+// search in the global table.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kCoarseMax = 4096;
+
+struct Splitters {
+    const uint64_t *spl;  // p-1 sorted keys
+    uint32_t n_spl;       // p-1
+    uint32_t stride;      // coarse stride
+    uint32_t n_coarse;    // ceil(n_spl / stride)
+};
+
+// Number of splitters <= key.
+__device__ __forceinline__ uint32_t upper_bound_coarse(const Splitters &S, const uint64_t *coarse, uint64_t key) {
+    // coarse[c] = spl[c*stride]; find the number of coarse entries <= key.
+    uint32_t lo = 0, hi = S.n_coarse;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (coarse[mid] <= key) lo = mid + 1;
+        else hi = mid;
+    }
+    // all splitters before index (lo-1)*stride+1 .. are known; refine in [base, base+stride)
+    if (lo == 0) return 0;
+    uint32_t b = (lo - 1) * S.stride + 1;
+    uint32_t e = b + S.stride - 1;
+    if (e > S.n_spl) e = S.n_spl;
+    while (b < e) {
+        uint32_t mid = (b + e) >> 1;
+        if (S.spl[mid] <= key) b = mid + 1;
+        else e = mid;
+    }
+    return b;
+}
+
+// Learned router: a monotone two-stage piecewise-linear CDF model.  Stage 1
+// maps the key to one of `n_seg` segments, stage 2 maps it to a predicted
+// bucket inside the segment's clamp range.  The prediction is then checked
+// against its two neighbouring splitters and corrected by a bounded gallop,
+// with the exact binary search as the fallback and the judge.
+struct Router {
+    const float *seg_a;      // per-segment slope
+    const float *seg_b;      // per-segment intercept
+    const uint32_t *seg_lo;  // per-segment clamp range
+    const uint32_t *seg_hi;
+    float root_a, root_b;    // stage-1 linear model on x in [0,1)
+    uint64_t kmin;           // key normalisation
+    float kscale;            // 1 / (kmax - kmin) as float
+    uint32_t n_seg;
+    uint32_t window;         // gallop bound before falling back
+    uint32_t enabled;
+    unsigned long long *stats;  // [0]=sum |err|, [1]=max |err|, [2]=fallbacks, [3]=count (nullable)
+};
+
+__device__ __forceinline__ float router_x(const Router &R, uint64_t key) {
+    uint64_t d = key > R.kmin ? key - R.kmin : 0;
+    return (float)d * R.kscale;
+}
+
+__device__ __forceinline__ uint32_t router_predict(const Router &R, const float *sa, const float *sb,
+                                                   const uint32_t *slo, const uint32_t *shi, uint64_t key) {
+    float x = router_x(R, key);
+    float sf = R.root_a * x + R.root_b;
+    int sidx = (int)sf;
+    if (sf < 0.f) sidx = 0;
+    if (sidx >= (int)R.n_seg) sidx = (int)R.n_seg - 1;
+    float y = sa[sidx] * x + sb[sidx];
+    int b = (int)(y + 0.5f);
+    if (y < 0.f) b = 0;
+    uint32_t lo = slo[sidx], hi = shi[sidx];
+    uint32_t ub = (uint32_t)b;
+    if (ub < lo) ub = lo;
+    if (ub > hi) ub = hi;
+    return ub;
+}
+
+// Exact bucket given a predicted one: verify spl[b-1] <= key < spl[b],
+// otherwise gallop up to `window` steps, otherwise binary search.
+__device__ __forceinline__ uint32_t router_correct(const Router &R, const Splitters &S, const uint64_t *coarse,
+                                                   uint64_t key, uint32_t b, uint32_t *fallback) {
+    const uint64_t *spl = S.spl;
+    uint32_t np = S.n_spl;
+    bool lo_ok = (b == 0) || spl[b - 1] <= key;
+    bool hi_ok = (b == np) || key < spl[b];
+    if (lo_ok && hi_ok) return b;
+    if (!hi_ok) {  // true bucket is above b
+        uint32_t lo = b + 1, step = 1, hi = lo;
+        for (uint32_t it = 0; it < R.window; ++it) {
+            hi = lo + step;
+            if (hi > np) hi = np;
+            if (hi == np || key < spl[hi]) {
+                // answer in [lo, hi]: number of splitters <= key
+                uint32_t a = lo, e = hi;
+                while (a < e) {
+                    uint32_t mid = (a + e) >> 1;
+                    if (spl[mid] <= key) a = mid + 1;
+                    else e = mid;
+                }
+                return a;
+            }
+            lo = hi + 1;
+            step <<= 1;
+        }
+    } else {  // true bucket is below b
+        uint32_t hi = b - 1, step = 1, lo = hi;
+        for (uint32_t it = 0; it < R.window; ++it) {
+            lo = hi >= step ? hi - step : 0;
+            if (lo == 0 || spl[lo - 1] <= key) {
+                uint32_t a = lo, e = hi;
+                while (a < e) {
+                    uint32_t mid = (a + e) >> 1;
+                    if (spl[mid] <= key) a = mid + 1;
+                    else e = mid;
+                }
+                return a;
+            }
+            hi = lo - 1;
+            step <<= 1;
+        }
+    }
+    *fallback = 1;
+    return upper_bound_coarse(S, coarse, key);
+}
+
+template <typename S, typename I, bool ROUTER>
+__global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeMap<S> cm, KeyGeom g, DevLimits L,
+                                                  uint64_t max_ctx, const I *__restrict__ pos, uint64_t n_pos,
+                                                  Splitters SP, Router R, uint32_t *__restrict__ bid,
+                                                  uint32_t *__restrict__ hist) {
+    __shared__ uint64_t coarse[kCoarseMax];
+    __shared__ float ra[ROUTER ? 1024 : 1], rb[ROUTER ? 1024 : 1];
+    __shared__ uint32_t rlo[ROUTER ? 1024 : 1], rhi[ROUTER ? 1024 : 1];
+    __shared__ unsigned long long st_sum, st_max, st_fb, st_cnt;
+    for (uint32_t c = threadIdx.x; c < SP.n_coarse; c += blockDim.x) coarse[c] = SP.spl[(uint64_t)c * SP.stride];
+    if (ROUTER) {
+        for (uint32_t c = threadIdx.x; c < R.n_seg; c += blockDim.x) {
+            ra[c] = R.seg_a[c];
+            rb[c] = R.seg_b[c];
+            rlo[c] = R.seg_lo[c];
+            rhi[c] = R.seg_hi[c];
+        }
+        if (threadIdx.x == 0) st_sum = st_max = st_fb = st_cnt = 0;
+    }
+    __syncthreads();
+    unsigned long long my_sum = 0, my_max = 0, my_fb = 0, my_cnt = 0;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pos; i += stride) {
+        uint64_t p = pos ? (uint64_t)pos[i] : i;
+        uint64_t key = key_at(t, cm, g, p, 0, cap_at(L, p, max_ctx));
+        uint32_t b;
+        if (ROUTER) {
+            uint32_t pred = router_predict(R, ra, rb, rlo, rhi, key);
+            uint32_t fb = 0;
+            b = router_correct(R, SP, coarse, key, pred, &fb);
+            if (R.stats) {
+                unsigned long long e = b > pred ? b - pred : pred - b;
+                my_sum += e;
+                my_max = e > my_max ? e : my_max;
+                my_fb += fb;
+                my_cnt += 1;
+            }
+        } else {
+            b = upper_bound_coarse(SP, coarse, key);
+        }
+        bid[i] = b;
+        atomicAdd(&hist[b], 1u);
+    }
+    if (ROUTER && R.stats) {
+        atomicAdd(&st_sum, my_sum);
+        atomicMax(&st_max, my_max);
+        atomicAdd(&st_fb, my_fb);
+        atomicAdd(&st_cnt, my_cnt);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            atomicAdd(&R.stats[0], st_sum);
+            atomicMax(&R.stats[1], st_max);
+            atomicAdd(&R.stats[2], st_fb);
+            atomicAdd(&R.stats[3], st_cnt);
+        }
+    }
+}
+
+// Exclusive scan of m u32 counts into m+1 u64 offsets (one workgroup).
+__global__ __launch_bounds__(1024) void k_scan_u32(const uint32_t *__restrict__ in, uint64_t m,
+                                                   uint64_t *__restrict__ out) {
+    __shared__ uint64_t s[1024];
+    uint32_t tid = threadIdx.x;
+    uint64_t per = (m + 1023) / 1024;
+    uint64_t b = tid * per, e = b + per;
+    if (b > m) b = m;
+    if (e > m) e = m;
+    uint64_t sum = 0;
+    for (uint64_t i = b; i < e; ++i) sum += in[i];
+    s[tid] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        uint64_t v = tid >= off ? s[tid - off] : 0;
+        __syncthreads();
+        s[tid] += v;
+        __syncthreads();
+    }
+    uint64_t run = tid ? s[tid - 1] : 0;
+    for (uint64_t i = b; i < e; ++i) {
+        out[i] = run;
+        run += in[i];
+    }
+    if (tid == 1023) out[m] = s[1023];
+}
+
+template <typename S, typename I>
+__global__ __launch_bounds__(256) void k_scatter(const S *__restrict__ t, CodeMap<S> cm, KeyGeom g, DevLimits L,
+                                                 uint64_t max_ctx, const I *__restrict__ pos, uint64_t n_pos,
+                                                 const uint32_t *__restrict__ bid, unsigned long long *__restrict__ cursor,
+                                                 uint64_t *__restrict__ keys, I *__restrict__ idx) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pos; i += stride) {
+        uint64_t p = pos ? (uint64_t)pos[i] : i;
+        uint64_t key = key_at(t, cm, g, p, 0, cap_at(L, p, max_ctx));
+        unsigned long long slot = atomicAdd(&cursor[bid[i]], 1ull);
+        keys[slot] = key;
+        idx[slot] = (I)p;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tiles: every bucket is cut into tiles of at most T records.
+// ---------------------------------------------------------------------------
+struct Tile {
+    uint64_t start;
+    uint32_t len;
+    uint32_t pad;
+};
+
+__global__ void k_tile_counts(const uint64_t *__restrict__ off, uint32_t p, uint32_t T, uint32_t *__restrict__ cnt) {
+    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= p) return;
+    uint64_t sz = off[b + 1] - off[b];
+    cnt[b] = (uint32_t)((sz + T - 1) / T);
+}
+
+struct Range {
+    uint64_t start;
+    uint64_t len;
+};
+
+__global__ void k_tile_emit(const uint64_t *__restrict__ off, uint32_t p, uint32_t T,
+                            const uint64_t *__restrict__ toff, Tile *__restrict__ tiles, Range *__restrict__ big,
+                            unsigned int *__restrict__ n_big) {
+    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= p) return;
+    uint64_t s = off[b], sz = off[b + 1] - s;
+    uint64_t o = toff[b];
+    for (uint64_t x = 0; x < sz; x += T) {
+        Tile tl;
+        tl.start = s + x;
+        tl.len = (uint32_t)(sz - x < T ? sz - x : T);
+        tl.pad = 0;
+        tiles[o++] = tl;
+    }
+    if (sz > T) {
+        unsigned int k = atomicAdd(n_big, 1u);
+        big[k].start = s;
+        big[k].len = sz;
+    }
+}
+
+// LDS bitonic sort of one tile by (key, original slot): stable, so equal
+// keys keep their scatter order.
+template <typename I, int T, int B>
+__global__ __launch_bounds__(B) void k_tile_sort(uint64_t *__restrict__ keys, I *__restrict__ idx,
+                                                 const Tile *__restrict__ tiles) {
+    __shared__ uint64_t sk[T];
+    __shared__ uint16_t sl[T];
+    __shared__ I si[T];
+    Tile tl = tiles[blockIdx.x];
+    uint32_t len = tl.len;
+    if (len <= 1) return;
+    uint32_t P = 1;
+    while (P < len) P <<= 1;
+    for (uint32_t j = threadIdx.x; j < P; j += B) {
+        if (j < len) {
+            sk[j] = keys[tl.start + j];
+            sl[j] = (uint16_t)j;
+            si[j] = idx[tl.start + j];
+        } else {
+            sk[j] = ~0ull;
+            sl[j] = 0xFFFF;
+        }
+    }
+    __syncthreads();
+    for (uint32_t size = 2; size <= P; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t t = threadIdx.x; t < P / 2; t += B) {
+                uint32_t i = 2 * t - (t & (stride - 1));
+                uint32_t j = i + stride;
+                bool up = (i & size) == 0;
+                uint64_t ki = sk[i], kj = sk[j];
+                uint16_t li = sl[i], lj = sl[j];
+                bool gt = ki > kj || (ki == kj && li > lj);
+                if (gt == up) {
+                    sk[i] = kj;
+                    sk[j] = ki;
+                    sl[i] = lj;
+                    sl[j] = li;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t j = threadIdx.x; j < len; j += B) {
+        keys[tl.start + j] = sk[j];
+        idx[tl.start + j] = si[sl[j]];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Merge-path key merge of sorted runs of length `run` inside ranges
+// (oversized buckets and the splitter sample).  Ties take the left run.
+// ---------------------------------------------------------------------------
+template <typename I>
+__global__ __launch_bounds__(256) void k_key_merge(const Range *__restrict__ ranges, uint64_t run,
+                                                   const uint64_t *__restrict__ ks, const I *__restrict__ is,
+                                                   uint64_t *__restrict__ kd, I *__restrict__ id) {
+    constexpr uint32_t ITEMS = 8;
+    Range r = ranges[blockIdx.y];
+    uint64_t o = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * ITEMS;
+    if (o >= r.len) return;
+    uint64_t base = (o / (2 * run)) * (2 * run);
+    uint64_t nA = r.len - base < run ? r.len - base : run;
+    uint64_t restB = r.len - base - nA;
+    uint64_t nB = restB < run ? restB : run;
+    const uint64_t *A = ks + r.start + base;
+    const uint64_t *Bk = A + nA;
+    const I *Ai = is + r.start + base;
+    const I *Bi = Ai + nA;
+    uint64_t d = o - base;
+    uint64_t lo = d > nB ? d - nB : 0, hi = d < nA ? d : nA;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (A[mid] <= Bk[d - mid - 1]) lo = mid + 1;
+        else hi = mid;
+    }
+    uint64_t i = lo, j = d - lo;
+    uint64_t *dk = kd + r.start + o;
+    I *di = id + r.start + o;
+    uint64_t cnt = r.len - o < ITEMS ? r.len - o : ITEMS;
+    for (uint64_t q = 0; q < cnt; ++q) {
+        bool takeA = j >= nB || (i < nA && A[i] <= Bk[j]);
+        if (takeA) {
+            dk[q] = A[i];
+            di[q] = Ai[i];
+            ++i;
+        } else {
+            dk[q] = Bk[j];
+            di[q] = Bi[j];
+            ++j;
+        }
+    }
+}
+
+template <typename I>
+__global__ __launch_bounds__(256) void k_copy_ranges(const Range *__restrict__ ranges, const uint64_t *__restrict__ ks,
+                                                     const I *__restrict__ is, uint64_t *__restrict__ kd,
+                                                     I *__restrict__ id) {
+    Range r = ranges[blockIdx.y];
+    for (uint64_t o = (uint64_t)blockIdx.x * 256 + threadIdx.x; o < r.len; o += (uint64_t)gridDim.x * 256) {
+        kd[r.start + o] = ks[r.start + o];
+        id[r.start + o] = is[r.start + o];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Group detection over the key-sorted records.  Between different keys the
+// LCP follows from the keys (pack.rs:307-317); runs of equal keys are groups
+// refined at depth k.
+// ---------------------------------------------------------------------------
+struct Group {
+    uint64_t start;
+    uint64_t size;
+};
+
+struct GroupLists {
+    uint64_t *pairs;
+    Group *small;  // 3..64
+    Group *mid;    // 65..kMidMax
+    Group *giant;  // > kMidMax
+    unsigned int *counts;  // [pairs, small, mid, giant]
+};
+
+constexpr uint32_t kSmallMax = 64;
+constexpr uint32_t kMidMax = 2048;
+
+template <typename I>
+__global__ __launch_bounds__(256) void k_detect(const uint64_t *__restrict__ keys, uint64_t N, GroupLists gl) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
+        uint64_t k0 = keys[i];
+        bool start = i == 0 || keys[i - 1] != k0;
+        if (!start) continue;
+        if (i + 1 < N && keys[i + 1] == k0) {
+            uint64_t j = i + 2;
+            while (j < N && keys[j] == k0) ++j;
+            uint64_t sz = j - i;
+            if (sz == 2) {
+                unsigned int q = atomicAdd(&gl.counts[0], 1u);
+                gl.pairs[q] = i;
+            } else if (sz <= kSmallMax) {
+                unsigned int q = atomicAdd(&gl.counts[1], 1u);
+                gl.small[q] = Group{i, sz};
+            } else if (sz <= kMidMax) {
+                unsigned int q = atomicAdd(&gl.counts[2], 1u);
+                gl.mid[q] = Group{i, sz};
+            } else {
+                unsigned int q = atomicAdd(&gl.counts[3], 1u);
+                gl.giant[q] = Group{i, sz};
+            }
+        }
+    }
+}
+
+// LCPs across key boundaries, once refinement has fixed which member of a
+// group sits next to the boundary: min(shared fields, both windows)
+// (pack.rs:307-317 with max_context).
+template <typename I>
+__global__ __launch_bounds__(256) void k_boundary_lcp(const uint64_t *__restrict__ keys, const I *__restrict__ idx,
+                                                      uint64_t N, DevLimits L, uint64_t max_ctx, KeyGeom g,
+                                                      I *__restrict__ lcp) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
+        if (i == 0) {
+            lcp[0] = 0;
+            continue;
+        }
+        uint64_t k0 = keys[i - 1], k1 = keys[i];
+        if (k0 == k1) continue;
+        uint64_t v = shared_fields(g, k0, k1);
+        uint64_t ca = cap_at(L, idx[i - 1], max_ctx), cb = cap_at(L, idx[i], max_ctx);
+        v = v < ca ? v : ca;
+        v = v < cb ? v : cb;
+        lcp[i] = (I)v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pair refinement: one wavefront per two-member group, compared with the
+// 64-lane LCP scan from the key depth on.
+// ---------------------------------------------------------------------------
+template <typename S, typename I>
+__global__ __launch_bounds__(256) void k_pairs(const S *__restrict__ t, DevLimits L, uint64_t max_ctx,
+                                               const uint64_t *__restrict__ pairs, uint32_t n_pairs, uint64_t known,
+                                               I *__restrict__ idx, I *__restrict__ lcp) {
+    uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= n_pairs) return;
+    uint64_t s = pairs[w];
+    uint64_t a = idx[s], b = idx[s + 1];
+    uint64_t l;
+    int c = wave_cmp(t, L, max_ctx, a, b, known, &l);
+    if ((threadIdx.x & 63) == 0) {
+        if (c > 0) {
+            idx[s] = (I)b;
+            idx[s + 1] = (I)a;
+        }
+        if (lcp) lcp[s + 1] = (I)l;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Group refinement: one workgroup per group of equal keys.  Rounds extend
+// every unresolved member by the next k symbols (a fresh packed key at the
+// current depth), plus a boundary value for members whose window ends inside
+// the round, and re-sort the group by (subgroup, chunk, boundary, position).
+// A subgroup is settled once it is a single member or its members all ended
+// (comparator-Equal; position order).
+// ---------------------------------------------------------------------------
+constexpr uint64_t kAuxCont = ~0ull;
+
+template <typename S, typename I, int B, int M>
+__global__ __launch_bounds__(B) void k_groups(const S *__restrict__ t, CodeMap<S> cm, KeyGeom g, DevLimits L,
+                                              uint64_t max_ctx, const Group *__restrict__ groups, uint64_t depth0,
+                                              I *__restrict__ idx, I *__restrict__ lcp, int run_mode,
+                                              uint32_t chunk_len) {
+    __shared__ uint64_t s_pos[M], s_chunk[M], s_aux[M], s_cap[M], s_lim[M], s_lcp[M];
+    __shared__ uint32_t s_sub[M];
+    __shared__ uint16_t s_perm[M];
+    __shared__ uint32_t s_scan[M];
+    __shared__ uint32_t s_unres;
+    Group gr = groups[blockIdx.x];
+    uint64_t gstart = gr.start, gsize = gr.size;
+    if (chunk_len) {  // giant group cut into runs of chunk_len members
+        gstart += (uint64_t)blockIdx.y * chunk_len;
+        if (gstart >= gr.start + gr.size) return;
+        gsize = gr.start + gr.size - gstart;
+        if (gsize > chunk_len) gsize = chunk_len;
+    }
+    const uint32_t m = (uint32_t)gsize;
+    uint32_t P = 1;
+    while (P < m) P <<= 1;
+    const bool star = L.order == 1;
+    for (uint32_t s = threadIdx.x; s < m; s += B) {
+        uint64_t p = idx[gstart + s];
+        uint64_t lim = lim_at(L, p);
+        s_pos[s] = p;
+        s_lim[s] = lim;
+        s_cap[s] = lim < max_ctx ? lim : max_ctx;
+        s_sub[s] = 0;
+        s_lcp[s] = 0;
+    }
+    for (uint32_t r = threadIdx.x; r < P; r += B) s_perm[r] = (uint16_t)r;
+    __syncthreads();
+    uint64_t D = depth0;
+    for (;;) {
+        // (1) fresh chunk + boundary value for unresolved members
+        for (uint32_t s = threadIdx.x; s < m; s += B) {
+            if (s_sub[s] & 0x80000000u) continue;  // settled
+            uint64_t cap = s_cap[s];
+            s_chunk[s] = key_at(t, cm, g, s_pos[s], D, cap);
+            uint64_t rem = cap > D ? cap - D : 0;
+            uint64_t aux;
+            if (rem >= g.k) aux = star ? 0 : kAuxCont;
+            else aux = star ? (1 + (kAuxCont - 1 - s_lim[s])) : s_lim[s];
+            s_aux[s] = aux;
+        }
+        __syncthreads();
+        // (2) bitonic sort of the permutation by (sub, chunk, aux, pos)
+        for (uint32_t size = 2; size <= P; size <<= 1) {
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                for (uint32_t q = threadIdx.x; q < P / 2; q += B) {
+                    uint32_t i = 2 * q - (q & (stride - 1));
+                    uint32_t j = i + stride;
+                    bool up = (i & size) == 0;
+                    uint32_t a = s_perm[i], b = s_perm[j];
+                    bool gt;
+                    if (a >= m) gt = true;
+                    else if (b >= m) gt = false;
+                    else {
+                        uint32_t sa = s_sub[a] & 0x7FFFFFFFu, sb = s_sub[b] & 0x7FFFFFFFu;
+                        if (sa != sb) gt = sa > sb;
+                        else if (s_chunk[a] != s_chunk[b]) gt = s_chunk[a] > s_chunk[b];
+                        else if (s_aux[a] != s_aux[b]) gt = s_aux[a] > s_aux[b];
+                        else gt = s_pos[a] > s_pos[b];
+                    }
+                    if (gt == up) {
+                        s_perm[i] = (uint16_t)b;
+                        s_perm[j] = (uint16_t)a;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // (3) new subgroup starts.  Sorted positions of subgroup boundaries
+        // never move once formed, so the depth at which neighbours r-1, r
+        // part is recorded per position; the window caps of whichever
+        // members end up there are applied at the end.
+        for (uint32_t r = threadIdx.x; r < m; r += B) {
+            uint32_t s = s_perm[r];
+            bool f = true;
+            if (r > 0) {
+                uint32_t q = s_perm[r - 1];
+                bool same_old = (s_sub[s] & 0x7FFFFFFFu) == (s_sub[q] & 0x7FFFFFFFu);
+                f = !(same_old && s_chunk[s] == s_chunk[q] && s_aux[s] == s_aux[q]);
+                if (same_old && !(s_sub[s] & 0x80000000u)) s_lcp[r] = D + shared_fields(g, s_chunk[q], s_chunk[s]);
+            }
+            s_scan[r] = f ? r : 0;
+        }
+        __syncthreads();
+        for (uint32_t off = 1; off < m; off <<= 1) {
+            uint32_t v[ (M + B - 1) / B ];
+            uint32_t c = 0;
+            for (uint32_t r = threadIdx.x; r < m; r += B, ++c) v[c] = (r >= off && s_scan[r - off] > s_scan[r]) ? s_scan[r - off] : s_scan[r];
+            __syncthreads();
+            c = 0;
+            for (uint32_t r = threadIdx.x; r < m; r += B, ++c) s_scan[r] = v[c];
+            __syncthreads();
+        }
+        // (4) settle: singleton subgroups and ended subgroups
+        if (threadIdx.x == 0) s_unres = 0;
+        __syncthreads();
+        uint32_t unres = 0;
+        for (uint32_t r = threadIdx.x; r < m; r += B) {
+            uint32_t s = s_perm[r];
+            uint32_t ns = s_scan[r];
+            bool has_nb = (r + 1 < m && s_scan[r + 1] == ns) || (r > 0 && s_scan[r - 1] == ns);
+            bool ended = s_aux[s] != (star ? 0 : kAuxCont);
+            bool settled = !has_nb || ended || (s_sub[s] & 0x80000000u);
+            s_sub[s] = ns | (settled ? 0x80000000u : 0u);
+            unres += settled ? 0 : 1;
+        }
+        if (unres) atomicAdd(&s_unres, unres);
+        __syncthreads();
+        if (s_unres == 0) break;
+        D += g.k;
+        __syncthreads();
+    }
+    for (uint32_t r = threadIdx.x; r < m; r += B) {
+        uint32_t s = s_perm[r];
+        idx[gstart + r] = (I)s_pos[s];
+        if (r > 0) {
+            if (lcp) {
+                uint32_t q = s_perm[r - 1];
+                uint64_t v = s_lcp[r];
+                v = v < s_cap[s] ? v : s_cap[s];
+                v = v < s_cap[q] ? v : s_cap[q];
+                lcp[gstart + r] = (I)v;
+            }
+        } else if (run_mode && lcp) {
+            lcp[gstart] = 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Merge-path LCP merge (sample_sort.rs:397-522 restated for a partitioned
+// GPU merge).  Each thread owns W consecutive outputs of one merge job: a
+// merge-path search with the full comparator finds its (i, j) split, one
+// explicit LCP rebuilds the merge state m = LCP(last output, other head),
+// and the three-case LCP merge runs sequentially from there.  Every output
+// LCP is the true LCP of neighbours, so the result is identical to the
+// sequential merge.
+// ---------------------------------------------------------------------------
+struct MergeJob {
+    uint64_t start;  // x at start, y at start + nx, output at start
+    uint64_t nx, ny;
+};
+
+template <typename S, typename I, int W>
+__global__ __launch_bounds__(128) void k_lcp_merge(const S *__restrict__ t, DevLimits L, uint64_t max_ctx,
+                                                   uint64_t known, const MergeJob *__restrict__ jobs,
+                                                   const I *__restrict__ si, const I *__restrict__ sl,
+                                                   I *__restrict__ di, I *__restrict__ dl) {
+    MergeJob jb = jobs[blockIdx.y];
+    uint64_t rlen = jb.nx + jb.ny;
+    uint64_t d = ((uint64_t)blockIdx.x * 128 + threadIdx.x) * W;
+    if (d >= rlen) return;
+    const uint64_t nX = jb.nx, nY = jb.ny;
+    const I *X = si + jb.start, *Y = X + nX;
+    const I *LX = sl + jb.start, *LY = LX + nX;
+    uint64_t cnt = rlen - d < W ? rlen - d : W;
+    I *oz = di + jb.start + d;
+    I *ol = dl + jb.start + d;
+    if (nY == 0 || nX == 0) {  // a lone run is carried over
+        const I *Z = nX ? X : Y, *LZ = nX ? LX : LY;
+        for (uint64_t q = 0; q < cnt; ++q) {
+            oz[q] = Z[d + q];
+            ol[q] = LZ[d + q];
+        }
+        return;
+    }
+    uint64_t lo = d > nY ? d - nY : 0, hi = d < nX ? d : nX;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        uint64_t l;
+        if (thread_cmp(t, L, max_ctx, (uint64_t)X[mid], (uint64_t)Y[d - mid - 1], known, &l) < 0) lo = mid + 1;
+        else hi = mid;
+    }
+    uint64_t i = lo, j = d - lo;
+    // A = stream of the last output, B = the other; m = LCP(last output, B head)
+    const I *A = X, *Bs = Y, *LA = LX, *LB = LY;
+    uint64_t iA = i, iB = j, nA = nX, nB = nY;
+    uint64_t m = 0;
+    if (d > 0) {
+        bool lastX;
+        if (i == 0) lastX = false;
+        else if (j == 0) lastX = true;
+        else {
+            uint64_t l;
+            lastX = thread_cmp(t, L, max_ctx, (uint64_t)X[i - 1], (uint64_t)Y[j - 1], known, &l) > 0;
+        }
+        if (!lastX) {
+            A = Y, Bs = X, LA = LY, LB = LX;
+            iA = j, iB = i, nA = nY, nB = nX;
+        }
+        if (iB < nB) {
+            uint64_t l;
+            thread_cmp(t, L, max_ctx, (uint64_t)A[iA - 1], (uint64_t)Bs[iB], known, &l);
+            m = l;
+        }
+    }
+    for (uint64_t q = 0; q < cnt; ++q) {
+        bool hasA = iA < nA, hasB = iB < nB;
+        bool outA;
+        uint64_t lo_out, new_m = m;
+        if (hasA && hasB) {
+            uint64_t la = (d + q == 0) ? 0 : (uint64_t)LA[iA];
+            if (la > m) {
+                outA = true, lo_out = la;
+            } else if (la < m) {
+                outA = false, lo_out = m, new_m = la;
+            } else {
+                uint64_t l;
+                int c = thread_cmp(t, L, max_ctx, (uint64_t)A[iA], (uint64_t)Bs[iB], m > known ? m : known, &l);
+                outA = c < 0, lo_out = m, new_m = l;
+            }
+        } else if (hasA) {
+            outA = true, lo_out = (uint64_t)LA[iA];
+        } else {
+            outA = false, lo_out = m;
+        }
+        if (outA) {
+            oz[q] = A[iA];
+            ol[q] = (I)lo_out;
+            ++iA;
+        } else {
+            oz[q] = Bs[iB];
+            ol[q] = (I)lo_out;
+            ++iB;
+            const I *tp;
+            tp = A, A = Bs, Bs = tp;
+            tp = LA, LA = LB, LB = tp;
+            uint64_t u;
+            u = iA, iA = iB, iB = u;
+            u = nA, nA = nB, nB = u;
+        }
+        m = new_m;
+    }
+}
+
+template <typename I>
+__global__ __launch_bounds__(256) void k_copy_idx_lcp(uint64_t start, uint64_t len, const I *__restrict__ si,
+                                                      const I *__restrict__ sl, I *__restrict__ di,
+                                                      I *__restrict__ dl) {
+    for (uint64_t o = (uint64_t)blockIdx.x * 256 + threadIdx.x; o < len; o += (uint64_t)gridDim.x * 256) {
+        di[start + o] = si[start + o];
+        dl[start + o] = sl[start + o];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kernels exposed for parity tests.
+// ---------------------------------------------------------------------------
+// LcpDispatch::lcp (lcp.rs:125-153): window min(n-p, n-q, max_ctx).
+template <typename S>
+__global__ __launch_bounds__(256) void k_lcp_pairs(const S *__restrict__ t, uint64_t n, const uint64_t *__restrict__ pq,
+                                                   uint64_t count, uint64_t max_ctx, uint64_t *__restrict__ out) {
+    uint64_t w = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= count) return;
+    uint64_t p = pq[2 * w], q = pq[2 * w + 1];
+    uint64_t lp = n > p ? n - p : 0, lq = n > q ? n - q : 0;
+    uint64_t cap = lp < lq ? lp : lq;
+    cap = cap < max_ctx ? cap : max_ctx;
+    uint64_t r = wave_lcp(t, p, q, 0, cap);
+    if ((threadIdx.x & 63) == 0) out[w] = r;
+}
+
+// LcpDispatch::suffix_cmp_with (lcp.rs:184-201), boundary_order without the
+// position tie for the default convention.
+template <typename S>
+__global__ __launch_bounds__(256) void k_cmp_pairs(const S *__restrict__ t, DevLimits L, const uint64_t *__restrict__ pq,
+                                                   uint64_t count, uint64_t max_ctx, int32_t *__restrict__ out) {
+    uint64_t w = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= count) return;
+    uint64_t a = pq[2 * w], b = pq[2 * w + 1];
+    uint64_t la = lim_at(L, a), lb = lim_at(L, b);
+    uint64_t ca = la < max_ctx ? la : max_ctx, cb = lb < max_ctx ? lb : max_ctx;
+    uint64_t cap = ca < cb ? ca : cb;
+    uint64_t c = wave_lcp(t, a, b, 0, cap);
+    if ((threadIdx.x & 63) != 0) return;
+    int r;
+    if (c < cap) r = t[a + c] < t[b + c] ? -1 : (t[a + c] > t[b + c] ? 1 : 0);
+    else if (L.order == 1) r = la != lb ? (lb < la ? -1 : 1) : (a < b ? -1 : (a > b ? 1 : 0));
+    else r = la < lb ? -1 : (la > lb ? 1 : 0);
+    out[w] = r;
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_seed_keys(const S *__restrict__ t, CodeMap<S> cm, KeyGeom g, DevLimits L,
+                                                   const uint64_t *__restrict__ pos, uint64_t count,
+                                                   uint64_t *__restrict__ keys) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    uint64_t p = pos[i];
+    keys[i] = key_at(t, cm, g, p, 0, lim_at(L, p));
+}
+
+// Writes positions 0..n-1 (or a copy of the caller's positions).
+template <typename I>
+__global__ __launch_bounds__(256) void k_iota(I *__restrict__ out, uint64_t n) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = (I)i;
+}
+
+}  // namespace caps

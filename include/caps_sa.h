@@ -1,0 +1,184 @@
+/*
+ * caps_sa.h — C ABI of the MI355X suffix-array constructor (gfx950, HIP).
+ *
+ * Drop-in boundary for the in-memory suffix-array path of caps-sa 0.7.0.
+ * Every entry point names the Rust item it replaces.  A Rust facade with the
+ * crate's generic signatures monomorphizes onto these symbols (S = u8 | u16,
+ * I = u32 | u64, L = PlainText | SegmentedText | the STAR provider); see
+ * INTEGRATION.md for the binding.
+ *
+ * Conventions
+ *   - Host-pointer entry points borrow their inputs for the call only and
+ *     never retain a pointer after returning.
+ *   - *_device entry points take device pointers (HBM-resident inputs and
+ *     outputs) and a hipStream_t passed as void*; they return after the work
+ *     on that stream has completed.
+ *   - Return codes are CAPS_OK or a CAPS_E_* code; caps_last_error() gives a
+ *     message (thread-local).  Nothing falls back to a CPU path: an input the
+ *     GPU build does not cover returns CAPS_E_UNSUPPORTED.
+ *   - Symbols are compared as unsigned integers of sym_bytes bytes.
+ *   - Among suffixes the limit provider calls Equal (possible only with
+ *     SegmentedText under the default boundary order), output is in
+ *     ascending position order.
+ */
+#ifndef CAPS_SA_H
+#define CAPS_SA_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CAPS_OK 0
+#define CAPS_E_INVALID 1     /* bad arguments: the reference's panics        */
+#define CAPS_E_DEVICE 2      /* HIP/device failure: BuildError::Io           */
+#define CAPS_E_EMIT 3        /* the emit callback aborted: BuildError::Emit  */
+#define CAPS_E_UNSUPPORTED 4 /* symbol/index/provider outside the GPU build  */
+#define CAPS_E_NOMEM 5
+
+/* LimitProvider descriptor (limits.rs:34-105).
+ *   kind PLAIN:     PlainText::new(n)                    (limits.rs:130-156)
+ *   kind SEGMENTED: SegmentedText::from_ends(n, ends)    (limits.rs:261-281)
+ *   order SHORTER_FIRST: default boundary_order lim_a.cmp(&lim_b)
+ *                        (limits.rs:66-75), BoundaryRank::ShorterFirst
+ *   order STAR: lim_b.cmp(&lim_a).then(p_a.cmp(&p_b)), BoundaryRank::LongerFirst
+ *               (sample_sort.rs:998-1015, pack.rs:329-351) */
+#define CAPS_LIMITS_PLAIN 0u
+#define CAPS_LIMITS_SEGMENTED 1u
+#define CAPS_ORDER_SHORTER_FIRST 0u
+#define CAPS_ORDER_STAR 1u
+typedef struct caps_limits {
+    uint32_t kind;
+    uint32_t order;
+    const uint64_t *ends; /* host pointer, strictly increasing, last == n */
+    uint64_t n_ends;
+} caps_limits_t;
+
+/* Bucket routers for the classification pass. */
+#define CAPS_ROUTER_BSEARCH 0u /* binary search over the splitter table   */
+#define CAPS_ROUTER_PWL 1u     /* learned piecewise-linear CDF + check     */
+#define CAPS_ROUTER_MLP 2u     /* learned 64-32-1 MLP + check              */
+
+/* Packed key geometry. */
+#define CAPS_KEY_DENSE 0u     /* ceil(log2(sigma))-bit fields, weakly order preserving */
+#define CAPS_KEY_REFERENCE 1u /* pack.rs geometry (u8 only; dense when ineligible)    */
+
+/* Opts (sample_sort.rs:71-85) + ExtMemOpts (ext_mem.rs:58-239) + GPU knobs. */
+typedef struct caps_opts {
+    uint64_t max_context;      /* Opts::max_context; UINT64_MAX = unbounded            */
+    uint64_t subproblem_count; /* ExtMemOpts::subproblem_count (the output never
+                                  depends on it; kept for the API)                     */
+    uint32_t router;           /* CAPS_ROUTER_*                                         */
+    uint32_t key_geometry;     /* CAPS_KEY_*                                            */
+    uint32_t gpu_buckets;      /* 0 = auto (one bucket per tile_keys/2 suffixes)        */
+    uint32_t tile_keys;        /* 0 = autotuned; else 1024 | 2048 | 4096                */
+    uint32_t want_lcp;         /* fill the LCP array where an entry point takes one     */
+    uint32_t flags;            /* reserved, 0                                           */
+} caps_opts_t;
+
+/* Fills opts with the reference defaults (max_context unbounded, p auto). */
+void caps_opts_default(caps_opts_t *opts);
+
+/* Thread-local message for the last nonzero return code. */
+const char *caps_last_error(void);
+
+/* Number of HIP devices; 0 when none is usable. */
+int caps_device_count(void);
+
+/* ---- in-memory builders: sample_sort.rs:97-206 --------------------------- */
+
+/* build_in_memory / build_in_memory_with_opts / build_in_memory_with
+ * (sample_sort.rs:97-127).  out_sa holds n entries of idx_bytes (4 | 8);
+ * out_lcp (nullable, same width) receives lcp[i] = LCP(sa[i-1], sa[i]) capped
+ * by both limits and max_context, lcp[0] = 0 — merge_sort's postcondition
+ * (sample_sort.rs:208-216). */
+int caps_build_in_memory(const void *text, uint64_t n, uint32_t sym_bytes, uint32_t idx_bytes,
+                         const caps_limits_t *limits, const caps_opts_t *opts, void *out_sa, void *out_lcp);
+
+/* build_in_memory_for_positions(_with_opts|_with) (sample_sort.rs:143-206):
+ * sorts `positions` (n_pos entries of idx_bytes) by their suffixes.  out_sa
+ * may alias positions (the reference reuses the moved-in Vec). */
+int caps_build_in_memory_for_positions(const void *text, uint64_t n, uint32_t sym_bytes, const void *positions,
+                                       uint64_t n_pos, uint32_t idx_bytes, const caps_limits_t *limits,
+                                       const caps_opts_t *opts, void *out_sa, void *out_lcp);
+
+/* ---- streaming builders: ext_mem.rs:289-871 ------------------------------ */
+
+/* Emit callback: called sequentially on the caller's thread, in lexicographic
+ * order, with batches of positions widened to u64.  A nonzero return aborts
+ * the build; caps_* then returns CAPS_E_EMIT and caps_last_emit_code() gives
+ * the callback's value (BuildError::Emit, ext_mem.rs:257-277). */
+typedef int (*caps_emit_fn)(void *ctx, const uint64_t *pos, uint64_t count);
+int caps_last_emit_code(void);
+
+/* (try_)build_in_memory_sample_sort(_with|_for_positions|_for_positions_with)
+ * (ext_mem.rs:711-871).  positions == NULL means every position 0..n-1. */
+int caps_build_in_memory_sample_sort(const void *text, uint64_t n, uint32_t sym_bytes, const uint64_t *positions,
+                                     uint64_t n_pos, const caps_limits_t *limits, const caps_opts_t *opts,
+                                     caps_emit_fn emit, void *ctx);
+
+/* (try_)build_ext_mem_for_positions(_with) (ext_mem.rs:370-449).  The GPU
+ * build keeps text and records in HBM instead of spilling to disk; work_dir
+ * and physical_file_count have no counterpart.  Same contract as above. */
+int caps_build_ext_mem_for_positions(const void *text, uint64_t n, uint32_t sym_bytes, const uint64_t *positions,
+                                     uint64_t n_pos, const caps_limits_t *limits, const caps_opts_t *opts,
+                                     caps_emit_fn emit, void *ctx);
+
+/* ---- device-resident builder (inputs and outputs already in HBM) ---------- */
+
+/* Same as caps_build_in_memory_for_positions with device pointers.
+ * d_positions == NULL means 0..n-1.  d_out_lcp nullable. */
+int caps_build_device(const void *d_text, uint64_t n, uint32_t sym_bytes, const void *d_positions, uint64_t n_pos,
+                      uint32_t idx_bytes, const caps_limits_t *limits, const caps_opts_t *opts, void *d_out_sa,
+                      void *d_out_lcp, void *stream);
+
+/* Per-phase device times (ms) and counters of the last build on this thread. */
+typedef struct caps_stats {
+    double ms_total, ms_alphabet, ms_sample, ms_classify, ms_scatter, ms_tile_sort, ms_merge, ms_detect,
+        ms_refine, ms_giant;
+    uint64_t buckets, tiles, oversized, pairs, small_groups, mid_groups, giant_groups;
+    uint64_t router_err_sum, router_err_max, router_fallbacks, router_count;
+    uint32_t key_bits, key_k;
+} caps_stats_t;
+int caps_last_stats(caps_stats_t *out);
+
+/* ---- kernels exposed for parity tests (device pointers) ------------------ */
+
+/* LcpDispatch::lcp (lcp.rs:125-153) for `count` pairs (p, q): the 64-lane
+ * wave compare.  d_pairs: 2*count u64; d_out: count u64. */
+int caps_lcp_pairs_device(const void *d_text, uint64_t n, uint32_t sym_bytes, const uint64_t *d_pairs,
+                          uint64_t count, uint64_t max_ctx, uint64_t *d_out, void *stream);
+
+/* LcpDispatch::suffix_cmp_with (lcp.rs:184-201) for `count` pairs, without
+ * the position tie: d_out[i] in {-1, 0, 1}. */
+int caps_suffix_cmp_pairs_device(const void *d_text, uint64_t n, uint32_t sym_bytes, const caps_limits_t *limits,
+                                 const uint64_t *d_pairs, uint64_t count, uint64_t max_ctx, int32_t *d_out,
+                                 void *stream);
+
+/* Packer::key_at_bounded (pack.rs:187-203) for `count` positions, in the
+ * geometry `key_geometry` (u8 text).  rank: 0 shorter first, 1 longer first.
+ * Geometry out: bits[0], k[1]. Returns CAPS_E_UNSUPPORTED when the reference
+ * geometry is ineligible (pack.rs:217-234). */
+int caps_seed_keys_device(const void *d_text, uint64_t n, uint32_t sym_bytes, const caps_limits_t *limits,
+                          uint32_t key_geometry, uint32_t rank, const uint64_t *d_pos, uint64_t count,
+                          uint64_t *d_keys, uint32_t *geometry_out, void *stream);
+
+/* merge (sample_sort.rs:505-522): LCP-enhanced merge of two sorted runs with
+ * LCP arrays (lcp_*[0] = 0) into z / lcp_z, as a merge-path GPU merge.
+ * Host pointers; all arrays u64. */
+int caps_merge_lcp(const void *text, uint64_t n, uint32_t sym_bytes, const caps_limits_t *limits, const uint64_t *x,
+                   const uint64_t *lcp_x, uint64_t nx, const uint64_t *y, const uint64_t *lcp_y, uint64_t ny,
+                   uint64_t max_ctx, uint64_t *z, uint64_t *lcp_z);
+
+/* CascadeWorkspace::cascade_merge (ext_mem.rs:2144-2243): merge the sorted
+ * runs delimited by boundaries = [0, e1, ..., n] level by level.  Host
+ * pointers; arrays u64. */
+int caps_cascade_merge(const void *text, uint64_t n, uint32_t sym_bytes, const caps_limits_t *limits,
+                       const uint64_t *sa, const uint64_t *lcp, uint64_t n_sa, const uint64_t *boundaries,
+                       uint64_t n_boundaries, uint64_t max_ctx, uint64_t *out_sa, uint64_t *out_lcp);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,163 @@
+"""GPU build vs the oracle: bit-exact suffix arrays and LCP arrays.
+
+The oracle is the C restatement of caps-sa's build_in_memory (merge sort
+with the LCP merge); for plain texts it is cross-checked against brute force
+and prefix doubling in test_oracle.py.  Under SegmentedText with the default
+boundary order, comparator-Equal suffixes are compared in ascending position
+order (the oracle's order inside those classes is canonicalized first).
+"""
+import numpy as np
+import pytest
+
+import caps_sa
+import oracle as O
+from caps_sa import gen
+
+pytestmark = pytest.mark.gpu
+
+MAX = caps_sa.MAX_CONTEXT
+
+
+def _lp(n, ends, order):
+    if ends is None:
+        return None
+    return caps_sa.StarSegmented(n, ends) if order == 1 else caps_sa.SegmentedText(n, ends)
+
+
+def check(text, positions=None, ends=None, order=0, max_ctx=MAX, idx=np.uint32, with_lcp=True, label=""):
+    text = np.ascontiguousarray(text)
+    n = len(text)
+    print(f"case {label} n={n} dtype={text.dtype} pos={'all' if positions is None else len(positions)} "
+          f"ends={None if ends is None else len(ends)} order={order} max_ctx={max_ctx}", flush=True)
+    ctx = O.Ctx(text, ends, order)
+    pos = np.arange(n, dtype=np.uint64) if positions is None else np.asarray(positions, dtype=np.uint64)
+    got, got_lcp = caps_sa.build_in_memory_for_positions_with(
+        text, pos.astype(idx), _lp(n, ends, order), caps_sa.Opts(max_ctx), idx, return_lcp=True)
+    want = ctx.build_in_memory(pos, max_ctx=max_ctx, threads=4)
+    if ends is not None and order == 0:
+        want = ctx.canonicalize(want, max_ctx)
+    got = got.astype(np.uint64)
+    if not np.array_equal(got, want):
+        i = int(np.argmax(got != want))
+        raise AssertionError(f"{label}: SA differs at {i}: got {got[max(0, i - 2):i + 3]} want {want[max(0, i - 2):i + 3]}")
+    if with_lcp:
+        want_lcp = ctx.lcp_array(want, max_ctx)
+        got_lcp = got_lcp.astype(np.uint64)
+        if not np.array_equal(got_lcp, want_lcp):
+            i = int(np.argmax(got_lcp != want_lcp))
+            raise AssertionError(f"{label}: LCP differs at {i}: got {got_lcp[i]} want {want_lcp[i]}")
+
+
+@pytest.mark.parametrize("t", [b"banana", b"mississippi", b"abracadabra", b"a", b"aa", b"ab", b"ba",
+                               bytes([7]), bytes([0, 1, 2, 0, 1, 5, 0, 2, 1, 6]), b"a" * 36, b"ab" * 14,
+                               b"ACGTACGTACGTACGTACGTACGTACGT" * 100 + bytes([200])],
+                         ids=lambda t: f"len{len(t)}-{t[:8].hex()}")
+def test_fixtures(t):
+    check(np.frombuffer(t, dtype=np.uint8), label=repr(t[:20]))
+
+
+def test_empty():
+    out = caps_sa.build_in_memory(np.zeros(0, dtype=np.uint8))
+    assert out.size == 0
+
+
+@pytest.mark.parametrize("sigma", [2, 4, 6, 255])
+def test_random_u8(sigma):
+    for n in [2, 3, 7, 16, 17, 63, 64, 65, 200, 1000, 5000]:
+        check(gen.uniform_u8(n, sigma, seed=n * 31 + sigma), label=f"u8 sigma={sigma}")
+
+
+@pytest.mark.parametrize("max_ctx", [0, 1, 2, 4, 16, 40])
+def test_max_context(max_ctx):
+    for n in [2, 3, 7, 64, 500, 3000]:
+        check(gen.uniform_u8(n, 3, seed=n + max_ctx), max_ctx=max_ctx, label=f"max_ctx={max_ctx}")
+
+
+def test_u16_random():
+    for sigma in [3, 16, 300, 65535]:
+        for n in [5, 100, 4000]:
+            check(gen.uniform_u16(n, sigma, seed=sigma + n), label=f"u16 sigma={sigma}")
+
+
+def test_u16_zipf():
+    check(gen.zipf_u16(20000, seed=3), label="u16 zipf")
+
+
+def test_long_runs_and_periodic():
+    check(np.full(2000, 7, dtype=np.uint8), label="run 2000")
+    check((np.arange(2000) % 3).astype(np.uint8), label="periodic 3")
+    mixed = np.concatenate([(np.arange(500) % 4), np.full(1500, 4), (np.arange(500) % 4)]).astype(np.uint8)
+    check(mixed, label="mixed run")
+
+
+def test_giant_groups():
+    # 9000 suffixes share their first 32 symbols: exercises chunked
+    # refinement and the merge-path LCP merge.
+    check(np.zeros(9000, dtype=np.uint8), label="zeros 9000")
+    check((np.arange(12000) % 2).astype(np.uint8), label="period 2, 12000")
+    t = gen.uniform_u8(6000, 4, seed=5)
+    check(np.tile(t[:700], 12), label="tandem 700x12")
+
+
+def test_positions_subsets():
+    rng = np.random.default_rng(0xFEED)
+    for n in [33, 200, 1000, 20000]:
+        t = gen.uniform_u8(n, 6, seed=n)
+        pos = np.flatnonzero(rng.integers(0, 10, n) < 7).astype(np.uint64)
+        rng.shuffle(pos)
+        check(t, positions=pos, label="subset")
+    t = np.frombuffer(b"mississippi", dtype=np.uint8)
+    check(t, positions=np.arange(0, 11, 2), label="even positions")
+
+
+def test_segmented_shorter_first():
+    rng = np.random.default_rng(0x5E6)
+    t = np.frombuffer(b"helloworldbananamississippi", dtype=np.uint8)
+    check(t, ends=np.cumsum([5, 5, 6, 11]), label="seg small")
+    for _ in range(20):
+        lengths = rng.integers(5, 50, rng.integers(1, 10))
+        n = int(lengths.sum())
+        check(rng.integers(0, 3, n).astype(np.uint8), ends=np.cumsum(lengths), label="seg random")
+
+
+def test_segmented_star():
+    rng = np.random.default_rng(0xCAFE)
+    check(np.frombuffer(b"AAAA", dtype=np.uint8), ends=[4], order=1, label="star AAAA")
+    t = np.frombuffer(b"helloworldbananamississippi", dtype=np.uint8)
+    check(t, ends=np.cumsum([5, 5, 6, 11]), order=1, label="star small")
+    for _ in range(20):
+        lengths = rng.integers(5, 50, rng.integers(1, 10))
+        n = int(lengths.sum())
+        check(rng.integers(0, 3, n).astype(np.uint8), ends=np.cumsum(lengths), order=1, label="star random")
+
+
+def test_segmented_genome_like():
+    text, ends, pos = gen.genome(200000, 80, median=2500.0, n_frac=0.058, n_mean=500.0, seed=11)
+    check(text, positions=pos, ends=ends, label="genome shorter-first", with_lcp=False)
+    check(text, positions=pos, ends=ends, order=1, label="genome star", with_lcp=False)
+
+
+def test_u64_index():
+    check(gen.uniform_u8(5000, 4, seed=9), idx=np.uint64, label="u64")
+    check(gen.markov_repeats(30000, seed=2, mean_len=300.0), idx=np.uint64, label="u64 markov")
+
+
+def test_c0_uniform_1m():
+    """The 1,048,576-symbol uniform {0..3} text, u32 index, vs the oracle."""
+    text = gen.uniform_u8(1 << 20, 4, seed=0xC0)
+    ctx = O.Ctx(text)
+    got, got_lcp = caps_sa.build_in_memory_with(text, return_lcp=True)
+    want = ctx.build_in_memory(threads=8)
+    assert np.array_equal(got.astype(np.uint64), want)
+    assert np.array_equal(got_lcp.astype(np.uint64), ctx.kasai(want))
+
+
+def test_markov_repeats_4m_properties():
+    """Repeat-rich text: permutation + strictly increasing adjacent suffixes
+    (which pins the unique SA) + Kasai LCPs."""
+    text = gen.markov_repeats(1 << 22, seed=0xC1)
+    ctx = O.Ctx(text)
+    got, got_lcp = caps_sa.build_in_memory_with(text, return_lcp=True)
+    got = got.astype(np.uint64)
+    assert ctx.verify(got, strict=True, canon=True, threads=8) == -1
+    assert np.array_equal(got_lcp.astype(np.uint64), ctx.kasai(got))
