@@ -63,7 +63,8 @@ class Stats(ctypes.Structure):
         (name, ctypes.c_double)
         for name in (
             "ms_total", "ms_alphabet", "ms_sample", "ms_classify", "ms_scatter", "ms_tile_sort", "ms_merge",
-            "ms_detect", "ms_refine", "ms_giant",
+            "ms_detect", "ms_refine", "ms_giant", "k_classify", "k_scatter", "k_tile_sort", "k_detect", "k_pairs",
+            "k_groups_small", "k_groups_mid", "k_boundary_lcp",
         )
     ] + [
         (name, ctypes.c_uint64)

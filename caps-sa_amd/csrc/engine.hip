@@ -218,6 +218,33 @@ struct PhaseTimer {
 };
 enum { PH_ALPHA, PH_SAMPLE, PH_CLASSIFY, PH_SCATTER, PH_TILE, PH_MERGE, PH_DETECT, PH_REFINE, PH_GIANT };
 
+// Event pair around one kernel launch.
+struct KernelTimer {
+    hipStream_t st;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    std::vector<double *> dst;
+    explicit KernelTimer(hipStream_t s) : st(s) {}
+    void begin(double *slot) {
+        hipEvent_t a, b;
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
+        HIPCHK(hipEventRecord(a, st));
+        ev.push_back({a, b});
+        dst.push_back(slot);
+    }
+    void end() { HIPCHK(hipEventRecord(ev.back().second, st)); }
+    void collect() {
+        for (size_t i = 0; i < ev.size(); ++i) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, ev[i].first, ev[i].second));
+            *dst[i] += ms;
+        }
+    }
+    ~KernelTimer() {
+        for (auto &p : ev) (void)hipEventDestroy(p.first), (void)hipEventDestroy(p.second);
+    }
+};
+
 // Sort `count` keys (with u32 payload) in place: LDS tiles + key merges.
 template <int T>
 static void sort_keys_single_range(Workspace &W, uint64_t *keys, uint32_t *ids, uint64_t *keys2, uint32_t *ids2,
@@ -259,6 +286,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     Workspace &W = workspace();
     memset(&stats, 0, sizeof(stats));
     PhaseTimer tm(st);
+    KernelTimer kt(st);
     const uint64_t max_ctx = o.max_context;
     DevLimits L = upload_limits(W, lim, n, st);
     if (n_pos == 0) {
@@ -327,8 +355,10 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     HIPCHK(hipMemsetAsync(hist, 0, p * 4, st));
     Router R{};
     unsigned grid = grid_for(n_pos, 256, 8192);
+    kt.begin(&stats.k_classify);
     k_classify<S, I, false><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, bid, hist);
     LAUNCHCHK();
+    kt.end();
     k_scan_u32<<<1, 1024, 0, st>>>(hist, p, off);
     LAUNCHCHK();
     HIPCHK(hipMemcpyAsync(cursor, off, p * 8, hipMemcpyDeviceToDevice, st));
@@ -336,8 +366,10 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
 
     // ---- scatter ----
     uint64_t *keys = (uint64_t *)W.keys.get(n_pos * 8);
+    kt.begin(&stats.k_scatter);
     k_scatter<S, I><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, bid, cursor, keys, d_out);
     LAUNCHCHK();
+    kt.end();
     tm.mark(PH_SCATTER);
 
     // ---- tiles ----
@@ -362,8 +394,10 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     stats.tiles = n_tiles;
     stats.oversized = n_big;
     if (n_tiles) {
+        kt.begin(&stats.k_tile_sort);
         k_tile_sort<I, T, 256><<<(unsigned)n_tiles, 256, 0, st>>>(keys, d_out, tiles);
         LAUNCHCHK();
+        kt.end();
     }
     tm.mark(PH_TILE);
 
@@ -407,8 +441,10 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     gl.giant = (Group *)W.giant.get((n_pos / (kMidMax + 1) + 1) * sizeof(Group));
     gl.counts = nbig + 4;
     HIPCHK(hipMemsetAsync(gl.counts, 0, 16, st));
+    kt.begin(&stats.k_detect);
     k_detect<I><<<grid_for(n_pos, 256, 16384), 256, 0, st>>>(keys, n_pos, gl);
     LAUNCHCHK();
+    kt.end();
     unsigned int cnt[4];
     HIPCHK(hipMemcpyAsync(cnt, gl.counts, 16, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -421,19 +457,25 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     // ---- refinement ----
     const uint64_t depth0 = g.k;
     if (cnt[0]) {
+        kt.begin(&stats.k_pairs);
         k_pairs<S, I><<<(unsigned)ceil_div(cnt[0], 4), 256, 0, st>>>(d_text, L, max_ctx, gl.pairs, cnt[0], depth0,
                                                                       d_out, lcp);
         LAUNCHCHK();
+        kt.end();
     }
     if (cnt[1]) {
+        kt.begin(&stats.k_groups_small);
         k_groups<S, I, 64, kSmallMax><<<cnt[1], 64, 0, st>>>(d_text, cm, g, L, max_ctx, gl.small, depth0, d_out, lcp,
                                                               0, 0);
         LAUNCHCHK();
+        kt.end();
     }
     if (cnt[2]) {
+        kt.begin(&stats.k_groups_mid);
         k_groups<S, I, 256, kMidMax><<<cnt[2], 256, 0, st>>>(d_text, cm, g, L, max_ctx, gl.mid, depth0, d_out, lcp,
                                                               0, 0);
         LAUNCHCHK();
+        kt.end();
     }
     tm.mark(PH_REFINE);
 
@@ -481,12 +523,15 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         }
     }
     if (lcp) {
+        kt.begin(&stats.k_boundary_lcp);
         k_boundary_lcp<I><<<grid_for(n_pos, 256, 16384), 256, 0, st>>>(keys, d_out, n_pos, L, max_ctx, g, lcp);
         LAUNCHCHK();
+        kt.end();
     }
     tm.mark(PH_GIANT);
     HIPCHK(hipStreamSynchronize(st));
     tm.collect(stats);
+    kt.collect();
 }
 
 template void build_impl<uint8_t, uint32_t>(const uint8_t *, uint64_t, const uint32_t *, uint64_t,

@@ -137,6 +137,8 @@ int caps_build_device(const void *d_text, uint64_t n, uint32_t sym_bytes, const 
 typedef struct caps_stats {
     double ms_total, ms_alphabet, ms_sample, ms_classify, ms_scatter, ms_tile_sort, ms_merge, ms_detect,
         ms_refine, ms_giant;
+    /* single-kernel durations (event pairs around one launch on the build's stream) */
+    double k_classify, k_scatter, k_tile_sort, k_detect, k_pairs, k_groups_small, k_groups_mid, k_boundary_lcp;
     uint64_t buckets, tiles, oversized, pairs, small_groups, mid_groups, giant_groups;
     uint64_t router_err_sum, router_err_max, router_fallbacks, router_count;
     uint32_t key_bits, key_k;

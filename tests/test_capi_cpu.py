@@ -1,0 +1,92 @@
+"""CPU checks of the C ABI library and the host-side mirror: the library
+loads, exports every entry point include/caps_sa.h declares, and rejects
+invalid arguments before touching a device.  No compute call runs here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import caps_sa
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "caps_sa.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(caps_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = caps_sa.lib()
+    names = declared_symbols()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(lib, name), f"{name} declared in caps_sa.h but not exported"
+
+
+def test_python_binding_covers_every_declared_symbol():
+    assert set(declared_symbols()) <= set(caps_sa._SIGNATURES)
+
+
+def test_struct_layouts():
+    assert ctypes.sizeof(caps_sa._Limits) == 24
+    assert ctypes.sizeof(caps_sa._Opts) == 40
+
+
+def test_opts_default():
+    o = caps_sa._c_opts()
+    assert o.max_context == caps_sa.MAX_CONTEXT and o.subproblem_count == 0
+
+
+def _call_invalid(sym_bytes=1, idx_bytes=4, n=4, text=b"abcd", lim=None):
+    t = np.frombuffer(text, dtype=np.uint8) if text is not None else None
+    out = np.zeros(max(n, 1), dtype=np.uint64)
+    lim = lim or caps_sa.PlainText(n)._c()
+    o = caps_sa._c_opts()
+    return caps_sa.lib().caps_build_in_memory(t.ctypes.data if t is not None else None, n, sym_bytes, idx_bytes,
+                                             ctypes.byref(lim), ctypes.byref(o), out.ctypes.data, None)
+
+
+def test_argument_validation_before_device():
+    assert _call_invalid(sym_bytes=4) == caps_sa.CAPS_E_UNSUPPORTED
+    assert _call_invalid(idx_bytes=2) == caps_sa.CAPS_E_UNSUPPORTED
+    assert _call_invalid(text=None) == caps_sa.CAPS_E_INVALID
+    assert "NULL" in caps_sa.lib().caps_last_error().decode()
+
+
+def test_signed_symbols_are_unsupported():
+    with pytest.raises(caps_sa.CapsError) as e:
+        caps_sa.build_in_memory(np.array([0, 1, -1], dtype=np.int8))
+    assert e.value.code == caps_sa.CAPS_E_UNSUPPORTED
+
+
+def test_segmented_text_constructor_checks():
+    # limits.rs:247-281 panics -> ValueError here
+    s = caps_sa.SegmentedText.from_lengths(15, [3, 5, 7])
+    assert s.ends.tolist() == [3, 8, 15] and s.n_segments() == 3
+    assert [s.lim_at(p) for p in (0, 1, 2, 3, 5, 7, 8, 14, 15)] == [3, 2, 1, 5, 3, 1, 7, 1, 0]
+    with pytest.raises(ValueError, match="sum to"):
+        caps_sa.SegmentedText.from_lengths(20, [3, 5, 7])
+    with pytest.raises(ValueError):
+        caps_sa.SegmentedText.from_ends(15, [3, 3, 15])
+    with pytest.raises(ValueError):
+        caps_sa.SegmentedText.from_ends(16, [3, 8, 15])
+    assert caps_sa.SegmentedText.from_lengths(0, []).n_segments() == 0
+
+
+def test_boundary_rank():
+    assert caps_sa.PlainText(3).boundary_rank() == caps_sa.BoundaryRank.SHORTER_FIRST
+    assert caps_sa.StarSegmented(4, [4]).boundary_rank() == caps_sa.BoundaryRank.LONGER_FIRST
+
+
+def test_ext_mem_opts_from_env(monkeypatch):
+    monkeypatch.setenv("CAPS_SA_SUBPROBLEMS", "17")
+    monkeypatch.setenv("CAPS_SA_PACKED_PREFIX_SEED", "on")
+    o = caps_sa.ExtMemOpts.from_env()
+    assert o.subproblem_count == 17
+    assert o.packed_prefix_seed == caps_sa.PackedPrefixSeedPolicy.DENSE_ALPHABET_ONLY
+    monkeypatch.setenv("CAPS_SA_PACKED_PREFIX_REMAP_BYTES", "100")
+    assert caps_sa.ExtMemOpts.from_env().packed_prefix_seed == caps_sa.PackedPrefixSeedPolicy.REMAP
