@@ -278,6 +278,93 @@ static void sort_keys_single_range(Workspace &W, uint64_t *keys, uint32_t *ids, 
 }
 
 // ---------------------------------------------------------------------------
+// Router training (on the GPU, from the sample the build already draws).
+// ---------------------------------------------------------------------------
+static std::vector<float> mlp_init_params(uint64_t seed) {
+    std::vector<float> p(kMlpParams, 0.f);
+    uint64_t x = seed;
+    auto next = [&]() {
+        uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    auto unif = [&]() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0; };
+    for (int i = 0; i < kMlpIn * kMlpHid; ++i) p[i] = (float)(unif() * 0.25);         // W1
+    for (int j = 0; j < kMlpHid; ++j) p[kMlpIn * kMlpHid + j] = (float)(unif() * 0.1);  // b1
+    for (int j = 0; j < kMlpHid; ++j) p[kMlpIn * kMlpHid + kMlpHid + j] = (float)(unif() * 0.1);  // w2
+    p[kMlpParams - 1] = 0.5f;                                                         // b2
+    return p;
+}
+
+template <typename S, typename I>
+static RouterDev train_router(Workspace &W, uint32_t kind, uint32_t flags, const uint64_t *sorted, uint64_t s,
+                              const uint64_t *spl, uint64_t p, const S *d_text, CodeMap<S> cm, KeyGeom g,
+                              DevLimits L, uint64_t max_ctx, const I *d_pos, uint64_t n_pos, hipStream_t st) {
+    RouterDev R{};
+    R.kind = kind;
+    R.window = 8;
+    R.n_buckets = (uint32_t)p;
+    R.scale = (float)(p - 1);
+    unsigned long long *stats = (unsigned long long *)W.r_stats.get(64);
+    HIPCHK(hipMemsetAsync(stats, 0, 64, st));
+    R.stats = stats;
+    if (kind == CAPS_ROUTER_PWL) {
+        uint32_t ns = (uint32_t)std::min<uint64_t>(kPwlSeg, s);
+        uint32_t *labels = (uint32_t *)W.r_labels.get(s * 4);
+        k_router_labels<<<(unsigned)ceil_div(s, 256), 256, 0, st>>>(sorted, s, spl, (uint32_t)(p - 1), labels,
+                                                                     nullptr, 0.f);
+        LAUNCHCHK();
+        uint64_t *knots = (uint64_t *)W.r_knots.get((ns + 1) * 8);
+        float *a = (float *)W.r_a.get(ns * 4), *b = (float *)W.r_b.get(ns * 4), *inv = (float *)W.r_inv.get(ns * 4);
+        uint32_t *lo = (uint32_t *)W.r_lo.get(ns * 4), *hi = (uint32_t *)W.r_hi.get(ns * 4);
+        uint16_t *radix = (uint16_t *)W.r_radix.get(((1u << kRouterRadixBits) + 1) * 2);
+        k_pwl_fit<<<ns, 256, 0, st>>>(sorted, labels, s, ns, (uint32_t)p, knots, a, b, inv, lo, hi);
+        LAUNCHCHK();
+        k_pwl_radix<<<(unsigned)ceil_div((1u << kRouterRadixBits) + 1, 256), 256, 0, st>>>(knots, ns, radix);
+        LAUNCHCHK();
+        R.knots = knots, R.a = a, R.b = b, R.inv = inv, R.lo = lo, R.hi = hi, R.radix = radix, R.ns = ns;
+    } else if (kind == CAPS_ROUTER_MLP) {
+        // seeded 1% suffix sample (at least 64K), labels from the splitters
+        uint64_t m = std::max<uint64_t>(65536, n_pos / 100);
+        m = std::min<uint64_t>(m, std::max<uint64_t>(n_pos, 1) * 4);
+        m = std::max<uint64_t>(m, 8192);  // at least one minibatch (sampling is with replacement)
+        m = (m + 255) / 256 * 256;
+        uint64_t *keys = (uint64_t *)W.r_x.get(m * 8);
+        float *y = (float *)W.r_y.get(m * 4);
+        k_random_keys<S, I><<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, m,
+                                                                         0x5EEDull, keys);
+        LAUNCHCHK();
+        k_router_labels<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(keys, m, spl, (uint32_t)(p - 1), nullptr, y,
+                                                                     1.f / (float)std::max<uint64_t>(p - 1, 1));
+        LAUNCHCHK();
+        float *params = (float *)W.r_w.get(kMlpParams * 4 * 4);
+        float *grad = params + kMlpParams, *am = grad + kMlpParams, *av = am + kMlpParams;
+        W.h_mlp = mlp_init_params(0xC0FFEEull);
+        HIPCHK(hipMemcpyAsync(params, W.h_mlp.data(), kMlpParams * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemsetAsync(am, 0, 2 * kMlpParams * 4, st));
+        const uint32_t B = 8192;
+        const int steps = (flags & CAPS_FLAG_ROUTER_UNTRAINED) ? 0 : 300;
+        double b1p = 1.0, b2p = 1.0;
+        for (int it = 0; it < steps; ++it) {
+            uint64_t off = ((uint64_t)it * B) % (m - B + 1);
+            b1p *= 0.9, b2p *= 0.999;
+            HIPCHK(hipMemsetAsync(grad, 0, kMlpParams * 4, st));
+            k_mlp_grad<<<B / 256, 256, 0, st>>>(params, keys + off, y + off, B, B, grad, nullptr);
+            k_mlp_adam<<<(kMlpParams + 255) / 256, 256, 0, st>>>(params, grad, am, av, 3e-3f, (float)b1p,
+                                                                 (float)b2p);
+        }
+        LAUNCHCHK();
+        float *nib = (float *)W.r_nib.get(16 * 16 * kMlpHid * 4);
+        k_mlp_nibbles<<<16, 16 * kMlpHid, 0, st>>>(params, nib);
+        LAUNCHCHK();
+        R.nib = nib;
+        R.w2 = params + kMlpIn * kMlpHid + kMlpHid;
+    }
+    return R;
+}
+
+// ---------------------------------------------------------------------------
 // The build.
 // ---------------------------------------------------------------------------
 template <typename S, typename I>
@@ -327,16 +414,16 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     const uint64_t oversample = 16;
     uint64_t s = p > 1 ? std::min<uint64_t>(n_pos, p * oversample) : 0;
     uint64_t *spl = (uint64_t *)W.spl.get(std::max<uint64_t>(p, 1) * 8);
+    uint64_t *sorted_sample = nullptr;
     if (p > 1) {
         uint64_t *sk = (uint64_t *)W.skeys.get(s * 8), *sk2 = (uint64_t *)W.skeys2.get(s * 8);
         uint32_t *si = (uint32_t *)W.sids.get(s * 4), *si2 = (uint32_t *)W.sids2.get(s * 4);
         k_sample_keys<S, I><<<grid_for(s, 256, 1u << 30), 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, s,
                                                                          sk, si);
         LAUNCHCHK();
-        uint64_t *sorted;
         uint32_t *sorted_ids;
-        sort_keys_single_range<T>(W, sk, si, sk2, si2, s, st, &sorted, &sorted_ids);
-        k_pick_splitters<<<(unsigned)ceil_div(p, 256), 256, 0, st>>>(sorted, s, (uint32_t)p, spl);
+        sort_keys_single_range<T>(W, sk, si, sk2, si2, s, st, &sorted_sample, &sorted_ids);
+        k_pick_splitters<<<(unsigned)ceil_div(p, 256), 256, 0, st>>>(sorted_sample, s, (uint32_t)p, spl);
         LAUNCHCHK();
     }
     Splitters SP{};
@@ -345,7 +432,6 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     SP.stride = (uint32_t)std::max<uint64_t>(1, ceil_div(p - 1, kCoarseMax));
     SP.n_coarse = (uint32_t)ceil_div(p - 1, SP.stride);
     stats.buckets = p;
-    tm.mark(PH_SAMPLE);
 
     // ---- classify + histogram ----
     uint32_t *bid = (uint32_t *)W.bid.get(n_pos * 4);
@@ -353,10 +439,21 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     uint64_t *off = (uint64_t *)W.off.get((p + 1) * 8);
     unsigned long long *cursor = (unsigned long long *)W.cursor.get(p * 8);
     HIPCHK(hipMemsetAsync(hist, 0, p * 4, st));
-    Router R{};
+    RouterDev R{};
+    uint32_t rkind = p > 1 ? o.router : CAPS_ROUTER_BSEARCH;
+    if (rkind > CAPS_ROUTER_MLP) throw CapsError(CAPS_E_INVALID, "unknown router");
+    if (rkind != CAPS_ROUTER_BSEARCH)
+        R = train_router<S, I>(W, rkind, o.flags, sorted_sample, s, spl, p, d_text, cm, g, L, max_ctx, d_pos, n_pos,
+                               st);
+    tm.mark(PH_SAMPLE);
     unsigned grid = grid_for(n_pos, 256, 8192);
     kt.begin(&stats.k_classify);
-    k_classify<S, I, false><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, bid, hist);
+    if (rkind == CAPS_ROUTER_PWL)
+        k_classify<S, I, 1><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, bid, hist);
+    else if (rkind == CAPS_ROUTER_MLP)
+        k_classify<S, I, 2><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, bid, hist);
+    else
+        k_classify<S, I, 0><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, bid, hist);
     LAUNCHCHK();
     kt.end();
     k_scan_u32<<<1, 1024, 0, st>>>(hist, p, off);
@@ -529,6 +626,13 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         kt.end();
     }
     tm.mark(PH_GIANT);
+    if (R.stats) {
+        unsigned long long rs[4];
+        HIPCHK(hipMemcpyAsync(rs, R.stats, 32, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        stats.router_err_sum = rs[0], stats.router_err_max = rs[1], stats.router_fallbacks = rs[2];
+        stats.router_count = rs[3];
+    }
     HIPCHK(hipStreamSynchronize(st));
     tm.collect(stats);
     kt.collect();
@@ -858,6 +962,37 @@ static void merge_jobs_host(const void *text, uint64_t n, const caps_limits_t *l
 }  // namespace caps
 
 extern "C" {
+
+int caps_router_mlp_grad_device(const float *d_params, const uint64_t *d_keys, const float *d_targets, uint32_t count,
+                                float *d_grad, float *d_loss, void *stream) {
+    return guarded([&] {
+        hipStream_t st = (hipStream_t)stream;
+        HIPCHK(hipMemsetAsync(d_grad, 0, kMlpParams * 4, st));
+        HIPCHK(hipMemsetAsync(d_loss, 0, 4, st));
+        if (count) {
+            k_mlp_grad<<<(unsigned)ceil_div(count, 256), 256, 0, st>>>(d_params, d_keys, d_targets, count, count,
+                                                                        d_grad, d_loss);
+            LAUNCHCHK();
+        }
+        HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
+int caps_router_mlp_forward_device(const float *d_params, const uint64_t *d_keys, uint32_t count, float *d_out,
+                                   void *stream) {
+    return guarded([&] {
+        hipStream_t st = (hipStream_t)stream;
+        float *nib = (float *)workspace().r_tmp.get(16 * 16 * kMlpHid * 4);
+        k_mlp_nibbles<<<16, 16 * kMlpHid, 0, st>>>(d_params, nib);
+        LAUNCHCHK();
+        if (count) {
+            k_mlp_predict<<<(unsigned)ceil_div(count, 256), 256, 0, st>>>(nib, d_params + kMlpIn * kMlpHid + kMlpHid,
+                                                                           d_keys, count, d_out);
+            LAUNCHCHK();
+        }
+        HIPCHK(hipStreamSynchronize(st));
+    });
+}
 
 int caps_merge_lcp(const void *text, uint64_t n, uint32_t sym_bytes, const caps_limits_t *limits, const uint64_t *x,
                    const uint64_t *lcp_x, uint64_t nx, const uint64_t *y, const uint64_t *lcp_y, uint64_t ny,

@@ -50,7 +50,8 @@ struct Workspace {
     // standalone merges
     DevBuf m_a, m_al, m_b, m_bl;
     // router
-    DevBuf r_a, r_b, r_lo, r_hi, r_stats, r_x, r_y, r_w;
+    DevBuf r_a, r_b, r_inv, r_lo, r_hi, r_knots, r_radix, r_labels, r_stats, r_x, r_y, r_w, r_nib, r_tmp;
+    std::vector<float> h_mlp;
 };
 
 Workspace &workspace();

@@ -8,6 +8,7 @@
 //   giant groups (chunked refinement + merge-path LCP merge).
 #pragma once
 #include "device.hpp"
+#include "router.hpp"
 
 namespace caps {
 
@@ -44,6 +45,22 @@ __global__ __launch_bounds__(256) void k_sample_keys(const S *__restrict__ t, Co
     uint64_t p = pos ? (uint64_t)pos[j] : j;
     keys[i] = key_at(t, cm, g, p, 0, cap_at(L, p, max_ctx));
     ids[i] = (uint32_t)i;
+}
+
+// Keys of a seeded pseudo-random sample of the positions (router training).
+template <typename S, typename I>
+__global__ __launch_bounds__(256) void k_random_keys(const S *__restrict__ t, CodeMap<S> cm, KeyGeom g, DevLimits L,
+                                                     uint64_t max_ctx, const I *__restrict__ pos, uint64_t n_pos,
+                                                     uint64_t m, uint64_t seed, uint64_t *__restrict__ keys) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;  // SplitMix64 finaliser
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    uint64_t j = (uint64_t)(((unsigned __int128)z * n_pos) >> 64);
+    uint64_t p = pos ? (uint64_t)pos[j] : j;
+    keys[i] = key_at(t, cm, g, p, 0, cap_at(L, p, max_ctx));
 }
 
 // splitter j-1 = sorted_sample[(j * s) / p], j = 1..p-1 (ext_mem.rs:1341).
@@ -91,114 +108,92 @@ __device__ __forceinline__ uint32_t upper_bound_coarse(const Splitters &S, const
     return b;
 }
 
-// Learned router: a monotone two-stage piecewise-linear CDF model.  Stage 1
-// maps the key to one of `n_seg` segments, stage 2 maps it to a predicted
-// bucket inside the segment's clamp range.  The prediction is then checked
-// against its two neighbouring splitters and corrected by a bounded gallop,
-// with the exact binary search as the fallback and the judge.
-struct Router {
-    const float *seg_a;      // per-segment slope
-    const float *seg_b;      // per-segment intercept
-    const uint32_t *seg_lo;  // per-segment clamp range
-    const uint32_t *seg_hi;
-    float root_a, root_b;    // stage-1 linear model on x in [0,1)
-    uint64_t kmin;           // key normalisation
-    float kscale;            // 1 / (kmax - kmin) as float
-    uint32_t n_seg;
-    uint32_t window;         // gallop bound before falling back
-    uint32_t enabled;
-    unsigned long long *stats;  // [0]=sum |err|, [1]=max |err|, [2]=fallbacks, [3]=count (nullable)
-};
-
-__device__ __forceinline__ float router_x(const Router &R, uint64_t key) {
-    uint64_t d = key > R.kmin ? key - R.kmin : 0;
-    return (float)d * R.kscale;
-}
-
-__device__ __forceinline__ uint32_t router_predict(const Router &R, const float *sa, const float *sb,
-                                                   const uint32_t *slo, const uint32_t *shi, uint64_t key) {
-    float x = router_x(R, key);
-    float sf = R.root_a * x + R.root_b;
-    int sidx = (int)sf;
-    if (sf < 0.f) sidx = 0;
-    if (sidx >= (int)R.n_seg) sidx = (int)R.n_seg - 1;
-    float y = sa[sidx] * x + sb[sidx];
-    int b = (int)(y + 0.5f);
-    if (y < 0.f) b = 0;
-    uint32_t lo = slo[sidx], hi = shi[sidx];
-    uint32_t ub = (uint32_t)b;
-    if (ub < lo) ub = lo;
-    if (ub > hi) ub = hi;
-    return ub;
-}
-
 // Exact bucket given a predicted one: verify spl[b-1] <= key < spl[b],
 // otherwise gallop up to `window` steps, otherwise binary search.
-__device__ __forceinline__ uint32_t router_correct(const Router &R, const Splitters &S, const uint64_t *coarse,
-                                                   uint64_t key, uint32_t b, uint32_t *fallback) {
+__device__ __forceinline__ uint32_t router_correct(uint32_t window, const Splitters &S, uint64_t key, uint32_t b,
+                                                   uint32_t *fallback) {
     const uint64_t *spl = S.spl;
     uint32_t np = S.n_spl;
+    if (b > np) b = np;
     bool lo_ok = (b == 0) || spl[b - 1] <= key;
     bool hi_ok = (b == np) || key < spl[b];
     if (lo_ok && hi_ok) return b;
-    if (!hi_ok) {  // true bucket is above b
-        uint32_t lo = b + 1, step = 1, hi = lo;
-        for (uint32_t it = 0; it < R.window; ++it) {
-            hi = lo + step;
+    if (!hi_ok) {  // spl[b] <= key: the bucket is above b
+        uint32_t lo = b + 1, step = 1;
+        for (uint32_t it = 0; it < window; ++it) {
+            uint32_t hi = lo + step;
             if (hi > np) hi = np;
             if (hi == np || key < spl[hi]) {
-                // answer in [lo, hi]: number of splitters <= key
-                uint32_t a = lo, e = hi;
-                while (a < e) {
-                    uint32_t mid = (a + e) >> 1;
-                    if (spl[mid] <= key) a = mid + 1;
+                uint32_t x = lo, e = hi;
+                while (x < e) {
+                    uint32_t mid = (x + e) >> 1;
+                    if (spl[mid] <= key) x = mid + 1;
                     else e = mid;
                 }
-                return a;
+                return x;
             }
             lo = hi + 1;
             step <<= 1;
         }
-    } else {  // true bucket is below b
-        uint32_t hi = b - 1, step = 1, lo = hi;
-        for (uint32_t it = 0; it < R.window; ++it) {
-            lo = hi >= step ? hi - step : 0;
+    } else {  // spl[b-1] > key: the bucket is below b
+        uint32_t hi = b - 1, step = 1;
+        for (uint32_t it = 0; it < window; ++it) {
+            uint32_t lo = hi >= step ? hi - step : 0;
             if (lo == 0 || spl[lo - 1] <= key) {
-                uint32_t a = lo, e = hi;
-                while (a < e) {
-                    uint32_t mid = (a + e) >> 1;
-                    if (spl[mid] <= key) a = mid + 1;
+                uint32_t x = lo, e = hi;
+                while (x < e) {
+                    uint32_t mid = (x + e) >> 1;
+                    if (spl[mid] <= key) x = mid + 1;
                     else e = mid;
                 }
-                return a;
+                return x;
             }
             hi = lo - 1;
             step <<= 1;
         }
     }
     *fallback = 1;
-    return upper_bound_coarse(S, coarse, key);
+    uint32_t x = 0, e = np;
+    while (x < e) {
+        uint32_t mid = (x + e) >> 1;
+        if (spl[mid] <= key) x = mid + 1;
+        else e = mid;
+    }
+    return x;
 }
 
-template <typename S, typename I, bool ROUTER>
+// RK: 0 = binary search (LDS coarse table), 1 = PWL router, 2 = MLP router.
+template <typename S, typename I, int RK>
 __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeMap<S> cm, KeyGeom g, DevLimits L,
                                                   uint64_t max_ctx, const I *__restrict__ pos, uint64_t n_pos,
-                                                  Splitters SP, Router R, uint32_t *__restrict__ bid,
+                                                  Splitters SP, RouterDev R, uint32_t *__restrict__ bid,
                                                   uint32_t *__restrict__ hist) {
-    __shared__ uint64_t coarse[kCoarseMax];
-    __shared__ float ra[ROUTER ? 1024 : 1], rb[ROUTER ? 1024 : 1];
-    __shared__ uint32_t rlo[ROUTER ? 1024 : 1], rhi[ROUTER ? 1024 : 1];
+    constexpr uint32_t NS = kPwlSeg;
+    __shared__ uint64_t coarse[RK == 0 ? kCoarseMax : 1];
+    __shared__ uint64_t rknots[RK == 1 ? NS + 1 : 1];
+    __shared__ float ra[RK == 1 ? NS : 1], rb[RK == 1 ? NS : 1], rinv[RK == 1 ? NS : 1];
+    __shared__ uint32_t rlo[RK == 1 ? NS : 1], rhi[RK == 1 ? NS : 1];
+    __shared__ uint16_t rradix[RK == 1 ? (1u << kRouterRadixBits) + 1 : 1];
+    __shared__ float nib[RK == 2 ? 16 * 16 * kMlpHid : 1], w2[RK == 2 ? kMlpHid + 1 : 1];
     __shared__ unsigned long long st_sum, st_max, st_fb, st_cnt;
-    for (uint32_t c = threadIdx.x; c < SP.n_coarse; c += blockDim.x) coarse[c] = SP.spl[(uint64_t)c * SP.stride];
-    if (ROUTER) {
-        for (uint32_t c = threadIdx.x; c < R.n_seg; c += blockDim.x) {
-            ra[c] = R.seg_a[c];
-            rb[c] = R.seg_b[c];
-            rlo[c] = R.seg_lo[c];
-            rhi[c] = R.seg_hi[c];
+    if (RK == 0)
+        for (uint32_t c = threadIdx.x; c < SP.n_coarse; c += blockDim.x) coarse[c] = SP.spl[(uint64_t)c * SP.stride];
+    if (RK == 1) {
+        for (uint32_t c = threadIdx.x; c < R.ns; c += blockDim.x) {
+            rknots[c] = R.knots[c];
+            ra[c] = R.a[c];
+            rb[c] = R.b[c];
+            rinv[c] = R.inv[c];
+            rlo[c] = R.lo[c];
+            rhi[c] = R.hi[c];
         }
-        if (threadIdx.x == 0) st_sum = st_max = st_fb = st_cnt = 0;
+        for (uint32_t c = threadIdx.x; c <= (1u << kRouterRadixBits); c += blockDim.x) rradix[c] = R.radix[c];
     }
+    if (RK == 2) {
+        for (uint32_t c = threadIdx.x; c < 16 * 16 * kMlpHid; c += blockDim.x) nib[c] = R.nib[c];
+        for (uint32_t c = threadIdx.x; c <= (uint32_t)kMlpHid; c += blockDim.x) w2[c] = R.w2[c];
+    }
+    if (threadIdx.x == 0) st_sum = st_max = st_fb = st_cnt = 0;
     __syncthreads();
     unsigned long long my_sum = 0, my_max = 0, my_fb = 0, my_cnt = 0;
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -206,24 +201,23 @@ __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeM
         uint64_t p = pos ? (uint64_t)pos[i] : i;
         uint64_t key = key_at(t, cm, g, p, 0, cap_at(L, p, max_ctx));
         uint32_t b;
-        if (ROUTER) {
-            uint32_t pred = router_predict(R, ra, rb, rlo, rhi, key);
-            uint32_t fb = 0;
-            b = router_correct(R, SP, coarse, key, pred, &fb);
-            if (R.stats) {
-                unsigned long long e = b > pred ? b - pred : pred - b;
-                my_sum += e;
-                my_max = e > my_max ? e : my_max;
-                my_fb += fb;
-                my_cnt += 1;
-            }
-        } else {
+        if (RK == 0) {
             b = upper_bound_coarse(SP, coarse, key);
+        } else {
+            uint32_t pred = RK == 1 ? pwl_predict(R, rknots, rradix, ra, rb, rinv, rlo, rhi, key)
+                                    : mlp_predict(R, nib, w2, key);
+            uint32_t fb = 0;
+            b = router_correct(R.window, SP, key, pred, &fb);
+            unsigned long long e = b > pred ? b - pred : pred - b;
+            my_sum += e;
+            my_max = e > my_max ? e : my_max;
+            my_fb += fb;
+            my_cnt += 1;
         }
         bid[i] = b;
         atomicAdd(&hist[b], 1u);
     }
-    if (ROUTER && R.stats) {
+    if (RK != 0 && R.stats) {
         atomicAdd(&st_sum, my_sum);
         atomicMax(&st_max, my_max);
         atomicAdd(&st_fb, my_fb);

@@ -74,8 +74,11 @@ typedef struct caps_opts {
     uint32_t gpu_buckets;      /* 0 = auto (one bucket per tile_keys/2 suffixes)        */
     uint32_t tile_keys;        /* 0 = autotuned; else 1024 | 2048 | 4096                */
     uint32_t want_lcp;         /* fill the LCP array where an entry point takes one     */
-    uint32_t flags;            /* reserved, 0                                           */
+    uint32_t flags;            /* CAPS_FLAG_* bits, 0 by default                        */
 } caps_opts_t;
+/* Skip router training (the learned router runs with its initial parameters;
+ * used to show what training gains).  The output never changes. */
+#define CAPS_FLAG_ROUTER_UNTRAINED 1u
 
 /* Fills opts with the reference defaults (max_context unbounded, p auto). */
 void caps_opts_default(caps_opts_t *opts);
@@ -179,6 +182,20 @@ int caps_merge_lcp(const void *text, uint64_t n, uint32_t sym_bytes, const caps_
 int caps_cascade_merge(const void *text, uint64_t n, uint32_t sym_bytes, const caps_limits_t *limits,
                        const uint64_t *sa, const uint64_t *lcp, uint64_t n_sa, const uint64_t *boundaries,
                        uint64_t n_boundaries, uint64_t max_ctx, uint64_t *out_sa, uint64_t *out_lcp);
+
+/* ---- learned router kernels (device pointers; for gradient checks) ------- */
+
+/* Mean-squared-error gradient of the 64-32-1 router MLP over `count`
+ * (key, target) pairs.  params: W1[64][32] (input i = key bit 63-i), b1[32],
+ * w2[32], b2 (2113 floats).  d_grad (2113 floats) and d_loss (1 float) are
+ * overwritten. */
+int caps_router_mlp_grad_device(const float *d_params, const uint64_t *d_keys, const float *d_targets, uint32_t count,
+                                float *d_grad, float *d_loss, void *stream);
+
+/* Network output for `count` keys, evaluated the way classification does
+ * (nibble tables). */
+int caps_router_mlp_forward_device(const float *d_params, const uint64_t *d_keys, uint32_t count, float *d_out,
+                                   void *stream);
 
 #ifdef __cplusplus
 }
