@@ -254,11 +254,12 @@ static void sort_keys_single_range(Workspace &W, uint64_t *keys, uint32_t *ids, 
     for (uint64_t i = 0; i < nt; ++i) {
         W.h_tiles[i].start = i * T;
         W.h_tiles[i].len = (uint32_t)std::min<uint64_t>(T, count - i * T);
-        W.h_tiles[i].pad = 0;
+        W.h_tiles[i].whole = 0;
     }
     Tile *dt = (Tile *)W.stiles.get(nt * sizeof(Tile));
     HIPCHK(hipMemcpyAsync(dt, W.h_tiles.data(), nt * sizeof(Tile), hipMemcpyHostToDevice, st));
-    k_tile_sort<uint32_t, T, 256><<<(unsigned)nt, 256, 0, st>>>(keys, ids, dt);
+    GroupLists none{};
+    k_tile_radix<uint32_t, T, 256><<<(unsigned)nt, 256, 0, st>>>(keys, ids, dt, none, 0);
     LAUNCHCHK();
     Range r{0, count};
     W.h_range = r;
@@ -469,36 +470,40 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     kt.end();
     tm.mark(PH_SCATTER);
 
-    // ---- tiles ----
+    // ---- tiles: LDS radix sort per tile; whole-bucket tiles emit their tie groups ----
     uint32_t *tcnt = (uint32_t *)W.tcnt.get(p * 4);
     uint64_t *toff = (uint64_t *)W.toff.get((p + 1) * 8);
     uint64_t max_tiles = ceil_div(n_pos, T) + p + 1;
     Tile *tiles = (Tile *)W.tiles.get(max_tiles * sizeof(Tile));
     Range *big = (Range *)W.big.get(p * sizeof(Range));
-    unsigned int *nbig = (unsigned int *)W.counts.get(64);
-    HIPCHK(hipMemsetAsync(nbig, 0, 64, st));
+    unsigned int *counters = (unsigned int *)W.counts.get(64);  // [0] oversized, [8..16) group classes
+    HIPCHK(hipMemsetAsync(counters, 0, 64, st));
+    GroupLists gl{};
+    for (int c = 0; c < kGroupClasses; ++c)
+        gl.list[c] = (Group *)W.glist[c].get((n_pos / group_class_min(c) + 1) * sizeof(Group));
+    gl.counts = counters + 8;
     k_tile_counts<<<(unsigned)ceil_div(p, 256), 256, 0, st>>>(off, (uint32_t)p, T, tcnt);
     LAUNCHCHK();
     k_scan_u32<<<1, 1024, 0, st>>>(tcnt, p, toff);
     LAUNCHCHK();
-    k_tile_emit<<<(unsigned)ceil_div(p, 256), 256, 0, st>>>(off, (uint32_t)p, T, toff, tiles, big, nbig);
+    k_tile_emit<<<(unsigned)ceil_div(p, 256), 256, 0, st>>>(off, (uint32_t)p, T, toff, tiles, big, counters);
     LAUNCHCHK();
     uint64_t n_tiles = 0;
     unsigned int n_big = 0;
     HIPCHK(hipMemcpyAsync(&n_tiles, toff + p, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&n_big, nbig, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&n_big, counters, 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     stats.tiles = n_tiles;
     stats.oversized = n_big;
     if (n_tiles) {
         kt.begin(&stats.k_tile_sort);
-        k_tile_sort<I, T, 256><<<(unsigned)n_tiles, 256, 0, st>>>(keys, d_out, tiles);
+        k_tile_radix<I, T, 256><<<(unsigned)n_tiles, 256, 0, st>>>(keys, d_out, tiles, gl, 1);
         LAUNCHCHK();
         kt.end();
     }
     tm.mark(PH_TILE);
 
-    // ---- oversized buckets: merge the sorted tiles by key ----
+    // ---- oversized buckets: merge the sorted tiles by key, then find their groups ----
     if (n_big) {
         std::vector<Range> hb(n_big);
         HIPCHK(hipMemcpyAsync(hb.data(), big, n_big * sizeof(Range), hipMemcpyDeviceToHost, st));
@@ -526,68 +531,72 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
                 LAUNCHCHK();
             }
         }
+        for (unsigned y0 = 0; y0 < n_big; y0 += 65535) {
+            dim3 gr((unsigned)std::min<uint64_t>(ceil_div(maxlen, 256), 1024), std::min(65535u, n_big - y0));
+            k_detect_ranges<Range><<<gr, 256, 0, st>>>(keys, big + y0, gl);
+            LAUNCHCHK();
+        }
     }
     tm.mark(PH_MERGE);
 
-    // ---- groups of equal keys ----
-    I *lcp = want_lcp ? d_lcp_out : nullptr;
-    GroupLists gl{};
-    gl.pairs = (uint64_t *)W.pairs.get((n_pos / 2 + 1) * 8);
-    gl.small = (Group *)W.small.get((n_pos / 3 + 1) * sizeof(Group));
-    gl.mid = (Group *)W.mid.get((n_pos / (kSmallMax + 1) + 1) * sizeof(Group));
-    gl.giant = (Group *)W.giant.get((n_pos / (kMidMax + 1) + 1) * sizeof(Group));
-    gl.counts = nbig + 4;
-    HIPCHK(hipMemsetAsync(gl.counts, 0, 16, st));
-    kt.begin(&stats.k_detect);
-    k_detect<I><<<grid_for(n_pos, 256, 16384), 256, 0, st>>>(keys, n_pos, gl);
-    LAUNCHCHK();
-    kt.end();
-    unsigned int cnt[4];
-    HIPCHK(hipMemcpyAsync(cnt, gl.counts, 16, hipMemcpyDeviceToHost, st));
+    unsigned int cnt[kGroupClasses];
+    HIPCHK(hipMemcpyAsync(cnt, gl.counts, sizeof(cnt), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     stats.pairs = cnt[0];
-    stats.small_groups = cnt[1];
-    stats.mid_groups = cnt[2];
-    stats.giant_groups = cnt[3];
+    stats.small_groups = cnt[1] + cnt[2] + cnt[3] + cnt[4] + cnt[5];
+    stats.mid_groups = cnt[6];
+    stats.giant_groups = cnt[7];
     tm.mark(PH_DETECT);
 
     // ---- refinement ----
+    I *lcp = want_lcp ? d_lcp_out : nullptr;
     const uint64_t depth0 = g.k;
     if (cnt[0]) {
         kt.begin(&stats.k_pairs);
-        k_pairs<S, I><<<(unsigned)ceil_div(cnt[0], 4), 256, 0, st>>>(d_text, L, max_ctx, gl.pairs, cnt[0], depth0,
-                                                                      d_out, lcp);
+        k_pairs16<S, I><<<(unsigned)ceil_div(cnt[0], 16), 256, 0, st>>>(d_text, L, max_ctx, gl.list[0], cnt[0],
+                                                                         depth0, d_out, lcp);
         LAUNCHCHK();
         kt.end();
     }
-    if (cnt[1]) {
-        kt.begin(&stats.k_groups_small);
-        k_groups<S, I, 64, kSmallMax><<<cnt[1], 64, 0, st>>>(d_text, cm, g, L, max_ctx, gl.small, depth0, d_out, lcp,
-                                                              0, 0);
-        LAUNCHCHK();
-        kt.end();
-    }
-    if (cnt[2]) {
+    kt.begin(&stats.k_groups_small);
+    if (cnt[1])
+        k_wgroups<S, I, 4><<<(unsigned)ceil_div(cnt[1], 64), 256, 0, st>>>(d_text, cm, g, L, max_ctx, gl.list[1],
+                                                                            cnt[1], depth0, d_out, lcp);
+    if (cnt[2])
+        k_wgroups<S, I, 8><<<(unsigned)ceil_div(cnt[2], 32), 256, 0, st>>>(d_text, cm, g, L, max_ctx, gl.list[2],
+                                                                            cnt[2], depth0, d_out, lcp);
+    if (cnt[3])
+        k_wgroups<S, I, 16><<<(unsigned)ceil_div(cnt[3], 16), 256, 0, st>>>(d_text, cm, g, L, max_ctx, gl.list[3],
+                                                                             cnt[3], depth0, d_out, lcp);
+    if (cnt[4])
+        k_wgroups<S, I, 32><<<(unsigned)ceil_div(cnt[4], 8), 256, 0, st>>>(d_text, cm, g, L, max_ctx, gl.list[4],
+                                                                            cnt[4], depth0, d_out, lcp);
+    if (cnt[5])
+        k_wgroups<S, I, 64><<<(unsigned)ceil_div(cnt[5], 4), 256, 0, st>>>(d_text, cm, g, L, max_ctx, gl.list[5],
+                                                                            cnt[5], depth0, d_out, lcp);
+    LAUNCHCHK();
+    kt.end();
+    if (cnt[6]) {
         kt.begin(&stats.k_groups_mid);
-        k_groups<S, I, 256, kMidMax><<<cnt[2], 256, 0, st>>>(d_text, cm, g, L, max_ctx, gl.mid, depth0, d_out, lcp,
-                                                              0, 0);
+        k_groups<S, I, 256, kMidMax><<<cnt[6], 256, 0, st>>>(d_text, cm, g, L, max_ctx, gl.list[6], depth0, d_out,
+                                                              lcp, 0, 0);
         LAUNCHCHK();
         kt.end();
     }
     tm.mark(PH_REFINE);
 
     // ---- giant groups: chunked refinement + merge-path LCP merges ----
-    if (cnt[3]) {
-        std::vector<Group> hg(cnt[3]);
-        HIPCHK(hipMemcpyAsync(hg.data(), gl.giant, cnt[3] * sizeof(Group), hipMemcpyDeviceToHost, st));
+    if (cnt[7]) {
+        std::vector<Group> hg(cnt[7]);
+        HIPCHK(hipMemcpyAsync(hg.data(), gl.list[7], cnt[7] * sizeof(Group), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         I *lbuf = lcp ? lcp : (I *)W.lcp.get(n_pos * sizeof(I));
         I *i2 = (I *)W.idx2.get(n_pos * sizeof(I));
         I *l2 = (I *)W.lcp2.get(n_pos * sizeof(I));
-        for (unsigned gi = 0; gi < cnt[3]; ++gi) {
+        for (unsigned gi = 0; gi < cnt[7]; ++gi) {
             const Group gr = hg[gi];
             unsigned nch = (unsigned)ceil_div(gr.size, kMidMax);
-            k_groups<S, I, 256, kMidMax><<<dim3(1, nch), 256, 0, st>>>(d_text, cm, g, L, max_ctx, gl.giant + gi,
+            k_groups<S, I, 256, kMidMax><<<dim3(1, nch), 256, 0, st>>>(d_text, cm, g, L, max_ctx, gl.list[7] + gi,
                                                                          depth0, d_out, lbuf, 1, kMidMax);
             LAUNCHCHK();
             I *is = d_out, *ls = lbuf, *id = i2, *ld = l2;

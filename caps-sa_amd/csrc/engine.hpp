@@ -45,7 +45,7 @@ struct Workspace {
     // tiles
     DevBuf tcnt, toff, tiles, big, counts;
     // refinement
-    DevBuf pairs, small, mid, giant, lcp, lcp2, first, jobs;
+    DevBuf glist[kGroupClasses], lcp, lcp2, first, jobs;
     std::vector<MergeJob> h_jobs;
     // standalone merges
     DevBuf m_a, m_al, m_b, m_bl;

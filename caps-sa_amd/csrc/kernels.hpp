@@ -8,7 +8,9 @@
 //   giant groups (chunked refinement + merge-path LCP merge).
 #pragma once
 #include "device.hpp"
+#include "refine.hpp"
 #include "router.hpp"
+#include "tilesort.hpp"
 
 namespace caps {
 
@@ -277,11 +279,6 @@ __global__ __launch_bounds__(256) void k_scatter(const S *__restrict__ t, CodeMa
 // ---------------------------------------------------------------------------
 // Tiles: every bucket is cut into tiles of at most T records.
 // ---------------------------------------------------------------------------
-struct Tile {
-    uint64_t start;
-    uint32_t len;
-    uint32_t pad;
-};
 
 __global__ void k_tile_counts(const uint64_t *__restrict__ off, uint32_t p, uint32_t T, uint32_t *__restrict__ cnt) {
     uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -306,62 +303,13 @@ __global__ void k_tile_emit(const uint64_t *__restrict__ off, uint32_t p, uint32
         Tile tl;
         tl.start = s + x;
         tl.len = (uint32_t)(sz - x < T ? sz - x : T);
-        tl.pad = 0;
+        tl.whole = sz <= T ? 1u : 0u;
         tiles[o++] = tl;
     }
     if (sz > T) {
         unsigned int k = atomicAdd(n_big, 1u);
         big[k].start = s;
         big[k].len = sz;
-    }
-}
-
-// LDS bitonic sort of one tile by (key, original slot): stable, so equal
-// keys keep their scatter order.
-template <typename I, int T, int B>
-__global__ __launch_bounds__(B) void k_tile_sort(uint64_t *__restrict__ keys, I *__restrict__ idx,
-                                                 const Tile *__restrict__ tiles) {
-    __shared__ uint64_t sk[T];
-    __shared__ uint16_t sl[T];
-    __shared__ I si[T];
-    Tile tl = tiles[blockIdx.x];
-    uint32_t len = tl.len;
-    if (len <= 1) return;
-    uint32_t P = 1;
-    while (P < len) P <<= 1;
-    for (uint32_t j = threadIdx.x; j < P; j += B) {
-        if (j < len) {
-            sk[j] = keys[tl.start + j];
-            sl[j] = (uint16_t)j;
-            si[j] = idx[tl.start + j];
-        } else {
-            sk[j] = ~0ull;
-            sl[j] = 0xFFFF;
-        }
-    }
-    __syncthreads();
-    for (uint32_t size = 2; size <= P; size <<= 1) {
-        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-            for (uint32_t t = threadIdx.x; t < P / 2; t += B) {
-                uint32_t i = 2 * t - (t & (stride - 1));
-                uint32_t j = i + stride;
-                bool up = (i & size) == 0;
-                uint64_t ki = sk[i], kj = sk[j];
-                uint16_t li = sl[i], lj = sl[j];
-                bool gt = ki > kj || (ki == kj && li > lj);
-                if (gt == up) {
-                    sk[i] = kj;
-                    sk[j] = ki;
-                    sl[i] = lj;
-                    sl[j] = li;
-                }
-            }
-            __syncthreads();
-        }
-    }
-    for (uint32_t j = threadIdx.x; j < len; j += B) {
-        keys[tl.start + j] = sk[j];
-        idx[tl.start + j] = si[sl[j]];
     }
 }
 
@@ -426,50 +374,6 @@ __global__ __launch_bounds__(256) void k_copy_ranges(const Range *__restrict__ r
 // LCP follows from the keys (pack.rs:307-317); runs of equal keys are groups
 // refined at depth k.
 // ---------------------------------------------------------------------------
-struct Group {
-    uint64_t start;
-    uint64_t size;
-};
-
-struct GroupLists {
-    uint64_t *pairs;
-    Group *small;  // 3..64
-    Group *mid;    // 65..kMidMax
-    Group *giant;  // > kMidMax
-    unsigned int *counts;  // [pairs, small, mid, giant]
-};
-
-constexpr uint32_t kSmallMax = 64;
-constexpr uint32_t kMidMax = 2048;
-
-template <typename I>
-__global__ __launch_bounds__(256) void k_detect(const uint64_t *__restrict__ keys, uint64_t N, GroupLists gl) {
-    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
-        uint64_t k0 = keys[i];
-        bool start = i == 0 || keys[i - 1] != k0;
-        if (!start) continue;
-        if (i + 1 < N && keys[i + 1] == k0) {
-            uint64_t j = i + 2;
-            while (j < N && keys[j] == k0) ++j;
-            uint64_t sz = j - i;
-            if (sz == 2) {
-                unsigned int q = atomicAdd(&gl.counts[0], 1u);
-                gl.pairs[q] = i;
-            } else if (sz <= kSmallMax) {
-                unsigned int q = atomicAdd(&gl.counts[1], 1u);
-                gl.small[q] = Group{i, sz};
-            } else if (sz <= kMidMax) {
-                unsigned int q = atomicAdd(&gl.counts[2], 1u);
-                gl.mid[q] = Group{i, sz};
-            } else {
-                unsigned int q = atomicAdd(&gl.counts[3], 1u);
-                gl.giant[q] = Group{i, sz};
-            }
-        }
-    }
-}
-
 // LCPs across key boundaries, once refinement has fixed which member of a
 // group sits next to the boundary: min(shared fields, both windows)
 // (pack.rs:307-317 with max_context).
@@ -494,29 +398,6 @@ __global__ __launch_bounds__(256) void k_boundary_lcp(const uint64_t *__restrict
 }
 
 // ---------------------------------------------------------------------------
-// Pair refinement: one wavefront per two-member group, compared with the
-// 64-lane LCP scan from the key depth on.
-// ---------------------------------------------------------------------------
-template <typename S, typename I>
-__global__ __launch_bounds__(256) void k_pairs(const S *__restrict__ t, DevLimits L, uint64_t max_ctx,
-                                               const uint64_t *__restrict__ pairs, uint32_t n_pairs, uint64_t known,
-                                               I *__restrict__ idx, I *__restrict__ lcp) {
-    uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (w >= n_pairs) return;
-    uint64_t s = pairs[w];
-    uint64_t a = idx[s], b = idx[s + 1];
-    uint64_t l;
-    int c = wave_cmp(t, L, max_ctx, a, b, known, &l);
-    if ((threadIdx.x & 63) == 0) {
-        if (c > 0) {
-            idx[s] = (I)b;
-            idx[s + 1] = (I)a;
-        }
-        if (lcp) lcp[s + 1] = (I)l;
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Group refinement: one workgroup per group of equal keys.  Rounds extend
 // every unresolved member by the next k symbols (a fresh packed key at the
 // current depth), plus a boundary value for members whose window ends inside
@@ -524,8 +405,6 @@ __global__ __launch_bounds__(256) void k_pairs(const S *__restrict__ t, DevLimit
 // A subgroup is settled once it is a single member or its members all ended
 // (comparator-Equal; position order).
 // ---------------------------------------------------------------------------
-constexpr uint64_t kAuxCont = ~0ull;
-
 template <typename S, typename I, int B, int M>
 __global__ __launch_bounds__(B) void k_groups(const S *__restrict__ t, CodeMap<S> cm, KeyGeom g, DevLimits L,
                                               uint64_t max_ctx, const Group *__restrict__ groups, uint64_t depth0,
