@@ -37,7 +37,10 @@ __device__ __forceinline__ uint64_t lim_at(const DevLimits &L, uint64_t p) {
         if (L.ends[mid] <= p) lo = mid + 1;
         else hi = mid;
     }
-    return lo < L.n_ends ? L.ends[lo] - p : L.n - p;
+    // select the value, not the address: a select of &ends[lo] and &L.n would
+    // turn the load into a generic load from the private copy of L
+    uint64_t e = L.ends[lo < L.n_ends ? lo : L.n_ends - 1];
+    return (lo < L.n_ends ? e : L.n) - p;
 }
 
 // Comparison window of a suffix: its limit, clipped by max_context.
@@ -84,20 +87,83 @@ struct CodeMap<uint16_t> {
     __device__ __forceinline__ uint32_t operator()(uint16_t c) const { return code ? code[c] : c; }
 };
 
+// Pad fields for the last `m` of the k fields: pad repeated m times.
+__device__ __forceinline__ uint64_t pad_fields(const KeyGeom &g, uint32_t m) {
+    if (g.pad == 0 || m == 0) return 0;
+    uint64_t r = 0;
+    for (uint32_t i = 0; i < m; ++i) r = (r << g.bits) | g.pad;
+    return r;
+}
+
 // Key of the `k` symbols starting at depth d of suffix p, whose comparison
-// window is `cap` symbols (limit and max_context already applied).
+// window is `cap` symbols (limit and max_context already applied).  All
+// aligned 8-byte words covering the window are loaded first (each holds at
+// least one in-range byte), then the fields are extracted with constant
+// indices, so the loads overlap instead of forming a dependent chain.
 template <typename S>
 __device__ __forceinline__ uint64_t key_at(const S *__restrict__ t, CodeMap<S> cm, const KeyGeom &g, uint64_t p,
                                            uint64_t d, uint64_t cap) {
+    constexpr uint32_t SZ = sizeof(S);
+    constexpr int NW = 9;  // 64 bytes of window + up to 7 bytes of misalignment
     uint64_t rem = cap > d ? cap - d : 0;
     uint32_t take = rem < g.k ? (uint32_t)rem : g.k;
     uint64_t key = 0;
-    const S *s = t + p + d;
-    for (uint32_t i = 0; i < g.k; ++i) {
-        uint64_t f = i < take ? (uint64_t)(cm(s[i]) + g.bias) : (uint64_t)g.pad;
-        key = (key << g.bits) | f;
+    if (take) {
+        const uint8_t *base = (const uint8_t *)t;
+        uint64_t b0 = (p + d) * SZ, b1 = b0 + (uint64_t)take * SZ;
+        uint64_t a0 = b0 & ~7ull;
+        uint32_t nw = (uint32_t)(((b1 + 7) & ~7ull) - a0) >> 3;
+        uint64_t w[NW];
+#pragma unroll
+        for (int i = 0; i < NW; ++i) w[i] = i < (int)nw ? *(const uint64_t *)(base + a0 + 8 * i) : 0ull;
+        uint32_t lo = (uint32_t)(b0 - a0), hi = lo + take * SZ;
+#pragma unroll
+        for (int j = 0; j < NW * 8; j += SZ) {
+            if ((uint32_t)j >= lo && (uint32_t)j < hi) {
+                uint32_t sym = SZ == 1 ? (uint32_t)(w[j >> 3] >> (8 * (j & 7))) & 0xFFu
+                                       : (uint32_t)(w[j >> 3] >> (8 * (j & 7))) & 0xFFFFu;
+                key = (key << g.bits) | (uint64_t)(cm((S)sym) + g.bias);
+            }
+        }
     }
+    if (take == 0) key = pad_fields(g, g.k);
+    else if (take < g.k) key = (key << (g.bits * (g.k - take))) | pad_fields(g, g.k - take);
     return key << g.shift;
+}
+
+// Keys of the consecutive positions base+first .. base+first+cnt-1 from a
+// text window staged in LDS (win[i] = text[base + i]), rolling the key by one
+// field per position while the whole window is inside the suffix's limit.
+// caps[i] is the comparison window of position base+first+i.
+template <typename S, int CNT>
+__device__ __forceinline__ void keys_from_window(const S *win, CodeMap<S> cm, const KeyGeom &g, uint32_t first,
+                                                 const uint64_t *caps, uint64_t *out) {
+    const uint32_t k = g.k, bits = g.bits;
+    const uint64_t mask = (k * bits >= 64) ? ~0ull : ((1ull << (k * bits)) - 1);
+    uint64_t ku = 0;
+    bool rolling = false;
+#pragma unroll
+    for (int i = 0; i < CNT; ++i) {
+        uint32_t q = first + i;
+        uint64_t cap = caps[i];
+        if (cap >= k) {
+            if (rolling) {
+                ku = ((ku << bits) & mask) | (uint64_t)(cm(win[q + k - 1]) + g.bias);
+            } else {
+                ku = 0;
+                for (uint32_t f = 0; f < k; ++f) ku = (ku << bits) | (uint64_t)(cm(win[q + f]) + g.bias);
+                rolling = true;
+            }
+            out[i] = ku << g.shift;
+        } else {
+            uint32_t take = (uint32_t)cap;
+            uint64_t key = 0;
+            for (uint32_t f = 0; f < take; ++f) key = (key << bits) | (uint64_t)(cm(win[q + f]) + g.bias);
+            key = take ? ((key << (bits * (k - take))) | pad_fields(g, k - take)) : pad_fields(g, k);
+            out[i] = key << g.shift;
+            rolling = false;
+        }
+    }
 }
 
 // Symbols two keys share, from the first differing field (pack.rs:207-209).

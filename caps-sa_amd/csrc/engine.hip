@@ -449,7 +449,16 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     tm.mark(PH_SAMPLE);
     unsigned grid = grid_for(n_pos, 256, 8192);
     kt.begin(&stats.k_classify);
-    if (rkind == CAPS_ROUTER_PWL)
+    const bool seq = d_pos == nullptr && n_pos == n;
+    const unsigned gseq = (unsigned)std::min<uint64_t>(ceil_div(n_pos, kSeqChunk), 65536);
+    if (seq) {
+        if (rkind == CAPS_ROUTER_PWL)
+            k_classify_seq<S, I, 1><<<gseq, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, bid, hist);
+        else if (rkind == CAPS_ROUTER_MLP)
+            k_classify_seq<S, I, 2><<<gseq, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, bid, hist);
+        else
+            k_classify_seq<S, I, 0><<<gseq, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, bid, hist);
+    } else if (rkind == CAPS_ROUTER_PWL)
         k_classify<S, I, 1><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, bid, hist);
     else if (rkind == CAPS_ROUTER_MLP)
         k_classify<S, I, 2><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, bid, hist);
@@ -465,7 +474,10 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     // ---- scatter ----
     uint64_t *keys = (uint64_t *)W.keys.get(n_pos * 8);
     kt.begin(&stats.k_scatter);
-    k_scatter<S, I><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, bid, cursor, keys, d_out);
+    if (seq)
+        k_scatter_seq<S, I><<<gseq, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, bid, cursor, keys, d_out);
+    else
+        k_scatter<S, I><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, bid, cursor, keys, d_out);
     LAUNCHCHK();
     kt.end();
     tm.mark(PH_SCATTER);

@@ -234,6 +234,148 @@ __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeM
     }
 }
 
+// ---------------------------------------------------------------------------
+// Contiguous positions (every suffix of the text): a block owns 4096
+// consecutive positions, stages their text window in LDS with coalesced
+// loads, and each thread rolls the packed key over 16 consecutive positions
+// into an LDS key array; classification / scatter then walk that array in
+// coalesced order.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kSeqItems = 16;
+constexpr uint32_t kSeqChunk = 256 * kSeqItems;
+
+template <typename S>
+__device__ __forceinline__ void block_keys_seq(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, const KeyGeom &g,
+                                               const DevLimits &L, uint64_t max_ctx, uint64_t base, uint32_t cnt,
+                                               S *win, uint64_t *skey) {
+    // window: positions [base, base + cnt + k - 1) clipped to the text
+    uint64_t wlen = (uint64_t)cnt + g.k - 1;
+    if (wlen > n - base) wlen = n - base;
+    const uint32_t wl = (uint32_t)wlen;
+    if (sizeof(S) == 1 && (((uintptr_t)(t + base)) & 3) == 0) {
+        const uint32_t *src = (const uint32_t *)(t + base);
+        uint32_t *dst = (uint32_t *)win;
+        for (uint32_t i = threadIdx.x; i < wl / 4; i += blockDim.x) dst[i] = src[i];
+        for (uint32_t i = (wl / 4) * 4 + threadIdx.x; i < wl; i += blockDim.x) win[i] = t[base + i];
+    } else {
+        for (uint32_t i = threadIdx.x; i < wl; i += blockDim.x) win[i] = t[base + i];
+    }
+    __syncthreads();
+    const uint32_t first = threadIdx.x * kSeqItems;
+    uint64_t caps[kSeqItems], keys[kSeqItems];
+    uint64_t seg_end = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kSeqItems; ++i) {
+        uint64_t q = base + first + i;
+        uint64_t lim;
+        if (q >= n) {
+            lim = 0;
+        } else if (L.ends == nullptr) {
+            lim = n - q;
+        } else {
+            if (q >= seg_end) seg_end = q + lim_at(L, q);
+            lim = seg_end - q;
+        }
+        caps[i] = lim < max_ctx ? lim : max_ctx;
+    }
+    if (first < cnt) keys_from_window<S, kSeqItems>(win, cm, g, first, caps, keys);
+#pragma unroll
+    for (uint32_t i = 0; i < kSeqItems; ++i)
+        if (first + i < cnt) skey[first + i] = keys[i];
+    __syncthreads();
+}
+
+template <typename S, typename I, int RK>
+__global__ __launch_bounds__(256) void k_classify_seq(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, KeyGeom g,
+                                                      DevLimits L, uint64_t max_ctx, Splitters SP, RouterDev R,
+                                                      uint32_t *__restrict__ bid, uint32_t *__restrict__ hist) {
+    constexpr uint32_t NS = kPwlSeg;
+    __shared__ S win[kSeqChunk + 72];
+    __shared__ uint64_t skey[kSeqChunk];
+    __shared__ uint64_t coarse[RK == 0 ? kCoarseMax : 1];
+    __shared__ uint64_t rknots[RK == 1 ? NS + 1 : 1];
+    __shared__ float ra[RK == 1 ? NS : 1], rb[RK == 1 ? NS : 1], rinv[RK == 1 ? NS : 1];
+    __shared__ uint32_t rlo[RK == 1 ? NS : 1], rhi[RK == 1 ? NS : 1];
+    __shared__ uint16_t rradix[RK == 1 ? (1u << kRouterRadixBits) + 1 : 1];
+    __shared__ float nib[RK == 2 ? 16 * 16 * kMlpHid : 1], w2[RK == 2 ? kMlpHid + 1 : 1];
+    __shared__ unsigned long long st_sum, st_max, st_fb, st_cnt;
+    if (RK == 0)
+        for (uint32_t c = threadIdx.x; c < SP.n_coarse; c += blockDim.x) coarse[c] = SP.spl[(uint64_t)c * SP.stride];
+    if (RK == 1) {
+        for (uint32_t c = threadIdx.x; c < R.ns; c += blockDim.x) {
+            rknots[c] = R.knots[c];
+            ra[c] = R.a[c];
+            rb[c] = R.b[c];
+            rinv[c] = R.inv[c];
+            rlo[c] = R.lo[c];
+            rhi[c] = R.hi[c];
+        }
+        for (uint32_t c = threadIdx.x; c <= (1u << kRouterRadixBits); c += blockDim.x) rradix[c] = R.radix[c];
+    }
+    if (RK == 2) {
+        for (uint32_t c = threadIdx.x; c < 16 * 16 * kMlpHid; c += blockDim.x) nib[c] = R.nib[c];
+        for (uint32_t c = threadIdx.x; c <= (uint32_t)kMlpHid; c += blockDim.x) w2[c] = R.w2[c];
+    }
+    if (threadIdx.x == 0) st_sum = st_max = st_fb = st_cnt = 0;
+    unsigned long long my_sum = 0, my_max = 0, my_fb = 0, my_cnt = 0;
+    for (uint64_t base = (uint64_t)blockIdx.x * kSeqChunk; base < n; base += (uint64_t)gridDim.x * kSeqChunk) {
+        uint32_t cnt = (uint32_t)(n - base < kSeqChunk ? n - base : kSeqChunk);
+        block_keys_seq<S>(t, n, cm, g, L, max_ctx, base, cnt, win, skey);
+        for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+            uint64_t key = skey[i];
+            uint32_t b;
+            if (RK == 0) {
+                b = upper_bound_coarse(SP, coarse, key);
+            } else {
+                uint32_t pred = RK == 1 ? pwl_predict(R, rknots, rradix, ra, rb, rinv, rlo, rhi, key)
+                                        : mlp_predict(R, nib, w2, key);
+                uint32_t fb = 0;
+                b = router_correct(R.window, SP, key, pred, &fb);
+                unsigned long long e = b > pred ? b - pred : pred - b;
+                my_sum += e;
+                my_max = e > my_max ? e : my_max;
+                my_fb += fb;
+                my_cnt += 1;
+            }
+            bid[base + i] = b;
+            atomicAdd(&hist[b], 1u);
+        }
+        __syncthreads();
+    }
+    if (RK != 0 && R.stats) {
+        atomicAdd(&st_sum, my_sum);
+        atomicMax(&st_max, my_max);
+        atomicAdd(&st_fb, my_fb);
+        atomicAdd(&st_cnt, my_cnt);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            atomicAdd(&R.stats[0], st_sum);
+            atomicMax(&R.stats[1], st_max);
+            atomicAdd(&R.stats[2], st_fb);
+            atomicAdd(&R.stats[3], st_cnt);
+        }
+    }
+}
+
+template <typename S, typename I>
+__global__ __launch_bounds__(256) void k_scatter_seq(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, KeyGeom g,
+                                                     DevLimits L, uint64_t max_ctx, const uint32_t *__restrict__ bid,
+                                                     unsigned long long *__restrict__ cursor,
+                                                     uint64_t *__restrict__ keys, I *__restrict__ idx) {
+    __shared__ S win[kSeqChunk + 72];
+    __shared__ uint64_t skey[kSeqChunk];
+    for (uint64_t base = (uint64_t)blockIdx.x * kSeqChunk; base < n; base += (uint64_t)gridDim.x * kSeqChunk) {
+        uint32_t cnt = (uint32_t)(n - base < kSeqChunk ? n - base : kSeqChunk);
+        block_keys_seq<S>(t, n, cm, g, L, max_ctx, base, cnt, win, skey);
+        for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+            unsigned long long slot = atomicAdd(&cursor[bid[base + i]], 1ull);
+            keys[slot] = skey[i];
+            idx[slot] = (I)(base + i);
+        }
+        __syncthreads();
+    }
+}
+
 // Exclusive scan of m u32 counts into m+1 u64 offsets (one workgroup).
 __global__ __launch_bounds__(1024) void k_scan_u32(const uint32_t *__restrict__ in, uint64_t m,
                                                    uint64_t *__restrict__ out) {
