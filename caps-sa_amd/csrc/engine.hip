@@ -259,7 +259,7 @@ static void sort_keys_single_range(Workspace &W, uint64_t *keys, uint32_t *ids, 
     Tile *dt = (Tile *)W.stiles.get(nt * sizeof(Tile));
     HIPCHK(hipMemcpyAsync(dt, W.h_tiles.data(), nt * sizeof(Tile), hipMemcpyHostToDevice, st));
     GroupLists none{};
-    k_tile_radix<uint32_t, T, 256><<<(unsigned)nt, 256, 0, st>>>(keys, ids, dt, none, 0);
+    k_tile_radix<uint32_t, T><<<(unsigned)nt, 256, 0, st>>>(keys, ids, dt, none, 0);
     LAUNCHCHK();
     Range r{0, count};
     W.h_range = r;
@@ -434,12 +434,17 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     SP.n_coarse = (uint32_t)ceil_div(p - 1, SP.stride);
     stats.buckets = p;
 
-    // ---- classify + histogram ----
+    // ---- classify + coarse histogram ----
+    uint32_t fshift = 7;
+    while (ceil_div(p, 1ull << fshift) > kCoarseBucketsMax) ++fshift;
+    if ((1u << fshift) > kFineMax) throw CapsError(CAPS_E_INVALID, "too many buckets");
+    const uint32_t pc = (uint32_t)ceil_div(p, 1ull << fshift);
     uint32_t *bid = (uint32_t *)W.bid.get(n_pos * 4);
-    uint32_t *hist = (uint32_t *)W.hist.get(p * 4);
+    uint32_t *hist = (uint32_t *)W.hist.get(pc * 4);
+    uint64_t *coff = (uint64_t *)W.toff.get((std::max<uint64_t>(pc, p) + 1) * 8);
     uint64_t *off = (uint64_t *)W.off.get((p + 1) * 8);
-    unsigned long long *cursor = (unsigned long long *)W.cursor.get(p * 8);
-    HIPCHK(hipMemsetAsync(hist, 0, p * 4, st));
+    unsigned long long *cursor = (unsigned long long *)W.cursor.get(pc * 8);
+    HIPCHK(hipMemsetAsync(hist, 0, pc * 4, st));
     RouterDev R{};
     uint32_t rkind = p > 1 ? o.router : CAPS_ROUTER_BSEARCH;
     if (rkind > CAPS_ROUTER_MLP) throw CapsError(CAPS_E_INVALID, "unknown router");
@@ -453,66 +458,69 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     const unsigned gseq = (unsigned)std::min<uint64_t>(ceil_div(n_pos, kSeqChunk), 65536);
     if (seq) {
         if (rkind == CAPS_ROUTER_PWL)
-            k_classify_seq<S, I, 1><<<gseq, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, bid, hist);
+            k_classify_seq<S, I, 1><<<gseq, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, bid, hist, fshift, pc);
         else if (rkind == CAPS_ROUTER_MLP)
-            k_classify_seq<S, I, 2><<<gseq, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, bid, hist);
+            k_classify_seq<S, I, 2><<<gseq, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, bid, hist, fshift, pc);
         else
-            k_classify_seq<S, I, 0><<<gseq, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, bid, hist);
+            k_classify_seq<S, I, 0><<<gseq, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, bid, hist, fshift, pc);
     } else if (rkind == CAPS_ROUTER_PWL)
-        k_classify<S, I, 1><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, bid, hist);
+        k_classify<S, I, 1><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, bid, hist, fshift,
+                                                   pc);
     else if (rkind == CAPS_ROUTER_MLP)
-        k_classify<S, I, 2><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, bid, hist);
+        k_classify<S, I, 2><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, bid, hist, fshift,
+                                                   pc);
     else
-        k_classify<S, I, 0><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, bid, hist);
+        k_classify<S, I, 0><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, bid, hist, fshift,
+                                                   pc);
     LAUNCHCHK();
     kt.end();
-    k_scan_u32<<<1, 1024, 0, st>>>(hist, p, off);
+    k_scan_u32<<<1, 1024, 0, st>>>(hist, pc, coff);
     LAUNCHCHK();
-    HIPCHK(hipMemcpyAsync(cursor, off, p * 8, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(cursor, coff, pc * 8, hipMemcpyDeviceToDevice, st));
     tm.mark(PH_CLASSIFY);
 
-    // ---- scatter ----
+    // ---- two-level scatter ----
     uint64_t *keys = (uint64_t *)W.keys.get(n_pos * 8);
+    Rec<I> *tmp = (Rec<I> *)W.recs.get(n_pos * sizeof(Rec<I>));
     kt.begin(&stats.k_scatter);
+    const unsigned g1 = (unsigned)ceil_div(n_pos, kScatterChunk);
     if (seq)
-        k_scatter_seq<S, I><<<gseq, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, bid, cursor, keys, d_out);
+        k_scatter1<S, I, true><<<g1, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, bid, fshift, pc, cursor,
+                                                   tmp);
     else
-        k_scatter<S, I><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, bid, cursor, keys, d_out);
+        k_scatter1<S, I, false><<<g1, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, bid, fshift, pc,
+                                                    cursor, tmp);
+    LAUNCHCHK();
+    k_scatter2<I><<<pc, 256, 0, st>>>(tmp, coff, fshift, (uint32_t)p, off, keys, d_out);
     LAUNCHCHK();
     kt.end();
     tm.mark(PH_SCATTER);
 
-    // ---- tiles: LDS radix sort per tile; whole-bucket tiles emit their tie groups ----
-    uint32_t *tcnt = (uint32_t *)W.tcnt.get(p * 4);
-    uint64_t *toff = (uint64_t *)W.toff.get((p + 1) * 8);
-    uint64_t max_tiles = ceil_div(n_pos, T) + p + 1;
-    Tile *tiles = (Tile *)W.tiles.get(max_tiles * sizeof(Tile));
+    // ---- tiles: LDS radix sort per tile (by size class); whole-bucket tiles emit their tie groups ----
+    uint64_t tcap = ceil_div(n_pos, 512) + p + 1;  // tiles per class (every tile but the last of a bucket is full)
+    Tile *tiles = (Tile *)W.tiles.get(kTileClasses * tcap * sizeof(Tile));
     Range *big = (Range *)W.big.get(p * sizeof(Range));
-    unsigned int *counters = (unsigned int *)W.counts.get(64);  // [0] oversized, [8..16) group classes
+    unsigned int *counters = (unsigned int *)W.counts.get(64);  // [0] oversized, [1..4) tile classes, [8..16) groups
     HIPCHK(hipMemsetAsync(counters, 0, 64, st));
     GroupLists gl{};
     for (int c = 0; c < kGroupClasses; ++c)
         gl.list[c] = (Group *)W.glist[c].get((n_pos / group_class_min(c) + 1) * sizeof(Group));
     gl.counts = counters + 8;
-    k_tile_counts<<<(unsigned)ceil_div(p, 256), 256, 0, st>>>(off, (uint32_t)p, T, tcnt);
+    k_tile_emit<<<(unsigned)ceil_div(p, 256), 256, 0, st>>>(off, (uint32_t)p, T, tiles, tcap, counters + 1, big,
+                                                             counters);
     LAUNCHCHK();
-    k_scan_u32<<<1, 1024, 0, st>>>(tcnt, p, toff);
-    LAUNCHCHK();
-    k_tile_emit<<<(unsigned)ceil_div(p, 256), 256, 0, st>>>(off, (uint32_t)p, T, toff, tiles, big, counters);
-    LAUNCHCHK();
-    uint64_t n_tiles = 0;
-    unsigned int n_big = 0;
-    HIPCHK(hipMemcpyAsync(&n_tiles, toff + p, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&n_big, counters, 4, hipMemcpyDeviceToHost, st));
+    unsigned int hc[4];
+    HIPCHK(hipMemcpyAsync(hc, counters, 16, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    stats.tiles = n_tiles;
+    unsigned int n_big = hc[0];
+    stats.tiles = (uint64_t)hc[1] + hc[2] + hc[3];
     stats.oversized = n_big;
-    if (n_tiles) {
-        kt.begin(&stats.k_tile_sort);
-        k_tile_radix<I, T, 256><<<(unsigned)n_tiles, 256, 0, st>>>(keys, d_out, tiles, gl, 1);
-        LAUNCHCHK();
-        kt.end();
-    }
+    kt.begin(&stats.k_tile_sort);
+    if (hc[1]) k_tile_radix<I, 1024><<<hc[1], 256, 0, st>>>(keys, d_out, tiles, gl, 1);
+    if (hc[2]) k_tile_radix<I, 2048><<<hc[2], 256, 0, st>>>(keys, d_out, tiles + tcap, gl, 1);
+    if (hc[3]) k_tile_radix<I, 4096><<<hc[3], 256, 0, st>>>(keys, d_out, tiles + 2 * tcap, gl, 1);
+    LAUNCHCHK();
+    kt.end();
     tm.mark(PH_TILE);
 
     // ---- oversized buckets: merge the sorted tiles by key, then find their groups ----

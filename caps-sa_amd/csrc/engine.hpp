@@ -43,7 +43,7 @@ struct Workspace {
     // classification / scatter
     DevBuf bid, hist, off, cursor, keys, keys2, idx2;
     // tiles
-    DevBuf tcnt, toff, tiles, big, counts;
+    DevBuf tcnt, toff, toff2, tiles, big, counts, recs;
     // refinement
     DevBuf glist[kGroupClasses], lcp, lcp2, first, jobs;
     std::vector<MergeJob> h_jobs;

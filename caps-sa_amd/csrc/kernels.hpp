@@ -80,6 +80,8 @@ __global__ void k_pick_splitters(const uint64_t *__restrict__ sorted, uint64_t s
 // search in the global table.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kCoarseMax = 4096;
+constexpr uint32_t kCoarseBucketsMax = 2048;  // coarse buckets of the two-level scatter
+constexpr uint32_t kFineMax = 4096;           // fine buckets per coarse bucket
 
 struct Splitters {
     const uint64_t *spl;  // p-1 sorted keys
@@ -169,8 +171,10 @@ template <typename S, typename I, int RK>
 __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeMap<S> cm, KeyGeom g, DevLimits L,
                                                   uint64_t max_ctx, const I *__restrict__ pos, uint64_t n_pos,
                                                   Splitters SP, RouterDev R, uint32_t *__restrict__ bid,
-                                                  uint32_t *__restrict__ hist) {
+                                                  uint32_t *__restrict__ hist, uint32_t fshift, uint32_t pc) {
     constexpr uint32_t NS = kPwlSeg;
+    __shared__ uint32_t chist[kCoarseBucketsMax];
+    for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) chist[c] = 0;
     __shared__ uint64_t coarse[RK == 0 ? kCoarseMax : 1];
     __shared__ uint64_t rknots[RK == 1 ? NS + 1 : 1];
     __shared__ float ra[RK == 1 ? NS : 1], rb[RK == 1 ? NS : 1], rinv[RK == 1 ? NS : 1];
@@ -217,8 +221,11 @@ __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeM
             my_cnt += 1;
         }
         bid[i] = b;
-        atomicAdd(&hist[b], 1u);
+        atomicAdd(&chist[b >> fshift], 1u);
     }
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x)
+        if (chist[c]) atomicAdd(&hist[c], chist[c]);
     if (RK != 0 && R.stats) {
         atomicAdd(&st_sum, my_sum);
         atomicMax(&st_max, my_max);
@@ -288,8 +295,11 @@ __device__ __forceinline__ void block_keys_seq(const S *__restrict__ t, uint64_t
 template <typename S, typename I, int RK>
 __global__ __launch_bounds__(256) void k_classify_seq(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, KeyGeom g,
                                                       DevLimits L, uint64_t max_ctx, Splitters SP, RouterDev R,
-                                                      uint32_t *__restrict__ bid, uint32_t *__restrict__ hist) {
+                                                      uint32_t *__restrict__ bid, uint32_t *__restrict__ hist,
+                                                      uint32_t fshift, uint32_t pc) {
     constexpr uint32_t NS = kPwlSeg;
+    __shared__ uint32_t chist[kCoarseBucketsMax];
+    for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) chist[c] = 0;
     __shared__ S win[kSeqChunk + 72];
     __shared__ uint64_t skey[kSeqChunk];
     __shared__ uint64_t coarse[RK == 0 ? kCoarseMax : 1];
@@ -338,10 +348,13 @@ __global__ __launch_bounds__(256) void k_classify_seq(const S *__restrict__ t, u
                 my_cnt += 1;
             }
             bid[base + i] = b;
-            atomicAdd(&hist[b], 1u);
+            atomicAdd(&chist[b >> fshift], 1u);
         }
         __syncthreads();
     }
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x)
+        if (chist[c]) atomicAdd(&hist[c], chist[c]);
     if (RK != 0 && R.stats) {
         atomicAdd(&st_sum, my_sum);
         atomicMax(&st_max, my_max);
@@ -354,25 +367,6 @@ __global__ __launch_bounds__(256) void k_classify_seq(const S *__restrict__ t, u
             atomicAdd(&R.stats[2], st_fb);
             atomicAdd(&R.stats[3], st_cnt);
         }
-    }
-}
-
-template <typename S, typename I>
-__global__ __launch_bounds__(256) void k_scatter_seq(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, KeyGeom g,
-                                                     DevLimits L, uint64_t max_ctx, const uint32_t *__restrict__ bid,
-                                                     unsigned long long *__restrict__ cursor,
-                                                     uint64_t *__restrict__ keys, I *__restrict__ idx) {
-    __shared__ S win[kSeqChunk + 72];
-    __shared__ uint64_t skey[kSeqChunk];
-    for (uint64_t base = (uint64_t)blockIdx.x * kSeqChunk; base < n; base += (uint64_t)gridDim.x * kSeqChunk) {
-        uint32_t cnt = (uint32_t)(n - base < kSeqChunk ? n - base : kSeqChunk);
-        block_keys_seq<S>(t, n, cm, g, L, max_ctx, base, cnt, win, skey);
-        for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
-            unsigned long long slot = atomicAdd(&cursor[bid[base + i]], 1ull);
-            keys[slot] = skey[i];
-            idx[slot] = (I)(base + i);
-        }
-        __syncthreads();
     }
 }
 
@@ -403,18 +397,171 @@ __global__ __launch_bounds__(1024) void k_scan_u32(const uint32_t *__restrict__ 
     if (tid == 1023) out[m] = s[1023];
 }
 
-template <typename S, typename I>
-__global__ __launch_bounds__(256) void k_scatter(const S *__restrict__ t, CodeMap<S> cm, KeyGeom g, DevLimits L,
-                                                 uint64_t max_ctx, const I *__restrict__ pos, uint64_t n_pos,
-                                                 const uint32_t *__restrict__ bid, unsigned long long *__restrict__ cursor,
-                                                 uint64_t *__restrict__ keys, I *__restrict__ idx) {
-    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pos; i += stride) {
-        uint64_t p = pos ? (uint64_t)pos[i] : i;
-        uint64_t key = key_at(t, cm, g, p, 0, cap_at(L, p, max_ctx));
-        unsigned long long slot = atomicAdd(&cursor[bid[i]], 1ull);
-        keys[slot] = key;
-        idx[slot] = (I)p;
+// ---------------------------------------------------------------------------
+// Two-level scatter.  Fine bucket b belongs to coarse bucket b >> fshift.
+// Pass 1: a block owns a chunk of 64K positions, counts its coarse buckets,
+// reserves one run per coarse bucket with a single global atomic each, and
+// writes (key, idx, fine bucket) records into those runs.  Pass 2: one
+// workgroup per coarse bucket histograms its fine buckets in LDS, writes the
+// fine bucket offsets, and places every record with LDS cursors.
+// ---------------------------------------------------------------------------
+template <typename I>
+struct Rec {
+    uint64_t key;
+    I idx;
+    uint32_t bid;
+};
+
+constexpr uint32_t kScatterChunk = 65536;
+
+template <typename S, typename I, bool SEQ>
+__global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, KeyGeom g,
+                                                  DevLimits L, uint64_t max_ctx, const I *__restrict__ pos,
+                                                  uint64_t n_pos, const uint32_t *__restrict__ bid, uint32_t fshift,
+                                                  uint32_t pc, unsigned long long *__restrict__ ccursor,
+                                                  Rec<I> *__restrict__ tmp) {
+    __shared__ S win[SEQ ? kSeqChunk + 72 : 1];
+    __shared__ uint64_t skey[SEQ ? kSeqChunk : 1];
+    __shared__ unsigned long long cbase[kCoarseBucketsMax];
+    __shared__ uint32_t lcnt[kCoarseBucketsMax];
+    const uint64_t c0 = (uint64_t)blockIdx.x * kScatterChunk;
+    if (c0 >= n_pos) return;
+    const uint64_t c1 = c0 + kScatterChunk < n_pos ? c0 + kScatterChunk : n_pos;
+    for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) lcnt[c] = 0;
+    __syncthreads();
+    for (uint64_t i0 = c0; i0 < c1; i0 += 8 * blockDim.x) {
+        uint32_t bb[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            uint64_t i = i0 + u * blockDim.x + threadIdx.x;
+            bb[u] = i < c1 ? bid[i] : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (bb[u] != 0xFFFFFFFFu) atomicAdd(&lcnt[bb[u] >> fshift], 1u);
+    }
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) {
+        uint32_t m = lcnt[c];
+        cbase[c] = m ? atomicAdd(&ccursor[c], (unsigned long long)m) : 0ull;
+        lcnt[c] = 0;
+    }
+    __syncthreads();
+    if (SEQ) {
+        for (uint64_t base = c0; base < c1; base += kSeqChunk) {
+            uint32_t cnt = (uint32_t)(c1 - base < kSeqChunk ? c1 - base : kSeqChunk);
+            block_keys_seq<S>(t, n, cm, g, L, max_ctx, base, cnt, win, skey);
+            constexpr int U = kSeqChunk / 256;
+            uint32_t bb[U];
+            unsigned long long slot[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                uint32_t i = u * 256 + threadIdx.x;
+                bb[u] = i < cnt ? bid[base + i] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                uint32_t i = u * 256 + threadIdx.x;
+                uint32_t c = bb[u] >> fshift;
+                slot[u] = i < cnt ? cbase[c] + atomicAdd(&lcnt[c], 1u) : 0ull;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                uint32_t i = u * 256 + threadIdx.x;
+                if (i < cnt) {
+                    Rec<I> r;
+                    r.key = skey[i];
+                    r.idx = (I)(base + i);
+                    r.bid = bb[u];
+                    tmp[slot[u]] = r;
+                }
+            }
+            __syncthreads();
+        }
+    } else {
+        for (uint64_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
+            uint64_t p = pos ? (uint64_t)pos[i] : i;
+            uint32_t b = bid[i];
+            uint32_t c = b >> fshift;
+            unsigned long long slot = cbase[c] + atomicAdd(&lcnt[c], 1u);
+            Rec<I> r;
+            r.key = key_at(t, cm, g, p, 0, cap_at(L, p, max_ctx));
+            r.idx = (I)p;
+            r.bid = b;
+            tmp[slot] = r;
+        }
+    }
+}
+
+template <typename I>
+__global__ __launch_bounds__(256) void k_scatter2(const Rec<I> *__restrict__ tmp, const uint64_t *__restrict__ coff,
+                                                  uint32_t fshift, uint32_t p, uint64_t *__restrict__ off,
+                                                  uint64_t *__restrict__ keys, I *__restrict__ idx) {
+    constexpr uint32_t FMAX = kFineMax;
+    __shared__ uint32_t cnt[FMAX];
+    __shared__ unsigned long long fbase[FMAX];
+    __shared__ unsigned long long part[256];
+    const uint32_t c = blockIdx.x;
+    const uint64_t s = coff[c], e = coff[c + 1];
+    const uint32_t f0 = c << fshift;
+    const uint32_t F = 1u << fshift;
+    const uint32_t nf = p - f0 < F ? p - f0 : F;
+    for (uint32_t j = threadIdx.x; j < nf; j += blockDim.x) cnt[j] = 0;
+    __syncthreads();
+    for (uint64_t i0 = s; i0 < e; i0 += 8 * blockDim.x) {
+        uint32_t bb[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            uint64_t i = i0 + u * blockDim.x + threadIdx.x;
+            bb[u] = i < e ? tmp[i].bid - f0 : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (bb[u] != 0xFFFFFFFFu) atomicAdd(&cnt[bb[u]], 1u);
+    }
+    __syncthreads();
+    // exclusive scan of cnt[0..nf): each thread scans a contiguous slice
+    const uint32_t per = (nf + blockDim.x - 1) / blockDim.x;
+    const uint32_t j0 = threadIdx.x * per, j1 = j0 + per < nf ? j0 + per : nf;
+    unsigned long long sum = 0;
+    for (uint32_t j = j0; j < j1; ++j) sum += cnt[j];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t o = 1; o < 256; o <<= 1) {
+        unsigned long long v = threadIdx.x >= o ? part[threadIdx.x - o] : 0ull;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    unsigned long long run = s + (threadIdx.x ? part[threadIdx.x - 1] : 0ull);
+    for (uint32_t j = j0; j < j1; ++j) {
+        fbase[j] = run;
+        off[f0 + j] = run;
+        run += cnt[j];
+        cnt[j] = 0;
+    }
+    if (f0 + nf == p && threadIdx.x == 0) off[p] = e;
+    __syncthreads();
+    for (uint64_t i0 = s; i0 < e; i0 += 8 * blockDim.x) {
+        Rec<I> r[8];
+        unsigned long long slot[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            uint64_t i = i0 + u * blockDim.x + threadIdx.x;
+            if (i < e) r[u] = tmp[i];
+            else r[u].bid = 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            uint32_t j = r[u].bid - f0;
+            slot[u] = r[u].bid != 0xFFFFFFFFu ? fbase[j] + atomicAdd(&cnt[j], 1u) : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (r[u].bid != 0xFFFFFFFFu) {
+                keys[slot[u]] = r[u].key;
+                idx[slot[u]] = r[u].idx;
+            }
     }
 }
 
@@ -422,36 +569,33 @@ __global__ __launch_bounds__(256) void k_scatter(const S *__restrict__ t, CodeMa
 // Tiles: every bucket is cut into tiles of at most T records.
 // ---------------------------------------------------------------------------
 
-__global__ void k_tile_counts(const uint64_t *__restrict__ off, uint32_t p, uint32_t T, uint32_t *__restrict__ cnt) {
-    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= p) return;
-    uint64_t sz = off[b + 1] - off[b];
-    cnt[b] = (uint32_t)((sz + T - 1) / T);
-}
-
 struct Range {
     uint64_t start;
     uint64_t len;
 };
 
-__global__ void k_tile_emit(const uint64_t *__restrict__ off, uint32_t p, uint32_t T,
-                            const uint64_t *__restrict__ toff, Tile *__restrict__ tiles, Range *__restrict__ big,
+// Cuts every bucket into tiles of at most T records and files each tile
+// under its size class (tiles[cls * cap + slot]); buckets larger than T are
+// also listed in `big` for the key merges.
+__global__ void k_tile_emit(const uint64_t *__restrict__ off, uint32_t p, uint32_t T, Tile *__restrict__ tiles,
+                            uint64_t cap, unsigned int *__restrict__ tcount, Range *__restrict__ big,
                             unsigned int *__restrict__ n_big) {
     uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= p) return;
     uint64_t s = off[b], sz = off[b + 1] - s;
-    uint64_t o = toff[b];
     for (uint64_t x = 0; x < sz; x += T) {
         Tile tl;
         tl.start = s + x;
         tl.len = (uint32_t)(sz - x < T ? sz - x : T);
         tl.whole = sz <= T ? 1u : 0u;
-        tiles[o++] = tl;
+        int c = tile_class(tl.len);
+        unsigned int slot = atomicAdd(&tcount[c], 1u);
+        tiles[(uint64_t)c * cap + slot] = tl;
     }
     if (sz > T) {
-        unsigned int k = atomicAdd(n_big, 1u);
-        big[k].start = s;
-        big[k].len = sz;
+        unsigned int q = atomicAdd(n_big, 1u);
+        big[q].start = s;
+        big[q].len = sz;
     }
 }
 

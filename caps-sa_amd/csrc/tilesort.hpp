@@ -2,10 +2,13 @@
 // that vary inside the tile, with tie-group emission for tiles that hold a
 // whole bucket.
 //
-// Records are held warp-striped (item i = wave*64*ITEMS + j*64 + lane), so
-// processing j in order and ranking lanes with a ballot-based match gives a
-// stable rank without any per-item atomics; per-wave digit counters are
-// scanned digit-major to place every item.  8-bit digits.
+// Items are held blocked (thread t owns items t*ITEMS .. t*ITEMS+ITEMS-1,
+// consecutive in the current order).  Each 4-bit digit pass counts digits in
+// thread-private LDS columns cnt[digit][thread], scans the counters
+// digit-major, and scatters: thread order = item order, so the sort is
+// stable without atomics or ballots.  LDS rows are padded by one slot per
+// thread's run to keep the blocked reads conflict-light.  Tiles are launched
+// by size class (1024 / 2048 / 4096 records).
 #pragma once
 #include "device.hpp"
 #include "refine.hpp"
@@ -18,38 +21,59 @@ struct Tile {
     uint32_t whole;  // 1 when the tile is a whole bucket (its tie groups are complete)
 };
 
-template <typename I, int T, int B>
-__global__ __launch_bounds__(B) void k_tile_radix(uint64_t *__restrict__ keys, I *__restrict__ idx,
-                                                  const Tile *__restrict__ tiles, GroupLists gl, int emit) {
-    constexpr int WAVES = B / 64;
+constexpr int kTileClasses = 3;  // capacities 1024, 2048, 4096
+__host__ __device__ __forceinline__ int tile_class(uint32_t len) { return len <= 1024 ? 0 : len <= 2048 ? 1 : 2; }
+
+// Exclusive scan of one u32 per thread across a 256-thread block.
+__device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t *wsum) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t before = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (q < w) before += wsum[q];
+    __syncthreads();
+    return before + x - v;
+}
+
+template <typename I, int T>
+__global__ __launch_bounds__(256) void k_tile_radix(uint64_t *__restrict__ keys, I *__restrict__ idx,
+                                                    const Tile *__restrict__ tiles, GroupLists gl, int emit) {
+    constexpr int B = 256;
     constexpr int ITEMS = T / B;
-    __shared__ uint64_t sk[T];
-    __shared__ uint16_t sl[T];
+    constexpr int PADT = T + B;  // one pad slot per thread run
+    __shared__ uint64_t sk[PADT];
+    __shared__ uint16_t sl[PADT];
     __shared__ I si[T];
-    __shared__ uint32_t cnt[WAVES * 256];
-    __shared__ uint32_t dsum[256];
+    __shared__ uint32_t cnt[16 * B];
+    __shared__ uint32_t wsum[4];
     __shared__ unsigned long long s_or;
     __shared__ uint32_t gcnt[kGroupClasses], gbase[kGroupClasses];
+    auto px = [](uint32_t i) { return i + i / ITEMS; };
 
     const Tile tl = tiles[blockIdx.x];
     const uint32_t len = tl.len;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int tid = threadIdx.x;
     if (len == 0) return;
     const uint64_t first = keys[tl.start];
-    uint64_t k[ITEMS];
-    uint16_t li[ITEMS];
     uint64_t my_or = 0;
-#pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-        uint32_t i = (uint32_t)(w * 64 * ITEMS + j * 64 + lane);
+    for (uint32_t i = tid; i < (uint32_t)T; i += B) {
         if (i < len) {
-            k[j] = keys[tl.start + i];
-            li[j] = (uint16_t)i;
+            uint64_t kk = keys[tl.start + i];
+            sk[px(i)] = kk;
+            sl[px(i)] = (uint16_t)i;
             si[i] = idx[tl.start + i];
-            my_or |= k[j] ^ first;
+            my_or |= kk ^ first;
         } else {
-            k[j] = ~0ull;
-            li[j] = 0xFFFF;
+            sk[px(i)] = ~0ull;
+            sl[px(i)] = 0xFFFF;
         }
     }
     if (tid == 0) s_or = 0;
@@ -60,88 +84,74 @@ __global__ __launch_bounds__(B) void k_tile_radix(uint64_t *__restrict__ keys, I
     if (diff) {
         const int lo = __builtin_ctzll(diff);
         const int hi = 64 - __builtin_clzll(diff);
-        const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-        for (int shift = lo; shift < hi; shift += 8) {
-            for (int d = lane; d < 256; d += 64) cnt[w * 256 + d] = 0;
-            __syncthreads();
-            uint32_t rank[ITEMS];
+        uint64_t k[ITEMS];
+        uint16_t li[ITEMS];
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            k[j] = sk[px(tid * ITEMS + j)];
+            li[j] = sl[px(tid * ITEMS + j)];
+        }
+        for (int shift = lo; shift < hi; shift += 4) {
+            // per-thread digit counts in registers: 16 x 8-bit fields in two words
+            uint64_t h0 = 0, h1 = 0;
+            uint32_t r[ITEMS];
 #pragma unroll
             for (int j = 0; j < ITEMS; ++j) {
-                uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
-                uint64_t peers = ~0ull;
-#pragma unroll
-                for (int b = 0; b < 8; ++b) {
-                    uint64_t bm = __ballot((d >> b) & 1u);
-                    peers &= ((d >> b) & 1u) ? bm : ~bm;
-                }
-                uint32_t below = (uint32_t)__popcll(peers & lt);
-                uint32_t base = cnt[w * 256 + d];
-                rank[j] = base + below;
-                if (below == 0) cnt[w * 256 + d] = base + (uint32_t)__popcll(peers);
+                uint32_t d = (uint32_t)(k[j] >> shift) & 15u;
+                uint32_t sh = (d & 7u) * 8u;
+                uint64_t hw = d < 8 ? h0 : h1;
+                r[j] = (uint32_t)(hw >> sh) & 0xFFu;
+                if (d < 8) h0 += 1ull << sh;
+                else h1 += 1ull << sh;
             }
+#pragma unroll
+            for (int d = 0; d < 16; ++d)
+                cnt[d * B + tid] = (uint32_t)(((d < 8 ? h0 : h1) >> ((d & 7) * 8)) & 0xFFu);
             __syncthreads();
-            // digit totals, exclusive scan over digits, per-wave offsets
-            for (int d = tid; d < 256; d += B) {
-                uint32_t s = 0;
+            // exclusive scan of the 16*B counters in digit-major order; thread
+            // u owns the flat range [16u, 16u + 16)
+            uint32_t v[16], s = 0;
 #pragma unroll
-                for (int ww = 0; ww < WAVES; ++ww) s += cnt[ww * 256 + d];
-                dsum[d] = s;
+            for (int q = 0; q < 16; ++q) {
+                v[q] = cnt[16 * tid + q];
+                s += v[q];
             }
-            __syncthreads();
-            for (int off = 1; off < 256; off <<= 1) {
-                uint32_t v[(256 + B - 1) / B];
-                int c = 0;
-                for (int d = tid; d < 256; d += B, ++c) v[c] = d >= off ? dsum[d - off] : 0u;
-                __syncthreads();
-                c = 0;
-                for (int d = tid; d < 256; d += B, ++c) dsum[d] += v[c];
-                __syncthreads();
-            }
-            for (int d = tid; d < 256; d += B) {
-                uint32_t run = d ? dsum[d - 1] : 0u;  // exclusive
+            uint32_t run = block_excl_scan256(s, wsum);
 #pragma unroll
-                for (int ww = 0; ww < WAVES; ++ww) {
-                    uint32_t c = cnt[ww * 256 + d];
-                    cnt[ww * 256 + d] = run;
-                    run += c;
-                }
+            for (int q = 0; q < 16; ++q) {
+                cnt[16 * tid + q] = run;
+                run += v[q];
             }
             __syncthreads();
 #pragma unroll
             for (int j = 0; j < ITEMS; ++j) {
-                uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
-                uint32_t p = cnt[w * 256 + d] + rank[j];
-                sk[p] = k[j];
-                sl[p] = li[j];
+                uint32_t d = (uint32_t)(k[j] >> shift) & 15u;
+                uint32_t p = cnt[d * B + tid] + r[j];
+                sk[px(p)] = k[j];
+                sl[px(p)] = li[j];
             }
             __syncthreads();
 #pragma unroll
             for (int j = 0; j < ITEMS; ++j) {
-                uint32_t i = (uint32_t)(w * 64 * ITEMS + j * 64 + lane);
-                k[j] = sk[i];
-                li[j] = sl[i];
+                k[j] = sk[px(tid * ITEMS + j)];
+                li[j] = sl[px(tid * ITEMS + j)];
             }
             __syncthreads();
         }
     }
-#pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-        uint32_t i = (uint32_t)(w * 64 * ITEMS + j * 64 + lane);
-        if (i < len) {
-            keys[tl.start + i] = k[j];
-            idx[tl.start + i] = si[li[j]];
-        }
-        sk[i] = k[j];
+    for (uint32_t i = tid; i < len; i += B) {
+        keys[tl.start + i] = sk[px(i)];
+        idx[tl.start + i] = si[sl[px(i)]];
     }
     if (!emit || !tl.whole || len < 2) return;
     // tie groups of the bucket: two passes (count, then place)
     if (tid < kGroupClasses) gcnt[tid] = 0;
     __syncthreads();
     for (uint32_t r = tid; r < len; r += B) {
-        uint64_t kr = sk[r];
-        if ((r == 0 || sk[r - 1] != kr) && r + 1 < len && sk[r + 1] == kr) {
+        uint64_t kr = sk[px(r)];
+        if ((r == 0 || sk[px(r - 1)] != kr) && r + 1 < len && sk[px(r + 1)] == kr) {
             uint32_t e = r + 2;
-            while (e < len && sk[e] == kr) ++e;
+            while (e < len && sk[px(e)] == kr) ++e;
             atomicAdd(&gcnt[group_class(e - r)], 1u);
         }
     }
@@ -152,10 +162,10 @@ __global__ __launch_bounds__(B) void k_tile_radix(uint64_t *__restrict__ keys, I
     }
     __syncthreads();
     for (uint32_t r = tid; r < len; r += B) {
-        uint64_t kr = sk[r];
-        if ((r == 0 || sk[r - 1] != kr) && r + 1 < len && sk[r + 1] == kr) {
+        uint64_t kr = sk[px(r)];
+        if ((r == 0 || sk[px(r - 1)] != kr) && r + 1 < len && sk[px(r + 1)] == kr) {
             uint32_t e = r + 2;
-            while (e < len && sk[e] == kr) ++e;
+            while (e < len && sk[px(e)] == kr) ++e;
             int c = group_class(e - r);
             uint32_t slot = atomicAdd(&gcnt[c], 1u);
             gl.list[c][gbase[c] + slot] = Group{tl.start + r, e - r};
