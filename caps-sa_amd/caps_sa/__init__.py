@@ -119,6 +119,11 @@ _SIGNATURES = {
                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p,
                                        ctypes.c_void_p]),
+    "caps_router_mlp_grad_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                    ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                                                    ctypes.c_void_p]),
+    "caps_router_mlp_forward_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                                       ctypes.c_void_p, ctypes.c_void_p]),
     "caps_cascade_merge": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
                                            ctypes.POINTER(_Limits), ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
@@ -302,8 +307,11 @@ class ExtMemOpts:
         return o
 
 
+FLAG_ROUTER_UNTRAINED = 1
+
+
 def _c_opts(max_context=MAX_CONTEXT, subproblem_count=0, router=0, key_geometry=0, gpu_buckets=0, tile_keys=0,
-            want_lcp=0) -> _Opts:
+            want_lcp=0, flags=0) -> _Opts:
     o = _Opts()
     lib().caps_opts_default(ctypes.byref(o))
     o.max_context = min(int(max_context), MAX_CONTEXT)
@@ -313,6 +321,7 @@ def _c_opts(max_context=MAX_CONTEXT, subproblem_count=0, router=0, key_geometry=
     o.gpu_buckets = int(gpu_buckets)
     o.tile_keys = int(tile_keys)
     o.want_lcp = int(want_lcp)
+    o.flags = int(flags)
     return o
 
 

@@ -31,7 +31,7 @@ def text_to_device(text: np.ndarray, device="cuda:0") -> torch.Tensor:
 
 def build_device(d_text: torch.Tensor, n: int, sym_bytes: int, d_positions=None, n_pos=None, idx_bytes=4,
                  lp: LimitProvider = None, max_context=MAX_CONTEXT, router=0, gpu_buckets=0, key_geometry=0,
-                 d_out=None, d_lcp=None, want_lcp=False):
+                 d_out=None, d_lcp=None, want_lcp=False, flags=0):
     """caps_build_device: all buffers already in HBM.  Returns (sa, lcp)."""
     n_pos = n if d_positions is None else (n_pos if n_pos is not None else d_positions.numel())
     if d_out is None:
@@ -40,7 +40,7 @@ def build_device(d_text: torch.Tensor, n: int, sym_bytes: int, d_positions=None,
         d_lcp = torch.empty(n_pos, dtype=_IDX[idx_bytes], device=d_text.device)
     lim = (lp or PlainText(n))._c()
     o = _c_opts(max_context, router=router, gpu_buckets=gpu_buckets, key_geometry=key_geometry,
-                want_lcp=int(d_lcp is not None))
+                want_lcp=int(d_lcp is not None), flags=flags)
     rc = lib().caps_build_device(d_text.data_ptr(), n, sym_bytes,
                                  d_positions.data_ptr() if d_positions is not None else None, n_pos, idx_bytes,
                                  ctypes.byref(lim), ctypes.byref(o), d_out.data_ptr(),
@@ -79,3 +79,29 @@ def seed_keys(d_text: torch.Tensor, n: int, sym_bytes: int, positions: np.ndarra
     _check(lib().caps_seed_keys_device(d_text.data_ptr(), n, sym_bytes, ctypes.byref(lim), geometry, rank,
                                        d_pos.data_ptr(), count, keys.data_ptr(), geo, _stream_ptr()))
     return keys.cpu().numpy().view(np.uint64)[:count], (geo[0], geo[1])
+
+
+MLP_IN, MLP_HID = 64, 32
+MLP_PARAMS = MLP_IN * MLP_HID + MLP_HID + MLP_HID + 1
+
+
+def mlp_grad(params: np.ndarray, keys: np.ndarray, targets: np.ndarray, device="cuda:0"):
+    """Router MLP mean-squared-error gradient on the GPU (caps_router_mlp_grad_device)."""
+    p = torch.from_numpy(np.ascontiguousarray(params, dtype=np.float32)).to(device)
+    k = torch.from_numpy(np.ascontiguousarray(keys, dtype=np.uint64).view(np.int64)).to(device)
+    y = torch.from_numpy(np.ascontiguousarray(targets, dtype=np.float32)).to(device)
+    g = torch.empty(MLP_PARAMS, dtype=torch.float32, device=device)
+    loss = torch.empty(1, dtype=torch.float32, device=device)
+    _check(lib().caps_router_mlp_grad_device(p.data_ptr(), k.data_ptr(), y.data_ptr(), len(keys), g.data_ptr(),
+                                             loss.data_ptr(), _stream_ptr()))
+    return g.cpu().numpy(), float(loss.cpu().numpy()[0])
+
+
+def mlp_forward(params: np.ndarray, keys: np.ndarray, device="cuda:0"):
+    """Router MLP output through the nibble tables used by classification."""
+    p = torch.from_numpy(np.ascontiguousarray(params, dtype=np.float32)).to(device)
+    k = torch.from_numpy(np.ascontiguousarray(keys, dtype=np.uint64).view(np.int64)).to(device)
+    out = torch.empty(max(len(keys), 1), dtype=torch.float32, device=device)
+    _check(lib().caps_router_mlp_forward_device(p.data_ptr(), k.data_ptr(), len(keys), out.data_ptr(),
+                                                _stream_ptr()))
+    return out.cpu().numpy()[: len(keys)]
