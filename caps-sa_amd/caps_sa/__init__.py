@@ -102,6 +102,11 @@ _SIGNATURES = {
     "caps_build_ext_mem_for_positions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
                                                          ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(_Limits),
                                                          ctypes.POINTER(_Opts), EMIT_FN, ctypes.c_void_p]),
+    "caps_build_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                         ctypes.POINTER(_Opts)]),
+    "caps_build_batch_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p,
+                                                ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(_Opts),
+                                                ctypes.c_void_p, ctypes.c_void_p]),
     "caps_build_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p,
                                           ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(_Limits),
                                           ctypes.POINTER(_Opts), ctypes.c_void_p, ctypes.c_void_p,
@@ -407,6 +412,30 @@ def build_in_memory_for_positions(text, positions, index_dtype=np.uint32):
 def build_in_memory_for_positions_with_opts(text, positions, opts: Opts, index_dtype=np.uint32):
     """sample_sort.rs:152-165."""
     return build_in_memory_for_positions_with(text, positions, None, opts, index_dtype)
+
+
+class _TextItem(ctypes.Structure):
+    _fields_ = [("text", ctypes.c_void_p), ("n", ctypes.c_uint64), ("out_sa", ctypes.c_void_p)]
+
+
+def build_batch(texts, index_dtype=np.uint32):
+    """build_in_memory for every text of the list, as one GPU build per symbol
+    width.  Returns the suffix arrays in input order."""
+    arrs = [_as_text(t) for t in texts]
+    idt = np.dtype(index_dtype)
+    outs = [np.empty(len(a), dtype=idt) for a in arrs]
+    for width in (1, 2):
+        sel = [i for i, a in enumerate(arrs) if a.dtype.itemsize == width]
+        if not sel:
+            continue
+        items = (_TextItem * len(sel))()
+        for j, i in enumerate(sel):
+            items[j].text = arrs[i].ctypes.data if len(arrs[i]) else None
+            items[j].n = len(arrs[i])
+            items[j].out_sa = outs[i].ctypes.data if len(arrs[i]) else None
+        o = _c_opts()
+        _check(lib().caps_build_batch(items, len(sel), width, idt.itemsize, ctypes.byref(o)))
+    return outs
 
 
 # ---------------------------------------------------------------------------

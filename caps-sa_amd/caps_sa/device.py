@@ -105,3 +105,16 @@ def mlp_forward(params: np.ndarray, keys: np.ndarray, device="cuda:0"):
     _check(lib().caps_router_mlp_forward_device(p.data_ptr(), k.data_ptr(), len(keys), out.data_ptr(),
                                                 _stream_ptr()))
     return out.cpu().numpy()[: len(keys)]
+
+
+def build_batch_device(d_text: torch.Tensor, n: int, sym_bytes: int, ends: np.ndarray, idx_bytes=4, d_out=None,
+                       router=0, gpu_buckets=0):
+    """caps_build_batch_device: texts back to back in HBM, `ends` their
+    cumulative ends.  Returns the global positions, text after text."""
+    ends = np.ascontiguousarray(ends, dtype=np.uint64)
+    if d_out is None:
+        d_out = torch.empty(n, dtype=_IDX[idx_bytes], device=d_text.device)
+    o = _c_opts(router=router, gpu_buckets=gpu_buckets)
+    _check(lib().caps_build_batch_device(d_text.data_ptr(), n, sym_bytes, ends.ctypes.data, len(ends), idx_bytes,
+                                         ctypes.byref(o), d_out.data_ptr(), _stream_ptr()))
+    return d_out

@@ -43,6 +43,30 @@ __device__ __forceinline__ uint64_t lim_at(const DevLimits &L, uint64_t p) {
     return (lo < L.n_ends ? e : L.n) - p;
 }
 
+// Segment index of position p (0 for PlainText) and its limit.
+__device__ __forceinline__ uint64_t lim_seg(const DevLimits &L, uint64_t p, uint64_t *seg) {
+    if (L.ends == nullptr) {
+        *seg = 0;
+        return L.n - p;
+    }
+    uint64_t lo = 0, hi = L.n_ends;
+    if (L.dir != nullptr && p < L.n) {
+        uint64_t b = p >> L.block_shift;
+        lo = L.dir[b];
+        hi = L.dir[b + 1];
+        if (hi < lo + 1) hi = lo + 1;
+        if (hi > L.n_ends) hi = L.n_ends;
+    }
+    while (lo < hi) {
+        uint64_t mid = lo + ((hi - lo) >> 1);
+        if (L.ends[mid] <= p) lo = mid + 1;
+        else hi = mid;
+    }
+    uint64_t e = L.ends[lo < L.n_ends ? lo : L.n_ends - 1];
+    *seg = lo;
+    return (lo < L.n_ends ? e : L.n) - p;
+}
+
 // Comparison window of a suffix: its limit, clipped by max_context.
 __device__ __forceinline__ uint64_t cap_at(const DevLimits &L, uint64_t p, uint64_t max_ctx) {
     uint64_t l = lim_at(L, p);
@@ -70,8 +94,10 @@ __device__ __forceinline__ int boundary_then_pos(const DevLimits &L, uint64_t pa
 //     every code is a symbol, so keys are weakly order-preserving: a strict
 //     key inequality still implies the same suffix inequality, and equal keys
 //     are settled by refinement.
+//   batched builds: the top `idbits` bits carry the text (segment) index, so
+//     one sort orders many texts text-major; the fields sit below them.
 struct KeyGeom {
-    uint32_t bits, k, bias, pad, shift;
+    uint32_t bits, k, bias, pad, shift, idbits;
 };
 
 template <typename S>
@@ -166,12 +192,18 @@ __device__ __forceinline__ void keys_from_window(const S *win, CodeMap<S> cm, co
     }
 }
 
-// Symbols two keys share, from the first differing field (pack.rs:207-209).
+// Symbols two keys share, from the first differing field (pack.rs:207-209);
+// keys of different texts (different id bits) share nothing.
 __device__ __forceinline__ uint32_t shared_fields(const KeyGeom &g, uint64_t a, uint64_t b) {
     uint64_t x = a ^ b;
     uint32_t lz = x ? (uint32_t)__builtin_clzll(x) : 64u;
-    uint32_t f = lz / g.bits;
+    if (lz < g.idbits) return 0;
+    uint32_t f = (lz - g.idbits) / g.bits;
     return f < g.k ? f : g.k;
+}
+
+__device__ __forceinline__ uint64_t with_id(const KeyGeom &g, uint64_t key, uint64_t id) {
+    return g.idbits ? (key | (id << (64 - g.idbits))) : key;
 }
 
 // Scalar LCP of text[p+start..] and text[q+start..], stopping at `cap`.

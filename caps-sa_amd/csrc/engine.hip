@@ -368,9 +368,13 @@ static RouterDev train_router(Workspace &W, uint32_t kind, uint32_t flags, const
 // ---------------------------------------------------------------------------
 // The build.
 // ---------------------------------------------------------------------------
+// idbits > 0 builds many texts at once: `lim` delimits the texts and the top
+// idbits bits of every packed key hold the text index, so suffixes sort
+// text-major and no comparison crosses a text end.
 template <typename S, typename I>
 void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, const caps_limits_t *lim,
-                const caps_opts_t &o, I *d_out, I *d_lcp_out, hipStream_t st, caps_stats_t &stats) {
+                const caps_opts_t &o, I *d_out, I *d_lcp_out, hipStream_t st, caps_stats_t &stats,
+                uint32_t idbits = 0) {
     Workspace &W = workspace();
     memset(&stats, 0, sizeof(stats));
     PhaseTimer tm(st);
@@ -400,8 +404,15 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     }
 
     // ---- alphabet and key geometry ----
-    Geometry G = compute_geometry<S>(W, d_text, n, o.key_geometry, L.order, st);
-    const KeyGeom g = G.g;
+    Geometry G = compute_geometry<S>(W, d_text, n, idbits ? CAPS_KEY_DENSE : o.key_geometry, L.order, st);
+    const KeyGeom gref = G.g;  // refinement chunks: full width, no text index
+    KeyGeom g = G.g;         // seed keys
+    if (idbits) {
+        if (idbits + G.g.bits > 64) throw CapsError(CAPS_E_INVALID, "too many texts in one batch");
+        g.idbits = idbits;
+        g.k = (64 - idbits) / g.bits;
+        g.shift = 64 - idbits - g.k * g.bits;
+    }
     const CodeMap<S> cm = codemap<S>(G);
     stats.key_bits = g.bits;
     stats.key_k = g.k;
@@ -580,25 +591,25 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     }
     kt.begin(&stats.k_groups_small);
     if (cnt[1])
-        k_wgroups<S, I, 4><<<(unsigned)ceil_div(cnt[1], 64), 256, 0, st>>>(d_text, cm, g, L, max_ctx, gl.list[1],
+        k_wgroups<S, I, 4><<<(unsigned)ceil_div(cnt[1], 64), 256, 0, st>>>(d_text, cm, gref, L, max_ctx, gl.list[1],
                                                                             cnt[1], depth0, d_out, lcp);
     if (cnt[2])
-        k_wgroups<S, I, 8><<<(unsigned)ceil_div(cnt[2], 32), 256, 0, st>>>(d_text, cm, g, L, max_ctx, gl.list[2],
+        k_wgroups<S, I, 8><<<(unsigned)ceil_div(cnt[2], 32), 256, 0, st>>>(d_text, cm, gref, L, max_ctx, gl.list[2],
                                                                             cnt[2], depth0, d_out, lcp);
     if (cnt[3])
-        k_wgroups<S, I, 16><<<(unsigned)ceil_div(cnt[3], 16), 256, 0, st>>>(d_text, cm, g, L, max_ctx, gl.list[3],
+        k_wgroups<S, I, 16><<<(unsigned)ceil_div(cnt[3], 16), 256, 0, st>>>(d_text, cm, gref, L, max_ctx, gl.list[3],
                                                                              cnt[3], depth0, d_out, lcp);
     if (cnt[4])
-        k_wgroups<S, I, 32><<<(unsigned)ceil_div(cnt[4], 8), 256, 0, st>>>(d_text, cm, g, L, max_ctx, gl.list[4],
+        k_wgroups<S, I, 32><<<(unsigned)ceil_div(cnt[4], 8), 256, 0, st>>>(d_text, cm, gref, L, max_ctx, gl.list[4],
                                                                             cnt[4], depth0, d_out, lcp);
     if (cnt[5])
-        k_wgroups<S, I, 64><<<(unsigned)ceil_div(cnt[5], 4), 256, 0, st>>>(d_text, cm, g, L, max_ctx, gl.list[5],
+        k_wgroups<S, I, 64><<<(unsigned)ceil_div(cnt[5], 4), 256, 0, st>>>(d_text, cm, gref, L, max_ctx, gl.list[5],
                                                                             cnt[5], depth0, d_out, lcp);
     LAUNCHCHK();
     kt.end();
     if (cnt[6]) {
         kt.begin(&stats.k_groups_mid);
-        k_groups<S, I, 256, kMidMax><<<cnt[6], 256, 0, st>>>(d_text, cm, g, L, max_ctx, gl.list[6], depth0, d_out,
+        k_groups<S, I, 256, kMidMax><<<cnt[6], 256, 0, st>>>(d_text, cm, gref, L, max_ctx, gl.list[6], depth0, d_out,
                                                               lcp, 0, 0);
         LAUNCHCHK();
         kt.end();
@@ -616,7 +627,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         for (unsigned gi = 0; gi < cnt[7]; ++gi) {
             const Group gr = hg[gi];
             unsigned nch = (unsigned)ceil_div(gr.size, kMidMax);
-            k_groups<S, I, 256, kMidMax><<<dim3(1, nch), 256, 0, st>>>(d_text, cm, g, L, max_ctx, gl.list[7] + gi,
+            k_groups<S, I, 256, kMidMax><<<dim3(1, nch), 256, 0, st>>>(d_text, cm, gref, L, max_ctx, gl.list[7] + gi,
                                                                          depth0, d_out, lbuf, 1, kMidMax);
             LAUNCHCHK();
             I *is = d_out, *ls = lbuf, *id = i2, *ld = l2;
@@ -666,10 +677,6 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     tm.collect(stats);
     kt.collect();
 }
-
-template void build_impl<uint8_t, uint32_t>(const uint8_t *, uint64_t, const uint32_t *, uint64_t,
-                                            const caps_limits_t *, const caps_opts_t &, uint32_t *, uint32_t *,
-                                            hipStream_t, caps_stats_t &);
 
 // ---------------------------------------------------------------------------
 // Small device helpers used by the C ABI.
@@ -882,6 +889,107 @@ int caps_build_device(const void *d_text, uint64_t n, uint32_t sym_bytes, const 
         else if (sym_bytes == 1) run_device<uint8_t, uint64_t>(d_text, n, d_positions, n_pos, limits, o, d_out_sa, d_out_lcp, st);
         else if (idx_bytes == 4) run_device<uint16_t, uint32_t>(d_text, n, d_positions, n_pos, limits, o, d_out_sa, d_out_lcp, st);
         else run_device<uint16_t, uint64_t>(d_text, n, d_positions, n_pos, limits, o, d_out_sa, d_out_lcp, st);
+    });
+}
+
+static uint32_t id_bits_for(uint64_t n_texts) {
+    uint32_t b = 1;
+    while ((1ull << b) < n_texts) ++b;
+    return b;
+}
+
+// Largest number of texts one batched build can index next to at least one
+// key field of `sym_bits` bits... resolved per build from the alphabet; the
+// host API keeps batches within 2^kBatchIdBitsMax texts.
+static constexpr uint32_t kBatchIdBitsMax = 16;
+
+int caps_build_batch_device(const void *d_text, uint64_t n, uint32_t sym_bytes, const uint64_t *ends,
+                            uint64_t n_texts, uint32_t idx_bytes, const caps_opts_t *opts, void *d_out, void *stream) {
+    return guarded([&] {
+        check_common(d_text, n, sym_bytes, nullptr);
+        if (idx_bytes != 4 && idx_bytes != 8) throw CapsError(CAPS_E_UNSUPPORTED, "index width must be 4 or 8");
+        if (idx_bytes == 4 && n > (1ull << 32)) throw CapsError(CAPS_E_INVALID, "u32 index cannot address the batch");
+        if (n_texts == 0 || !ends) throw CapsError(CAPS_E_INVALID, "empty batch");
+        if (id_bits_for(n_texts) > kBatchIdBitsMax) throw CapsError(CAPS_E_INVALID, "too many texts in one batch");
+        caps_limits_t lim{CAPS_LIMITS_SEGMENTED, CAPS_ORDER_SHORTER_FIRST, ends, n_texts};
+        caps_opts_t o = opts_or_default(opts);
+        o.max_context = ~0ull;
+        hipStream_t st = (hipStream_t)stream;
+        uint32_t ib = id_bits_for(n_texts);
+        if (sym_bytes == 1 && idx_bytes == 4)
+            build_impl<uint8_t, uint32_t>((const uint8_t *)d_text, n, nullptr, n, &lim, o, (uint32_t *)d_out, nullptr, st, g_stats, ib);
+        else if (sym_bytes == 1)
+            build_impl<uint8_t, uint64_t>((const uint8_t *)d_text, n, nullptr, n, &lim, o, (uint64_t *)d_out, nullptr, st, g_stats, ib);
+        else if (idx_bytes == 4)
+            build_impl<uint16_t, uint32_t>((const uint16_t *)d_text, n, nullptr, n, &lim, o, (uint32_t *)d_out, nullptr, st, g_stats, ib);
+        else
+            build_impl<uint16_t, uint64_t>((const uint16_t *)d_text, n, nullptr, n, &lim, o, (uint64_t *)d_out, nullptr, st, g_stats, ib);
+    });
+}
+
+int caps_build_batch(const caps_text_item_t *items, uint64_t count, uint32_t sym_bytes, uint32_t idx_bytes,
+                     const caps_opts_t *opts) {
+    return guarded([&] {
+        if (sym_bytes != 1 && sym_bytes != 2)
+            throw CapsError(CAPS_E_UNSUPPORTED, "the GPU build covers 1- and 2-byte symbols only");
+        if (idx_bytes != 4 && idx_bytes != 8) throw CapsError(CAPS_E_UNSUPPORTED, "index width must be 4 or 8");
+        if (count && !items) throw CapsError(CAPS_E_INVALID, "items is NULL");
+        caps_opts_t o = opts_or_default(opts);
+        o.max_context = ~0ull;
+        Workspace &W = workspace();
+        hipStream_t st = nullptr;
+        const uint64_t max_texts = 1ull << kBatchIdBitsMax;
+        uint64_t i = 0;
+        std::vector<uint64_t> ends;
+        std::vector<uint64_t> which;
+        std::vector<uint8_t> host;
+        while (i < count) {
+            // one sub-batch: up to max_texts non-empty texts
+            ends.clear();
+            which.clear();
+            uint64_t total = 0;
+            for (; i < count && which.size() < max_texts; ++i) {
+                if (items[i].n == 0) continue;
+                if (!items[i].text || !items[i].out_sa) throw CapsError(CAPS_E_INVALID, "batch item has NULL pointers");
+                if (idx_bytes == 4 && items[i].n > (1ull << 32))
+                    throw CapsError(CAPS_E_INVALID, "u32 index cannot address a text");
+                total += items[i].n;
+                ends.push_back(total);
+                which.push_back(i);
+            }
+            if (which.empty()) break;
+            host.resize(total * sym_bytes);
+            for (size_t t = 0; t < which.size(); ++t) {
+                uint64_t start = t ? ends[t - 1] : 0;
+                memcpy(host.data() + start * sym_bytes, items[which[t]].text, items[which[t]].n * sym_bytes);
+            }
+            const void *dt = upload_text(W, host.data(), total, sym_bytes, st);
+            caps_limits_t lim{CAPS_LIMITS_SEGMENTED, CAPS_ORDER_SHORTER_FIRST, ends.data(), ends.size()};
+            uint32_t ib = id_bits_for(ends.size());
+            bool wide = total > (1ull << 32);
+            size_t w = wide ? 8 : 4;
+            void *dout = W.out.get(total * w + 8);
+            if (sym_bytes == 1 && !wide)
+                build_impl<uint8_t, uint32_t>((const uint8_t *)dt, total, nullptr, total, &lim, o, (uint32_t *)dout, nullptr, st, g_stats, ib);
+            else if (sym_bytes == 1)
+                build_impl<uint8_t, uint64_t>((const uint8_t *)dt, total, nullptr, total, &lim, o, (uint64_t *)dout, nullptr, st, g_stats, ib);
+            else if (!wide)
+                build_impl<uint16_t, uint32_t>((const uint16_t *)dt, total, nullptr, total, &lim, o, (uint32_t *)dout, nullptr, st, g_stats, ib);
+            else
+                build_impl<uint16_t, uint64_t>((const uint16_t *)dt, total, nullptr, total, &lim, o, (uint64_t *)dout, nullptr, st, g_stats, ib);
+            std::vector<uint8_t> sa(total * w);
+            HIPCHK(hipMemcpy(sa.data(), dout, total * w, hipMemcpyDeviceToHost));
+            for (size_t t = 0; t < which.size(); ++t) {
+                uint64_t start = t ? ends[t - 1] : 0, len = ends[t] - start;
+                void *out = items[which[t]].out_sa;
+                for (uint64_t r = 0; r < len; ++r) {
+                    uint64_t v = wide ? ((const uint64_t *)sa.data())[start + r] : ((const uint32_t *)sa.data())[start + r];
+                    v -= start;
+                    if (idx_bytes == 4) ((uint32_t *)out)[r] = (uint32_t)v;
+                    else ((uint64_t *)out)[r] = v;
+                }
+            }
+        }
     });
 }
 

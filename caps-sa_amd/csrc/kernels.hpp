@@ -45,7 +45,9 @@ __global__ __launch_bounds__(256) void k_sample_keys(const S *__restrict__ t, Co
     if (i >= s) return;
     uint64_t j = ((2 * i + 1) * n_pos) / (2 * s);
     uint64_t p = pos ? (uint64_t)pos[j] : j;
-    keys[i] = key_at(t, cm, g, p, 0, cap_at(L, p, max_ctx));
+    uint64_t seg;
+    uint64_t lim = lim_seg(L, p, &seg);
+    keys[i] = with_id(g, key_at(t, cm, g, p, 0, lim < max_ctx ? lim : max_ctx), seg);
     ids[i] = (uint32_t)i;
 }
 
@@ -62,7 +64,9 @@ __global__ __launch_bounds__(256) void k_random_keys(const S *__restrict__ t, Co
     z ^= z >> 31;
     uint64_t j = (uint64_t)(((unsigned __int128)z * n_pos) >> 64);
     uint64_t p = pos ? (uint64_t)pos[j] : j;
-    keys[i] = key_at(t, cm, g, p, 0, cap_at(L, p, max_ctx));
+    uint64_t seg;
+    uint64_t lim = lim_seg(L, p, &seg);
+    keys[i] = with_id(g, key_at(t, cm, g, p, 0, lim < max_ctx ? lim : max_ctx), seg);
 }
 
 // splitter j-1 = sorted_sample[(j * s) / p], j = 1..p-1 (ext_mem.rs:1341).
@@ -205,7 +209,9 @@ __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeM
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pos; i += stride) {
         uint64_t p = pos ? (uint64_t)pos[i] : i;
-        uint64_t key = key_at(t, cm, g, p, 0, cap_at(L, p, max_ctx));
+        uint64_t seg;
+        uint64_t lim = lim_seg(L, p, &seg);
+        uint64_t key = with_id(g, key_at(t, cm, g, p, 0, lim < max_ctx ? lim : max_ctx), seg);
         uint32_t b;
         if (RK == 0) {
             b = upper_bound_coarse(SP, coarse, key);
@@ -269,8 +275,8 @@ __device__ __forceinline__ void block_keys_seq(const S *__restrict__ t, uint64_t
     }
     __syncthreads();
     const uint32_t first = threadIdx.x * kSeqItems;
-    uint64_t caps[kSeqItems], keys[kSeqItems];
-    uint64_t seg_end = 0;
+    uint64_t caps[kSeqItems], keys[kSeqItems], segs[kSeqItems];
+    uint64_t seg_end = 0, seg = 0;
 #pragma unroll
     for (uint32_t i = 0; i < kSeqItems; ++i) {
         uint64_t q = base + first + i;
@@ -280,15 +286,16 @@ __device__ __forceinline__ void block_keys_seq(const S *__restrict__ t, uint64_t
         } else if (L.ends == nullptr) {
             lim = n - q;
         } else {
-            if (q >= seg_end) seg_end = q + lim_at(L, q);
+            if (q >= seg_end) seg_end = q + lim_seg(L, q, &seg);
             lim = seg_end - q;
         }
         caps[i] = lim < max_ctx ? lim : max_ctx;
+        segs[i] = seg;
     }
     if (first < cnt) keys_from_window<S, kSeqItems>(win, cm, g, first, caps, keys);
 #pragma unroll
     for (uint32_t i = 0; i < kSeqItems; ++i)
-        if (first + i < cnt) skey[first + i] = keys[i];
+        if (first + i < cnt) skey[first + i] = with_id(g, keys[i], segs[i]);
     __syncthreads();
 }
 
@@ -485,7 +492,9 @@ __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint6
             uint32_t c = b >> fshift;
             unsigned long long slot = cbase[c] + atomicAdd(&lcnt[c], 1u);
             Rec<I> r;
-            r.key = key_at(t, cm, g, p, 0, cap_at(L, p, max_ctx));
+            uint64_t seg;
+            uint64_t lim = lim_seg(L, p, &seg);
+            r.key = with_id(g, key_at(t, cm, g, p, 0, lim < max_ctx ? lim : max_ctx), seg);
             r.idx = (I)p;
             r.bid = b;
             tmp[slot] = r;

@@ -136,6 +136,32 @@ int caps_build_device(const void *d_text, uint64_t n, uint32_t sym_bytes, const 
                       uint32_t idx_bytes, const caps_limits_t *limits, const caps_opts_t *opts, void *d_out_sa,
                       void *d_out_lcp, void *stream);
 
+/* ---- batched builds: many independent texts in one pipeline -------------- */
+
+/* One text of a batch (host pointers).  out_sa receives n entries of
+ * idx_bytes: the text's own suffix array (positions local to the text). */
+typedef struct caps_text_item {
+    const void *text;
+    uint64_t n;
+    void *out_sa;
+} caps_text_item_t;
+
+/* build_in_memory (sample_sort.rs:97-103) for every item, as one GPU build:
+ * the texts are laid back to back, each a segment whose end stops every
+ * comparison, and the packed keys carry the text index in their top bits so
+ * one sort orders all suffixes text-major.  All items share sym_bytes.
+ * Empty texts are skipped.  Batches larger than the key's text-index budget
+ * are split into several builds. */
+int caps_build_batch(const caps_text_item_t *items, uint64_t count, uint32_t sym_bytes, uint32_t idx_bytes,
+                     const caps_opts_t *opts);
+
+/* Device variant: d_text holds the texts back to back, ends (host pointer)
+ * their n_texts cumulative ends (strictly increasing, last == n).  d_out
+ * receives n positions: text after text, each text's suffix array as global
+ * positions (subtract the text's start). */
+int caps_build_batch_device(const void *d_text, uint64_t n, uint32_t sym_bytes, const uint64_t *ends,
+                            uint64_t n_texts, uint32_t idx_bytes, const caps_opts_t *opts, void *d_out, void *stream);
+
 /* Per-phase device times (ms) and counters of the last build on this thread. */
 typedef struct caps_stats {
     double ms_total, ms_alphabet, ms_sample, ms_classify, ms_scatter, ms_tile_sort, ms_merge, ms_detect,
