@@ -101,6 +101,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 1 GiB device to device three times (calibrates the HBM byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -116,6 +118,14 @@ def main() -> int:
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no GPU"}))
         return 1
+    if args.copy_probe:
+        src = torch.ones(1 << 28, dtype=torch.int32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 2 * src.numel() * 4}))
+        return 0
     kind, n_default, desc = WORKLOADS[args.workload]
     n = args.n or n_default
     print(f"[bench] generating {args.workload} n={n}", file=sys.stderr, flush=True)

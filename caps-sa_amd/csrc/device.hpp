@@ -135,13 +135,14 @@ __device__ __forceinline__ uint64_t key_at(const S *__restrict__ t, CodeMap<S> c
     uint32_t take = rem < g.k ? (uint32_t)rem : g.k;
     uint64_t key = 0;
     if (take) {
-        const uint8_t *base = (const uint8_t *)t;
-        uint64_t b0 = (p + d) * SZ, b1 = b0 + (uint64_t)take * SZ;
+        // words aligned on the absolute address: each loaded word holds an
+        // in-window byte, so none crosses into a page past the text
+        uint64_t b0 = (uint64_t)(uintptr_t)t + (p + d) * SZ, b1 = b0 + (uint64_t)take * SZ;
         uint64_t a0 = b0 & ~7ull;
         uint32_t nw = (uint32_t)(((b1 + 7) & ~7ull) - a0) >> 3;
         uint64_t w[NW];
 #pragma unroll
-        for (int i = 0; i < NW; ++i) w[i] = i < (int)nw ? *(const uint64_t *)(base + a0 + 8 * i) : 0ull;
+        for (int i = 0; i < NW; ++i) w[i] = i < (int)nw ? *(const uint64_t *)(a0 + 8 * i) : 0ull;
         uint32_t lo = (uint32_t)(b0 - a0), hi = lo + take * SZ;
 #pragma unroll
         for (int j = 0; j < NW * 8; j += SZ) {

@@ -515,7 +515,8 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     HIPCHK(hipMemsetAsync(counters, 0, 64, st));
     GroupLists gl{};
     for (int c = 0; c < kGroupClasses; ++c)
-        gl.list[c] = (Group *)W.glist[c].get((n_pos / group_class_min(c) + 1) * sizeof(Group));
+        gl.list[c] = (Group *)W.glist[c].get((n_pos / group_class_min(c) + 1) *
+                                             (c == 0 ? sizeof(PairRec<I>) : sizeof(Group)));
     gl.counts = counters + 8;
     k_tile_emit<<<(unsigned)ceil_div(p, 256), 256, 0, st>>>(off, (uint32_t)p, T, tiles, tcap, counters + 1, big,
                                                              counters);
@@ -564,7 +565,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         }
         for (unsigned y0 = 0; y0 < n_big; y0 += 65535) {
             dim3 gr((unsigned)std::min<uint64_t>(ceil_div(maxlen, 256), 1024), std::min(65535u, n_big - y0));
-            k_detect_ranges<Range><<<gr, 256, 0, st>>>(keys, big + y0, gl);
+            k_detect_ranges<Range, I><<<gr, 256, 0, st>>>(keys, big + y0, gl, d_out);
             LAUNCHCHK();
         }
     }
@@ -584,8 +585,9 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     const uint64_t depth0 = g.k;
     if (cnt[0]) {
         kt.begin(&stats.k_pairs);
-        k_pairs16<S, I><<<(unsigned)ceil_div(cnt[0], 16), 256, 0, st>>>(d_text, L, max_ctx, gl.list[0], cnt[0],
-                                                                         depth0, d_out, lcp);
+        k_pairs4<S, I><<<(unsigned)ceil_div(cnt[0], 64), 256, 0, st>>>(d_text, L, max_ctx,
+                                                                        (const PairRec<I> *)gl.list[0], cnt[0],
+                                                                        depth0, d_out, lcp);
         LAUNCHCHK();
         kt.end();
     }

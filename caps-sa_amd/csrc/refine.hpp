@@ -49,11 +49,20 @@ __device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// Two-member groups are listed with their positions, so the pair kernel reads
+// the text without first gathering the positions from the suffix array.
+template <typename I>
+struct PairRec {
+    uint64_t start;
+    I a, b;
+};
+
 // ---------------------------------------------------------------------------
 // Group detection inside key-sorted ranges (buckets larger than one tile,
 // after their tiles were merged).  Appends are aggregated per wavefront.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void wave_append(GroupLists &gl, bool have, int cls, Group gr) {
+template <typename I>
+__device__ __forceinline__ void wave_append(GroupLists &gl, bool have, int cls, Group gr, const I *__restrict__ idx) {
     for (int c = 0; c < kGroupClasses; ++c) {
         bool mine = have && cls == c;
         uint64_t mask = __ballot(mine);
@@ -65,16 +74,18 @@ __device__ __forceinline__ void wave_append(GroupLists &gl, bool have, int cls, 
         base = __shfl(base, leader);
         if (mine) {
             uint64_t below = mask & ((1ull << lane) - 1);
-            gl.list[c][base + __popcll(below)] = gr;
+            uint32_t slot = base + __popcll(below);
+            if (c == 0) ((PairRec<I> *)gl.list[0])[slot] = PairRec<I>{gr.start, idx[gr.start], idx[gr.start + 1]};
+            else gl.list[c][slot] = gr;
         }
     }
 }
 
 struct Range;  // defined in kernels.hpp
 
-template <typename RangeT>
+template <typename RangeT, typename I>
 __global__ __launch_bounds__(256) void k_detect_ranges(const uint64_t *__restrict__ keys, const RangeT *__restrict__ ranges,
-                                                       GroupLists gl) {
+                                                       GroupLists gl, const I *__restrict__ idx) {
     RangeT r = ranges[blockIdx.y];
     for (uint64_t base = (uint64_t)blockIdx.x * 256; base < r.len; base += (uint64_t)gridDim.x * 256) {
         uint64_t o = base + threadIdx.x;
@@ -91,59 +102,84 @@ __global__ __launch_bounds__(256) void k_detect_ranges(const uint64_t *__restric
                 have = true;
             }
         }
-        wave_append(gl, have, have ? group_class(gr.size) : 0, gr);
+        wave_append<I>(gl, have, have ? group_class(gr.size) : 0, gr, idx);
     }
 }
 
 // ---------------------------------------------------------------------------
-// Two-member groups: 16 lanes per pair, 4 symbols per lane per step (64
-// symbols), four pairs per wavefront; ballot + ffs find the first mismatch.
+// Two-member groups: four lanes per pair, sixteen pairs per wavefront.  Each
+// lane compares 16 bytes of both suffixes per step (three aligned 8-byte
+// loads per suffix, funnel-shifted), so a pair advances 64 bytes per step;
+// a ballot over the pair's four lanes and ffs find the first differing byte
+// (lcp.rs:292-360 as a lane-parallel compare).  Bytes past the comparison
+// window count as a mismatch at the window's end; no load touches a word
+// without an in-window byte.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void load16(uint64_t x, uint64_t end, uint64_t &lo, uint64_t &hi) {
+    // x, end: absolute byte addresses; words are aligned on the address, so a
+    // word that holds one in-window byte never crosses into another page
+    uint64_t a8 = x & ~7ull;
+    uint32_t sh = (uint32_t)(x & 7) * 8;
+    uint64_t w0 = a8 < end ? *(const uint64_t *)a8 : 0ull;
+    uint64_t w1 = a8 + 8 < end ? *(const uint64_t *)(a8 + 8) : 0ull;
+    uint64_t w2 = a8 + 16 < end ? *(const uint64_t *)(a8 + 16) : 0ull;
+    lo = sh ? (w0 >> sh) | (w1 << (64 - sh)) : w0;
+    hi = sh ? (w1 >> sh) | (w2 << (64 - sh)) : w1;
+}
+
 template <typename S, typename I>
-__global__ __launch_bounds__(256) void k_pairs16(const S *__restrict__ t, DevLimits L, uint64_t max_ctx,
-                                                 const Group *__restrict__ pairs, uint32_t n_pairs, uint64_t known,
-                                                 I *__restrict__ idx, I *__restrict__ lcp) {
-    const uint32_t lane = threadIdx.x & 63, sub = lane >> 4, sl = lane & 15;
-    const uint32_t pid = blockIdx.x * 16 + (threadIdx.x >> 4);
+__global__ __launch_bounds__(256) void k_pairs4(const S *__restrict__ t, DevLimits L, uint64_t max_ctx,
+                                                const PairRec<I> *__restrict__ pairs, uint32_t n_pairs, uint64_t known,
+                                                I *__restrict__ idx, I *__restrict__ lcp) {
+    constexpr uint64_t SZ = sizeof(S);
+    const uint32_t lane = threadIdx.x & 63, grp = lane >> 2, sl = lane & 3;
+    const uint32_t pid = blockIdx.x * 64 + (threadIdx.x >> 2);
     const bool valid = pid < n_pairs;
+    const uint64_t tb = (uint64_t)(uintptr_t)t;
     uint64_t s = 0, a = 0, b = 0, la = 0, lb = 0, cap = 0;
     if (valid) {
-        s = pairs[pid].start;
-        a = idx[s];
-        b = idx[s + 1];
+        PairRec<I> pr = pairs[pid];
+        s = pr.start;
+        a = pr.a;
+        b = pr.b;
         la = lim_at(L, a);
         lb = lim_at(L, b);
         uint64_t ca = la < max_ctx ? la : max_ctx, cb = lb < max_ctx ? lb : max_ctx;
         cap = ca < cb ? ca : cb;
     }
-    uint64_t o = known < cap ? known : cap;
-    bool done = !valid || o >= cap;
-    uint64_t res = cap;
+    const uint64_t capb = cap * SZ;
+    const uint64_t enda = tb + (a + cap) * SZ, endb = tb + (b + cap) * SZ;
+    uint64_t ob = (known < cap ? known : cap) * SZ;  // bytes known equal
+    bool done = !valid || ob >= capb;
+    uint64_t resb = capb;
     while (__any(!done)) {
-        uint32_t firstbad = 4;
+        uint32_t fb = 16;
         if (!done) {
-#pragma unroll
-            for (uint32_t q = 0; q < 4; ++q) {
-                uint64_t oo = o + 4 * sl + q;
-                bool mism = oo >= cap || t[a + oo] != t[b + oo];
-                if (mism && firstbad == 4) firstbad = q;
-            }
+            uint64_t rel = ob + 16 * sl;
+            uint64_t alo, ahi, blo, bhi;
+            load16(tb + a * SZ + rel, enda, alo, ahi);
+            load16(tb + b * SZ + rel, endb, blo, bhi);
+            uint64_t d0 = alo ^ blo, d1 = ahi ^ bhi;
+            fb = d0 ? (uint32_t)__builtin_ctzll(d0) >> 3 : d1 ? 8 + ((uint32_t)__builtin_ctzll(d1) >> 3) : 16u;
+            uint64_t in_win = capb > rel ? capb - rel : 0;
+            if (in_win < fb) fb = (uint32_t)in_win;
         }
-        uint64_t mask = __ballot(!done && firstbad < 4);
-        uint32_t gm = (uint32_t)(mask >> (sub * 16)) & 0xFFFFu;
+        uint64_t mask = __ballot(!done && fb < 16);
+        uint32_t gm = (uint32_t)(mask >> (grp * 4)) & 0xFu;
         int fl = gm ? __ffs(gm) - 1 : 0;
-        uint32_t fq = __shfl(firstbad, (int)(sub * 16 + fl));
+        uint32_t fq = __shfl(fb, (int)(grp * 4 + fl));
         if (!done) {
             if (gm) {
-                uint64_t r = o + 4 * (uint64_t)fl + fq;
-                res = r < cap ? r : cap;
+                uint64_t r = ob + 16 * (uint64_t)fl + fq;
+                resb = r < capb ? r : capb;
                 done = true;
             } else {
-                o += 64;
+                ob += 64;
             }
         }
     }
     if (valid && sl == 0) {
+        uint64_t res = resb / SZ;
         int c;
         if (res < cap) c = t[a + res] < t[b + res] ? -1 : 1;
         else c = boundary_then_pos(L, a, la, b, lb);

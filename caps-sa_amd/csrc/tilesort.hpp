@@ -4,9 +4,10 @@
 //
 // Items are held blocked (thread t owns items t*ITEMS .. t*ITEMS+ITEMS-1,
 // consecutive in the current order).  Each 4-bit digit pass counts digits in
-// thread-private LDS columns cnt[digit][thread], scans the counters
-// digit-major, and scatters: thread order = item order, so the sort is
-// stable without atomics or ballots.  LDS rows are padded by one slot per
+// registers (sixteen 8-bit fields in two words per thread), writes the
+// counts to LDS columns cnt[digit][thread], scans them digit-major, and
+// scatters: thread order = item order, so the sort is stable without atomics
+// or ballots.  LDS rows are padded by one slot per
 // thread's run to keep the blocked reads conflict-light.  Tiles are launched
 // by size class (1024 / 2048 / 4096 records).
 #pragma once
@@ -167,8 +168,9 @@ __global__ __launch_bounds__(256) void k_tile_radix(uint64_t *__restrict__ keys,
             uint32_t e = r + 2;
             while (e < len && sk[px(e)] == kr) ++e;
             int c = group_class(e - r);
-            uint32_t slot = atomicAdd(&gcnt[c], 1u);
-            gl.list[c][gbase[c] + slot] = Group{tl.start + r, e - r};
+            uint32_t slot = gbase[c] + atomicAdd(&gcnt[c], 1u);
+            if (c == 0) ((PairRec<I> *)gl.list[0])[slot] = PairRec<I>{tl.start + r, si[sl[px(r)]], si[sl[px(r + 1)]]};
+            else gl.list[c][slot] = Group{tl.start + r, e - r};
         }
     }
 }

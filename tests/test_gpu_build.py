@@ -161,3 +161,26 @@ def test_markov_repeats_4m_properties():
     got = got.astype(np.uint64)
     assert ctx.verify(got, strict=True, canon=True, threads=8) == -1
     assert np.array_equal(got_lcp.astype(np.uint64), ctx.kasai(got))
+
+
+@pytest.mark.parametrize("offset", [1, 3, 5])
+def test_unaligned_device_text(offset):
+    """A device text that starts at an odd address: the word loads of the key
+    gather and of the pair compare align on absolute addresses."""
+    import torch
+    from caps_sa import device as dev
+
+    for text, w in ((gen.markov_repeats(20000, seed=offset, mean_len=400.0), 1),
+                    (gen.zipf_u16(9000, 300, 1.1, seed=offset), 2)):
+        n = len(text)
+        raw = np.zeros((n + 8) * w, dtype=np.uint8)
+        raw[offset * w: offset * w + n * w] = text.view(np.uint8)
+        d_raw = torch.from_numpy(raw).to("cuda:0")
+        d_view = d_raw[offset * w:]
+        assert d_view.data_ptr() % 8 != 0
+        sa, lcp = dev.build_device(d_view, n, w, idx_bytes=4, want_lcp=True)
+        got = sa.cpu().numpy().view(np.uint32).astype(np.uint64)
+        ctx = O.Ctx(text)
+        want, want_lcp = ctx.build_in_memory(threads=4, want_lcp=True)
+        assert np.array_equal(got, want)
+        assert np.array_equal(lcp.cpu().numpy().view(np.uint32).astype(np.uint64), want_lcp)
