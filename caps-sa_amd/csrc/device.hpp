@@ -37,10 +37,10 @@ __device__ __forceinline__ uint64_t lim_at(const DevLimits &L, uint64_t p) {
         if (L.ends[mid] <= p) lo = mid + 1;
         else hi = mid;
     }
-    // select the value, not the address: a select of &ends[lo] and &L.n would
-    // turn the load into a generic load from the private copy of L
-    uint64_t e = L.ends[lo < L.n_ends ? lo : L.n_ends - 1];
-    return (lo < L.n_ends ? e : L.n) - p;
+    // the last end equals n (checked on the host), so p at or past it reads
+    // that end; no select between ends[] and L.n, which the compiler would
+    // turn into a generic load from a private copy of L
+    return L.ends[lo < L.n_ends ? lo : L.n_ends - 1] - p;
 }
 
 // Segment index of position p (0 for PlainText) and its limit.
@@ -62,9 +62,8 @@ __device__ __forceinline__ uint64_t lim_seg(const DevLimits &L, uint64_t p, uint
         if (L.ends[mid] <= p) lo = mid + 1;
         else hi = mid;
     }
-    uint64_t e = L.ends[lo < L.n_ends ? lo : L.n_ends - 1];
     *seg = lo;
-    return (lo < L.n_ends ? e : L.n) - p;
+    return L.ends[lo < L.n_ends ? lo : L.n_ends - 1] - p;
 }
 
 // Comparison window of a suffix: its limit, clipped by max_context.

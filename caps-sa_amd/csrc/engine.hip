@@ -8,6 +8,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/caps_sa.h"
@@ -371,11 +372,30 @@ static RouterDev train_router(Workspace &W, uint32_t kind, uint32_t flags, const
 // idbits > 0 builds many texts at once: `lim` delimits the texts and the top
 // idbits bits of every packed key hold the text index, so suffixes sort
 // text-major and no comparison crosses a text end.
+// Launch style per refinement size class (pairs, 3-4, 5-8, 9-16, 17-32,
+// 33-64 members).  CAPS_FLAG_PERSISTENT_REFINE queues every class; the
+// default queues the classes where slots finish at very different rounds.
+struct RefinePlan {
+    bool queue[6];
+};
+static RefinePlan refine_plan(const caps_opts_t &o) {
+    RefinePlan p{};
+    const bool all = (o.flags & CAPS_FLAG_PERSISTENT_REFINE) != 0;
+    for (int c = 0; c < 6; ++c) p.queue[c] = all || c == 1 || c == 2;
+    return p;
+}
+
 template <typename S, typename I>
 void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, const caps_limits_t *lim,
                 const caps_opts_t &o, I *d_out, I *d_lcp_out, hipStream_t st, caps_stats_t &stats,
                 uint32_t idbits = 0) {
     Workspace &W = workspace();
+    if (!W.n_cu) {
+        int dev = 0, cus = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        W.n_cu = cus > 0 ? (uint32_t)cus : 256u;
+    }
     memset(&stats, 0, sizeof(stats));
     PhaseTimer tm(st);
     KernelTimer kt(st);
@@ -511,8 +531,9 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     uint64_t tcap = ceil_div(n_pos, 512) + p + 1;  // tiles per class (every tile but the last of a bucket is full)
     Tile *tiles = (Tile *)W.tiles.get(kTileClasses * tcap * sizeof(Tile));
     Range *big = (Range *)W.big.get(p * sizeof(Range));
-    unsigned int *counters = (unsigned int *)W.counts.get(64);  // [0] oversized, [1..4) tile classes, [8..16) groups
-    HIPCHK(hipMemsetAsync(counters, 0, 64, st));
+    // [0] oversized, [1..4) tile classes, [8..16) group counts, [16..24) refinement work queues
+    unsigned int *counters = (unsigned int *)W.counts.get(128);
+    HIPCHK(hipMemsetAsync(counters, 0, 128, st));
     GroupLists gl{};
     for (int c = 0; c < kGroupClasses; ++c)
         gl.list[c] = (Group *)W.glist[c].get((n_pos / group_class_min(c) + 1) *
@@ -583,30 +604,47 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     // ---- refinement ----
     I *lcp = want_lcp ? d_lcp_out : nullptr;
     const uint64_t depth0 = g.k;
+    // refinement of groups of 2..64, per size class: one slot per group (grid
+    // launch), or persistent slots fed from a device work queue
+    const RefinePlan plan = refine_plan(o);
+    unsigned int *queues = counters + 16;
+    const uint32_t waves = W.n_cu * 32;  // persistent grid: 8 waves per SIMD
+    auto pgrid = [&](uint64_t items, uint32_t per_block) {
+        return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(items, per_block), waves / 4));
+    };
+    auto chunk_for = [&](uint64_t items, uint32_t slots) {  // about four claims per wavefront
+        return (uint32_t)std::max<uint64_t>(slots, std::min<uint64_t>(256, items / ((uint64_t)waves * 4)));
+    };
     if (cnt[0]) {
         kt.begin(&stats.k_pairs);
-        k_pairs4<S, I><<<(unsigned)ceil_div(cnt[0], 64), 256, 0, st>>>(d_text, L, max_ctx,
-                                                                        (const PairRec<I> *)gl.list[0], cnt[0],
-                                                                        depth0, d_out, lcp);
+        const PairRec<I> *pl = (const PairRec<I> *)gl.list[0];
+        if (plan.queue[0])
+            k_pairs_wq<S, I><<<pgrid(cnt[0], 64), 256, 0, st>>>(d_text, L, max_ctx, pl, cnt[0], depth0, d_out, lcp,
+                                                                 queues + 0, chunk_for(cnt[0], 16));
+        else
+            k_pairs4<S, I><<<(unsigned)ceil_div(cnt[0], 64), 256, 0, st>>>(d_text, L, max_ctx, pl, cnt[0], depth0,
+                                                                            d_out, lcp);
         LAUNCHCHK();
         kt.end();
     }
     kt.begin(&stats.k_groups_small);
-    if (cnt[1])
-        k_wgroups<S, I, 4><<<(unsigned)ceil_div(cnt[1], 64), 256, 0, st>>>(d_text, cm, gref, L, max_ctx, gl.list[1],
-                                                                            cnt[1], depth0, d_out, lcp);
-    if (cnt[2])
-        k_wgroups<S, I, 8><<<(unsigned)ceil_div(cnt[2], 32), 256, 0, st>>>(d_text, cm, gref, L, max_ctx, gl.list[2],
-                                                                            cnt[2], depth0, d_out, lcp);
-    if (cnt[3])
-        k_wgroups<S, I, 16><<<(unsigned)ceil_div(cnt[3], 16), 256, 0, st>>>(d_text, cm, gref, L, max_ctx, gl.list[3],
-                                                                             cnt[3], depth0, d_out, lcp);
-    if (cnt[4])
-        k_wgroups<S, I, 32><<<(unsigned)ceil_div(cnt[4], 8), 256, 0, st>>>(d_text, cm, gref, L, max_ctx, gl.list[4],
-                                                                            cnt[4], depth0, d_out, lcp);
-    if (cnt[5])
-        k_wgroups<S, I, 64><<<(unsigned)ceil_div(cnt[5], 4), 256, 0, st>>>(d_text, cm, gref, L, max_ctx, gl.list[5],
-                                                                            cnt[5], depth0, d_out, lcp);
+    auto wgroups = [&](auto wtag, int c) {
+        constexpr int Wd = decltype(wtag)::value;
+        if (!cnt[c]) return;
+        constexpr uint32_t per_block = 256 / Wd;
+        if (plan.queue[c])
+            k_wgroups_wq<S, I, Wd><<<pgrid(cnt[c], per_block), 256, 0, st>>>(
+                d_text, cm, gref, L, max_ctx, gl.list[c], cnt[c], depth0, d_out, lcp, queues + c,
+                chunk_for(cnt[c], 64 / Wd));
+        else
+            k_wgroups_grid<S, I, Wd><<<(unsigned)ceil_div(cnt[c], per_block), 256, 0, st>>>(
+                d_text, cm, gref, L, max_ctx, gl.list[c], cnt[c], depth0, d_out, lcp);
+    };
+    wgroups(std::integral_constant<int, 4>{}, 1);
+    wgroups(std::integral_constant<int, 8>{}, 2);
+    wgroups(std::integral_constant<int, 16>{}, 3);
+    wgroups(std::integral_constant<int, 32>{}, 4);
+    wgroups(std::integral_constant<int, 64>{}, 5);
     LAUNCHCHK();
     kt.end();
     if (cnt[6]) {

@@ -26,6 +26,7 @@ struct Geometry {
 };
 
 struct Workspace {
+    uint32_t n_cu = 0;  // compute units of the current device (sizes persistent grids)
     // inputs staged from host pointers
     DevBuf text, hpos, out, outlcp, pos_copy;
     // limits

@@ -204,8 +204,13 @@ __global__ __launch_bounds__(256) void k_pairs4(const S *__restrict__ t, DevLimi
 // ---------------------------------------------------------------------------
 constexpr uint64_t kAuxCont = ~0ull;
 
+// ---------------------------------------------------------------------------
+// Groups of 3..64 members, one W-lane slot per group (grid launch).  The
+// round structure is the one described for k_wgroups_wq below; the wavefront
+// runs until all of its slots' groups are settled.
+// ---------------------------------------------------------------------------
 template <typename S, typename I, int W>
-__global__ __launch_bounds__(256) void k_wgroups(const S *__restrict__ t, CodeMap<S> cm, KeyGeom g, DevLimits L,
+__global__ __launch_bounds__(256) void k_wgroups_grid(const S *__restrict__ t, CodeMap<S> cm, KeyGeom g, DevLimits L,
                                                  uint64_t max_ctx, const Group *__restrict__ groups,
                                                  uint32_t n_groups, uint64_t depth0, I *__restrict__ idx,
                                                  I *__restrict__ lcp) {
@@ -294,6 +299,289 @@ __global__ __launch_bounds__(256) void k_wgroups(const S *__restrict__ t, CodeMa
             v = v < cap ? v : cap;
             v = v < q_cap ? v : q_cap;
             lcp[gr.start + r] = (I)v;
+        }
+    }
+}
+
+
+// Work queue shared by the persistent refinement kernels.  A wavefront
+// claims `claim` consecutive items with one atomic and hands them to its
+// slots of W lanes in order (ballot + popcount, no further atomics); `cb`/`ce`
+// hold the wavefront's unclaimed range and are uniform across it.  The host
+// sizes `claim` (at least the slots per wavefront) so that the queue holds a
+// few claims per wavefront: large enough to keep the counter cold, small
+// enough that no wavefront is left with a long tail.  Returns, in every lane
+// of a slot that asked, the item it got (>= n: queue empty).
+template <int W>
+__device__ __forceinline__ uint32_t wq_take(unsigned int *next, uint32_t claim, bool ask, uint32_t &cb,
+                                            uint32_t &ce) {
+    const uint32_t lane = threadIdx.x & 63;
+    const bool leader_asks = ask && (lane & (W - 1)) == 0;
+    const uint64_t m = __ballot(leader_asks);
+    uint32_t got = 0xFFFFFFFFu;
+    if (m) {
+        const uint32_t k = (uint32_t)__popcll(m);
+        const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1));
+        const uint32_t avail = ce - cb;
+        if (k > avail) {
+            const uint32_t first = (uint32_t)__ffsll((unsigned long long)m) - 1;
+            uint32_t nb = 0;
+            if (lane == first) nb = atomicAdd(next, claim);
+            nb = __shfl(nb, (int)first);
+            got = rank < avail ? cb + rank : nb + (rank - avail);
+            cb = nb + (k - avail);
+            ce = nb + claim;
+        } else {
+            got = cb + rank;
+            cb += k;
+        }
+    }
+    return __shfl(got, (int)(lane & ~(uint32_t)(W - 1)));
+}
+
+// Groups of 3..64 members: W lanes per group (W = 4..64), persistent.  Each
+// slot of W lanes refines one group at a time, round by round, and moves to
+// the next as soon as it is done, so a wavefront never waits for its slowest
+// group.  The queue is software-pipelined two groups deep: the group record
+// of the slot's next-but-one group and the positions of its next group are
+// loaded in the same iteration as the current round's text, and used one
+// iteration later, so taking a new group adds no load latency of its own.
+//
+// Each round every unresolved member packs the next k symbols at the slot's
+// depth into a chunk key plus a boundary value for members whose window ends
+// inside the chunk; the group is bitonic-sorted across its W lanes by
+// (subgroup, chunk, boundary value, position); subgroups split where chunk or
+// boundary value differ.  A subgroup is settled when it has one member or its
+// members all ended (comparator-Equal: position order).  The depth at which
+// sorted neighbours r-1, r part stays with position r (subgroup boundaries
+// never move), and the final LCP caps it by both members' windows.  Every
+// wavefront exits once the queue is empty and its slots are idle; each group
+// ends because the depth grows by k per round until every window has ended.
+template <typename S, typename I, int W>
+__global__ __launch_bounds__(256) void k_wgroups_wq(const S *__restrict__ t, CodeMap<S> cm, KeyGeom g, DevLimits L,
+                                                 uint64_t max_ctx, const Group *__restrict__ groups,
+                                                 uint32_t n_groups, uint64_t depth0, I *__restrict__ idx,
+                                                 I *__restrict__ lcp, unsigned int *next, uint32_t claim) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t r = lane & (W - 1);
+    const bool star = L.order == 1;
+    const uint64_t cont = star ? 0 : kAuxCont;
+    uint32_t cb = 0, ce = 0;  // this wavefront's claimed, not yet handed out queue range
+    bool drained = false;
+    // stage 1: group record loaded; stage 2: its member positions loaded
+    bool v1 = false, v2 = false;
+    uint64_t s1 = 0, s2 = 0, p2 = 0;
+    uint32_t m1 = 0, m2 = 0;
+    // current group
+    bool busy = false;
+    uint64_t start = 0, D = 0;
+    uint32_t m = 0;
+    uint64_t pos = 0, lim = 0, cap = 0, chunk = ~0ull, aux = ~0ull, h = 0;
+    uint32_t sub = 0xFFFFFFFFu;
+    bool settled = true;
+    for (;;) {
+        // advance the pipeline (all conditions are uniform within a slot)
+        if (!busy && v2) {
+            start = s2;
+            m = m2;
+            busy = true;
+            v2 = false;
+            D = depth0;
+            h = 0;
+            chunk = ~0ull;
+            aux = ~0ull;
+            settled = r >= m;
+            sub = settled ? 0xFFFFFFFFu : 0;
+            if (!settled) {
+                pos = p2;
+                lim = lim_at(L, pos);
+                cap = lim < max_ctx ? lim : max_ctx;
+            }
+        }
+        if (v1 && !v2) {
+            s2 = s1;
+            m2 = m1;
+            if (r < m1) p2 = idx[s1 + r];
+            v2 = true;
+            v1 = false;
+        }
+        const bool ask = !v1 && !drained;
+        const uint32_t gid = wq_take<W>(next, claim, ask, cb, ce);
+        if (ask) {
+            if (gid < n_groups) {
+                const Group gr = groups[gid];
+                s1 = gr.start;
+                m1 = (uint32_t)gr.size;
+                v1 = true;
+            } else {
+                drained = true;
+            }
+        }
+        if (!__any(busy || v1 || v2)) break;
+        const bool act = busy && r < m;
+        if (!settled) {
+            chunk = key_at(t, cm, g, pos, D, cap);
+            uint64_t rem = cap > D ? cap - D : 0;
+            if (rem >= g.k) aux = cont;
+            else aux = star ? (1 + (kAuxCont - 1 - lim)) : lim;
+        }
+        // bitonic sort of the W-lane segment
+#pragma unroll
+        for (int size = 2; size <= W; size <<= 1) {
+#pragma unroll
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                uint32_t p_sub = __shfl_xor(sub, stride);
+                uint64_t p_chunk = shfl_xor64(chunk, stride);
+                uint64_t p_aux = shfl_xor64(aux, stride);
+                uint64_t p_pos = shfl_xor64(pos, stride);
+                uint64_t p_lim = shfl_xor64(lim, stride);
+                uint64_t p_cap = shfl_xor64(cap, stride);
+                int p_settled = __shfl_xor((int)settled, stride);
+                bool less;  // me < partner
+                if (sub != p_sub) less = sub < p_sub;
+                else if (chunk != p_chunk) less = chunk < p_chunk;
+                else if (aux != p_aux) less = aux < p_aux;
+                else less = pos < p_pos;
+                bool up = (r & size) == 0;
+                bool lower = (r & stride) == 0;
+                bool keep = (lower == up) ? less : !less;
+                if (!keep) {
+                    sub = p_sub, chunk = p_chunk, aux = p_aux, pos = p_pos, lim = p_lim, cap = p_cap;
+                    settled = p_settled != 0;
+                }
+            }
+        }
+        // neighbour r-1 after the sort
+        const int prev = lane - 1 < 0 ? 0 : lane - 1;
+        uint32_t q_sub = __shfl(sub, prev);
+        uint64_t q_chunk = shfl64(chunk, prev);
+        uint64_t q_aux = shfl64(aux, prev);
+        uint64_t q_cap = shfl64(cap, prev);
+        bool same_old = r > 0 && r < m && sub == q_sub;
+        bool f = !(same_old && chunk == q_chunk && aux == q_aux);
+        if (act && same_old && !settled) h = D + shared_fields(g, q_chunk, chunk);
+        // new subgroup id: inclusive max-scan of (f ? r : 0) within the segment
+        uint32_t ns = f ? r : 0;
+#pragma unroll
+        for (int o = 1; o < W; o <<= 1) {
+            uint32_t v = __shfl_up(ns, o, W);
+            if ((int)r >= o) ns = v > ns ? v : ns;
+        }
+        uint32_t ns_prev = __shfl(ns, prev);
+        uint32_t ns_next = __shfl(ns, lane + 1 < 64 ? lane + 1 : lane);
+        bool nb = (r > 0 && r < m && ns_prev == ns) || (r + 1 < m && ns_next == ns);
+        bool ended = aux != cont;
+        if (act) {
+            settled = settled || !nb || ended;
+            sub = ns;
+        }
+        // a slot is done when none of its lanes is unsettled
+        const uint64_t open = __ballot(!settled);
+        const uint64_t smask = (W == 64) ? ~0ull : (((1ull << W) - 1) << (lane & ~(W - 1)));
+        if (busy && (open & smask) == 0) {
+            if (act) {
+                idx[start + r] = (I)pos;
+                if (lcp && r > 0) {
+                    uint64_t v = h;
+                    v = v < cap ? v : cap;
+                    v = v < q_cap ? v : q_cap;
+                    lcp[start + r] = (I)v;
+                }
+            }
+            busy = false;
+            m = 0;
+            sub = 0xFFFFFFFFu;
+            settled = true;
+        } else {
+            D += g.k;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Two-member groups, persistent: the four-lane compare above, with slots that
+// move to the next pair as soon as theirs is resolved.  The next pair's
+// record is loaded one iteration ahead, beside the current compare's text.
+// ---------------------------------------------------------------------------
+template <typename S, typename I>
+__global__ __launch_bounds__(256) void k_pairs_wq(const S *__restrict__ t, DevLimits L, uint64_t max_ctx,
+                                                  const PairRec<I> *__restrict__ pairs, uint32_t n_pairs,
+                                                  uint64_t known, I *__restrict__ idx, I *__restrict__ lcp,
+                                                  unsigned int *next, uint32_t claim) {
+    constexpr uint64_t SZ = sizeof(S);
+    const uint32_t lane = threadIdx.x & 63, grp = lane >> 2, sl = lane & 3;
+    const uint64_t tb = (uint64_t)(uintptr_t)t;
+    bool busy = false, drained = false, v1 = false;
+    uint32_t cb = 0, ce = 0;
+    PairRec<I> nx{0, 0, 0};
+    uint64_t s = 0, a = 0, b = 0, la = 0, lb = 0, cap = 0, capb = 0, enda = 0, endb = 0, ob = 0, resb = 0;
+    for (;;) {
+        if (!busy && v1) {
+            s = nx.start;
+            a = nx.a;
+            b = nx.b;
+            la = lim_at(L, a);
+            lb = lim_at(L, b);
+            uint64_t ca = la < max_ctx ? la : max_ctx, cb2 = lb < max_ctx ? lb : max_ctx;
+            cap = ca < cb2 ? ca : cb2;
+            capb = cap * SZ;
+            enda = tb + (a + cap) * SZ;
+            endb = tb + (b + cap) * SZ;
+            ob = (known < cap ? known : cap) * SZ;
+            resb = capb;
+            busy = true;
+            v1 = false;
+        }
+        const bool ask = !v1 && !drained;
+        const uint32_t pid = wq_take<4>(next, claim, ask, cb, ce);
+        if (ask) {
+            if (pid < n_pairs) {
+                nx = pairs[pid];
+                v1 = true;
+            } else {
+                drained = true;
+            }
+        }
+        if (!__any(busy || v1)) break;
+        uint32_t fb = 16;
+        const bool work = busy && ob < capb;
+        if (work) {
+            uint64_t rel = ob + 16 * sl;
+            uint64_t alo, ahi, blo, bhi;
+            load16(tb + a * SZ + rel, enda, alo, ahi);
+            load16(tb + b * SZ + rel, endb, blo, bhi);
+            uint64_t d0 = alo ^ blo, d1 = ahi ^ bhi;
+            fb = d0 ? (uint32_t)__builtin_ctzll(d0) >> 3 : d1 ? 8 + ((uint32_t)__builtin_ctzll(d1) >> 3) : 16u;
+            uint64_t in_win = capb > rel ? capb - rel : 0;
+            if (in_win < fb) fb = (uint32_t)in_win;
+        }
+        uint64_t mask = __ballot(work && fb < 16);
+        uint32_t gm = (uint32_t)(mask >> (grp * 4)) & 0xFu;
+        int fl = gm ? __ffs(gm) - 1 : 0;
+        uint32_t fq = __shfl(fb, (int)(grp * 4 + fl));
+        bool finish = busy && !work;  // window already exhausted by the known prefix
+        if (work) {
+            if (gm) {
+                uint64_t rr = ob + 16 * (uint64_t)fl + fq;
+                resb = rr < capb ? rr : capb;
+                finish = true;
+            } else {
+                ob += 64;
+            }
+        }
+        if (finish) {
+            if (sl == 0) {
+                uint64_t res = resb / SZ;
+                int c;
+                if (res < cap) c = t[a + res] < t[b + res] ? -1 : 1;
+                else c = boundary_then_pos(L, a, la, b, lb);
+                if (c > 0) {
+                    idx[s] = (I)b;
+                    idx[s + 1] = (I)a;
+                }
+                if (lcp) lcp[s + 1] = (I)res;
+            }
+            busy = false;
         }
     }
 }

@@ -79,6 +79,10 @@ typedef struct caps_opts {
 /* Skip router training (the learned router runs with its initial parameters;
  * used to show what training gains).  The output never changes. */
 #define CAPS_FLAG_ROUTER_UNTRAINED 1u
+/* Refine groups of 2..64 members with persistent kernels fed from device work
+ * queues (slots take the next group when theirs is done) instead of one grid
+ * slot per group.  The output never changes. */
+#define CAPS_FLAG_PERSISTENT_REFINE 2u
 
 /* Fills opts with the reference defaults (max_context unbounded, p auto). */
 void caps_opts_default(caps_opts_t *opts);

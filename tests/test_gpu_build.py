@@ -184,3 +184,22 @@ def test_unaligned_device_text(offset):
         want, want_lcp = ctx.build_in_memory(threads=4, want_lcp=True)
         assert np.array_equal(got, want)
         assert np.array_equal(lcp.cpu().numpy().view(np.uint32).astype(np.uint64), want_lcp)
+
+
+def test_persistent_refinement_matches_grid():
+    """Work-queue refinement (persistent slots) gives the same SA and LCP as
+    the grid launch, on texts with every small-group class."""
+    import torch
+    from caps_sa import device as dev
+
+    for text in (gen.markov_repeats(300000, seed=31, mean_len=700.0), gen.zipf_u16(60000, 40, 1.1, seed=3),
+                 np.frombuffer(b"ab" * 3000 + b"abc" * 2000, dtype=np.uint8).copy()):
+        w = text.dtype.itemsize
+        d_text = dev.text_to_device(text)
+        a, la = dev.build_device(d_text, len(text), w, want_lcp=True)
+        b, lb = dev.build_device(d_text, len(text), w, want_lcp=True, flags=caps_sa.FLAG_PERSISTENT_REFINE)
+        torch.cuda.synchronize()
+        assert torch.equal(a, b) and torch.equal(la, lb)
+        want, want_lcp = O.Ctx(text).build_in_memory(threads=4, want_lcp=True)
+        assert np.array_equal(b.cpu().numpy().view(np.uint32).astype(np.uint64), want)
+        assert np.array_equal(lb.cpu().numpy().view(np.uint32).astype(np.uint64), want_lcp)
