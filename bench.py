@@ -97,8 +97,10 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=0, help="override the text length")
     ap.add_argument("--seed", type=int, default=0xC1)
     ap.add_argument("--router", default="bsearch", choices=["bsearch", "pwl", "mlp"])
-    ap.add_argument("--refine", default="grid", choices=["grid", "queue"],
-                    help="groups of 2..64: one slot per group, or persistent slots fed from work queues")
+    ap.add_argument("--refine", default="plan", choices=["plan", "queue"],
+                    help="groups of 2..64: the default per-class plan, or persistent work-queue slots for all")
+    ap.add_argument("--tile-sort", default="merge", choices=["merge", "radix"])
+    ap.add_argument("--bucket-keys", type=int, default=0, help="mean records per bucket (0: library default)")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -137,8 +139,9 @@ def main() -> int:
     router = {"bsearch": 0, "pwl": 1, "mlp": 2}[args.router]
 
     def step():
-        dev.build_device(d_text, n, 1, idx_bytes=4, router=router, d_out=d_out,
-                         flags=caps_sa.FLAG_PERSISTENT_REFINE if args.refine == "queue" else 0)
+        dev.build_device(d_text, n, 1, idx_bytes=4, router=router, d_out=d_out, tile_keys=args.bucket_keys,
+                         flags=(caps_sa.FLAG_PERSISTENT_REFINE if args.refine == "queue" else 0)
+                         | (caps_sa.FLAG_TILE_RADIX if args.tile_sort == "radix" else 0))
         return caps_sa.last_stats()
 
     for i in range(args.warmup):
@@ -198,7 +201,7 @@ def main() -> int:
         "dtype": "u8",
         "data": "synthetic",
         "config": {"workload": args.workload, "n": n, "text": desc, "index": "u32", "router": args.router,
-                   "refine": args.refine, "seed": args.seed, "buckets": last["buckets"]},
+                   "refine": args.refine, "tile_sort": args.tile_sort, "seed": args.seed, "buckets": last["buckets"]},
         "verified_bit_exact": verified,
         "phases_ms": {k: round(sum(s[k] for s in stats) / len(stats), 3)
                       for k in ("ms_alphabet", "ms_sample", "ms_classify", "ms_scatter", "ms_tile_sort", "ms_merge",

@@ -314,6 +314,7 @@ class ExtMemOpts:
 
 FLAG_ROUTER_UNTRAINED = 1
 FLAG_PERSISTENT_REFINE = 2
+FLAG_TILE_RADIX = 4
 
 
 def _c_opts(max_context=MAX_CONTEXT, subproblem_count=0, router=0, key_geometry=0, gpu_buckets=0, tile_keys=0,

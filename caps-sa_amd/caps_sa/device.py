@@ -31,7 +31,7 @@ def text_to_device(text: np.ndarray, device="cuda:0") -> torch.Tensor:
 
 def build_device(d_text: torch.Tensor, n: int, sym_bytes: int, d_positions=None, n_pos=None, idx_bytes=4,
                  lp: LimitProvider = None, max_context=MAX_CONTEXT, router=0, gpu_buckets=0, key_geometry=0,
-                 d_out=None, d_lcp=None, want_lcp=False, flags=0):
+                 d_out=None, d_lcp=None, want_lcp=False, flags=0, tile_keys=0):
     """caps_build_device: all buffers already in HBM.  Returns (sa, lcp)."""
     n_pos = n if d_positions is None else (n_pos if n_pos is not None else d_positions.numel())
     if d_out is None:
@@ -40,7 +40,7 @@ def build_device(d_text: torch.Tensor, n: int, sym_bytes: int, d_positions=None,
         d_lcp = torch.empty(n_pos, dtype=_IDX[idx_bytes], device=d_text.device)
     lim = (lp or PlainText(n))._c()
     o = _c_opts(max_context, router=router, gpu_buckets=gpu_buckets, key_geometry=key_geometry,
-                want_lcp=int(d_lcp is not None), flags=flags)
+                want_lcp=int(d_lcp is not None), flags=flags, tile_keys=tile_keys)
     rc = lib().caps_build_device(d_text.data_ptr(), n, sym_bytes,
                                  d_positions.data_ptr() if d_positions is not None else None, n_pos, idx_bytes,
                                  ctypes.byref(lim), ctypes.byref(o), d_out.data_ptr(),

@@ -260,7 +260,7 @@ static void sort_keys_single_range(Workspace &W, uint64_t *keys, uint32_t *ids, 
     Tile *dt = (Tile *)W.stiles.get(nt * sizeof(Tile));
     HIPCHK(hipMemcpyAsync(dt, W.h_tiles.data(), nt * sizeof(Tile), hipMemcpyHostToDevice, st));
     GroupLists none{};
-    k_tile_radix<uint32_t, T><<<(unsigned)nt, 256, 0, st>>>(keys, ids, dt, none, 0);
+    k_tile_merge<uint32_t, T><<<(unsigned)nt, 256, 0, st>>>(keys, ids, dt, none, 0);
     LAUNCHCHK();
     Range r{0, count};
     W.h_range = r;
@@ -372,6 +372,8 @@ static RouterDev train_router(Workspace &W, uint32_t kind, uint32_t flags, const
 // idbits > 0 builds many texts at once: `lim` delimits the texts and the top
 // idbits bits of every packed key hold the text index, so suffixes sort
 // text-major and no comparison crosses a text end.
+constexpr uint32_t kBucketTarget = 1280;
+
 // Launch style per refinement size class (pairs, 3-4, 5-8, 9-16, 17-32,
 // 33-64 members).  CAPS_FLAG_PERSISTENT_REFINE queues every class; the
 // default queues the classes where slots finish at very different rounds.
@@ -440,7 +442,10 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
 
     // ---- buckets and splitters ----
     constexpr uint32_t T = 4096;
-    uint64_t target = T / 2;
+    // mean bucket size the splitters aim at: small enough that nearly every
+    // bucket fits the 2048-record tile class (sample-based splitters spread
+    // bucket sizes by about a quarter around the mean)
+    const uint64_t target = o.tile_keys ? std::max<uint32_t>(64, std::min<uint32_t>(o.tile_keys, T)) : kBucketTarget;
     uint64_t p = o.gpu_buckets ? o.gpu_buckets : ceil_div(n_pos, target);
     p = std::max<uint64_t>(1, std::min<uint64_t>(p, std::min<uint64_t>(n_pos, 1u << 22)));
     const uint64_t oversample = 16;
@@ -549,9 +554,15 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     stats.tiles = (uint64_t)hc[1] + hc[2] + hc[3];
     stats.oversized = n_big;
     kt.begin(&stats.k_tile_sort);
-    if (hc[1]) k_tile_radix<I, 1024><<<hc[1], 256, 0, st>>>(keys, d_out, tiles, gl, 1);
-    if (hc[2]) k_tile_radix<I, 2048><<<hc[2], 256, 0, st>>>(keys, d_out, tiles + tcap, gl, 1);
-    if (hc[3]) k_tile_radix<I, 4096><<<hc[3], 256, 0, st>>>(keys, d_out, tiles + 2 * tcap, gl, 1);
+    if (o.flags & CAPS_FLAG_TILE_RADIX) {
+        if (hc[1]) k_tile_radix<I, 1024><<<hc[1], 256, 0, st>>>(keys, d_out, tiles, gl, 1);
+        if (hc[2]) k_tile_radix<I, 2048><<<hc[2], 256, 0, st>>>(keys, d_out, tiles + tcap, gl, 1);
+        if (hc[3]) k_tile_radix<I, 4096><<<hc[3], 256, 0, st>>>(keys, d_out, tiles + 2 * tcap, gl, 1);
+    } else {
+        if (hc[1]) k_tile_merge<I, 1024><<<hc[1], 256, 0, st>>>(keys, d_out, tiles, gl, 1);
+        if (hc[2]) k_tile_merge<I, 2048><<<hc[2], 256, 0, st>>>(keys, d_out, tiles + tcap, gl, 1);
+        if (hc[3]) k_tile_merge<I, 4096><<<hc[3], 256, 0, st>>>(keys, d_out, tiles + 2 * tcap, gl, 1);
+    }
     LAUNCHCHK();
     kt.end();
     tm.mark(PH_TILE);

@@ -176,3 +176,154 @@ __global__ __launch_bounds__(256) void k_tile_radix(uint64_t *__restrict__ keys,
 }
 
 }  // namespace caps
+
+namespace caps {
+
+// ---------------------------------------------------------------------------
+// Merge-path block merge sort of one tile (alternative to the radix sort):
+// every thread sorts ITEMS = T/256 records in registers (odd-even
+// transposition network), then eight merge levels double the sorted runs
+// (8 -> 2048 for T = 2048): each thread finds where its ITEMS outputs start on
+// the merge path of its two input runs (binary search in LDS) and merges them
+// serially.  Records compare by (key, index in the tile), a strict total
+// order, so the sort is stable, pads (key ~0, index >= len) stay behind every
+// real record, and no two records tie.  LDS holds the keys and tile indices
+// (blocked, one pad slot per thread run) and the positions.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool rec_less(uint64_t ka, uint32_t la, uint64_t kb, uint32_t lb) {
+    return ka < kb || (ka == kb && la < lb);
+}
+
+template <typename I, int T>
+__global__ __launch_bounds__(256) void k_tile_merge(uint64_t *__restrict__ keys, I *__restrict__ idx,
+                                                    const Tile *__restrict__ tiles, GroupLists gl, int emit) {
+    constexpr int B = 256;
+    constexpr int ITEMS = T / B;
+    constexpr int PADT = T + B;
+    __shared__ uint64_t sk[PADT];
+    __shared__ uint16_t sl[PADT];
+    __shared__ I si[T];
+    __shared__ uint32_t gcnt[kGroupClasses], gbase[kGroupClasses];
+    auto px = [](uint32_t i) { return i + i / ITEMS; };
+
+    const Tile tl = tiles[blockIdx.x];
+    const uint32_t len = tl.len;
+    const int tid = threadIdx.x;
+    if (len == 0) return;
+    for (uint32_t i = tid; i < (uint32_t)T; i += B) {
+        if (i < len) {
+            sk[px(i)] = keys[tl.start + i];
+            si[i] = idx[tl.start + i];
+        } else {
+            sk[px(i)] = ~0ull;
+        }
+        sl[px(i)] = (uint16_t)i;
+    }
+    __syncthreads();
+    uint64_t k[ITEMS];
+    uint32_t li[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        k[j] = sk[px(tid * ITEMS + j)];
+        li[j] = sl[px(tid * ITEMS + j)];
+    }
+    // register sort: odd-even transposition network over ITEMS records
+#pragma unroll
+    for (int round = 0; round < ITEMS; ++round) {
+#pragma unroll
+        for (int j = round & 1; j + 1 < ITEMS; j += 2) {
+            if (rec_less(k[j + 1], li[j + 1], k[j], li[j])) {
+                uint64_t tk = k[j];
+                k[j] = k[j + 1];
+                k[j + 1] = tk;
+                uint32_t tli = li[j];
+                li[j] = li[j + 1];
+                li[j + 1] = tli;
+            }
+        }
+    }
+    // merge levels: runs of w records -> runs of 2w
+#pragma unroll 1
+    for (uint32_t w = ITEMS; w < (uint32_t)T; w *= 2) {
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            sk[px(tid * ITEMS + j)] = k[j];
+            sl[px(tid * ITEMS + j)] = (uint16_t)li[j];
+        }
+        __syncthreads();
+        const uint32_t per = (2 * w) / ITEMS;  // threads per merged run
+        const uint32_t a0 = (tid / per) * 2 * w, b0 = a0 + w;
+        const uint32_t diag = (tid % per) * ITEMS;
+        uint32_t lo = diag > w ? diag - w : 0, hi = diag < w ? diag : w;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi) >> 1;
+            uint32_t bi = b0 + diag - 1 - mid;
+            if (!rec_less(sk[px(bi)], sl[px(bi)], sk[px(a0 + mid)], sl[px(a0 + mid)])) lo = mid + 1;
+            else hi = mid;
+        }
+        uint32_t a = a0 + lo, b = b0 + (diag - lo);
+        const uint32_t ae = a0 + w, be = b0 + w;
+        uint64_t ka = a < ae ? sk[px(a)] : ~0ull, kb = b < be ? sk[px(b)] : ~0ull;
+        uint32_t la = a < ae ? sl[px(a)] : 0xFFFFFFFFu, lb = b < be ? sl[px(b)] : 0xFFFFFFFFu;
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const bool take_a = b >= be || (a < ae && !rec_less(kb, lb, ka, la));
+            if (take_a) {
+                k[j] = ka;
+                li[j] = la;
+                ++a;
+                ka = a < ae ? sk[px(a)] : ~0ull;
+                la = a < ae ? sl[px(a)] : 0xFFFFFFFFu;
+            } else {
+                k[j] = kb;
+                li[j] = lb;
+                ++b;
+                kb = b < be ? sk[px(b)] : ~0ull;
+                lb = b < be ? sl[px(b)] : 0xFFFFFFFFu;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        sk[px(tid * ITEMS + j)] = k[j];
+        sl[px(tid * ITEMS + j)] = (uint16_t)li[j];
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < len; i += B) {
+        keys[tl.start + i] = sk[px(i)];
+        idx[tl.start + i] = si[sl[px(i)]];
+    }
+    if (!emit || !tl.whole || len < 2) return;
+    // tie groups of the bucket: two passes (count, then place)
+    if (tid < kGroupClasses) gcnt[tid] = 0;
+    __syncthreads();
+    for (uint32_t r = tid; r < len; r += B) {
+        uint64_t kr = sk[px(r)];
+        if ((r == 0 || sk[px(r - 1)] != kr) && r + 1 < len && sk[px(r + 1)] == kr) {
+            uint32_t e = r + 2;
+            while (e < len && sk[px(e)] == kr) ++e;
+            atomicAdd(&gcnt[group_class(e - r)], 1u);
+        }
+    }
+    __syncthreads();
+    if (tid < kGroupClasses) {
+        gbase[tid] = gcnt[tid] ? atomicAdd(&gl.counts[tid], gcnt[tid]) : 0u;
+        gcnt[tid] = 0;
+    }
+    __syncthreads();
+    for (uint32_t r = tid; r < len; r += B) {
+        uint64_t kr = sk[px(r)];
+        if ((r == 0 || sk[px(r - 1)] != kr) && r + 1 < len && sk[px(r + 1)] == kr) {
+            uint32_t e = r + 2;
+            while (e < len && sk[px(e)] == kr) ++e;
+            int c = group_class(e - r);
+            uint32_t slot = gbase[c] + atomicAdd(&gcnt[c], 1u);
+            if (c == 0) ((PairRec<I> *)gl.list[0])[slot] = PairRec<I>{tl.start + r, si[sl[px(r)]], si[sl[px(r + 1)]]};
+            else gl.list[c][slot] = Group{tl.start + r, e - r};
+        }
+    }
+}
+
+}  // namespace caps

@@ -72,7 +72,8 @@ typedef struct caps_opts {
     uint32_t router;           /* CAPS_ROUTER_*                                         */
     uint32_t key_geometry;     /* CAPS_KEY_*                                            */
     uint32_t gpu_buckets;      /* 0 = auto (one bucket per tile_keys/2 suffixes)        */
-    uint32_t tile_keys;        /* 0 = autotuned; else 1024 | 2048 | 4096                */
+    uint32_t tile_keys;        /* mean records per bucket the splitters aim at
+                                  (0 = 1280; 64..4096); ignored with gpu_buckets       */
     uint32_t want_lcp;         /* fill the LCP array where an entry point takes one     */
     uint32_t flags;            /* CAPS_FLAG_* bits, 0 by default                        */
 } caps_opts_t;
@@ -83,6 +84,9 @@ typedef struct caps_opts {
  * queues (slots take the next group when theirs is done) instead of one grid
  * slot per group.  The output never changes. */
 #define CAPS_FLAG_PERSISTENT_REFINE 2u
+/* Sort tiles with the 4-bit LDS radix sort instead of the merge-path block
+ * merge sort.  The output never changes. */
+#define CAPS_FLAG_TILE_RADIX 4u
 
 /* Fills opts with the reference defaults (max_context unbounded, p auto). */
 void caps_opts_default(caps_opts_t *opts);

@@ -203,3 +203,21 @@ def test_persistent_refinement_matches_grid():
         want, want_lcp = O.Ctx(text).build_in_memory(threads=4, want_lcp=True)
         assert np.array_equal(b.cpu().numpy().view(np.uint32).astype(np.uint64), want)
         assert np.array_equal(lb.cpu().numpy().view(np.uint32).astype(np.uint64), want_lcp)
+
+
+def test_radix_tile_sort_matches_merge():
+    """Both tile sorts (merge path, 4-bit radix) give the oracle's SA and LCP,
+    including tiles with all-ones keys next to the pads."""
+    import torch
+    from caps_sa import device as dev
+
+    star_text = np.full(9000, 255, dtype=np.uint8)
+    star_text[::7] = 3
+    for text in (gen.markov_repeats(200000, seed=41, mean_len=500.0), star_text,
+                 gen.uniform_u8(70000, 256, seed=4)):
+        d_text = dev.text_to_device(text)
+        want, want_lcp = O.Ctx(text).build_in_memory(threads=4, want_lcp=True)
+        for flags in (0, caps_sa.FLAG_TILE_RADIX):
+            sa, lcp = dev.build_device(d_text, len(text), 1, want_lcp=True, flags=flags)
+            assert np.array_equal(sa.cpu().numpy().view(np.uint32).astype(np.uint64), want), flags
+            assert np.array_equal(lcp.cpu().numpy().view(np.uint32).astype(np.uint64), want_lcp), flags
