@@ -208,6 +208,14 @@ def main() -> int:
                                 "ms_detect", "ms_refine", "ms_giant")},
         "kernels_ms": {k: round(v, 3) for k, v in avg.items()},
         "groups": {k: last[k] for k in ("pairs", "small_groups", "mid_groups", "giant_groups", "oversized")},
+        "buckets": {"count": last["buckets"], "max": last["bucket_max"],
+                    "imbalance": round(last["bucket_max"] * last["buckets"] / n, 3)},
+        "router": None if args.router == "bsearch" else {
+            "kind": args.router,
+            "mean_mispredict": round(last["router_err_sum"] / max(last["router_count"], 1), 4),
+            "max_mispredict": last["router_err_max"],
+            "fallback_frac": round(last["router_fallbacks"] / max(last["router_count"], 1), 6),
+            "train_and_sample_ms": round(sum(s["ms_sample"] for s in stats) / len(stats), 3)},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

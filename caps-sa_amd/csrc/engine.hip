@@ -60,6 +60,16 @@ Workspace &workspace() {
 
 static inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
+// Blocks of `kernel` that fit on the device at once (one resident wave of
+// blocks), for grid-stride kernels that stage tables once per block.
+template <typename K>
+static unsigned resident_blocks(K kernel, unsigned block) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)block, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    return (unsigned)per_cu * std::max<uint32_t>(workspace().n_cu, 1);
+}
+
 static unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 65536u * 4) {
     uint64_t g = ceil_div(n ? n : 1, block);
     return (unsigned)std::min<uint64_t>(g, cap);
@@ -301,8 +311,15 @@ static std::vector<float> mlp_init_params(uint64_t seed) {
 
 template <typename S, typename I>
 static RouterDev train_router(Workspace &W, uint32_t kind, uint32_t flags, const uint64_t *sorted, uint64_t s,
-                              const uint64_t *spl, uint64_t p, const S *d_text, CodeMap<S> cm, KeyGeom g,
-                              DevLimits L, uint64_t max_ctx, const I *d_pos, uint64_t n_pos, hipStream_t st) {
+                              const Splitters &SP, const S *d_text, CodeMap<S> cm, KeyGeom g, DevLimits L,
+                              uint64_t max_ctx, const I *d_pos, uint64_t n_pos, hipStream_t st) {
+    // the router predicts the position among the coarse splitters (0..n_coarse)
+    const uint32_t nc = SP.n_coarse;
+    uint64_t *ctab = (uint64_t *)W.r_tmp.get((uint64_t)std::max<uint32_t>(nc, 1) * 8);
+    k_coarse_table<<<(unsigned)ceil_div(std::max<uint32_t>(nc, 1), 256), 256, 0, st>>>(SP.spl, SP.stride, nc, ctab);
+    LAUNCHCHK();
+    const uint64_t *spl = ctab;
+    const uint64_t p = (uint64_t)nc + 1;
     RouterDev R{};
     R.kind = kind;
     R.window = 8;
@@ -468,6 +485,8 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     SP.n_spl = (uint32_t)(p - 1);
     SP.stride = (uint32_t)std::max<uint64_t>(1, ceil_div(p - 1, kCoarseMax));
     SP.n_coarse = (uint32_t)ceil_div(p - 1, SP.stride);
+    while ((1ull << SP.coarse_steps) <= SP.n_coarse) ++SP.coarse_steps;
+    while ((1ull << SP.fine_steps) < SP.stride) ++SP.fine_steps;
     stats.buckets = p;
 
     // ---- classify + coarse histogram ----
@@ -485,20 +504,25 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     uint32_t rkind = p > 1 ? o.router : CAPS_ROUTER_BSEARCH;
     if (rkind > CAPS_ROUTER_MLP) throw CapsError(CAPS_E_INVALID, "unknown router");
     if (rkind != CAPS_ROUTER_BSEARCH)
-        R = train_router<S, I>(W, rkind, o.flags, sorted_sample, s, spl, p, d_text, cm, g, L, max_ctx, d_pos, n_pos,
-                               st);
+        R = train_router<S, I>(W, rkind, o.flags, sorted_sample, s, SP, d_text, cm, g, L, max_ctx, d_pos, n_pos, st);
     tm.mark(PH_SAMPLE);
     unsigned grid = grid_for(n_pos, 256, 8192);
     kt.begin(&stats.k_classify);
     const bool seq = d_pos == nullptr && n_pos == n;
-    const unsigned gseq = (unsigned)std::min<uint64_t>(ceil_div(n_pos, kSeqChunk), 65536);
+    // one resident wave of blocks (two per CU, bounded by LDS) striding over the
+    // chunks: the coarse splitter table is staged once per block, not per chunk
+    const uint64_t nchunks = ceil_div(n_pos, kSeqChunk);
+    auto gseq = [&](auto kern) { return (unsigned)std::min<uint64_t>(nchunks, resident_blocks(kern, 256)); };
     if (seq) {
         if (rkind == CAPS_ROUTER_PWL)
-            k_classify_seq<S, I, 1><<<gseq, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, bid, hist, fshift, pc);
+            k_classify_seq<S, I, 1><<<gseq(k_classify_seq<S, I, 1>), 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R,
+                                                                                 bid, hist, fshift, pc);
         else if (rkind == CAPS_ROUTER_MLP)
-            k_classify_seq<S, I, 2><<<gseq, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, bid, hist, fshift, pc);
+            k_classify_seq<S, I, 2><<<gseq(k_classify_seq<S, I, 2>), 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R,
+                                                                                 bid, hist, fshift, pc);
         else
-            k_classify_seq<S, I, 0><<<gseq, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, bid, hist, fshift, pc);
+            k_classify_seq<S, I, 0><<<gseq(k_classify_seq<S, I, 0>), 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R,
+                                                                                 bid, hist, fshift, pc);
     } else if (rkind == CAPS_ROUTER_PWL)
         k_classify<S, I, 1><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, bid, hist, fshift,
                                                    pc);
@@ -519,13 +543,15 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     uint64_t *keys = (uint64_t *)W.keys.get(n_pos * 8);
     Rec<I> *tmp = (Rec<I> *)W.recs.get(n_pos * sizeof(Rec<I>));
     kt.begin(&stats.k_scatter);
-    const unsigned g1 = (unsigned)ceil_div(n_pos, kScatterChunk);
+    // chunk per scatter block: a multiple of the key window, about one resident wave of blocks
+    const uint64_t schunk = std::max<uint64_t>(kSeqChunk, ceil_div(ceil_div(n_pos, (uint64_t)W.n_cu * 2), kSeqChunk) * kSeqChunk);
+    const unsigned g1 = (unsigned)ceil_div(n_pos, schunk);
     if (seq)
         k_scatter1<S, I, true><<<g1, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, bid, fshift, pc, cursor,
-                                                   tmp);
+                                                   tmp, schunk);
     else
         k_scatter1<S, I, false><<<g1, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, bid, fshift, pc,
-                                                    cursor, tmp);
+                                                    cursor, tmp, schunk);
     LAUNCHCHK();
     k_scatter2<I><<<pc, 256, 0, st>>>(tmp, coff, fshift, (uint32_t)p, off, keys, d_out);
     LAUNCHCHK();
@@ -536,7 +562,8 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     uint64_t tcap = ceil_div(n_pos, 512) + p + 1;  // tiles per class (every tile but the last of a bucket is full)
     Tile *tiles = (Tile *)W.tiles.get(kTileClasses * tcap * sizeof(Tile));
     Range *big = (Range *)W.big.get(p * sizeof(Range));
-    // [0] oversized, [1..4) tile classes, [8..16) group counts, [16..24) refinement work queues
+    // [0] oversized, [1..4) tile classes, [4] largest bucket, [8..16) group counts,
+    // [16..24) refinement work queues
     unsigned int *counters = (unsigned int *)W.counts.get(128);
     HIPCHK(hipMemsetAsync(counters, 0, 128, st));
     GroupLists gl{};
@@ -545,11 +572,12 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
                                              (c == 0 ? sizeof(PairRec<I>) : sizeof(Group)));
     gl.counts = counters + 8;
     k_tile_emit<<<(unsigned)ceil_div(p, 256), 256, 0, st>>>(off, (uint32_t)p, T, tiles, tcap, counters + 1, big,
-                                                             counters);
+                                                             counters, counters + 4);
     LAUNCHCHK();
-    unsigned int hc[4];
-    HIPCHK(hipMemcpyAsync(hc, counters, 16, hipMemcpyDeviceToHost, st));
+    unsigned int hc[5];
+    HIPCHK(hipMemcpyAsync(hc, counters, 20, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    stats.bucket_max = hc[4];
     unsigned int n_big = hc[0];
     stats.tiles = (uint64_t)hc[1] + hc[2] + hc[3];
     stats.oversized = n_big;

@@ -88,10 +88,12 @@ constexpr uint32_t kCoarseBucketsMax = 2048;  // coarse buckets of the two-level
 constexpr uint32_t kFineMax = 4096;           // fine buckets per coarse bucket
 
 struct Splitters {
-    const uint64_t *spl;  // p-1 sorted keys
-    uint32_t n_spl;       // p-1
-    uint32_t stride;      // coarse stride
-    uint32_t n_coarse;    // ceil(n_spl / stride)
+    const uint64_t *spl;    // p-1 sorted keys
+    uint32_t n_spl;         // p-1
+    uint32_t stride;        // coarse stride
+    uint32_t n_coarse;      // ceil(n_spl / stride)
+    uint32_t coarse_steps;  // ceil(log2(n_coarse + 1)): binary-lifting steps over the coarse table
+    uint32_t fine_steps;    // ceil(log2(stride)): steps inside one coarse interval
 };
 
 // Number of splitters <= key.
@@ -116,26 +118,33 @@ __device__ __forceinline__ uint32_t upper_bound_coarse(const Splitters &S, const
     return b;
 }
 
-// Exact bucket given a predicted one: verify spl[b-1] <= key < spl[b],
-// otherwise gallop up to `window` steps, otherwise binary search.
-__device__ __forceinline__ uint32_t router_correct(uint32_t window, const Splitters &S, uint64_t key, uint32_t b,
-                                                   uint32_t *fallback) {
-    const uint64_t *spl = S.spl;
-    uint32_t np = S.n_spl;
-    if (b > np) b = np;
-    bool lo_ok = (b == 0) || spl[b - 1] <= key;
-    bool hi_ok = (b == np) || key < spl[b];
+// The coarse splitter table: every stride-th splitter (what the learned router
+// is trained to index).
+__global__ void k_coarse_table(const uint64_t *__restrict__ spl, uint32_t stride, uint32_t n_coarse,
+                               uint64_t *__restrict__ ctab) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n_coarse) ctab[c] = spl[(uint64_t)c * stride];
+}
+
+// Exact position given a predicted one in a sorted table `tab` of n keys (the
+// number of entries <= key): verify tab[b-1] <= key < tab[b], otherwise
+// gallop up to `window` steps, otherwise binary search (the fallback).
+__device__ __forceinline__ uint32_t router_correct(uint32_t window, const uint64_t *tab, uint32_t n, uint64_t key,
+                                                   uint32_t b, uint32_t *fallback) {
+    if (b > n) b = n;
+    bool lo_ok = (b == 0) || tab[b - 1] <= key;
+    bool hi_ok = (b == n) || key < tab[b];
     if (lo_ok && hi_ok) return b;
-    if (!hi_ok) {  // spl[b] <= key: the bucket is above b
+    if (!hi_ok) {  // tab[b] <= key: the answer is above b
         uint32_t lo = b + 1, step = 1;
         for (uint32_t it = 0; it < window; ++it) {
             uint32_t hi = lo + step;
-            if (hi > np) hi = np;
-            if (hi == np || key < spl[hi]) {
+            if (hi > n) hi = n;
+            if (hi == n || key < tab[hi]) {
                 uint32_t x = lo, e = hi;
                 while (x < e) {
                     uint32_t mid = (x + e) >> 1;
-                    if (spl[mid] <= key) x = mid + 1;
+                    if (tab[mid] <= key) x = mid + 1;
                     else e = mid;
                 }
                 return x;
@@ -143,15 +152,15 @@ __device__ __forceinline__ uint32_t router_correct(uint32_t window, const Splitt
             lo = hi + 1;
             step <<= 1;
         }
-    } else {  // spl[b-1] > key: the bucket is below b
+    } else {  // tab[b-1] > key: the answer is below b
         uint32_t hi = b - 1, step = 1;
         for (uint32_t it = 0; it < window; ++it) {
             uint32_t lo = hi >= step ? hi - step : 0;
-            if (lo == 0 || spl[lo - 1] <= key) {
+            if (lo == 0 || tab[lo - 1] <= key) {
                 uint32_t x = lo, e = hi;
                 while (x < e) {
                     uint32_t mid = (x + e) >> 1;
-                    if (spl[mid] <= key) x = mid + 1;
+                    if (tab[mid] <= key) x = mid + 1;
                     else e = mid;
                 }
                 return x;
@@ -161,16 +170,89 @@ __device__ __forceinline__ uint32_t router_correct(uint32_t window, const Splitt
         }
     }
     *fallback = 1;
-    uint32_t x = 0, e = np;
+    uint32_t x = 0, e = n;
     while (x < e) {
         uint32_t mid = (x + e) >> 1;
-        if (spl[mid] <= key) x = mid + 1;
+        if (tab[mid] <= key) x = mid + 1;
         else e = mid;
     }
     return x;
 }
 
-// RK: 0 = binary search (LDS coarse table), 1 = PWL router, 2 = MLP router.
+// Bucket of a key = the number of splitters <= key, exactly the upper bound
+// of ext_mem.rs:1705-1734 on packed keys.  It is found in two steps: the
+// position among the coarse splitters (every stride-th splitter, staged in
+// LDS), then among the stride-1 splitters between two coarse entries (global
+// memory).  The coarse position comes from a binary search or from the
+// learned router (prediction, neighbour check in LDS, bounded local search,
+// binary search as the fallback).  N keys are handled in lockstep, branch
+// free (binary lifting over a fixed number of steps), so every step has N
+// independent loads in flight.
+template <int N>
+__device__ __forceinline__ void coarse_bsearch(const Splitters &SP, const uint64_t *coarse, const uint64_t (&key)[N],
+                                               uint32_t (&pos)[N]) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) pos[j] = 0;
+    for (uint32_t step = SP.coarse_steps ? 1u << (SP.coarse_steps - 1) : 0; step; step >>= 1) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            uint32_t pr = pos[j] + step;
+            if (pr <= SP.n_coarse && coarse[pr - 1] <= key[j]) pos[j] = pr;
+        }
+    }
+}
+
+template <int RK, int N>
+__device__ __forceinline__ void coarse_routed(const Splitters &SP, const uint64_t *coarse, const RouterDev &R,
+                                              const uint64_t *rknots, const uint16_t *rradix, const float *ra,
+                                              const float *rb, const float *rinv, const uint32_t *rlo,
+                                              const uint32_t *rhi, const float *nib, const float *w2,
+                                              const uint64_t (&key)[N], const bool (&valid)[N], uint32_t (&pos)[N],
+                                              unsigned long long &sum, unsigned long long &mx,
+                                              unsigned long long &fbs, unsigned long long &cnt) {
+    const uint32_t nc = SP.n_coarse;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        uint32_t pred = RK == 1 ? pwl_predict(R, rknots, rradix, ra, rb, rinv, rlo, rhi, key[j])
+                                : mlp_predict(R, nib, w2, key[j]);
+        pred = pred > nc ? nc : pred;
+        const bool ok = (pred == 0 || coarse[pred - 1] <= key[j]) && (pred == nc || key[j] < coarse[pred]);
+        uint32_t fb = 0;
+        pos[j] = ok ? pred : router_correct(R.window, coarse, nc, key[j], pred, &fb);
+        if (valid[j]) {
+            unsigned long long e = pos[j] > pred ? pos[j] - pred : pred - pos[j];
+            sum += e;
+            mx = e > mx ? e : mx;
+            fbs += fb;
+            cnt += 1;
+        }
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void fine_from_coarse(const Splitters &SP, const uint64_t (&key)[N],
+                                                 const uint32_t (&pos)[N], uint32_t (&b)[N]) {
+    uint32_t hi[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        if (pos[j] == 0) {
+            b[j] = 0;
+            hi[j] = 0;
+        } else {
+            b[j] = (pos[j] - 1) * SP.stride + 1;
+            uint32_t e = b[j] + SP.stride - 1;
+            hi[j] = e > SP.n_spl ? SP.n_spl : e;
+        }
+    }
+    for (uint32_t step = SP.fine_steps ? 1u << (SP.fine_steps - 1) : 0; step; step >>= 1) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            uint32_t pr = b[j] + step;
+            if (pr <= hi[j] && SP.spl[pr - 1] <= key[j]) b[j] = pr;
+        }
+    }
+}
+
 template <typename S, typename I, int RK>
 __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeMap<S> cm, KeyGeom g, DevLimits L,
                                                   uint64_t max_ctx, const I *__restrict__ pos, uint64_t n_pos,
@@ -179,15 +261,14 @@ __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeM
     constexpr uint32_t NS = kPwlSeg;
     __shared__ uint32_t chist[kCoarseBucketsMax];
     for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) chist[c] = 0;
-    __shared__ uint64_t coarse[RK == 0 ? kCoarseMax : 1];
+    __shared__ uint64_t coarse[kCoarseMax];
     __shared__ uint64_t rknots[RK == 1 ? NS + 1 : 1];
     __shared__ float ra[RK == 1 ? NS : 1], rb[RK == 1 ? NS : 1], rinv[RK == 1 ? NS : 1];
     __shared__ uint32_t rlo[RK == 1 ? NS : 1], rhi[RK == 1 ? NS : 1];
     __shared__ uint16_t rradix[RK == 1 ? (1u << kRouterRadixBits) + 1 : 1];
     __shared__ float nib[RK == 2 ? 16 * 16 * kMlpHid : 1], w2[RK == 2 ? kMlpHid + 1 : 1];
     __shared__ unsigned long long st_sum, st_max, st_fb, st_cnt;
-    if (RK == 0)
-        for (uint32_t c = threadIdx.x; c < SP.n_coarse; c += blockDim.x) coarse[c] = SP.spl[(uint64_t)c * SP.stride];
+    for (uint32_t c = threadIdx.x; c < SP.n_coarse; c += blockDim.x) coarse[c] = SP.spl[(uint64_t)c * SP.stride];
     if (RK == 1) {
         for (uint32_t c = threadIdx.x; c < R.ns; c += blockDim.x) {
             rknots[c] = R.knots[c];
@@ -206,28 +287,38 @@ __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeM
     if (threadIdx.x == 0) st_sum = st_max = st_fb = st_cnt = 0;
     __syncthreads();
     unsigned long long my_sum = 0, my_max = 0, my_fb = 0, my_cnt = 0;
-    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pos; i += stride) {
-        uint64_t p = pos ? (uint64_t)pos[i] : i;
-        uint64_t seg;
-        uint64_t lim = lim_seg(L, p, &seg);
-        uint64_t key = with_id(g, key_at(t, cm, g, p, 0, lim < max_ctx ? lim : max_ctx), seg);
-        uint32_t b;
-        if (RK == 0) {
-            b = upper_bound_coarse(SP, coarse, key);
-        } else {
-            uint32_t pred = RK == 1 ? pwl_predict(R, rknots, rradix, ra, rb, rinv, rlo, rhi, key)
-                                    : mlp_predict(R, nib, w2, key);
-            uint32_t fb = 0;
-            b = router_correct(R.window, SP, key, pred, &fb);
-            unsigned long long e = b > pred ? b - pred : pred - b;
-            my_sum += e;
-            my_max = e > my_max ? e : my_max;
-            my_fb += fb;
-            my_cnt += 1;
+    constexpr int NB = RK == 2 ? 1 : 8;  // the MLP's hidden layer does not fit eight keys in registers
+    const uint64_t step_all = (uint64_t)gridDim.x * blockDim.x * NB;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x * NB; base < n_pos; base += step_all) {
+        uint64_t key[NB];
+        bool valid[NB];
+        uint32_t b[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const uint64_t i = base + threadIdx.x + (uint64_t)j * blockDim.x;
+            valid[j] = i < n_pos;
+            key[j] = 0;
+            if (valid[j]) {
+                uint64_t p = pos ? (uint64_t)pos[i] : i;
+                uint64_t seg;
+                uint64_t lim = lim_seg(L, p, &seg);
+                key[j] = with_id(g, key_at(t, cm, g, p, 0, lim < max_ctx ? lim : max_ctx), seg);
+            }
         }
-        bid[i] = b;
-        atomicAdd(&chist[b >> fshift], 1u);
+        uint32_t cpos[NB];
+        if (RK == 0) coarse_bsearch<NB>(SP, coarse, key, cpos);
+        else
+            coarse_routed<RK, NB>(SP, coarse, R, rknots, rradix, ra, rb, rinv, rlo, rhi, nib, w2, key, valid, cpos,
+                                  my_sum, my_max, my_fb, my_cnt);
+        fine_from_coarse<NB>(SP, key, cpos, b);
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const uint64_t i = base + threadIdx.x + (uint64_t)j * blockDim.x;
+            if (valid[j]) {
+                bid[i] = b[j];
+                atomicAdd(&chist[b[j] >> fshift], 1u);
+            }
+        }
     }
     __syncthreads();
     for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x)
@@ -258,10 +349,11 @@ constexpr uint32_t kSeqItems = 16;
 constexpr uint32_t kSeqChunk = 256 * kSeqItems;
 
 template <typename S>
-__device__ __forceinline__ void block_keys_seq(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, const KeyGeom &g,
-                                               const DevLimits &L, uint64_t max_ctx, uint64_t base, uint32_t cnt,
-                                               S *win, uint64_t *skey) {
+__device__ __forceinline__ void thread_keys_seq(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, const KeyGeom &g,
+                                                const DevLimits &L, uint64_t max_ctx, uint64_t base, uint32_t cnt,
+                                                S *win, uint64_t (&out)[kSeqItems]) {
     // window: positions [base, base + cnt + k - 1) clipped to the text
+    __syncthreads();  // the previous chunk's readers of win are done
     uint64_t wlen = (uint64_t)cnt + g.k - 1;
     if (wlen > n - base) wlen = n - base;
     const uint32_t wl = (uint32_t)wlen;
@@ -294,8 +386,20 @@ __device__ __forceinline__ void block_keys_seq(const S *__restrict__ t, uint64_t
     }
     if (first < cnt) keys_from_window<S, kSeqItems>(win, cm, g, first, caps, keys);
 #pragma unroll
+    for (uint32_t i = 0; i < kSeqItems; ++i) out[i] = first + i < cnt ? with_id(g, keys[i], segs[i]) : 0ull;
+}
+
+// The same keys staged in LDS (skey[i] = key of position base + i).
+template <typename S>
+__device__ __forceinline__ void block_keys_seq(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, const KeyGeom &g,
+                                               const DevLimits &L, uint64_t max_ctx, uint64_t base, uint32_t cnt,
+                                               S *win, uint64_t *skey) {
+    uint64_t keys[kSeqItems];
+    thread_keys_seq<S>(t, n, cm, g, L, max_ctx, base, cnt, win, keys);
+    const uint32_t first = threadIdx.x * kSeqItems;
+#pragma unroll
     for (uint32_t i = 0; i < kSeqItems; ++i)
-        if (first + i < cnt) skey[first + i] = with_id(g, keys[i], segs[i]);
+        if (first + i < cnt) skey[first + i] = keys[i];
     __syncthreads();
 }
 
@@ -308,16 +412,14 @@ __global__ __launch_bounds__(256) void k_classify_seq(const S *__restrict__ t, u
     __shared__ uint32_t chist[kCoarseBucketsMax];
     for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) chist[c] = 0;
     __shared__ S win[kSeqChunk + 72];
-    __shared__ uint64_t skey[kSeqChunk];
-    __shared__ uint64_t coarse[RK == 0 ? kCoarseMax : 1];
+    __shared__ uint64_t coarse[kCoarseMax];
     __shared__ uint64_t rknots[RK == 1 ? NS + 1 : 1];
     __shared__ float ra[RK == 1 ? NS : 1], rb[RK == 1 ? NS : 1], rinv[RK == 1 ? NS : 1];
     __shared__ uint32_t rlo[RK == 1 ? NS : 1], rhi[RK == 1 ? NS : 1];
     __shared__ uint16_t rradix[RK == 1 ? (1u << kRouterRadixBits) + 1 : 1];
     __shared__ float nib[RK == 2 ? 16 * 16 * kMlpHid : 1], w2[RK == 2 ? kMlpHid + 1 : 1];
     __shared__ unsigned long long st_sum, st_max, st_fb, st_cnt;
-    if (RK == 0)
-        for (uint32_t c = threadIdx.x; c < SP.n_coarse; c += blockDim.x) coarse[c] = SP.spl[(uint64_t)c * SP.stride];
+    for (uint32_t c = threadIdx.x; c < SP.n_coarse; c += blockDim.x) coarse[c] = SP.spl[(uint64_t)c * SP.stride];
     if (RK == 1) {
         for (uint32_t c = threadIdx.x; c < R.ns; c += blockDim.x) {
             rknots[c] = R.knots[c];
@@ -337,26 +439,31 @@ __global__ __launch_bounds__(256) void k_classify_seq(const S *__restrict__ t, u
     unsigned long long my_sum = 0, my_max = 0, my_fb = 0, my_cnt = 0;
     for (uint64_t base = (uint64_t)blockIdx.x * kSeqChunk; base < n; base += (uint64_t)gridDim.x * kSeqChunk) {
         uint32_t cnt = (uint32_t)(n - base < kSeqChunk ? n - base : kSeqChunk);
-        block_keys_seq<S>(t, n, cm, g, L, max_ctx, base, cnt, win, skey);
-        for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
-            uint64_t key = skey[i];
-            uint32_t b;
-            if (RK == 0) {
-                b = upper_bound_coarse(SP, coarse, key);
-            } else {
-                uint32_t pred = RK == 1 ? pwl_predict(R, rknots, rradix, ra, rb, rinv, rlo, rhi, key)
-                                        : mlp_predict(R, nib, w2, key);
-                uint32_t fb = 0;
-                b = router_correct(R.window, SP, key, pred, &fb);
-                unsigned long long e = b > pred ? b - pred : pred - b;
-                my_sum += e;
-                my_max = e > my_max ? e : my_max;
-                my_fb += fb;
-                my_cnt += 1;
-            }
-            bid[base + i] = b;
-            atomicAdd(&chist[b >> fshift], 1u);
+        uint64_t key[kSeqItems];
+        bool valid[kSeqItems];
+        uint32_t b[kSeqItems], cpos[kSeqItems];
+        thread_keys_seq<S>(t, n, cm, g, L, max_ctx, base, cnt, win, key);
+        const uint32_t first = threadIdx.x * kSeqItems;
+#pragma unroll
+        for (uint32_t j = 0; j < kSeqItems; ++j) valid[j] = first + j < cnt;
+        if (RK == 0) coarse_bsearch<kSeqItems>(SP, coarse, key, cpos);
+        else
+            coarse_routed<RK, kSeqItems>(SP, coarse, R, rknots, rradix, ra, rb, rinv, rlo, rhi, nib, w2, key, valid,
+                                         cpos, my_sum, my_max, my_fb, my_cnt);
+        fine_from_coarse<kSeqItems>(SP, key, cpos, b);
+        if (first + kSeqItems <= cnt) {  // 64 contiguous, 64-byte aligned bytes per thread
+            uint4 *dst = (uint4 *)(bid + base + first);
+#pragma unroll
+            for (uint32_t q = 0; q < kSeqItems / 4; ++q)
+                dst[q] = make_uint4(b[4 * q], b[4 * q + 1], b[4 * q + 2], b[4 * q + 3]);
+        } else {
+#pragma unroll
+            for (uint32_t j = 0; j < kSeqItems; ++j)
+                if (valid[j]) bid[base + first + j] = b[j];
         }
+#pragma unroll
+        for (uint32_t j = 0; j < kSeqItems; ++j)
+            if (valid[j]) atomicAdd(&chist[b[j] >> fshift], 1u);
         __syncthreads();
     }
     __syncthreads();
@@ -419,21 +526,20 @@ struct Rec {
     uint32_t bid;
 };
 
-constexpr uint32_t kScatterChunk = 65536;
-
 template <typename S, typename I, bool SEQ>
 __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, KeyGeom g,
                                                   DevLimits L, uint64_t max_ctx, const I *__restrict__ pos,
                                                   uint64_t n_pos, const uint32_t *__restrict__ bid, uint32_t fshift,
                                                   uint32_t pc, unsigned long long *__restrict__ ccursor,
-                                                  Rec<I> *__restrict__ tmp) {
+                                                  Rec<I> *__restrict__ tmp, uint64_t chunk) {
     __shared__ S win[SEQ ? kSeqChunk + 72 : 1];
     __shared__ uint64_t skey[SEQ ? kSeqChunk : 1];
     __shared__ unsigned long long cbase[kCoarseBucketsMax];
     __shared__ uint32_t lcnt[kCoarseBucketsMax];
-    const uint64_t c0 = (uint64_t)blockIdx.x * kScatterChunk;
+    // chunk: a multiple of kSeqChunk, so key windows never straddle two blocks
+    const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
     if (c0 >= n_pos) return;
-    const uint64_t c1 = c0 + kScatterChunk < n_pos ? c0 + kScatterChunk : n_pos;
+    const uint64_t c1 = c0 + chunk < n_pos ? c0 + chunk : n_pos;
     for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) lcnt[c] = 0;
     __syncthreads();
     for (uint64_t i0 = c0; i0 < c1; i0 += 8 * blockDim.x) {
@@ -588,10 +694,11 @@ struct Range {
 // also listed in `big` for the key merges.
 __global__ void k_tile_emit(const uint64_t *__restrict__ off, uint32_t p, uint32_t T, Tile *__restrict__ tiles,
                             uint64_t cap, unsigned int *__restrict__ tcount, Range *__restrict__ big,
-                            unsigned int *__restrict__ n_big) {
+                            unsigned int *__restrict__ n_big, unsigned int *__restrict__ max_bucket) {
     uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= p) return;
     uint64_t s = off[b], sz = off[b + 1] - s;
+    atomicMax(max_bucket, (unsigned int)(sz < 0xFFFFFFFFull ? sz : 0xFFFFFFFFull));
     for (uint64_t x = 0; x < sz; x += T) {
         Tile tl;
         tl.start = s + x;

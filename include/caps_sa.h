@@ -179,6 +179,7 @@ typedef struct caps_stats {
     uint64_t buckets, tiles, oversized, pairs, small_groups, mid_groups, giant_groups;
     uint64_t router_err_sum, router_err_max, router_fallbacks, router_count;
     uint32_t key_bits, key_k;
+    uint64_t bucket_max; /* records in the largest bucket (imbalance = bucket_max * buckets / n) */
 } caps_stats_t;
 int caps_last_stats(caps_stats_t *out);
 
