@@ -34,6 +34,8 @@ WORKLOADS = {
     # name: (generator, n, description)
     "repeat-rich-268M": ("markov", 1 << 28, "order-3 Markov {0..3} + 40% planted repeats (mean 5000, 1% subst)"),
     "uniform-1M": ("uniform", 1 << 20, "i.i.d. uniform {0..3}"),
+    "batch-4096": ("batch", 0, "4,096 texts, lognormal lengths (median 65,536, sigma 1.5, clipped to [1,024, "
+                              "16,777,216]); even texts u8 i.i.d. {0..3}, odd texts u16 Zipf(1.1) over 50,000 symbols"),
 }
 
 
@@ -88,6 +90,99 @@ def cpu_baseline(text, seconds_budget: float, threads: int):
                       f"median of {len(runs)} runs ({med:.2f} s each)"}
 
 
+def run_batch(args, count: int) -> dict:
+    """Batched builds: every text's suffix array from one pipeline per symbol
+    width (caps_build_batch_device), texts resident in HBM back to back."""
+    import numpy as np
+    import torch
+
+    import caps_sa
+    from caps_sa import device as dev
+    from caps_sa import gen
+
+    lengths = gen.lognormal_lengths(count, 65536.0, 1.5, 1024, 1 << 24, seed=args.seed).astype(np.int64)
+    texts = []
+    for i, m in enumerate(lengths.tolist()):
+        sd = args.seed * 1_000_003 + i
+        texts.append(gen.uniform_u8(m, 4, seed=sd) if i % 2 == 0 else gen.zipf_u16(m, 50000, 1.1, seed=sd))
+    groups = []
+    for w, sel in ((1, [i for i in range(count) if i % 2 == 0]), (2, [i for i in range(count) if i % 2 == 1])):
+        if not sel:
+            continue
+        cat = np.concatenate([texts[i] for i in sel])
+        ends = np.cumsum([len(texts[i]) for i in sel]).astype(np.uint64)
+        d_text = dev.text_to_device(cat)
+        d_out = torch.empty(len(cat), dtype=torch.int32, device="cuda:0")
+        groups.append((w, sel, ends, d_text, d_out, len(cat)))
+    total = sum(g[5] for g in groups)
+    print(f"[bench] batch: {count} texts, {total} symbols", file=sys.stderr, flush=True)
+
+    def step():
+        out = []
+        for w, sel, ends, d_text, d_out, m in groups:
+            dev.build_batch_device(d_text, m, w, ends, 4, d_out)
+            out.append(caps_sa.last_stats())
+        return out
+
+    for i in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = [step() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = wall * 1e3 / max(args.steps, 1)
+    verified = None
+    sample = []
+    if not args.no_verify:
+        import oracle as O
+
+        rng = np.random.default_rng(args.seed)
+        sample = sorted(rng.choice(count, size=min(count, 48), replace=False).tolist())
+        ok = True
+        for w, sel, ends, d_text, d_out, m in groups:
+            got = d_out.cpu().numpy().view(np.uint32).astype(np.uint64)
+            starts = np.concatenate([[0], ends[:-1]]).astype(np.int64)
+            for k, i in enumerate(sel):
+                if i not in sample:
+                    continue
+                a, b = int(starts[k]), int(ends[k])
+                want = O.Ctx(texts[i]).build_in_memory(threads=args.cpu_threads)
+                ok = ok and bool(np.array_equal(got[a:b] - a, want))
+        verified = ok
+        print(f"[bench] verify {len(sample)} texts: {'ok' if ok else 'FAILED'}", file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+
+        t_end = time.perf_counter() + args.cpu_seconds
+        done, t_cpu, k = 0, 0.0, 0
+        order = np.random.default_rng(args.seed + 1).permutation(count)
+        while time.perf_counter() < t_end and k < count:
+            tx = texts[int(order[k])]
+            t1 = time.perf_counter()
+            O.Ctx(tx).build_in_memory(threads=args.cpu_threads)
+            t_cpu += time.perf_counter() - t1
+            done += len(tx)
+            k += 1
+        cpu = {"value": round(done / t_cpu / 1e6, 3), "unit": "M suffixes/s", "cores": args.cpu_threads,
+               "kind": "port", "sample": f"build_in_memory on {k} texts of the same batch ({done} symbols)"}
+    kern = ["k_classify", "k_scatter", "k_tile_sort", "k_detect", "k_pairs", "k_groups_small", "k_groups_mid",
+            "k_boundary_lcp"]
+    avg = {kk: sum(sum(x[kk] for x in st) for st in stats) / len(stats) for kk in kern}
+    phases = ("ms_alphabet", "ms_sample", "ms_classify", "ms_scatter", "ms_tile_sort", "ms_merge", "ms_detect",
+              "ms_refine", "ms_giant")
+    per_width = []
+    for gi, (w, sel, ends, d_text, d_out, m) in enumerate(groups):
+        last = stats[-1][gi]
+        per_width.append({"sym_bytes": w, "texts": len(sel), "symbols": m, "key_k": last["key_k"],
+                          "phases_ms": {ph: round(sum(st[gi][ph] for st in stats) / len(stats), 3) for ph in phases},
+                          "groups": {k: last[k] for k in ("pairs", "small_groups", "mid_groups", "giant_groups",
+                                                           "oversized", "buckets", "bucket_max")}})
+    return {"ms_step": ms_step, "n": total, "verified": verified, "sample_texts": len(sample), "cpu": cpu,
+            "kernels": avg, "last": stats[-1], "per_width": per_width}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +196,8 @@ def main() -> int:
                     help="groups of 2..64: the default per-class plan, or persistent work-queue slots for all")
     ap.add_argument("--tile-sort", default="merge", choices=["merge", "radix"])
     ap.add_argument("--bucket-keys", type=int, default=0, help="mean records per bucket (0: library default)")
+    ap.add_argument("--buckets", type=int, default=0, help="explicit bucket count (0: from --bucket-keys)")
+    ap.add_argument("--texts", type=int, default=4096, help="batch workload: number of texts")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -131,6 +228,31 @@ def main() -> int:
         print(json.dumps({"copy_probe_bytes": 2 * src.numel() * 4}))
         return 0
     kind, n_default, desc = WORKLOADS[args.workload]
+    if kind == "batch":
+        r = run_batch(args, args.texts)
+        n = r["n"]
+        value = n / (r["ms_step"] / 1e3) / 1e6
+        avg = r["kernels"]
+        dom = max(avg, key=lambda k: avg[k] if algorithmic_bytes_per_suffix(k, 4) else -1)
+        bpsuf = algorithmic_bytes_per_suffix(dom, 4)
+        achieved = n * bpsuf / (avg[dom] / 1e3) / 1e9 if avg[dom] else 0.0
+        line = {
+            "metric": "M suffixes/s (bit-exact SA), learned-router vs plain GPU; HBM GB/s % peak",
+            "value": round(value, 2), "unit": "M suffixes/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(r["ms_step"], 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u8+u16", "data": "synthetic",
+            "config": {"workload": args.workload, "texts": args.texts, "symbols": n, "text": desc, "index": "u32",
+                       "router": args.router, "seed": args.seed},
+            "verified_bit_exact": r["verified"], "verified_texts": r["sample_texts"],
+            "kernels_ms": {k: round(v, 3) for k, v in avg.items()},
+            "builds": r["per_width"],
+            "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                         "bytes_per_suffix": bpsuf, "kernel_ms": round(avg[dom], 4)},
+            "cpu_baseline": r["cpu"],
+        }
+        print(json.dumps(line))
+        return 0 if r["verified"] in (None, True) else 3
     n = args.n or n_default
     print(f"[bench] generating {args.workload} n={n}", file=sys.stderr, flush=True)
     text = make_text(kind, n, args.seed)
@@ -140,6 +262,7 @@ def main() -> int:
 
     def step():
         dev.build_device(d_text, n, 1, idx_bytes=4, router=router, d_out=d_out, tile_keys=args.bucket_keys,
+                         gpu_buckets=args.buckets,
                          flags=(caps_sa.FLAG_PERSISTENT_REFINE if args.refine == "queue" else 0)
                          | (caps_sa.FLAG_TILE_RADIX if args.tile_sort == "radix" else 0))
         return caps_sa.last_stats()

@@ -696,6 +696,10 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     tm.mark(PH_REFINE);
 
     // ---- giant groups: chunked refinement + merge-path LCP merges ----
+    // All giant groups at once: one launch sorts every chunk of every group,
+    // then one launch per merge level merges the runs of every group (a group
+    // already in one run is carried over by a copy job).  The job lists of all
+    // levels are built and uploaded together, so no level waits for the host.
     if (cnt[7]) {
         std::vector<Group> hg(cnt[7]);
         HIPCHK(hipMemcpyAsync(hg.data(), gl.list[7], cnt[7] * sizeof(Group), hipMemcpyDeviceToHost, st));
@@ -703,16 +707,21 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         I *lbuf = lcp ? lcp : (I *)W.lcp.get(n_pos * sizeof(I));
         I *i2 = (I *)W.idx2.get(n_pos * sizeof(I));
         I *l2 = (I *)W.lcp2.get(n_pos * sizeof(I));
-        for (unsigned gi = 0; gi < cnt[7]; ++gi) {
-            const Group gr = hg[gi];
-            unsigned nch = (unsigned)ceil_div(gr.size, kMidMax);
-            k_groups<S, I, 256, kMidMax><<<dim3(1, nch), 256, 0, st>>>(d_text, cm, gref, L, max_ctx, gl.list[7] + gi,
-                                                                         depth0, d_out, lbuf, 1, kMidMax);
-            LAUNCHCHK();
-            I *is = d_out, *ls = lbuf, *id = i2, *ld = l2;
-            int levels = 0;
-            for (uint64_t run = kMidMax; run < gr.size; run *= 2, ++levels) {
-                W.h_jobs.clear();
+        std::vector<Group> hc;
+        uint64_t maxsize = 0;
+        for (const Group &gr : hg) {
+            maxsize = std::max<uint64_t>(maxsize, gr.size);
+            for (uint64_t c = 0; c < gr.size; c += kMidMax)
+                hc.push_back(Group{gr.start + c, std::min<uint64_t>(kMidMax, gr.size - c)});
+        }
+        int levels = 0;
+        for (uint64_t run = kMidMax; run < maxsize; run *= 2) ++levels;
+        std::vector<uint64_t> level_off(levels + 1, 0);
+        W.h_jobs.clear();
+        for (int lv = 0; lv < levels; ++lv) {
+            const uint64_t run = (uint64_t)kMidMax << lv;
+            level_off[lv] = W.h_jobs.size();
+            for (const Group &gr : hg)
                 for (uint64_t b = 0; b < gr.size; b += 2 * run) {
                     MergeJob jb;
                     jb.start = gr.start + b;
@@ -720,20 +729,38 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
                     jb.ny = std::min<uint64_t>(run, gr.size - b - jb.nx);
                     W.h_jobs.push_back(jb);
                 }
-                MergeJob *dj = (MergeJob *)W.jobs.get(W.h_jobs.size() * sizeof(MergeJob));
-                HIPCHK(hipMemcpyAsync(dj, W.h_jobs.data(), W.h_jobs.size() * sizeof(MergeJob), hipMemcpyHostToDevice,
-                                      st));
-                constexpr int MW = 32;
-                dim3 gr2((unsigned)ceil_div(ceil_div(2 * run, MW), 128), (unsigned)W.h_jobs.size());
-                k_lcp_merge<S, I, MW><<<gr2, 128, 0, st>>>(d_text, L, max_ctx, depth0, dj, is, ls, id, ld);
+        }
+        level_off[levels] = W.h_jobs.size();
+        Group *dc = (Group *)W.chunks.get(hc.size() * sizeof(Group));
+        HIPCHK(hipMemcpyAsync(dc, hc.data(), hc.size() * sizeof(Group), hipMemcpyHostToDevice, st));
+        MergeJob *dj = (MergeJob *)W.jobs.get(std::max<size_t>(W.h_jobs.size(), 1) * sizeof(MergeJob));
+        if (!W.h_jobs.empty())
+            HIPCHK(hipMemcpyAsync(dj, W.h_jobs.data(), W.h_jobs.size() * sizeof(MergeJob), hipMemcpyHostToDevice,
+                                  st));
+        for (uint64_t c0 = 0; c0 < hc.size(); c0 += 65535u * 16) {
+            const unsigned nb = (unsigned)std::min<uint64_t>(hc.size() - c0, 65535u * 16);
+            k_groups<S, I, 256, kMidMax><<<nb, 256, 0, st>>>(d_text, cm, gref, L, max_ctx, dc + c0, depth0, d_out, lbuf,
+                                                             1, 0);
+            LAUNCHCHK();
+        }
+        I *is = d_out, *ls = lbuf, *id = i2, *ld = l2;
+        constexpr int MW = 32;
+        for (int lv = 0; lv < levels; ++lv) {
+            const uint64_t run = (uint64_t)kMidMax << lv;
+            const uint64_t span = std::min<uint64_t>(2 * run, maxsize);
+            for (uint64_t j0 = level_off[lv]; j0 < level_off[lv + 1]; j0 += 65535) {
+                const unsigned nj = (unsigned)std::min<uint64_t>(level_off[lv + 1] - j0, 65535);
+                dim3 gr2((unsigned)ceil_div(ceil_div(span, MW), 128), nj);
+                k_lcp_merge<S, I, MW><<<gr2, 128, 0, st>>>(d_text, L, max_ctx, depth0, dj + j0, is, ls, id, ld);
                 LAUNCHCHK();
-                HIPCHK(hipStreamSynchronize(st));  // h_jobs is reused by the next level
-                std::swap(is, id);
-                std::swap(ls, ld);
             }
-            if (levels & 1) {
-                k_copy_idx_lcp<I><<<grid_for(gr.size, 256, 4096), 256, 0, st>>>(gr.start, gr.size, i2, l2, d_out,
-                                                                                 lbuf);
+            std::swap(is, id);
+            std::swap(ls, ld);
+        }
+        if (levels & 1) {
+            for (uint64_t c0 = 0; c0 < hc.size(); c0 += 65535u * 16) {
+                const unsigned nb = (unsigned)std::min<uint64_t>(hc.size() - c0, 65535u * 16);
+                k_copy_chunks<I><<<nb, 256, 0, st>>>(dc + c0, i2, l2, d_out, lbuf);
                 LAUNCHCHK();
             }
         }

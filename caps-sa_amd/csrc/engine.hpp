@@ -46,7 +46,7 @@ struct Workspace {
     // tiles
     DevBuf tcnt, toff, toff2, tiles, big, counts, recs;
     // refinement
-    DevBuf glist[kGroupClasses], lcp, lcp2, first, jobs;
+    DevBuf glist[kGroupClasses], lcp, lcp2, first, jobs, chunks;
     std::vector<MergeJob> h_jobs;
     // standalone merges
     DevBuf m_a, m_al, m_b, m_bl;

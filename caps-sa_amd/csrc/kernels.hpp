@@ -941,6 +941,19 @@ __global__ __launch_bounds__(B) void k_groups(const S *__restrict__ t, CodeMap<S
     }
 }
 
+// Copies the (idx, lcp) ranges of a list of chunks back from the ping-pong
+// buffers (one block per chunk).
+template <typename I>
+__global__ __launch_bounds__(256) void k_copy_chunks(const Group *__restrict__ chunks, const I *__restrict__ si,
+                                                     const I *__restrict__ sl, I *__restrict__ di,
+                                                     I *__restrict__ dl) {
+    const Group c = chunks[blockIdx.x];
+    for (uint64_t q = threadIdx.x; q < c.size; q += 256) {
+        di[c.start + q] = si[c.start + q];
+        dl[c.start + q] = sl[c.start + q];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Merge-path LCP merge (sample_sort.rs:397-522 restated for a partitioned
 // GPU merge).  Each thread owns W consecutive outputs of one merge job: a
