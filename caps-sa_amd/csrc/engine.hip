@@ -489,12 +489,10 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     while ((1ull << SP.fine_steps) < SP.stride) ++SP.fine_steps;
     stats.buckets = p;
 
-    // ---- classify + coarse histogram ----
-    uint32_t fshift = 7;
-    while (ceil_div(p, 1ull << fshift) > kCoarseBucketsMax) ++fshift;
-    if ((1u << fshift) > kFineMax) throw CapsError(CAPS_E_INVALID, "too many buckets");
-    const uint32_t pc = (uint32_t)ceil_div(p, 1ull << fshift);
-    uint32_t *bid = (uint32_t *)W.bid.get(n_pos * 4);
+    // ---- classify (coarse interval) + histogram ----
+    if (SP.stride > kFineMax) throw CapsError(CAPS_E_INVALID, "too many buckets");
+    const uint32_t pc = SP.n_coarse + 1;
+    uint16_t *cid = (uint16_t *)W.bid.get(n_pos * 2);
     uint32_t *hist = (uint32_t *)W.hist.get(pc * 4);
     uint64_t *coff = (uint64_t *)W.toff.get((std::max<uint64_t>(pc, p) + 1) * 8);
     uint64_t *off = (uint64_t *)W.off.get((p + 1) * 8);
@@ -516,22 +514,19 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     if (seq) {
         if (rkind == CAPS_ROUTER_PWL)
             k_classify_seq<S, I, 1><<<gseq(k_classify_seq<S, I, 1>), 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R,
-                                                                                 bid, hist, fshift, pc);
+                                                                                 cid, hist, pc);
         else if (rkind == CAPS_ROUTER_MLP)
             k_classify_seq<S, I, 2><<<gseq(k_classify_seq<S, I, 2>), 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R,
-                                                                                 bid, hist, fshift, pc);
+                                                                                 cid, hist, pc);
         else
             k_classify_seq<S, I, 0><<<gseq(k_classify_seq<S, I, 0>), 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R,
-                                                                                 bid, hist, fshift, pc);
+                                                                                 cid, hist, pc);
     } else if (rkind == CAPS_ROUTER_PWL)
-        k_classify<S, I, 1><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, bid, hist, fshift,
-                                                   pc);
+        k_classify<S, I, 1><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hist, pc);
     else if (rkind == CAPS_ROUTER_MLP)
-        k_classify<S, I, 2><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, bid, hist, fshift,
-                                                   pc);
+        k_classify<S, I, 2><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hist, pc);
     else
-        k_classify<S, I, 0><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, bid, hist, fshift,
-                                                   pc);
+        k_classify<S, I, 0><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hist, pc);
     LAUNCHCHK();
     kt.end();
     k_scan_u32<<<1, 1024, 0, st>>>(hist, pc, coff);
@@ -541,19 +536,22 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
 
     // ---- two-level scatter ----
     uint64_t *keys = (uint64_t *)W.keys.get(n_pos * 8);
-    Rec<I> *tmp = (Rec<I> *)W.recs.get(n_pos * sizeof(Rec<I>));
+    uint64_t *tk = (uint64_t *)W.recs.get(n_pos * (8 + sizeof(I)));
+    I *ti = (I *)(tk + n_pos);
     kt.begin(&stats.k_scatter);
     // chunk per scatter block: a multiple of the key window, about one resident wave of blocks
-    const uint64_t schunk = std::max<uint64_t>(kSeqChunk, ceil_div(ceil_div(n_pos, (uint64_t)W.n_cu * 2), kSeqChunk) * kSeqChunk);
+    const uint64_t schunk =
+        std::max<uint64_t>(kSeqChunk, ceil_div(ceil_div(n_pos, (uint64_t)W.n_cu * 2), kSeqChunk) * kSeqChunk);
     const unsigned g1 = (unsigned)ceil_div(n_pos, schunk);
     if (seq)
-        k_scatter1<S, I, true><<<g1, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, bid, fshift, pc, cursor,
-                                                   tmp, schunk);
+        k_scatter1<S, I, true><<<g1, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, cid, pc, cursor, tk,
+                                                   ti, schunk);
     else
-        k_scatter1<S, I, false><<<g1, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, bid, fshift, pc,
-                                                    cursor, tmp, schunk);
+        k_scatter1<S, I, false><<<g1, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, cid, pc, cursor, tk,
+                                                    ti, schunk);
     LAUNCHCHK();
-    k_scatter2<I><<<pc, 256, 0, st>>>(tmp, coff, fshift, (uint32_t)p, off, keys, d_out);
+    // the coarse ids are consumed: their buffer holds the per-record bucket within its interval
+    k_scatter2<I><<<pc, 256, 0, st>>>(tk, ti, cid, coff, SP, (uint32_t)p, off, keys, d_out);
     LAUNCHCHK();
     kt.end();
     tm.mark(PH_SCATTER);

@@ -79,13 +79,16 @@ __global__ void k_pick_splitters(const uint64_t *__restrict__ sorted, uint64_t s
 
 // ---------------------------------------------------------------------------
 // Bucket classification.  Bucket of a key = number of splitters <= key, so
-// equal keys always share a bucket.  The splitter table is searched with an
-// LDS-resident coarse table (every COARSE-th splitter) followed by a short
-// search in the global table.
+// equal keys always share a bucket.  It is found in two steps, which are also
+// the two levels of the scatter: the classify pass finds the key's coarse
+// interval (the number of coarse splitters <= key, coarse splitter j being
+// splitter j*stride), and the second scatter pass places each record of one
+// coarse interval into its bucket by a search over the interval's stride-1
+// inner splitters, staged in LDS.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kCoarseMax = 4096;
-constexpr uint32_t kCoarseBucketsMax = 2048;  // coarse buckets of the two-level scatter
-constexpr uint32_t kFineMax = 4096;           // fine buckets per coarse bucket
+constexpr uint32_t kCoarseMax = 2047;         // coarse splitters (LDS table of the classify pass)
+constexpr uint32_t kCoarseBucketsMax = 2048;  // coarse intervals = kCoarseMax + 1
+constexpr uint32_t kFineMax = 4096;           // buckets per coarse interval (the stride)
 
 struct Splitters {
     const uint64_t *spl;    // p-1 sorted keys
@@ -229,35 +232,12 @@ __device__ __forceinline__ void coarse_routed(const Splitters &SP, const uint64_
     }
 }
 
-template <int N>
-__device__ __forceinline__ void fine_from_coarse(const Splitters &SP, const uint64_t (&key)[N],
-                                                 const uint32_t (&pos)[N], uint32_t (&b)[N]) {
-    uint32_t hi[N];
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        if (pos[j] == 0) {
-            b[j] = 0;
-            hi[j] = 0;
-        } else {
-            b[j] = (pos[j] - 1) * SP.stride + 1;
-            uint32_t e = b[j] + SP.stride - 1;
-            hi[j] = e > SP.n_spl ? SP.n_spl : e;
-        }
-    }
-    for (uint32_t step = SP.fine_steps ? 1u << (SP.fine_steps - 1) : 0; step; step >>= 1) {
-#pragma unroll
-        for (int j = 0; j < N; ++j) {
-            uint32_t pr = b[j] + step;
-            if (pr <= hi[j] && SP.spl[pr - 1] <= key[j]) b[j] = pr;
-        }
-    }
-}
 
 template <typename S, typename I, int RK>
 __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeMap<S> cm, KeyGeom g, DevLimits L,
                                                   uint64_t max_ctx, const I *__restrict__ pos, uint64_t n_pos,
-                                                  Splitters SP, RouterDev R, uint32_t *__restrict__ bid,
-                                                  uint32_t *__restrict__ hist, uint32_t fshift, uint32_t pc) {
+                                                  Splitters SP, RouterDev R, uint16_t *__restrict__ cid,
+                                                  uint32_t *__restrict__ hist, uint32_t pc) {
     constexpr uint32_t NS = kPwlSeg;
     __shared__ uint32_t chist[kCoarseBucketsMax];
     for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) chist[c] = 0;
@@ -305,18 +285,16 @@ __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeM
                 key[j] = with_id(g, key_at(t, cm, g, p, 0, lim < max_ctx ? lim : max_ctx), seg);
             }
         }
-        uint32_t cpos[NB];
-        if (RK == 0) coarse_bsearch<NB>(SP, coarse, key, cpos);
+        if (RK == 0) coarse_bsearch<NB>(SP, coarse, key, b);
         else
-            coarse_routed<RK, NB>(SP, coarse, R, rknots, rradix, ra, rb, rinv, rlo, rhi, nib, w2, key, valid, cpos,
+            coarse_routed<RK, NB>(SP, coarse, R, rknots, rradix, ra, rb, rinv, rlo, rhi, nib, w2, key, valid, b,
                                   my_sum, my_max, my_fb, my_cnt);
-        fine_from_coarse<NB>(SP, key, cpos, b);
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
             const uint64_t i = base + threadIdx.x + (uint64_t)j * blockDim.x;
             if (valid[j]) {
-                bid[i] = b[j];
-                atomicAdd(&chist[b[j] >> fshift], 1u);
+                cid[i] = (uint16_t)b[j];
+                atomicAdd(&chist[b[j]], 1u);
             }
         }
     }
@@ -406,8 +384,8 @@ __device__ __forceinline__ void block_keys_seq(const S *__restrict__ t, uint64_t
 template <typename S, typename I, int RK>
 __global__ __launch_bounds__(256) void k_classify_seq(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, KeyGeom g,
                                                       DevLimits L, uint64_t max_ctx, Splitters SP, RouterDev R,
-                                                      uint32_t *__restrict__ bid, uint32_t *__restrict__ hist,
-                                                      uint32_t fshift, uint32_t pc) {
+                                                      uint16_t *__restrict__ cid, uint32_t *__restrict__ hist,
+                                                      uint32_t pc) {
     constexpr uint32_t NS = kPwlSeg;
     __shared__ uint32_t chist[kCoarseBucketsMax];
     for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) chist[c] = 0;
@@ -441,29 +419,29 @@ __global__ __launch_bounds__(256) void k_classify_seq(const S *__restrict__ t, u
         uint32_t cnt = (uint32_t)(n - base < kSeqChunk ? n - base : kSeqChunk);
         uint64_t key[kSeqItems];
         bool valid[kSeqItems];
-        uint32_t b[kSeqItems], cpos[kSeqItems];
+        uint32_t b[kSeqItems];
         thread_keys_seq<S>(t, n, cm, g, L, max_ctx, base, cnt, win, key);
         const uint32_t first = threadIdx.x * kSeqItems;
 #pragma unroll
         for (uint32_t j = 0; j < kSeqItems; ++j) valid[j] = first + j < cnt;
-        if (RK == 0) coarse_bsearch<kSeqItems>(SP, coarse, key, cpos);
+        if (RK == 0) coarse_bsearch<kSeqItems>(SP, coarse, key, b);
         else
             coarse_routed<RK, kSeqItems>(SP, coarse, R, rknots, rradix, ra, rb, rinv, rlo, rhi, nib, w2, key, valid,
-                                         cpos, my_sum, my_max, my_fb, my_cnt);
-        fine_from_coarse<kSeqItems>(SP, key, cpos, b);
-        if (first + kSeqItems <= cnt) {  // 64 contiguous, 64-byte aligned bytes per thread
-            uint4 *dst = (uint4 *)(bid + base + first);
+                                         b, my_sum, my_max, my_fb, my_cnt);
+        if (first + kSeqItems <= cnt) {  // 32 contiguous, 32-byte aligned bytes per thread
+            uint4 *dst = (uint4 *)(cid + base + first);
 #pragma unroll
-            for (uint32_t q = 0; q < kSeqItems / 4; ++q)
-                dst[q] = make_uint4(b[4 * q], b[4 * q + 1], b[4 * q + 2], b[4 * q + 3]);
+            for (uint32_t q = 0; q < kSeqItems / 8; ++q)
+                dst[q] = make_uint4(b[8 * q] | (b[8 * q + 1] << 16), b[8 * q + 2] | (b[8 * q + 3] << 16),
+                                    b[8 * q + 4] | (b[8 * q + 5] << 16), b[8 * q + 6] | (b[8 * q + 7] << 16));
         } else {
 #pragma unroll
             for (uint32_t j = 0; j < kSeqItems; ++j)
-                if (valid[j]) bid[base + first + j] = b[j];
+                if (valid[j]) cid[base + first + j] = (uint16_t)b[j];
         }
 #pragma unroll
         for (uint32_t j = 0; j < kSeqItems; ++j)
-            if (valid[j]) atomicAdd(&chist[b[j] >> fshift], 1u);
+            if (valid[j]) atomicAdd(&chist[b[j]], 1u);
         __syncthreads();
     }
     __syncthreads();
@@ -512,26 +490,20 @@ __global__ __launch_bounds__(1024) void k_scan_u32(const uint32_t *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Two-level scatter.  Fine bucket b belongs to coarse bucket b >> fshift.
-// Pass 1: a block owns a chunk of 64K positions, counts its coarse buckets,
-// reserves one run per coarse bucket with a single global atomic each, and
-// writes (key, idx, fine bucket) records into those runs.  Pass 2: one
-// workgroup per coarse bucket histograms its fine buckets in LDS, writes the
-// fine bucket offsets, and places every record with LDS cursors.
+// Two-level scatter.  Pass 1: a block owns a chunk of positions, counts its
+// coarse intervals, reserves one run per interval with a single global atomic
+// each, and writes (key, position) into those runs (two arrays).  Pass 2: one
+// workgroup per coarse interval stages the interval's inner splitters in LDS,
+// finds each record's bucket by binary lifting over them (kept as a u16 for
+// the placing pass), histograms and scans the buckets, writes the bucket
+// offsets, and places every record with LDS cursors.
 // ---------------------------------------------------------------------------
-template <typename I>
-struct Rec {
-    uint64_t key;
-    I idx;
-    uint32_t bid;
-};
-
 template <typename S, typename I, bool SEQ>
 __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, KeyGeom g,
                                                   DevLimits L, uint64_t max_ctx, const I *__restrict__ pos,
-                                                  uint64_t n_pos, const uint32_t *__restrict__ bid, uint32_t fshift,
-                                                  uint32_t pc, unsigned long long *__restrict__ ccursor,
-                                                  Rec<I> *__restrict__ tmp, uint64_t chunk) {
+                                                  uint64_t n_pos, const uint16_t *__restrict__ cid, uint32_t pc,
+                                                  unsigned long long *__restrict__ ccursor, uint64_t *__restrict__ tk,
+                                                  I *__restrict__ ti, uint64_t chunk) {
     __shared__ S win[SEQ ? kSeqChunk + 72 : 1];
     __shared__ uint64_t skey[SEQ ? kSeqChunk : 1];
     __shared__ unsigned long long cbase[kCoarseBucketsMax];
@@ -543,15 +515,15 @@ __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint6
     for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) lcnt[c] = 0;
     __syncthreads();
     for (uint64_t i0 = c0; i0 < c1; i0 += 8 * blockDim.x) {
-        uint32_t bb[8];
+        uint32_t cc[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             uint64_t i = i0 + u * blockDim.x + threadIdx.x;
-            bb[u] = i < c1 ? bid[i] : 0xFFFFFFFFu;
+            cc[u] = i < c1 ? cid[i] : 0xFFFFFFFFu;
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-            if (bb[u] != 0xFFFFFFFFu) atomicAdd(&lcnt[bb[u] >> fshift], 1u);
+            if (cc[u] != 0xFFFFFFFFu) atomicAdd(&lcnt[cc[u]], 1u);
     }
     __syncthreads();
     for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) {
@@ -565,28 +537,24 @@ __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint6
             uint32_t cnt = (uint32_t)(c1 - base < kSeqChunk ? c1 - base : kSeqChunk);
             block_keys_seq<S>(t, n, cm, g, L, max_ctx, base, cnt, win, skey);
             constexpr int U = kSeqChunk / 256;
-            uint32_t bb[U];
+            uint32_t cc[U];
             unsigned long long slot[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 uint32_t i = u * 256 + threadIdx.x;
-                bb[u] = i < cnt ? bid[base + i] : 0u;
+                cc[u] = i < cnt ? cid[base + i] : 0u;
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 uint32_t i = u * 256 + threadIdx.x;
-                uint32_t c = bb[u] >> fshift;
-                slot[u] = i < cnt ? cbase[c] + atomicAdd(&lcnt[c], 1u) : 0ull;
+                slot[u] = i < cnt ? cbase[cc[u]] + atomicAdd(&lcnt[cc[u]], 1u) : 0ull;
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 uint32_t i = u * 256 + threadIdx.x;
                 if (i < cnt) {
-                    Rec<I> r;
-                    r.key = skey[i];
-                    r.idx = (I)(base + i);
-                    r.bid = bb[u];
-                    tmp[slot[u]] = r;
+                    tk[slot[u]] = skey[i];
+                    ti[slot[u]] = (I)(base + i);
                 }
             }
             __syncthreads();
@@ -594,50 +562,68 @@ __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint6
     } else {
         for (uint64_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
             uint64_t p = pos ? (uint64_t)pos[i] : i;
-            uint32_t b = bid[i];
-            uint32_t c = b >> fshift;
+            uint32_t c = cid[i];
             unsigned long long slot = cbase[c] + atomicAdd(&lcnt[c], 1u);
-            Rec<I> r;
             uint64_t seg;
             uint64_t lim = lim_seg(L, p, &seg);
-            r.key = with_id(g, key_at(t, cm, g, p, 0, lim < max_ctx ? lim : max_ctx), seg);
-            r.idx = (I)p;
-            r.bid = b;
-            tmp[slot] = r;
+            tk[slot] = with_id(g, key_at(t, cm, g, p, 0, lim < max_ctx ? lim : max_ctx), seg);
+            ti[slot] = (I)p;
         }
     }
 }
 
 template <typename I>
-__global__ __launch_bounds__(256) void k_scatter2(const Rec<I> *__restrict__ tmp, const uint64_t *__restrict__ coff,
-                                                  uint32_t fshift, uint32_t p, uint64_t *__restrict__ off,
+__global__ __launch_bounds__(256) void k_scatter2(const uint64_t *__restrict__ tk, const I *__restrict__ ti,
+                                                  uint16_t *__restrict__ fid, const uint64_t *__restrict__ coff,
+                                                  Splitters SP, uint32_t p, uint64_t *__restrict__ off,
                                                   uint64_t *__restrict__ keys, I *__restrict__ idx) {
     constexpr uint32_t FMAX = kFineMax;
+    __shared__ uint64_t sspl[FMAX];
     __shared__ uint32_t cnt[FMAX];
     __shared__ unsigned long long fbase[FMAX];
     __shared__ unsigned long long part[256];
     const uint32_t c = blockIdx.x;
     const uint64_t s = coff[c], e = coff[c + 1];
-    const uint32_t f0 = c << fshift;
-    const uint32_t F = 1u << fshift;
-    const uint32_t nf = p - f0 < F ? p - f0 : F;
-    for (uint32_t j = threadIdx.x; j < nf; j += blockDim.x) cnt[j] = 0;
+    // buckets of interval c: b0 .. b0 + nb - 1; its inner splitters spl[b0 .. b0 + nb - 1)
+    const uint32_t b0 = c == 0 ? 0u : (c - 1) * SP.stride + 1;
+    const uint32_t nb = c == 0 ? 1u : (SP.stride < p - b0 ? SP.stride : p - b0);
+    const uint32_t ninner = nb - 1;
+    uint32_t fsteps = 0;
+    while ((1u << fsteps) <= ninner) ++fsteps;
+    for (uint32_t j = threadIdx.x; j < nb; j += blockDim.x) {
+        cnt[j] = 0;
+        if (j < ninner) sspl[j] = SP.spl[b0 + j];
+    }
     __syncthreads();
     for (uint64_t i0 = s; i0 < e; i0 += 8 * blockDim.x) {
-        uint32_t bb[8];
+        uint64_t kk[8];
+        uint32_t f[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             uint64_t i = i0 + u * blockDim.x + threadIdx.x;
-            bb[u] = i < e ? tmp[i].bid - f0 : 0xFFFFFFFFu;
+            kk[u] = i < e ? tk[i] : 0ull;
+            f[u] = 0;
+        }
+        for (uint32_t step = fsteps ? 1u << (fsteps - 1) : 0; step; step >>= 1) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                uint32_t pr = f[u] + step;
+                if (pr <= ninner && sspl[pr - 1] <= kk[u]) f[u] = pr;
+            }
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (bb[u] != 0xFFFFFFFFu) atomicAdd(&cnt[bb[u]], 1u);
+        for (int u = 0; u < 8; ++u) {
+            uint64_t i = i0 + u * blockDim.x + threadIdx.x;
+            if (i < e) {
+                fid[i] = (uint16_t)f[u];
+                atomicAdd(&cnt[f[u]], 1u);
+            }
+        }
     }
     __syncthreads();
-    // exclusive scan of cnt[0..nf): each thread scans a contiguous slice
-    const uint32_t per = (nf + blockDim.x - 1) / blockDim.x;
-    const uint32_t j0 = threadIdx.x * per, j1 = j0 + per < nf ? j0 + per : nf;
+    // exclusive scan of cnt[0..nb): each thread scans a contiguous slice
+    const uint32_t per = (nb + blockDim.x - 1) / blockDim.x;
+    const uint32_t j0 = threadIdx.x * per, j1 = j0 + per < nb ? j0 + per : nb;
     unsigned long long sum = 0;
     for (uint32_t j = j0; j < j1; ++j) sum += cnt[j];
     part[threadIdx.x] = sum;
@@ -651,31 +637,34 @@ __global__ __launch_bounds__(256) void k_scatter2(const Rec<I> *__restrict__ tmp
     unsigned long long run = s + (threadIdx.x ? part[threadIdx.x - 1] : 0ull);
     for (uint32_t j = j0; j < j1; ++j) {
         fbase[j] = run;
-        off[f0 + j] = run;
+        off[b0 + j] = run;
         run += cnt[j];
         cnt[j] = 0;
     }
-    if (f0 + nf == p && threadIdx.x == 0) off[p] = e;
+    if (b0 + nb == p && threadIdx.x == 0) off[p] = e;
     __syncthreads();
     for (uint64_t i0 = s; i0 < e; i0 += 8 * blockDim.x) {
-        Rec<I> r[8];
+        uint64_t kk[8];
+        I ix[8];
+        uint32_t f[8];
         unsigned long long slot[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             uint64_t i = i0 + u * blockDim.x + threadIdx.x;
-            if (i < e) r[u] = tmp[i];
-            else r[u].bid = 0xFFFFFFFFu;
+            f[u] = 0xFFFFFFFFu;
+            if (i < e) {
+                kk[u] = tk[i];
+                ix[u] = ti[i];
+                f[u] = fid[i];
+            }
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            uint32_t j = r[u].bid - f0;
-            slot[u] = r[u].bid != 0xFFFFFFFFu ? fbase[j] + atomicAdd(&cnt[j], 1u) : 0ull;
-        }
+        for (int u = 0; u < 8; ++u) slot[u] = f[u] != 0xFFFFFFFFu ? fbase[f[u]] + atomicAdd(&cnt[f[u]], 1u) : 0ull;
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-            if (r[u].bid != 0xFFFFFFFFu) {
-                keys[slot[u]] = r[u].key;
-                idx[slot[u]] = r[u].idx;
+            if (f[u] != 0xFFFFFFFFu) {
+                keys[slot[u]] = kk[u];
+                idx[slot[u]] = ix[u];
             }
     }
 }
