@@ -220,12 +220,15 @@ def main() -> int:
         print(json.dumps({"error": "no GPU"}))
         return 1
     if args.copy_probe:
+        # 1 GiB read + 1 GiB written per launch: a device copy (runtime copy
+        # kernel) and an elementwise kernel (src + 0 into dst)
         src = torch.ones(1 << 28, dtype=torch.int32, device="cuda:0")
         dst = torch.empty_like(src)
         for _ in range(3):
             dst.copy_(src)
+            torch.add(src, 0, out=dst)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes": 2 * src.numel() * 4}))
+        print(json.dumps({"copy_probe_bytes_read": src.numel() * 4, "copy_probe_bytes_written": src.numel() * 4}))
         return 0
     kind, n_default, desc = WORKLOADS[args.workload]
     if kind == "batch":

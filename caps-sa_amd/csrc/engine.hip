@@ -536,22 +536,21 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
 
     // ---- two-level scatter ----
     uint64_t *keys = (uint64_t *)W.keys.get(n_pos * 8);
-    uint64_t *tk = (uint64_t *)W.recs.get(n_pos * (8 + sizeof(I)));
-    I *ti = (I *)(tk + n_pos);
+    KRec<I> *tmp = (KRec<I> *)W.recs.get(n_pos * sizeof(KRec<I>));
     kt.begin(&stats.k_scatter);
     // chunk per scatter block: a multiple of the key window, about one resident wave of blocks
     const uint64_t schunk =
         std::max<uint64_t>(kSeqChunk, ceil_div(ceil_div(n_pos, (uint64_t)W.n_cu * 2), kSeqChunk) * kSeqChunk);
     const unsigned g1 = (unsigned)ceil_div(n_pos, schunk);
     if (seq)
-        k_scatter1<S, I, true><<<g1, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, cid, pc, cursor, tk,
-                                                   ti, schunk);
+        k_scatter1<S, I, true><<<g1, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, cid, pc, cursor, tmp,
+                                                   schunk);
     else
-        k_scatter1<S, I, false><<<g1, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, cid, pc, cursor, tk,
-                                                    ti, schunk);
+        k_scatter1<S, I, false><<<g1, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, cid, pc, cursor, tmp,
+                                                    schunk);
     LAUNCHCHK();
     // the coarse ids are consumed: their buffer holds the per-record bucket within its interval
-    k_scatter2<I><<<pc, 256, 0, st>>>(tk, ti, cid, coff, SP, (uint32_t)p, off, keys, d_out);
+    k_scatter2<I><<<pc, 256, SP.stride * 20, st>>>(tmp, cid, coff, SP, (uint32_t)p, off, keys, d_out);
     LAUNCHCHK();
     kt.end();
     tm.mark(PH_SCATTER);

@@ -492,18 +492,25 @@ __global__ __launch_bounds__(1024) void k_scan_u32(const uint32_t *__restrict__ 
 // ---------------------------------------------------------------------------
 // Two-level scatter.  Pass 1: a block owns a chunk of positions, counts its
 // coarse intervals, reserves one run per interval with a single global atomic
-// each, and writes (key, position) into those runs (two arrays).  Pass 2: one
+// each, and writes (key, position) records into those runs.  Pass 2: one
 // workgroup per coarse interval stages the interval's inner splitters in LDS,
 // finds each record's bucket by binary lifting over them (kept as a u16 for
 // the placing pass), histograms and scans the buckets, writes the bucket
 // offsets, and places every record with LDS cursors.
 // ---------------------------------------------------------------------------
+// A record between the two scatter passes: one 16-byte store per record.
+template <typename I>
+struct KRec {
+    uint64_t key;
+    I idx;
+};
+
 template <typename S, typename I, bool SEQ>
 __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, KeyGeom g,
                                                   DevLimits L, uint64_t max_ctx, const I *__restrict__ pos,
                                                   uint64_t n_pos, const uint16_t *__restrict__ cid, uint32_t pc,
-                                                  unsigned long long *__restrict__ ccursor, uint64_t *__restrict__ tk,
-                                                  I *__restrict__ ti, uint64_t chunk) {
+                                                  unsigned long long *__restrict__ ccursor, KRec<I> *__restrict__ tmp,
+                                                  uint64_t chunk) {
     __shared__ S win[SEQ ? kSeqChunk + 72 : 1];
     __shared__ uint64_t skey[SEQ ? kSeqChunk : 1];
     __shared__ unsigned long long cbase[kCoarseBucketsMax];
@@ -552,10 +559,7 @@ __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint6
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 uint32_t i = u * 256 + threadIdx.x;
-                if (i < cnt) {
-                    tk[slot[u]] = skey[i];
-                    ti[slot[u]] = (I)(base + i);
-                }
+                if (i < cnt) tmp[slot[u]] = KRec<I>{skey[i], (I)(base + i)};
             }
             __syncthreads();
         }
@@ -566,22 +570,21 @@ __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint6
             unsigned long long slot = cbase[c] + atomicAdd(&lcnt[c], 1u);
             uint64_t seg;
             uint64_t lim = lim_seg(L, p, &seg);
-            tk[slot] = with_id(g, key_at(t, cm, g, p, 0, lim < max_ctx ? lim : max_ctx), seg);
-            ti[slot] = (I)p;
+            tmp[slot] = KRec<I>{with_id(g, key_at(t, cm, g, p, 0, lim < max_ctx ? lim : max_ctx), seg), (I)p};
         }
     }
 }
 
 template <typename I>
-__global__ __launch_bounds__(256) void k_scatter2(const uint64_t *__restrict__ tk, const I *__restrict__ ti,
-                                                  uint16_t *__restrict__ fid, const uint64_t *__restrict__ coff,
+__global__ __launch_bounds__(256) void k_scatter2(const KRec<I> *__restrict__ tmp, uint16_t *__restrict__ fid, const uint64_t *__restrict__ coff,
                                                   Splitters SP, uint32_t p, uint64_t *__restrict__ off,
                                                   uint64_t *__restrict__ keys, I *__restrict__ idx) {
-    constexpr uint32_t FMAX = kFineMax;
-    __shared__ uint64_t sspl[FMAX];
-    __shared__ uint32_t cnt[FMAX];
-    __shared__ unsigned long long fbase[FMAX];
+    // LDS sized by the launch for SP.stride buckets: inner splitters, cursors, counts
+    extern __shared__ unsigned long long scatter2_lds[];
     __shared__ unsigned long long part[256];
+    uint64_t *sspl = (uint64_t *)scatter2_lds;
+    unsigned long long *fbase = scatter2_lds + SP.stride;
+    uint32_t *cnt = (uint32_t *)(scatter2_lds + 2 * SP.stride);
     const uint32_t c = blockIdx.x;
     const uint64_t s = coff[c], e = coff[c + 1];
     // buckets of interval c: b0 .. b0 + nb - 1; its inner splitters spl[b0 .. b0 + nb - 1)
@@ -601,7 +604,7 @@ __global__ __launch_bounds__(256) void k_scatter2(const uint64_t *__restrict__ t
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             uint64_t i = i0 + u * blockDim.x + threadIdx.x;
-            kk[u] = i < e ? tk[i] : 0ull;
+            kk[u] = i < e ? tmp[i].key : 0ull;
             f[u] = 0;
         }
         for (uint32_t step = fsteps ? 1u << (fsteps - 1) : 0; step; step >>= 1) {
@@ -653,8 +656,9 @@ __global__ __launch_bounds__(256) void k_scatter2(const uint64_t *__restrict__ t
             uint64_t i = i0 + u * blockDim.x + threadIdx.x;
             f[u] = 0xFFFFFFFFu;
             if (i < e) {
-                kk[u] = tk[i];
-                ix[u] = ti[i];
+                const KRec<I> r = tmp[i];
+                kk[u] = r.key;
+                ix[u] = r.idx;
                 f[u] = fid[i];
             }
         }
@@ -685,9 +689,16 @@ __global__ void k_tile_emit(const uint64_t *__restrict__ off, uint32_t p, uint32
                             uint64_t cap, unsigned int *__restrict__ tcount, Range *__restrict__ big,
                             unsigned int *__restrict__ n_big, unsigned int *__restrict__ max_bucket) {
     uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t s = b < p ? off[b] : 0, sz = b < p ? off[b + 1] - s : 0;
+    // largest bucket: wavefront maximum first, one atomic per wavefront
+    uint32_t mx = (uint32_t)(sz < 0xFFFFFFFFull ? sz : 0xFFFFFFFFull);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        uint32_t v = __shfl_xor(mx, o);
+        mx = v > mx ? v : mx;
+    }
+    if ((threadIdx.x & 63) == 0 && mx) atomicMax(max_bucket, mx);
     if (b >= p) return;
-    uint64_t s = off[b], sz = off[b + 1] - s;
-    atomicMax(max_bucket, (unsigned int)(sz < 0xFFFFFFFFull ? sz : 0xFFFFFFFFull));
     for (uint64_t x = 0; x < sz; x += T) {
         Tile tl;
         tl.start = s + x;
