@@ -47,15 +47,34 @@ def make_text(kind: str, n: int, seed: int):
     return gen.uniform_u8(n, 4, seed=seed)
 
 
-# Algorithmic HBM bytes per suffix of each kernel (u8 text, index of I bytes).
+# Algorithmic HBM bytes per suffix of each single-launch kernel (u8 text,
+# index of I bytes): what the kernel must read and write at the least.
 def algorithmic_bytes_per_suffix(kernel: str, I: int) -> float:
     return {
-        "k_classify": 1 + 4,            # text once + bucket id write
-        "k_scatter": 1 + 4 + 8 + I,     # text once + bucket id read + (key, idx) write
-        "k_tile_sort": 2 * (8 + I),     # (key, idx) read + write
-        "k_detect": 8,                  # keys read once
-        "k_boundary_lcp": 8 + I + I,    # keys, idx, lcp write
+        "k_classify": 1 + 2,            # text once + coarse interval id (u16) write
+        "k_scatter1": 1 + 2 + 8 + I,    # text once + interval id read + (key, position) record write
+        "k_scatter2": 2 * (8 + I),      # record read + record written to its bucket
     }.get(kernel, 0.0)
+
+
+# rocprofv3 kernel names of those kernels on the default workload.
+ROCPROF_NAMES = {"k_classify": "k_classify_seq<u8, u32, 0>", "k_scatter1": "k_scatter1<u8, u32, true>",
+                 "k_scatter2": "k_scatter2<u32>"}
+
+
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE runs
+    (profiles/hbm_traffic.json), each scaled by what the counter reads for a
+    plain 1 GiB copy; None when no measurement is committed."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        k = d["workloads"][workload][ROCPROF_NAMES[kernel]]
+        cal = d["calibration"]
+        return round(k["fetch_kib"] * 1024 * cal["fetch"] + k["write_kib"] * 1024 * cal["write"])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(text, seconds_budget: float, threads: int):
@@ -167,8 +186,8 @@ def run_batch(args, count: int) -> dict:
             k += 1
         cpu = {"value": round(done / t_cpu / 1e6, 3), "unit": "M suffixes/s", "cores": args.cpu_threads,
                "kind": "port", "sample": f"build_in_memory on {k} texts of the same batch ({done} symbols)"}
-    kern = ["k_classify", "k_scatter", "k_tile_sort", "k_detect", "k_pairs", "k_groups_small", "k_groups_mid",
-            "k_boundary_lcp"]
+    kern = ["k_classify", "k_scatter1", "k_scatter2", "k_scatter", "k_tile_sort", "k_pairs", "k_groups_small",
+            "k_groups_mid", "k_boundary_lcp"]
     avg = {kk: sum(sum(x[kk] for x in st) for st in stats) / len(stats) for kk in kern}
     phases = ("ms_alphabet", "ms_sample", "ms_classify", "ms_scatter", "ms_tile_sort", "ms_merge", "ms_detect",
               "ms_refine", "ms_giant")
@@ -236,7 +255,7 @@ def main() -> int:
         n = r["n"]
         value = n / (r["ms_step"] / 1e3) / 1e6
         avg = r["kernels"]
-        dom = max(avg, key=lambda k: avg[k] if algorithmic_bytes_per_suffix(k, 4) else -1)
+        dom = max(("k_classify", "k_scatter1", "k_scatter2"), key=lambda k: avg[k])
         bpsuf = algorithmic_bytes_per_suffix(dom, 4)
         achieved = n * bpsuf / (avg[dom] / 1e3) / 1e9 if avg[dom] else 0.0
         line = {
@@ -286,16 +305,20 @@ def main() -> int:
     print(f"[bench] {args.steps} steps: {ms_step:.2f} ms/step (device events {ev_ms:.2f} ms)", file=sys.stderr,
           flush=True)
 
-    # roofline of the dominant kernel
-    kernels = ["k_classify", "k_scatter", "k_tile_sort", "k_detect", "k_pairs", "k_groups_small", "k_groups_mid",
-               "k_boundary_lcp"]
+    # roofline of the dominant single-launch kernel (HIP events on the build's stream)
+    kernels = ["k_classify", "k_scatter1", "k_scatter2", "k_scatter", "k_tile_sort", "k_pairs", "k_groups_small",
+               "k_groups_mid", "k_boundary_lcp"]
     avg = {k: sum(s[k] for s in stats) / len(stats) for k in kernels}
-    dom = max(kernels, key=lambda k: avg[k] if algorithmic_bytes_per_suffix(k, 4) else -1)
-    bpsuf = algorithmic_bytes_per_suffix(dom, 4)
-    achieved = n * bpsuf / (avg[dom] / 1e3) / 1e9
-    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "bytes_per_suffix": bpsuf, "kernel_ms": round(avg[dom], 4)}
+    rooflines = {}
+    for k in ("k_classify", "k_scatter1", "k_scatter2"):
+        bpsuf = algorithmic_bytes_per_suffix(k, 4)
+        achieved = n * bpsuf / (avg[k] / 1e3) / 1e9 if avg[k] else 0.0
+        rooflines[k] = {"bound": "hbm", "kernel": k, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": measured_traffic(args.workload, k) if args.router == "bsearch" else None,
+                        "bytes_per_suffix": bpsuf, "kernel_ms": round(avg[k], 4)}
+    dom = max(rooflines, key=lambda k: avg[k])
+    roof = rooflines[dom]
 
     verified = None
     if not args.no_verify:
@@ -343,6 +366,7 @@ def main() -> int:
             "fallback_frac": round(last["router_fallbacks"] / max(last["router_count"], 1), 6),
             "train_and_sample_ms": round(sum(s["ms_sample"] for s in stats) / len(stats), 3)},
         "roofline": roof,
+        "rooflines": rooflines,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

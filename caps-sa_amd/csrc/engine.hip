@@ -537,7 +537,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     // ---- two-level scatter ----
     uint64_t *keys = (uint64_t *)W.keys.get(n_pos * 8);
     KRec<I> *tmp = (KRec<I> *)W.recs.get(n_pos * sizeof(KRec<I>));
-    kt.begin(&stats.k_scatter);
+    kt.begin(&stats.k_scatter1);
     // chunk per scatter block: a multiple of the key window, about one resident wave of blocks
     const uint64_t schunk =
         std::max<uint64_t>(kSeqChunk, ceil_div(ceil_div(n_pos, (uint64_t)W.n_cu * 2), kSeqChunk) * kSeqChunk);
@@ -549,7 +549,9 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         k_scatter1<S, I, false><<<g1, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, cid, pc, cursor, tmp,
                                                     schunk);
     LAUNCHCHK();
+    kt.end();
     // the coarse ids are consumed: their buffer holds the per-record bucket within its interval
+    kt.begin(&stats.k_scatter2);
     k_scatter2<I><<<pc, 256, SP.stride * 20, st>>>(tmp, cid, coff, SP, (uint32_t)p, off, keys, d_out);
     LAUNCHCHK();
     kt.end();
@@ -779,6 +781,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     HIPCHK(hipStreamSynchronize(st));
     tm.collect(stats);
     kt.collect();
+    stats.k_scatter = stats.k_scatter1 + stats.k_scatter2;
 }
 
 // ---------------------------------------------------------------------------

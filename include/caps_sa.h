@@ -180,6 +180,7 @@ typedef struct caps_stats {
     uint64_t router_err_sum, router_err_max, router_fallbacks, router_count;
     uint32_t key_bits, key_k;
     uint64_t bucket_max; /* records in the largest bucket (imbalance = bucket_max * buckets / n) */
+    double k_scatter1, k_scatter2; /* the two scatter passes (their sum is k_scatter) */
 } caps_stats_t;
 int caps_last_stats(caps_stats_t *out);
 
