@@ -552,7 +552,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     kt.end();
     // the coarse ids are consumed: their buffer holds the per-record bucket within its interval
     kt.begin(&stats.k_scatter2);
-    k_scatter2<I><<<pc, 256, SP.stride * 20, st>>>(tmp, cid, coff, SP, (uint32_t)p, off, keys, d_out);
+    k_scatter2<I><<<pc, 256, SP.stride * 28, st>>>(tmp, cid, coff, SP, (uint32_t)p, off, keys, d_out);
     LAUNCHCHK();
     kt.end();
     tm.mark(PH_SCATTER);

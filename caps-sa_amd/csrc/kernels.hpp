@@ -575,16 +575,53 @@ __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint6
     }
 }
 
+// Exclusive scan of cnt[0..nb) into out[0..nb) (+ base) across a 256-thread
+// block; each thread scans a contiguous slice.  Returns the total in every
+// thread.  part: 256 u64 of LDS.
+__device__ __forceinline__ unsigned long long block_scan_slices(const uint32_t *cnt, uint32_t nb,
+                                                                unsigned long long base,
+                                                                unsigned long long *out,
+                                                                unsigned long long *part) {
+    const uint32_t per = (nb + blockDim.x - 1) / blockDim.x;
+    const uint32_t j0 = threadIdx.x * per, j1 = j0 + per < nb ? j0 + per : nb;
+    unsigned long long sum = 0;
+    for (uint32_t j = j0; j < j1; ++j) sum += cnt[j];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t o = 1; o < 256; o <<= 1) {
+        unsigned long long v = threadIdx.x >= o ? part[threadIdx.x - o] : 0ull;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    unsigned long long run = base + (threadIdx.x ? part[threadIdx.x - 1] : 0ull);
+    for (uint32_t j = j0; j < j1; ++j) {
+        out[j] = run;
+        run += cnt[j];
+    }
+    unsigned long long total = part[255];
+    __syncthreads();
+    return total;
+}
+
+constexpr uint32_t kScatterTile = 2048;  // records staged per placing step of pass 2
+
 template <typename I>
-__global__ __launch_bounds__(256) void k_scatter2(const KRec<I> *__restrict__ tmp, uint16_t *__restrict__ fid, const uint64_t *__restrict__ coff,
-                                                  Splitters SP, uint32_t p, uint64_t *__restrict__ off,
-                                                  uint64_t *__restrict__ keys, I *__restrict__ idx) {
-    // LDS sized by the launch for SP.stride buckets: inner splitters, cursors, counts
+__global__ __launch_bounds__(256) void k_scatter2(const KRec<I> *__restrict__ tmp, uint16_t *__restrict__ fid,
+                                                  const uint64_t *__restrict__ coff, Splitters SP, uint32_t p,
+                                                  uint64_t *__restrict__ off, uint64_t *__restrict__ keys,
+                                                  I *__restrict__ idx) {
+    // LDS sized by the launch for SP.stride buckets: inner splitters, global
+    // cursors, tile offsets, counts
     extern __shared__ unsigned long long scatter2_lds[];
     __shared__ unsigned long long part[256];
+    __shared__ uint64_t tkey[kScatterTile];
+    __shared__ I tidx[kScatterTile];
+    __shared__ uint16_t tf[kScatterTile];
     uint64_t *sspl = (uint64_t *)scatter2_lds;
-    unsigned long long *fbase = scatter2_lds + SP.stride;
-    uint32_t *cnt = (uint32_t *)(scatter2_lds + 2 * SP.stride);
+    unsigned long long *gcur = scatter2_lds + SP.stride;
+    unsigned long long *lbase = scatter2_lds + 2 * SP.stride;
+    uint32_t *cnt = (uint32_t *)(scatter2_lds + 3 * SP.stride);
     const uint32_t c = blockIdx.x;
     const uint64_t s = coff[c], e = coff[c + 1];
     // buckets of interval c: b0 .. b0 + nb - 1; its inner splitters spl[b0 .. b0 + nb - 1)
@@ -598,6 +635,7 @@ __global__ __launch_bounds__(256) void k_scatter2(const KRec<I> *__restrict__ tm
         if (j < ninner) sspl[j] = SP.spl[b0 + j];
     }
     __syncthreads();
+    // counting pass: bucket of every record (kept in fid) and bucket sizes
     for (uint64_t i0 = s; i0 < e; i0 += 8 * blockDim.x) {
         uint64_t kk[8];
         uint32_t f[8];
@@ -624,52 +662,54 @@ __global__ __launch_bounds__(256) void k_scatter2(const KRec<I> *__restrict__ tm
         }
     }
     __syncthreads();
-    // exclusive scan of cnt[0..nb): each thread scans a contiguous slice
-    const uint32_t per = (nb + blockDim.x - 1) / blockDim.x;
-    const uint32_t j0 = threadIdx.x * per, j1 = j0 + per < nb ? j0 + per : nb;
-    unsigned long long sum = 0;
-    for (uint32_t j = j0; j < j1; ++j) sum += cnt[j];
-    part[threadIdx.x] = sum;
-    __syncthreads();
-    for (uint32_t o = 1; o < 256; o <<= 1) {
-        unsigned long long v = threadIdx.x >= o ? part[threadIdx.x - o] : 0ull;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    unsigned long long run = s + (threadIdx.x ? part[threadIdx.x - 1] : 0ull);
-    for (uint32_t j = j0; j < j1; ++j) {
-        fbase[j] = run;
-        off[b0 + j] = run;
-        run += cnt[j];
+    block_scan_slices(cnt, nb, s, gcur, part);
+    for (uint32_t j = threadIdx.x; j < nb; j += blockDim.x) {
+        off[b0 + j] = gcur[j];
         cnt[j] = 0;
     }
     if (b0 + nb == p && threadIdx.x == 0) off[p] = e;
     __syncthreads();
-    for (uint64_t i0 = s; i0 < e; i0 += 8 * blockDim.x) {
-        uint64_t kk[8];
-        I ix[8];
-        uint32_t f[8];
-        unsigned long long slot[8];
+    // placing pass, one tile of kScatterTile records at a time: count the
+    // tile's buckets, scan, stage the records in LDS grouped by bucket, then
+    // write each bucket's run of the tile with consecutive threads
+    for (uint64_t t0 = s; t0 < e; t0 += kScatterTile) {
+        const uint32_t tl = (uint32_t)(e - t0 < kScatterTile ? e - t0 : kScatterTile);
+        constexpr int U = kScatterTile / 256;
+        uint32_t f[U];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            uint64_t i = i0 + u * blockDim.x + threadIdx.x;
-            f[u] = 0xFFFFFFFFu;
-            if (i < e) {
-                const KRec<I> r = tmp[i];
-                kk[u] = r.key;
-                ix[u] = r.idx;
-                f[u] = fid[i];
+        for (int u = 0; u < U; ++u) {
+            uint32_t q = u * 256 + threadIdx.x;
+            f[u] = q < tl ? fid[t0 + q] : 0xFFFFFFFFu;
+            if (q < tl) atomicAdd(&cnt[f[u]], 1u);
+        }
+        __syncthreads();
+        block_scan_slices(cnt, nb, 0, lbase, part);
+        for (uint32_t j = threadIdx.x; j < nb; j += blockDim.x) cnt[j] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            uint32_t q = u * 256 + threadIdx.x;
+            if (q < tl) {
+                const KRec<I> r = tmp[t0 + q];
+                const uint32_t slot = (uint32_t)lbase[f[u]] + atomicAdd(&cnt[f[u]], 1u);
+                tkey[slot] = r.key;
+                tidx[slot] = r.idx;
+                tf[slot] = (uint16_t)f[u];
             }
         }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) slot[u] = f[u] != 0xFFFFFFFFu ? fbase[f[u]] + atomicAdd(&cnt[f[u]], 1u) : 0ull;
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (f[u] != 0xFFFFFFFFu) {
-                keys[slot[u]] = kk[u];
-                idx[slot[u]] = ix[u];
-            }
+        __syncthreads();
+        for (uint32_t q = threadIdx.x; q < tl; q += 256) {
+            const uint32_t fb = tf[q];
+            const uint64_t dst = gcur[fb] + (q - lbase[fb]);
+            keys[dst] = tkey[q];
+            idx[dst] = tidx[q];
+        }
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < nb; j += blockDim.x) {
+            gcur[j] += cnt[j];
+            cnt[j] = 0;
+        }
+        __syncthreads();
     }
 }
 
@@ -698,17 +738,30 @@ __global__ void k_tile_emit(const uint64_t *__restrict__ off, uint32_t p, uint32
         mx = v > mx ? v : mx;
     }
     if ((threadIdx.x & 63) == 0 && mx) atomicMax(max_bucket, mx);
-    if (b >= p) return;
-    for (uint64_t x = 0; x < sz; x += T) {
-        Tile tl;
-        tl.start = s + x;
-        tl.len = (uint32_t)(sz - x < T ? sz - x : T);
-        tl.whole = sz <= T ? 1u : 0u;
-        int c = tile_class(tl.len);
-        unsigned int slot = atomicAdd(&tcount[c], 1u);
-        tiles[(uint64_t)c * cap + slot] = tl;
+    // tiles: one list slot reservation per wavefront and class (ballot + popcount)
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t below = (1ull << lane) - 1;
+    for (uint64_t x = 0; __any(x < sz); x += T) {
+        const bool have = x < sz;
+        Tile tl{};
+        int cls = -1;
+        if (have) {
+            tl.start = s + x;
+            tl.len = (uint32_t)(sz - x < T ? sz - x : T);
+            tl.whole = sz <= T ? 1u : 0u;
+            cls = tile_class(tl.len);
+        }
+        for (int c = 0; c < kTileClasses; ++c) {
+            const uint64_t m = __ballot(cls == c);
+            if (!m) continue;
+            const uint32_t first = (uint32_t)__ffsll((unsigned long long)m) - 1;
+            unsigned int base = 0;
+            if (lane == first) base = atomicAdd(&tcount[c], (unsigned)__popcll(m));
+            base = __shfl(base, (int)first);
+            if (cls == c) tiles[(uint64_t)c * cap + base + __popcll(m & below)] = tl;
+        }
     }
-    if (sz > T) {
+    if (b < p && sz > T) {
         unsigned int q = atomicAdd(n_big, 1u);
         big[q].start = s;
         big[q].len = sz;

@@ -152,6 +152,25 @@ __global__ __launch_bounds__(256) void k_pairs4(const S *__restrict__ t, DevLimi
     uint64_t ob = (known < cap ? known : cap) * SZ;  // bytes known equal
     bool done = !valid || ob >= capb;
     uint64_t resb = capb;
+    // first step: only the pair's first lane compares 16 bytes (one cache
+    // line per suffix in most cases; most pairs part there); then all four
+    // lanes, 64 bytes per step
+    if (!done && sl == 0) {
+        uint64_t alo, ahi, blo, bhi;
+        load16(tb + a * SZ + ob, enda, alo, ahi);
+        load16(tb + b * SZ + ob, endb, blo, bhi);
+        uint64_t d0 = alo ^ blo, d1 = ahi ^ bhi;
+        uint32_t fb = d0 ? (uint32_t)__builtin_ctzll(d0) >> 3 : d1 ? 8 + ((uint32_t)__builtin_ctzll(d1) >> 3) : 16u;
+        uint64_t in_win = capb - ob;
+        if (in_win < fb) fb = (uint32_t)in_win;
+        if (fb < 16) {
+            resb = ob + fb;
+            done = true;
+        }
+    }
+    done = __shfl(done ? 1 : 0, (int)(lane & ~3u)) != 0;
+    resb = shfl64(resb, (int)(lane & ~3u));
+    if (!done) ob += 16;
     while (__any(!done)) {
         uint32_t fb = 16;
         if (!done) {
