@@ -194,62 +194,44 @@ __device__ __forceinline__ bool rec_less(uint64_t ka, uint32_t la, uint64_t kb, 
     return ka < kb || (ka == kb && la < lb);
 }
 
-template <typename I, int T>
-__global__ __launch_bounds__(256) void k_tile_merge(uint64_t *__restrict__ keys, I *__restrict__ idx,
-                                                    const Tile *__restrict__ tiles, GroupLists gl, int emit) {
-    constexpr int B = 256;
-    constexpr int ITEMS = T / B;
-    constexpr int PADT = T + B;
-    __shared__ uint64_t sk[PADT];
-    __shared__ uint16_t sl[PADT];
-    __shared__ I si[T];
-    __shared__ uint32_t gcnt[kGroupClasses], gbase[kGroupClasses];
+// Block merge sort of B*ITEMS records held blocked in registers (thread t
+// holds records t*ITEMS .. t*ITEMS+ITEMS-1), through the padded LDS arrays
+// sk/sl.  PACKED: the records are unique u64 values (key bits and tile index
+// packed together; li unused); otherwise (key, li) pairs compared as above.
+template <int B, int ITEMS, bool PACKED>
+__device__ __forceinline__ void block_merge_sort(uint64_t (&k)[ITEMS], uint32_t (&li)[ITEMS], uint64_t *sk,
+                                                 uint16_t *sl) {
+    constexpr uint32_t T = B * ITEMS;
     auto px = [](uint32_t i) { return i + i / ITEMS; };
-
-    const Tile tl = tiles[blockIdx.x];
-    const uint32_t len = tl.len;
+    auto less = [](uint64_t ka, uint32_t la, uint64_t kb, uint32_t lb) {
+        return PACKED ? ka < kb : rec_less(ka, la, kb, lb);
+    };
     const int tid = threadIdx.x;
-    if (len == 0) return;
-    for (uint32_t i = tid; i < (uint32_t)T; i += B) {
-        if (i < len) {
-            sk[px(i)] = keys[tl.start + i];
-            si[i] = idx[tl.start + i];
-        } else {
-            sk[px(i)] = ~0ull;
-        }
-        sl[px(i)] = (uint16_t)i;
-    }
-    __syncthreads();
-    uint64_t k[ITEMS];
-    uint32_t li[ITEMS];
-#pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-        k[j] = sk[px(tid * ITEMS + j)];
-        li[j] = sl[px(tid * ITEMS + j)];
-    }
     // register sort: odd-even transposition network over ITEMS records
 #pragma unroll
     for (int round = 0; round < ITEMS; ++round) {
 #pragma unroll
         for (int j = round & 1; j + 1 < ITEMS; j += 2) {
-            if (rec_less(k[j + 1], li[j + 1], k[j], li[j])) {
+            if (less(k[j + 1], li[j + 1], k[j], li[j])) {
                 uint64_t tk = k[j];
                 k[j] = k[j + 1];
                 k[j + 1] = tk;
-                uint32_t tli = li[j];
-                li[j] = li[j + 1];
-                li[j + 1] = tli;
+                if (!PACKED) {
+                    uint32_t tli = li[j];
+                    li[j] = li[j + 1];
+                    li[j + 1] = tli;
+                }
             }
         }
     }
     // merge levels: runs of w records -> runs of 2w
 #pragma unroll 1
-    for (uint32_t w = ITEMS; w < (uint32_t)T; w *= 2) {
+    for (uint32_t w = ITEMS; w < T; w *= 2) {
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
             sk[px(tid * ITEMS + j)] = k[j];
-            sl[px(tid * ITEMS + j)] = (uint16_t)li[j];
+            if (!PACKED) sl[px(tid * ITEMS + j)] = (uint16_t)li[j];
         }
         __syncthreads();
         const uint32_t per = (2 * w) / ITEMS;  // threads per merged run
@@ -259,36 +241,110 @@ __global__ __launch_bounds__(256) void k_tile_merge(uint64_t *__restrict__ keys,
         while (lo < hi) {
             uint32_t mid = (lo + hi) >> 1;
             uint32_t bi = b0 + diag - 1 - mid;
-            if (!rec_less(sk[px(bi)], sl[px(bi)], sk[px(a0 + mid)], sl[px(a0 + mid)])) lo = mid + 1;
+            const bool bl = PACKED ? sk[px(bi)] < sk[px(a0 + mid)]
+                                   : rec_less(sk[px(bi)], sl[px(bi)], sk[px(a0 + mid)], sl[px(a0 + mid)]);
+            if (!bl) lo = mid + 1;
             else hi = mid;
         }
         uint32_t a = a0 + lo, b = b0 + (diag - lo);
         const uint32_t ae = a0 + w, be = b0 + w;
         uint64_t ka = a < ae ? sk[px(a)] : ~0ull, kb = b < be ? sk[px(b)] : ~0ull;
-        uint32_t la = a < ae ? sl[px(a)] : 0xFFFFFFFFu, lb = b < be ? sl[px(b)] : 0xFFFFFFFFu;
+        uint32_t la = 0, lb = 0;
+        if (!PACKED) {
+            la = a < ae ? sl[px(a)] : 0xFFFFFFFFu;
+            lb = b < be ? sl[px(b)] : 0xFFFFFFFFu;
+        }
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
-            const bool take_a = b >= be || (a < ae && !rec_less(kb, lb, ka, la));
+            const bool take_a = b >= be || (a < ae && !less(kb, lb, ka, la));
             if (take_a) {
                 k[j] = ka;
-                li[j] = la;
+                if (!PACKED) li[j] = la;
                 ++a;
                 ka = a < ae ? sk[px(a)] : ~0ull;
-                la = a < ae ? sl[px(a)] : 0xFFFFFFFFu;
+                if (!PACKED) la = a < ae ? sl[px(a)] : 0xFFFFFFFFu;
             } else {
                 k[j] = kb;
-                li[j] = lb;
+                if (!PACKED) li[j] = lb;
                 ++b;
                 kb = b < be ? sk[px(b)] : ~0ull;
-                lb = b < be ? sl[px(b)] : 0xFFFFFFFFu;
+                if (!PACKED) lb = b < be ? sl[px(b)] : 0xFFFFFFFFu;
             }
         }
     }
     __syncthreads();
+}
+
+template <typename I, int T>
+__global__ __launch_bounds__(256) void k_tile_merge(uint64_t *__restrict__ keys, I *__restrict__ idx,
+                                                    const Tile *__restrict__ tiles, GroupLists gl, int emit) {
+    constexpr int B = 256;
+    constexpr int ITEMS = T / B;
+    constexpr int PADT = T + B;
+    constexpr int LB = __builtin_ctz(T);  // tile index bits
+    __shared__ uint64_t sk[PADT];
+    __shared__ uint16_t sl[PADT];
+    __shared__ I si[T];
+    __shared__ unsigned long long s_or;
+    __shared__ uint32_t gcnt[kGroupClasses], gbase[kGroupClasses];
+    auto px = [](uint32_t i) { return i + i / ITEMS; };
+
+    const Tile tl = tiles[blockIdx.x];
+    const uint32_t len = tl.len;
+    const int tid = threadIdx.x;
+    if (len == 0) return;
+    const uint64_t first = keys[tl.start];
+    uint64_t my_or = 0;
+    for (uint32_t i = tid; i < (uint32_t)T; i += B) {
+        if (i < len) {
+            const uint64_t kk = keys[tl.start + i];
+            sk[px(i)] = kk;
+            si[i] = idx[tl.start + i];
+            my_or |= kk ^ first;
+        } else {
+            sk[px(i)] = ~0ull;
+        }
+        sl[px(i)] = (uint16_t)i;
+    }
+    if (tid == 0) s_or = 0;
+    __syncthreads();
+    if (my_or) atomicOr(&s_or, (unsigned long long)my_or);
+    __syncthreads();
+    const uint64_t diff = s_or;  // bits that vary inside the tile
+    const int hi = diff ? 64 - __builtin_clzll(diff) : 0;
+    uint64_t k[ITEMS];
+    uint32_t li[ITEMS];
+    if (diff == 0) {
+        // one key: already in order
+    } else if (hi + LB <= 64) {
+        // varying bits below `hi`, tile index below them: unique u64 records
+        // (a pad, ~0, is above every real record: when hi + LB = 64 a real
+        // record reaches ~0 only with index T-1, and then the tile is full)
+        const uint64_t mask = hi == 64 ? ~0ull : ((1ull << hi) - 1);
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-        sk[px(tid * ITEMS + j)] = k[j];
-        sl[px(tid * ITEMS + j)] = (uint16_t)li[j];
+        for (int j = 0; j < ITEMS; ++j) {
+            const uint32_t i = tid * ITEMS + j;
+            k[j] = i < len ? ((sk[px(i)] & mask) << LB) | i : ~0ull;
+        }
+        block_merge_sort<B, ITEMS, true>(k, li, sk, sl);
+        const uint64_t top = first & ~mask;
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            sk[px(tid * ITEMS + j)] = top | (k[j] >> LB);  // pads (positions >= len) are never read
+            sl[px(tid * ITEMS + j)] = (uint16_t)(k[j] & (T - 1));
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            k[j] = sk[px(tid * ITEMS + j)];
+            li[j] = sl[px(tid * ITEMS + j)];
+        }
+        block_merge_sort<B, ITEMS, false>(k, li, sk, sl);
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            sk[px(tid * ITEMS + j)] = k[j];
+            sl[px(tid * ITEMS + j)] = (uint16_t)li[j];
+        }
     }
     __syncthreads();
     for (uint32_t i = tid; i < len; i += B) {
