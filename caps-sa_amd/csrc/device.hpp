@@ -101,14 +101,21 @@ struct KeyGeom {
 
 template <typename S>
 struct CodeMap;
+// Symbol codes.  `packed`, when set, holds the whole text as codes of
+// g.bits (a power of two) each, MSB-first in u64 words (2^lgf codes a word,
+// zero words past the end): a key is then two aligned loads and a shift.
 template <>
 struct CodeMap<uint8_t> {
-    const uint8_t *code;  // nullptr = identity
+    const uint8_t *code = nullptr;  // nullptr = identity
+    const uint64_t *packed = nullptr;
+    uint32_t lgf = 0;
     __device__ __forceinline__ uint32_t operator()(uint8_t c) const { return code ? code[c] : c; }
 };
 template <>
 struct CodeMap<uint16_t> {
-    const uint16_t *code;  // nullptr = identity
+    const uint16_t *code = nullptr;  // nullptr = identity
+    const uint64_t *packed = nullptr;
+    uint32_t lgf = 0;
     __device__ __forceinline__ uint32_t operator()(uint16_t c) const { return code ? code[c] : c; }
 };
 
@@ -125,9 +132,33 @@ __device__ __forceinline__ uint64_t pad_fields(const KeyGeom &g, uint32_t m) {
 // aligned 8-byte words covering the window are loaded first (each holds at
 // least one in-range byte), then the fields are extracted with constant
 // indices, so the loads overlap instead of forming a dependent chain.
+// key_at from the packed code text (dense geometry: no bias, power-of-two
+// field width, so a word holds exactly 64 / bits codes).
+template <typename CM>
+__device__ __forceinline__ uint64_t key_at_packed(const CM &cm, const KeyGeom &g, uint64_t p, uint64_t d,
+                                                  uint64_t cap) {
+    const uint64_t rem = cap > d ? cap - d : 0;
+    const uint32_t take = rem < g.k ? (uint32_t)rem : g.k;
+    if (take == 0) return pad_fields(g, g.k) << g.shift;
+    const uint32_t bits = g.bits;
+    const uint64_t q = p + d;
+    const uint64_t w = q >> cm.lgf;
+    const uint32_t rb = (uint32_t)(q & ((1ull << cm.lgf) - 1)) * bits;
+    uint64_t v = cm.packed[w];
+    if (rb) v = (v << rb) | (cm.packed[w + 1] >> (64 - rb));
+    const uint32_t kb = g.k * bits;
+    uint64_t fields = kb >= 64 ? v : (v >> (64 - kb));
+    if (take < g.k) {
+        const uint32_t drop = (g.k - take) * bits;
+        fields = ((fields >> drop) << drop) | pad_fields(g, g.k - take);
+    }
+    return fields << g.shift;
+}
+
 template <typename S>
 __device__ __forceinline__ uint64_t key_at(const S *__restrict__ t, CodeMap<S> cm, const KeyGeom &g, uint64_t p,
                                            uint64_t d, uint64_t cap) {
+    if (cm.packed) return key_at_packed(cm, g, p, d, cap);
     constexpr uint32_t SZ = sizeof(S);
     constexpr int NW = 9;  // 64 bytes of window + up to 7 bytes of misalignment
     uint64_t rem = cap > d ? cap - d : 0;

@@ -161,8 +161,10 @@ Geometry compute_geometry(Workspace &W, const S *d_text, uint64_t n, uint32_t mo
         }
     }
     if (!ref) {
+        // power-of-two field width (1, 2, 4, 8, 16), as the reference's
+        // packer: codes then tile u64 words exactly
         uint32_t bits = 1;
-        while ((1ull << bits) < distinct) ++bits;
+        while ((1ull << bits) < distinct) bits <<= 1;
         G.g.bits = bits;
         G.g.k = 64 / bits;
         G.g.shift = 64 - G.g.k * bits;
@@ -452,7 +454,17 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         g.k = (64 - idbits) / g.bits;
         g.shift = 64 - idbits - g.k * g.bits;
     }
-    const CodeMap<S> cm = codemap<S>(G);
+    CodeMap<S> cm = codemap<S>(G);
+    if (!G.reference) {  // packed code text for the key gathers
+        uint32_t lgf = 0;
+        while ((g.bits << lgf) < 64) ++lgf;
+        const uint64_t nwords = (n >> lgf) + 2;
+        uint64_t *pk = (uint64_t *)W.packed.get(nwords * 8);
+        k_pack_codes<S><<<grid_for(nwords, 256, 8192), 256, 0, st>>>(d_text, n, cm, g.bits, lgf, pk, nwords);
+        LAUNCHCHK();
+        cm.packed = pk;
+        cm.lgf = lgf;
+    }
     stats.key_bits = g.bits;
     stats.key_k = g.k;
     tm.mark(PH_ALPHA);

@@ -34,7 +34,7 @@ struct Workspace {
     std::vector<uint64_t> h_ends;
     std::vector<uint32_t> h_dir;
     // key geometry
-    DevBuf bitmap, code;
+    DevBuf bitmap, code, packed;
     std::vector<uint8_t> h_code8;
     std::vector<uint16_t> h_code16;
     // sample and splitters

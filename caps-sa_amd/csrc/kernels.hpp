@@ -121,6 +121,25 @@ __device__ __forceinline__ uint32_t upper_bound_coarse(const Splitters &S, const
     return b;
 }
 
+// Packed code text: word w holds the codes of symbols w*F .. w*F+F-1
+// (F = 2^lgf = 64 / bits), the first in the most significant bits.
+template <typename S>
+__global__ void k_pack_codes(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, uint32_t bits, uint32_t lgf,
+                             uint64_t *__restrict__ out, uint64_t nwords) {
+    const uint32_t F = 1u << lgf;
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords;
+         w += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t j0 = w << lgf;
+        uint64_t v = 0;
+        for (uint32_t f = 0; f < F; ++f) {
+            const uint64_t j = j0 + f;
+            const uint32_t c = j < n ? cm(t[j]) : 0u;
+            v |= (uint64_t)c << (64 - (f + 1) * bits);
+        }
+        out[w] = v;
+    }
+}
+
 // The coarse splitter table: every stride-th splitter (what the learned router
 // is trained to index).
 __global__ void k_coarse_table(const uint64_t *__restrict__ spl, uint32_t stride, uint32_t n_coarse,
