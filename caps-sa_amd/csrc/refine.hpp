@@ -224,6 +224,113 @@ __global__ __launch_bounds__(256) void k_pairs4(const S *__restrict__ t, DevLimi
 constexpr uint64_t kAuxCont = ~0ull;
 
 // ---------------------------------------------------------------------------
+// Two-member groups on the packed code text: four lanes per pair, each lane
+// compares 128 bits of codes (128 / bits symbols) of both suffixes per step,
+// loaded as three aligned words and shifted; the first step uses the pair's
+// first lane only.  The first differing bit gives the first differing code,
+// so the LCP (in symbols) is exact; codes order like symbols.  A word is
+// loaded only if it holds a symbol inside the comparison window.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void load128_codes(const uint64_t *__restrict__ pk, uint32_t lgf, uint32_t bits,
+                                              uint64_t q, uint64_t end, uint64_t &hi, uint64_t &lo) {
+    const uint64_t w = q >> lgf;
+    const uint32_t rb = (uint32_t)(q & ((1ull << lgf) - 1)) * bits;
+    const uint64_t w0 = (w << lgf) < end ? pk[w] : 0ull;
+    const uint64_t w1 = ((w + 1) << lgf) < end ? pk[w + 1] : 0ull;
+    const uint64_t w2 = rb && ((w + 2) << lgf) < end ? pk[w + 2] : 0ull;
+    hi = rb ? (w0 << rb) | (w1 >> (64 - rb)) : w0;
+    lo = rb ? (w1 << rb) | (w2 >> (64 - rb)) : w1;
+}
+
+// first differing code of two 128-bit windows (128 / bits if none)
+__device__ __forceinline__ uint32_t first_diff_code(uint64_t ahi, uint64_t alo, uint64_t bhi, uint64_t blo,
+                                                    uint32_t bits) {
+    const uint64_t x = ahi ^ bhi, y = alo ^ blo;
+    if (x) return (uint32_t)__builtin_clzll(x) / bits;
+    if (y) return (64 + (uint32_t)__builtin_clzll(y)) / bits;
+    return 128 / bits;
+}
+
+template <typename S, typename I>
+__global__ __launch_bounds__(256) void k_pairs_packed(const S *__restrict__ t, const uint64_t *__restrict__ pk,
+                                                      uint32_t lgf, uint32_t bits, DevLimits L, uint64_t max_ctx,
+                                                      const PairRec<I> *__restrict__ pairs, uint32_t n_pairs,
+                                                      uint64_t known, I *__restrict__ idx, I *__restrict__ lcp) {
+    const uint32_t lane = threadIdx.x & 63, grp = lane >> 2, sl = lane & 3;
+    const uint32_t pid = blockIdx.x * 64 + (threadIdx.x >> 2);
+    const bool valid = pid < n_pairs;
+    const uint32_t FL = 128 / bits;  // codes per lane and step
+    uint64_t s = 0, a = 0, b = 0, la = 0, lb = 0, cap = 0;
+    if (valid) {
+        PairRec<I> pr = pairs[pid];
+        s = pr.start;
+        a = pr.a;
+        b = pr.b;
+        la = lim_at(L, a);
+        lb = lim_at(L, b);
+        uint64_t ca = la < max_ctx ? la : max_ctx, cb = lb < max_ctx ? lb : max_ctx;
+        cap = ca < cb ? ca : cb;
+    }
+    uint64_t o = known < cap ? known : cap;  // symbols known equal
+    bool done = !valid || o >= cap;
+    uint64_t res = cap;
+    if (!done && sl == 0) {
+        uint64_t ahi, alo, bhi, blo;
+        load128_codes(pk, lgf, bits, a + o, a + cap, ahi, alo);
+        load128_codes(pk, lgf, bits, b + o, b + cap, bhi, blo);
+        uint32_t fc = first_diff_code(ahi, alo, bhi, blo, bits);
+        const uint64_t in_win = cap - o;
+        if (in_win < fc) fc = (uint32_t)in_win;
+        if (fc < FL) {
+            res = o + fc;
+            done = true;
+        }
+    }
+    done = __shfl(done ? 1 : 0, (int)(lane & ~3u)) != 0;
+    res = shfl64(res, (int)(lane & ~3u));
+    if (!done) o += FL;
+    while (__any(!done)) {
+        uint32_t fc = FL;
+        if (!done) {
+            const uint64_t rel = o + (uint64_t)sl * FL;
+            if (rel < cap) {
+                uint64_t ahi, alo, bhi, blo;
+                load128_codes(pk, lgf, bits, a + rel, a + cap, ahi, alo);
+                load128_codes(pk, lgf, bits, b + rel, b + cap, bhi, blo);
+                fc = first_diff_code(ahi, alo, bhi, blo, bits);
+            } else {
+                fc = 0;
+            }
+            const uint64_t in_win = cap > rel ? cap - rel : 0;
+            if (in_win < fc) fc = (uint32_t)in_win;
+        }
+        const uint64_t mask = __ballot(!done && fc < FL);
+        const uint32_t gm = (uint32_t)(mask >> (grp * 4)) & 0xFu;
+        const int fl = gm ? __ffs(gm) - 1 : 0;
+        const uint32_t fq = __shfl(fc, (int)(grp * 4 + fl));
+        if (!done) {
+            if (gm) {
+                const uint64_t r = o + (uint64_t)fl * FL + fq;
+                res = r < cap ? r : cap;
+                done = true;
+            } else {
+                o += 4ull * FL;
+            }
+        }
+    }
+    if (valid && sl == 0) {
+        int c;
+        if (res < cap) c = t[a + res] < t[b + res] ? -1 : 1;
+        else c = boundary_then_pos(L, a, la, b, lb);
+        if (c > 0) {
+            idx[s] = (I)b;
+            idx[s + 1] = (I)a;
+        }
+        if (lcp) lcp[s + 1] = (I)res;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Groups of 3..64 members, one W-lane slot per group (grid launch).  The
 // round structure is the one described for k_wgroups_wq below; the wavefront
 // runs until all of its slots' groups are settled.
