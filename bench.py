@@ -135,11 +135,21 @@ def run_batch(args, count: int) -> dict:
         groups.append((w, sel, ends, d_text, d_out, len(cat)))
     total = sum(g[5] for g in groups)
     print(f"[bench] batch: {count} texts, {total} symbols", file=sys.stderr, flush=True)
+    knobs = [{"tile_keys": 0, "flags": 0} for _ in groups]
+    tuned = None
+    if args.autotune:
+        tuned = []
+        for gi, (w, sel, ends, d_text, d_out, m) in enumerate(groups):
+            choice, report = dev.autotune_batch(d_text, m, w, ends, 4)
+            knobs[gi] = choice
+            tuned.append({"sym_bytes": w, "choice": choice, "report": report})
+        print(f"[bench] autotune: {knobs}", file=sys.stderr, flush=True)
 
     def step():
         out = []
-        for w, sel, ends, d_text, d_out, m in groups:
-            dev.build_batch_device(d_text, m, w, ends, 4, d_out)
+        for gi, (w, sel, ends, d_text, d_out, m) in enumerate(groups):
+            dev.build_batch_device(d_text, m, w, ends, 4, d_out, tile_keys=knobs[gi]["tile_keys"],
+                                   flags=knobs[gi]["flags"])
             out.append(caps_sa.last_stats())
         return out
 
@@ -199,7 +209,7 @@ def run_batch(args, count: int) -> dict:
                           "groups": {k: last[k] for k in ("pairs", "small_groups", "mid_groups", "giant_groups",
                                                            "oversized", "buckets", "bucket_max")}})
     return {"ms_step": ms_step, "n": total, "verified": verified, "sample_texts": len(sample), "cpu": cpu,
-            "kernels": avg, "last": stats[-1], "per_width": per_width}
+            "kernels": avg, "last": stats[-1], "per_width": per_width, "autotune": tuned}
 
 
 def main() -> int:
@@ -217,6 +227,8 @@ def main() -> int:
     ap.add_argument("--bucket-keys", type=int, default=0, help="mean records per bucket (0: library default)")
     ap.add_argument("--buckets", type=int, default=0, help="explicit bucket count (0: from --bucket-keys)")
     ap.add_argument("--texts", type=int, default=4096, help="batch workload: number of texts")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick bucket size / tile sort / refinement style / merge width by timing builds first")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -268,6 +280,7 @@ def main() -> int:
             "verified_bit_exact": r["verified"], "verified_texts": r["sample_texts"],
             "kernels_ms": {k: round(v, 3) for k, v in avg.items()},
             "builds": r["per_width"],
+            "autotune": r["autotune"],
             "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                          "bytes_per_suffix": bpsuf, "kernel_ms": round(avg[dom], 4)},
@@ -282,11 +295,19 @@ def main() -> int:
     d_out = torch.empty(n, dtype=torch.int32, device="cuda:0")
     router = {"bsearch": 0, "pwl": 1, "mlp": 2}[args.router]
 
+    tuned = None
+    tile_keys = args.bucket_keys
+    flags = ((caps_sa.FLAG_PERSISTENT_REFINE if args.refine == "queue" else 0)
+             | (caps_sa.FLAG_TILE_RADIX if args.tile_sort == "radix" else 0))
+    if args.autotune:
+        choice, report = dev.autotune(d_text, n, 1, idx_bytes=4, router=router)
+        tile_keys, flags = choice["tile_keys"], choice["flags"]
+        tuned = {"choice": choice, "report": report}
+        print(f"[bench] autotune: {choice}", file=sys.stderr, flush=True)
+
     def step():
-        dev.build_device(d_text, n, 1, idx_bytes=4, router=router, d_out=d_out, tile_keys=args.bucket_keys,
-                         gpu_buckets=args.buckets,
-                         flags=(caps_sa.FLAG_PERSISTENT_REFINE if args.refine == "queue" else 0)
-                         | (caps_sa.FLAG_TILE_RADIX if args.tile_sort == "radix" else 0))
+        dev.build_device(d_text, n, 1, idx_bytes=4, router=router, d_out=d_out, tile_keys=tile_keys,
+                         gpu_buckets=args.buckets, flags=flags)
         return caps_sa.last_stats()
 
     for i in range(args.warmup):
@@ -365,6 +386,7 @@ def main() -> int:
             "max_mispredict": last["router_err_max"],
             "fallback_frac": round(last["router_fallbacks"] / max(last["router_count"], 1), 6),
             "train_and_sample_ms": round(sum(s["ms_sample"] for s in stats) / len(stats), 3)},
+        "autotune": tuned,
         "roofline": roof,
         "rooflines": rooflines,
         "cpu_baseline": cpu,

@@ -58,6 +58,17 @@ class _Opts(ctypes.Structure):
     ]
 
 
+class TuneReport(ctypes.Structure):
+    _fields_ = [("n_tried", ctypes.c_uint32), ("best_ms", ctypes.c_double), ("ms", ctypes.c_double * 16),
+                ("tile_keys", ctypes.c_uint32 * 16), ("flags", ctypes.c_uint32 * 16)]
+
+    def as_dict(self) -> dict:
+        k = self.n_tried
+        return {"best_ms": self.best_ms,
+                "tried": [{"tile_keys": self.tile_keys[i], "flags": self.flags[i], "ms": self.ms[i]}
+                          for i in range(k)]}
+
+
 class Stats(ctypes.Structure):
     _fields_ = [
         (name, ctypes.c_double)
@@ -108,6 +119,13 @@ _SIGNATURES = {
     "caps_build_batch_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p,
                                                 ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(_Opts),
                                                 ctypes.c_void_p, ctypes.c_void_p]),
+    "caps_autotune_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p,
+                                             ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32,
+                                             ctypes.POINTER(_Opts), ctypes.c_void_p, ctypes.c_void_p]),
+    "caps_autotune_batch_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                                                   ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                                                   ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(_Opts),
+                                                   ctypes.c_void_p, ctypes.c_void_p]),
     "caps_build_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p,
                                           ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(_Limits),
                                           ctypes.POINTER(_Opts), ctypes.c_void_p, ctypes.c_void_p,
@@ -316,6 +334,9 @@ class ExtMemOpts:
 FLAG_ROUTER_UNTRAINED = 1
 FLAG_PERSISTENT_REFINE = 2
 FLAG_TILE_RADIX = 4
+FLAG_MERGE_W16 = 8
+FLAG_MERGE_W64 = 16
+TUNE_MAX = 16
 
 
 def _c_opts(max_context=MAX_CONTEXT, subproblem_count=0, router=0, key_geometry=0, gpu_buckets=0, tile_keys=0,

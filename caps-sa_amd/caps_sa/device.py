@@ -11,7 +11,7 @@ import ctypes
 import numpy as np
 import torch
 
-from . import (CAPS_OK, MAX_CONTEXT, CapsError, LimitProvider, PlainText, _c_opts, _check, lib)
+from . import (CAPS_OK, MAX_CONTEXT, CapsError, LimitProvider, PlainText, TuneReport, _c_opts, _check, lib)
 
 _IDX = {4: torch.int32, 8: torch.int64}
 
@@ -108,13 +108,37 @@ def mlp_forward(params: np.ndarray, keys: np.ndarray, device="cuda:0"):
 
 
 def build_batch_device(d_text: torch.Tensor, n: int, sym_bytes: int, ends: np.ndarray, idx_bytes=4, d_out=None,
-                       router=0, gpu_buckets=0):
+                       router=0, gpu_buckets=0, tile_keys=0, flags=0):
     """caps_build_batch_device: texts back to back in HBM, `ends` their
     cumulative ends.  Returns the global positions, text after text."""
     ends = np.ascontiguousarray(ends, dtype=np.uint64)
     if d_out is None:
         d_out = torch.empty(n, dtype=_IDX[idx_bytes], device=d_text.device)
-    o = _c_opts(router=router, gpu_buckets=gpu_buckets)
+    o = _c_opts(router=router, gpu_buckets=gpu_buckets, tile_keys=tile_keys, flags=flags)
     _check(lib().caps_build_batch_device(d_text.data_ptr(), n, sym_bytes, ends.ctypes.data, len(ends), idx_bytes,
                                          ctypes.byref(o), d_out.data_ptr(), _stream_ptr()))
     return d_out
+
+
+def autotune(d_text: torch.Tensor, n: int, sym_bytes: int, lp: LimitProvider = None, idx_bytes=4, reps=2,
+             router=0):
+    """caps_autotune_device: returns ({"tile_keys", "flags"}, report dict)."""
+    scratch = torch.empty(n, dtype=_IDX[idx_bytes], device=d_text.device)
+    lim = (lp or PlainText(n))._c()
+    o = _c_opts(router=router)
+    rep = TuneReport()
+    _check(lib().caps_autotune_device(d_text.data_ptr(), n, sym_bytes, ctypes.byref(lim), idx_bytes,
+                                      scratch.data_ptr(), reps, ctypes.byref(o), _stream_ptr(), ctypes.byref(rep)))
+    return {"tile_keys": o.tile_keys, "flags": o.flags}, rep.as_dict()
+
+
+def autotune_batch(d_text: torch.Tensor, n: int, sym_bytes: int, ends: np.ndarray, idx_bytes=4, reps=2, router=0):
+    """caps_autotune_batch_device: returns ({"tile_keys", "flags"}, report dict)."""
+    ends = np.ascontiguousarray(ends, dtype=np.uint64)
+    scratch = torch.empty(n, dtype=_IDX[idx_bytes], device=d_text.device)
+    o = _c_opts(router=router)
+    rep = TuneReport()
+    _check(lib().caps_autotune_batch_device(d_text.data_ptr(), n, sym_bytes, ends.ctypes.data, len(ends), idx_bytes,
+                                            scratch.data_ptr(), reps, ctypes.byref(o), _stream_ptr(),
+                                            ctypes.byref(rep)))
+    return {"tile_keys": o.tile_keys, "flags": o.flags}, rep.as_dict()

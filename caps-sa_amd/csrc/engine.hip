@@ -20,7 +20,9 @@ namespace caps {
 thread_local std::string g_err;
 thread_local int g_emit_code = 0;
 thread_local caps_stats_t g_stats;
-std::mutex g_mu;  // one build at a time per process (the device context is serialized)
+// one build at a time per process (the device context is serialized);
+// recursive: the autotuner's entry points run builds through the public ones
+std::recursive_mutex g_mu;
 
 struct CapsError : std::runtime_error {
     int code;
@@ -758,14 +760,21 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
             LAUNCHCHK();
         }
         I *is = d_out, *ls = lbuf, *id = i2, *ld = l2;
-        constexpr int MW = 32;
+        // merge-path partition width: outputs per thread (a tuning knob)
+        const int mw = (o.flags & CAPS_FLAG_MERGE_W16) ? 16 : (o.flags & CAPS_FLAG_MERGE_W64) ? 64 : 32;
+        auto merge_level = [&](auto wtag, dim3 grid, const MergeJob *jobs) {
+            constexpr int MW = decltype(wtag)::value;
+            k_lcp_merge<S, I, MW><<<grid, 128, 0, st>>>(d_text, L, max_ctx, depth0, jobs, is, ls, id, ld);
+        };
         for (int lv = 0; lv < levels; ++lv) {
             const uint64_t run = (uint64_t)kMidMax << lv;
             const uint64_t span = std::min<uint64_t>(2 * run, maxsize);
             for (uint64_t j0 = level_off[lv]; j0 < level_off[lv + 1]; j0 += 65535) {
                 const unsigned nj = (unsigned)std::min<uint64_t>(level_off[lv + 1] - j0, 65535);
-                dim3 gr2((unsigned)ceil_div(ceil_div(span, MW), 128), nj);
-                k_lcp_merge<S, I, MW><<<gr2, 128, 0, st>>>(d_text, L, max_ctx, depth0, dj + j0, is, ls, id, ld);
+                dim3 gr2((unsigned)ceil_div(ceil_div(span, mw), 128), nj);
+                if (mw == 16) merge_level(std::integral_constant<int, 16>{}, gr2, dj + j0);
+                else if (mw == 64) merge_level(std::integral_constant<int, 64>{}, gr2, dj + j0);
+                else merge_level(std::integral_constant<int, 32>{}, gr2, dj + j0);
                 LAUNCHCHK();
             }
             std::swap(is, id);
@@ -853,7 +862,7 @@ static void check_common(const void *text, uint64_t n, uint32_t sym_bytes, const
 template <typename F>
 static int guarded(F &&f) {
     try {
-        std::lock_guard<std::mutex> lk(g_mu);
+        std::lock_guard<std::recursive_mutex> lk(g_mu);
         f();
         return CAPS_OK;
     } catch (const CapsError &e) {
@@ -1283,6 +1292,103 @@ int caps_cascade_merge(const void *text, uint64_t n, uint32_t sym_bytes, const c
         if (sym_bytes == 1) merge_jobs_host<uint8_t>(text, n, limits, sa, lcp, n_sa, runs, max_ctx, out_sa, out_lcp);
         else merge_jobs_host<uint16_t>(text, n, limits, sa, lcp, n_sa, runs, max_ctx, out_sa, out_lcp);
     });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Autotuner: times real builds of the caller's input over a small grid of
+// knobs and writes the fastest into *opts: the mean bucket size the splitters
+// aim at (keys per LDS tile), the tile sort (merge path or 4-bit radix), the
+// refinement launch style (per-class plan or persistent work queues for every
+// class) and the merge-path partition width of the giant-group merges.  Each
+// configuration is built `reps` times after one warm build; the device time
+// of the fastest build counts.  The output of every build is the same suffix
+// array, so tuning only moves time.
+// ---------------------------------------------------------------------------
+namespace {
+struct TuneCand {
+    uint32_t tile_keys, flags;
+};
+template <typename RUN>
+static void tune(caps_opts_t &o, uint32_t reps, caps_tune_report_t *rep, RUN &&run) {
+    static const uint32_t kKeys[] = {1024, 1280, 1536};
+    static const uint32_t kFlags[] = {0, CAPS_FLAG_TILE_RADIX, CAPS_FLAG_PERSISTENT_REFINE, CAPS_FLAG_MERGE_W16,
+                                      CAPS_FLAG_MERGE_W64};
+    std::vector<TuneCand> cands;
+    const uint32_t keep = o.flags & ~(CAPS_FLAG_TILE_RADIX | CAPS_FLAG_PERSISTENT_REFINE | CAPS_FLAG_MERGE_W16 |
+                                      CAPS_FLAG_MERGE_W64);
+    for (uint32_t k : kKeys) cands.push_back({k, keep});
+    for (size_t f = 1; f < sizeof(kFlags) / sizeof(kFlags[0]); ++f) cands.push_back({0, keep | kFlags[f]});
+    double best = 1e300;
+    TuneCand pick{o.tile_keys, o.flags};
+    uint32_t tried = 0;
+    for (size_t ci = 0; ci < cands.size(); ++ci) {
+        // the flag candidates take the best bucket size found so far
+        TuneCand c = cands[ci];
+        if (c.tile_keys == 0) c.tile_keys = pick.tile_keys;
+        caps_opts_t t = o;
+        t.tile_keys = c.tile_keys;
+        t.flags = c.flags;
+        run(t);  // warm: buffers sized, code loaded
+        double ms = 1e300;
+        for (uint32_t r = 0; r < std::max<uint32_t>(reps, 1); ++r) {
+            run(t);
+            ms = std::min(ms, g_stats.ms_total);
+        }
+        if (rep && tried < CAPS_TUNE_MAX) {
+            rep->ms[tried] = ms;
+            rep->tile_keys[tried] = c.tile_keys;
+            rep->flags[tried] = c.flags;
+        }
+        ++tried;
+        if (ms < best) {
+            best = ms;
+            pick = c;
+        }
+    }
+    o.tile_keys = pick.tile_keys;
+    o.flags = pick.flags;
+    if (rep) {
+        rep->n_tried = std::min<uint32_t>(tried, CAPS_TUNE_MAX);
+        rep->best_ms = best;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int caps_autotune_device(const void *d_text, uint64_t n, uint32_t sym_bytes, const caps_limits_t *limits,
+                         uint32_t idx_bytes, void *d_scratch_sa, uint32_t reps, caps_opts_t *opts, void *stream,
+                         caps_tune_report_t *report) {
+    if (!opts) return guarded([] { throw CapsError(CAPS_E_INVALID, "opts is required"); });
+    caps_opts_t o = *opts;
+    int rc = guarded([&] {
+        if (report) memset(report, 0, sizeof(*report));
+        tune(o, reps, report, [&](const caps_opts_t &t) {
+            int r = caps_build_device(d_text, n, sym_bytes, nullptr, n, idx_bytes, limits, &t, d_scratch_sa, nullptr,
+                                      stream);
+            if (r != CAPS_OK) throw CapsError(r, caps_last_error());
+        });
+    });
+    if (rc == CAPS_OK) *opts = o;
+    return rc;
+}
+
+int caps_autotune_batch_device(const void *d_text, uint64_t n, uint32_t sym_bytes, const uint64_t *ends,
+                               uint64_t n_texts, uint32_t idx_bytes, void *d_scratch_sa, uint32_t reps,
+                               caps_opts_t *opts, void *stream, caps_tune_report_t *report) {
+    if (!opts) return guarded([] { throw CapsError(CAPS_E_INVALID, "opts is required"); });
+    caps_opts_t o = *opts;
+    int rc = guarded([&] {
+        if (report) memset(report, 0, sizeof(*report));
+        tune(o, reps, report, [&](const caps_opts_t &t) {
+            int r = caps_build_batch_device(d_text, n, sym_bytes, ends, n_texts, idx_bytes, &t, d_scratch_sa, stream);
+            if (r != CAPS_OK) throw CapsError(r, caps_last_error());
+        });
+    });
+    if (rc == CAPS_OK) *opts = o;
+    return rc;
 }
 
 }  // extern "C"

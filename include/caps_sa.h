@@ -87,6 +87,10 @@ typedef struct caps_opts {
 /* Sort tiles with the 4-bit LDS radix sort instead of the merge-path block
  * merge sort.  The output never changes. */
 #define CAPS_FLAG_TILE_RADIX 4u
+/* Merge-path partition width of the giant-group LCP merges: 16 or 64 outputs
+ * per thread instead of 32.  The output never changes. */
+#define CAPS_FLAG_MERGE_W16 8u
+#define CAPS_FLAG_MERGE_W64 16u
 
 /* Fills opts with the reference defaults (max_context unbounded, p auto). */
 void caps_opts_default(caps_opts_t *opts);
@@ -183,6 +187,30 @@ typedef struct caps_stats {
     double k_scatter1, k_scatter2; /* the two scatter passes (their sum is k_scatter) */
 } caps_stats_t;
 int caps_last_stats(caps_stats_t *out);
+
+/* ---- autotuner (no counterpart in the reference) ------------------------- */
+
+/* Times builds of the caller's own input over a small grid of knobs (mean
+ * bucket size = keys per LDS tile, tile sort, refinement launch style,
+ * merge-path width) and writes the fastest into *opts (its other fields are
+ * kept).  d_scratch_sa: n entries of idx_bytes, overwritten.  reps: timed
+ * builds per configuration after one warm build.  The suffix array does not
+ * depend on any of these knobs. */
+#define CAPS_TUNE_MAX 16
+typedef struct caps_tune_report {
+    uint32_t n_tried;
+    double best_ms;
+    double ms[CAPS_TUNE_MAX];
+    uint32_t tile_keys[CAPS_TUNE_MAX];
+    uint32_t flags[CAPS_TUNE_MAX];
+} caps_tune_report_t;
+int caps_autotune_device(const void *d_text, uint64_t n, uint32_t sym_bytes, const caps_limits_t *limits,
+                         uint32_t idx_bytes, void *d_scratch_sa, uint32_t reps, caps_opts_t *opts, void *stream,
+                         caps_tune_report_t *report);
+/* The same for a batch (caps_build_batch_device's arguments). */
+int caps_autotune_batch_device(const void *d_text, uint64_t n, uint32_t sym_bytes, const uint64_t *ends,
+                               uint64_t n_texts, uint32_t idx_bytes, void *d_scratch_sa, uint32_t reps,
+                               caps_opts_t *opts, void *stream, caps_tune_report_t *report);
 
 /* ---- kernels exposed for parity tests (device pointers) ------------------ */
 

@@ -221,3 +221,22 @@ def test_radix_tile_sort_matches_merge():
             sa, lcp = dev.build_device(d_text, len(text), 1, want_lcp=True, flags=flags)
             assert np.array_equal(sa.cpu().numpy().view(np.uint32).astype(np.uint64), want), flags
             assert np.array_equal(lcp.cpu().numpy().view(np.uint32).astype(np.uint64), want_lcp), flags
+
+
+def test_autotune_then_build_is_exact():
+    """The autotuner times the candidate knobs on the input and returns one of
+    them; builds with the tuned knobs (and with every candidate it timed) give
+    the oracle's suffix array."""
+    import torch
+    from caps_sa import device as dev
+
+    text = gen.markov_repeats(600000, seed=51, mean_len=700.0)
+    d_text = dev.text_to_device(text)
+    choice, report = dev.autotune(d_text, len(text), 1, reps=1)
+    assert 0 < len(report["tried"]) <= caps_sa.TUNE_MAX
+    assert report["best_ms"] == min(t["ms"] for t in report["tried"])
+    assert any(t["tile_keys"] == choice["tile_keys"] and t["flags"] == choice["flags"] for t in report["tried"])
+    want = O.Ctx(text).build_in_memory(threads=4)
+    for t in report["tried"] + [choice]:
+        sa, _ = dev.build_device(d_text, len(text), 1, tile_keys=t["tile_keys"], flags=t["flags"])
+        assert np.array_equal(sa.cpu().numpy().view(np.uint32).astype(np.uint64), want), t
