@@ -34,6 +34,10 @@ WORKLOADS = {
     # name: (generator, n, description)
     "repeat-rich-268M": ("markov", 1 << 28, "order-3 Markov {0..3} + 40% planted repeats (mean 5000, 1% subst)"),
     "uniform-1M": ("uniform", 1 << 20, "i.i.d. uniform {0..3}"),
+    "genome-share-820M": ("genome", 820_000_000,
+                          "one GPU's eighth of the 6.56e9-symbol genome: 175,000 lognormal segments (median 2,500, "
+                          "sigma 1.6) of i.i.d. {0..3} ending in spacer 4, ~5.8% N runs (symbol 5, geometric mean "
+                          "10,000); positions {p : text[p] < 4}, SegmentedText ShorterFirst, u64 index"),
     "batch-4096": ("batch", 0, "4,096 texts, lognormal lengths (median 65,536, sigma 1.5, clipped to [1,024, "
                               "16,777,216]); even texts u8 i.i.d. {0..3}, odd texts u16 Zipf(1.1) over 50,000 symbols"),
 }
@@ -212,6 +216,56 @@ def run_batch(args, count: int) -> dict:
             "kernels": avg, "last": stats[-1], "per_width": per_width, "autotune": tuned}
 
 
+def run_genome(args, n: int) -> dict:
+    """Segmented + filtered build at genome scale on one GPU: positions and
+    suffix array in HBM (u64), SegmentedText limits."""
+    import numpy as np
+    import torch
+
+    import caps_sa
+    from caps_sa import device as dev
+    from caps_sa import gen
+
+    n_seg = max(1, round(n * 1.40e6 / 6.56e9))
+    text, ends, pos = gen.genome(n, n_seg, seed=args.seed)
+    print(f"[bench] genome share: n={n} segments={n_seg} positions={len(pos)}", file=sys.stderr, flush=True)
+    lp = caps_sa.SegmentedText(n, ends)
+    d_text = dev.text_to_device(text)
+    d_pos = torch.from_numpy(pos.view(np.int64)).to("cuda:0")
+    d_out = torch.empty(len(pos), dtype=torch.int64, device="cuda:0")
+
+    def step():
+        dev.build_device(d_text, n, 1, d_positions=d_pos, n_pos=len(pos), idx_bytes=8, lp=lp, d_out=d_out)
+        return caps_sa.last_stats()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = [step() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t0) * 1e3 / max(args.steps, 1)
+    verified = None
+    if not args.no_verify:
+        import oracle as O
+
+        sa = d_out.cpu().numpy().view(np.uint64)
+        bad = O.Ctx(text, ends, O.ORDER_SHORTER_FIRST).verify(sa, positions=pos, strict=True, canon=True,
+                                                               threads=args.cpu_threads)
+        verified = bool(bad == -1)
+        print(f"[bench] verify: {'ok' if verified else f'FAILED at {bad}'}", file=sys.stderr, flush=True)
+    kern = ["k_classify", "k_scatter1", "k_scatter2", "k_scatter", "k_tile_sort", "k_pairs", "k_groups_small",
+            "k_groups_mid", "k_boundary_lcp"]
+    avg = {k: sum(st[k] for st in stats) / len(stats) for k in kern}
+    phases = ("ms_alphabet", "ms_sample", "ms_classify", "ms_scatter", "ms_tile_sort", "ms_merge", "ms_detect",
+              "ms_refine", "ms_giant")
+    last = stats[-1]
+    return {"ms_step": ms_step, "n_pos": len(pos), "n": n, "segments": n_seg, "verified": verified,
+            "kernels": avg, "phases": {ph: round(sum(st[ph] for st in stats) / len(stats), 3) for ph in phases},
+            "groups": {k: last[k] for k in ("pairs", "small_groups", "mid_groups", "giant_groups", "oversized",
+                                             "buckets", "bucket_max")}, "key_k": last["key_k"]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -285,6 +339,32 @@ def main() -> int:
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                          "bytes_per_suffix": bpsuf, "kernel_ms": round(avg[dom], 4)},
             "cpu_baseline": r["cpu"],
+        }
+        print(json.dumps(line))
+        return 0 if r["verified"] in (None, True) else 3
+    if kind == "genome":
+        r = run_genome(args, args.n or n_default)
+        n_pos = r["n_pos"]
+        value = n_pos / (r["ms_step"] / 1e3) / 1e6
+        avg = r["kernels"]
+        rooflines = {}
+        for k in ("k_classify", "k_scatter1", "k_scatter2"):
+            bpsuf = {"k_classify": 8 + 1 + 2, "k_scatter1": 8 + 2 + 16, "k_scatter2": 2 * 16}[k]
+            achieved = n_pos * bpsuf / (avg[k] / 1e3) / 1e9 if avg[k] else 0.0
+            rooflines[k] = {"bound": "hbm", "kernel": k, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                            "bytes_per_suffix": bpsuf, "kernel_ms": round(avg[k], 4)}
+        dom = max(rooflines, key=lambda k: avg[k])
+        line = {
+            "metric": "M suffixes/s (bit-exact SA), learned-router vs plain GPU; HBM GB/s % peak",
+            "value": round(value, 2), "unit": "M suffixes/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(r["ms_step"], 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+            "config": {"workload": args.workload, "n": r["n"], "segments": r["segments"], "positions": n_pos,
+                       "text": desc, "index": "u64", "key_symbols": r["key_k"], "seed": args.seed},
+            "verified_bit_exact": r["verified"], "phases_ms": r["phases"],
+            "kernels_ms": {k: round(v, 3) for k, v in avg.items()}, "groups": r["groups"],
+            "roofline": rooflines[dom], "rooflines": rooflines, "cpu_baseline": None,
         }
         print(json.dumps(line))
         return 0 if r["verified"] in (None, True) else 3
