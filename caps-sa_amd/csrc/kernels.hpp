@@ -872,20 +872,98 @@ __global__ __launch_bounds__(256) void k_boundary_lcp(const uint64_t *__restrict
 }
 
 // ---------------------------------------------------------------------------
-// Group refinement: one workgroup per group of equal keys.  Rounds extend
-// every unresolved member by the next k symbols (a fresh packed key at the
-// current depth), plus a boundary value for members whose window ends inside
-// the round, and re-sort the group by (subgroup, chunk, boundary, position).
-// A subgroup is settled once it is a single member or its members all ended
-// (comparator-Equal; position order).
+// Group refinement: one workgroup per group of suffixes known to agree on
+// their first depth0 symbols (65..2048 members, or one chunk of a larger
+// group).  The first kChunkRounds rounds sort each subgroup by the next k
+// symbols at its depth (chunk key, plus the boundary value of members whose
+// window ends inside the chunk), which splits short common prefixes into many
+// classes at once.  Later rounds compare every unresolved member with the first
+// member (in the current order) of its subgroup, the subgroup's reference:
+// one LCP scan from the subgroup's depth D over the packed codes, as long as
+// the windows allow, gives l, the offset where member and reference part.
+// The comparator's outcome against the reference then fixes the member's
+// class:
+//   - the codes differ at D+l: the member lies before or after the reference
+//     (by the codes), and ties among members are only possible with the same
+//     l and the same code; such ties continue at depth D+l+1;
+//   - the member's window ends first (at D+l): before the reference when
+//     shorter suffixes come first, after it in the STAR order; settled;
+//   - both windows end together: the reference's class, ordered by
+//     boundary_then_pos (limit, then position); settled;
+//   - the reference's window ends first: after the reference (shorter first)
+//     or before it (STAR); such members continue together at depth D+l.
+// On the "before" side members order by l ascending, on the "after" side by
+// l descending (a member that parts from the reference later is closer to
+// it), then by code.  So one round orders whole staircases of nested common
+// prefixes (e.g. suffixes before runs of one symbol of different lengths),
+// which rounds of k symbols would peel off one member at a time.  Sorted
+// neighbours in different classes share D + min(l, l') symbols; the final
+// LCP caps that by both windows.
 // ---------------------------------------------------------------------------
+constexpr uint64_t kLMax = (1ull << 44) - 1;
+constexpr uint32_t kChunkRounds = 2;
+
+// LCP of the codes of suffixes a and b from offset 0, at most `limit`.
+template <typename S>
+__device__ __forceinline__ uint64_t lcp_codes(const S *__restrict__ t, const CodeMap<S> &cm, const KeyGeom &g,
+                                              uint64_t a, uint64_t b, uint64_t limit) {
+    if (!cm.packed) return thread_lcp(t, a, b, 0, limit);
+    const uint32_t bits = g.bits, F = 1u << cm.lgf;
+    uint64_t o = 0;
+    while (o < limit) {
+        uint64_t va[2], vb[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            va[u] = vb[u] = 0;
+            if (u == 1 && o + F >= limit) break;  // every word read holds a symbol inside the window
+            const uint64_t qa = a + o + (uint64_t)u * F, qb = b + o + (uint64_t)u * F;
+            const uint64_t wa = qa >> cm.lgf, wb = qb >> cm.lgf;
+            const uint32_t ra = (uint32_t)(qa & (F - 1)) * bits, rb = (uint32_t)(qb & (F - 1)) * bits;
+            va[u] = ra ? (cm.packed[wa] << ra) | (cm.packed[wa + 1] >> (64 - ra)) : cm.packed[wa];
+            vb[u] = rb ? (cm.packed[wb] << rb) | (cm.packed[wb + 1] >> (64 - rb)) : cm.packed[wb];
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const uint64_t x = va[u] ^ vb[u];
+            if (x) {
+                const uint64_t r = o + (uint64_t)u * F + (uint32_t)__builtin_clzll(x) / bits;
+                return r < limit ? r : limit;
+            }
+        }
+        o += 2ull * F;
+    }
+    return limit;
+}
+
+// Class keys that members can share and refine further: a differing code
+// (symbol field 1..2^16), or "the reference's window ended first" (field 0 on
+// the after side when shorter suffixes come first, field 0x1FFFF on the before
+// side in the STAR order).  Window-ended and both-ended members never tie.
+__device__ __forceinline__ bool group_key_continues(uint64_t k, bool star) {
+    const uint64_t cls = k >> 62, sym = k & 0x1FFFFull;
+    if (cls == 1) return false;
+    if (sym != 0 && sym != 0x1FFFFull) return true;
+    return star ? (cls == 0 && sym == 0x1FFFFull) : (cls == 2 && sym == 0);
+}
+
+template <typename S>
+__device__ __forceinline__ uint32_t code_at(const S *__restrict__ t, const CodeMap<S> &cm, const KeyGeom &g,
+                                            uint64_t q) {
+    if (!cm.packed) return cm(t[q]);
+    const uint64_t w = cm.packed[q >> cm.lgf];
+    const uint32_t f = (uint32_t)(q & ((1ull << cm.lgf) - 1));
+    return (uint32_t)(w >> (64 - (f + 1) * g.bits)) & ((1u << g.bits) - 1);
+}
+
 template <typename S, typename I, int B, int M>
 __global__ __launch_bounds__(B) void k_groups(const S *__restrict__ t, CodeMap<S> cm, KeyGeom g, DevLimits L,
                                               uint64_t max_ctx, const Group *__restrict__ groups, uint64_t depth0,
                                               I *__restrict__ idx, I *__restrict__ lcp, int run_mode,
                                               uint32_t chunk_len) {
-    __shared__ uint64_t s_pos[M], s_chunk[M], s_aux[M], s_cap[M], s_lim[M], s_lcp[M];
-    __shared__ uint32_t s_sub[M];
+    // member-indexed: position, window, class key, tie key, subgroup depth, l
+    __shared__ uint64_t s_pos[M], s_cap[M], s_key[M], s_aux[M], s_dep[M], s_l[M];
+    __shared__ uint64_t s_lcp[M];  // split depth of sorted neighbours r-1, r
+    __shared__ uint32_t s_sub[M];  // subgroup (its first sorted position) | settled bit
     __shared__ uint16_t s_perm[M];
     __shared__ uint32_t s_scan[M];
     __shared__ uint32_t s_unres;
@@ -905,28 +983,67 @@ __global__ __launch_bounds__(B) void k_groups(const S *__restrict__ t, CodeMap<S
         uint64_t p = idx[gstart + s];
         uint64_t lim = lim_at(L, p);
         s_pos[s] = p;
-        s_lim[s] = lim;
         s_cap[s] = lim < max_ctx ? lim : max_ctx;
         s_sub[s] = 0;
+        s_dep[s] = depth0;
         s_lcp[s] = 0;
     }
     for (uint32_t r = threadIdx.x; r < P; r += B) s_perm[r] = (uint16_t)r;
     __syncthreads();
-    uint64_t D = depth0;
-    for (;;) {
-        // (1) fresh chunk + boundary value for unresolved members
+    const uint64_t cont = star ? 0 : kAuxCont;
+    for (uint32_t round = 0;; ++round) {
+        // the first rounds split by chunk keys (k symbols at the subgroup's
+        // depth: many classes at once when members part early); later rounds
+        // by the reference's LCP (one round for nested common prefixes)
+        const bool chunk_round = round < kChunkRounds;
+        // (1) sort key of every unresolved member
         for (uint32_t s = threadIdx.x; s < m; s += B) {
             if (s_sub[s] & 0x80000000u) continue;  // settled
-            uint64_t cap = s_cap[s];
-            s_chunk[s] = key_at(t, cm, g, s_pos[s], D, cap);
-            uint64_t rem = cap > D ? cap - D : 0;
-            uint64_t aux;
-            if (rem >= g.k) aux = star ? 0 : kAuxCont;
-            else aux = star ? (1 + (kAuxCont - 1 - s_lim[s])) : s_lim[s];
+            const uint64_t D = s_dep[s];
+            if (chunk_round) {
+                const uint64_t cap = s_cap[s];
+                s_key[s] = key_at(t, cm, g, s_pos[s], D, cap);
+                const uint64_t rem = cap > D ? cap - D : 0;
+                uint64_t aux = cont;
+                if (rem < g.k) {
+                    const uint64_t lm = lim_at(L, s_pos[s]);
+                    aux = star ? (1 + (kAuxCont - 1 - lm)) : lm;
+                }
+                s_aux[s] = aux;
+                continue;
+            }
+            const uint32_t ref = s_perm[s_sub[s]];
+            const uint64_t pa = s_pos[s], pr = s_pos[ref];
+            const uint64_t cs = s_cap[s], cref = s_cap[ref];
+            const uint64_t cmin = cs < cref ? cs : cref;
+            const uint64_t lim2 = cmin > D ? cmin - D : 0;  // common window from depth D
+            const uint64_t l = s == ref ? lim2 : lcp_codes(t, cm, g, pa + D, pr + D, lim2);
+            // A: absolute depth where the member parts from the reference (or
+            // its window ends, possibly above D); classes order by it
+            uint64_t key, aux = 0;
+            if (l < lim2) {  // codes differ
+                const uint64_t A = D + l;
+                const uint32_t sa = code_at(t, cm, g, pa + A), sr = code_at(t, cm, g, pr + A);
+                key = sa < sr ? ((A << 17) | (sa + 1)) : ((2ull << 62) | ((kLMax - A) << 17) | (sa + 1));
+            } else if (cs < cref) {  // the member's window ends first: settled
+                const uint64_t A = cs;
+                key = star ? ((2ull << 62) | ((kLMax - A) << 17) | 0x1FFFFull) : ((A << 17) | 0u);
+                const uint64_t lm = lim_at(L, pa);
+                aux = star ? ~lm : lm;
+            } else if (cs == cref) {  // both end here: the boundary order decides; settled
+                key = 1ull << 62;
+                const uint64_t lm = lim_at(L, pa);
+                aux = star ? ~lm : lm;
+            } else {  // the reference's window ends first: these continue together
+                const uint64_t A = D + l;
+                key = star ? ((A << 17) | 0x1FFFFull) : ((2ull << 62) | ((kLMax - A) << 17) | 0u);
+            }
+            s_key[s] = key;
             s_aux[s] = aux;
+            s_l[s] = l;
         }
         __syncthreads();
-        // (2) bitonic sort of the permutation by (sub, chunk, aux, pos)
+        // (2) bitonic sort of the permutation by (sub, key, aux, pos)
         for (uint32_t size = 2; size <= P; size <<= 1) {
             for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
                 for (uint32_t q = threadIdx.x; q < P / 2; q += B) {
@@ -940,9 +1057,16 @@ __global__ __launch_bounds__(B) void k_groups(const S *__restrict__ t, CodeMap<S
                     else {
                         uint32_t sa = s_sub[a] & 0x7FFFFFFFu, sb = s_sub[b] & 0x7FFFFFFFu;
                         if (sa != sb) gt = sa > sb;
-                        else if (s_chunk[a] != s_chunk[b]) gt = s_chunk[a] > s_chunk[b];
-                        else if (s_aux[a] != s_aux[b]) gt = s_aux[a] > s_aux[b];
-                        else gt = s_pos[a] > s_pos[b];
+                        else {
+                            // settled members keep their place: they are alone in their subgroup
+                            uint64_t ka = (s_sub[a] & 0x80000000u) ? 0 : s_key[a];
+                            uint64_t kb = (s_sub[b] & 0x80000000u) ? 0 : s_key[b];
+                            uint64_t xa = (s_sub[a] & 0x80000000u) ? 0 : s_aux[a];
+                            uint64_t xb = (s_sub[b] & 0x80000000u) ? 0 : s_aux[b];
+                            if (ka != kb) gt = ka > kb;
+                            else if (xa != xb) gt = xa > xb;
+                            else gt = s_pos[a] > s_pos[b];
+                        }
                     }
                     if (gt == up) {
                         s_perm[i] = (uint16_t)b;
@@ -952,32 +1076,38 @@ __global__ __launch_bounds__(B) void k_groups(const S *__restrict__ t, CodeMap<S
                 __syncthreads();
             }
         }
-        // (3) new subgroup starts.  Sorted positions of subgroup boundaries
-        // never move once formed, so the depth at which neighbours r-1, r
-        // part is recorded per position; the window caps of whichever
-        // members end up there are applied at the end.
+        // (3) new subgroups: a member starts one unless it continues a tie
+        // (same old subgroup, same class key, a class that can tie)
         for (uint32_t r = threadIdx.x; r < m; r += B) {
             uint32_t s = s_perm[r];
             bool f = true;
-            if (r > 0) {
+            if (r > 0 && !(s_sub[s] & 0x80000000u)) {
                 uint32_t q = s_perm[r - 1];
                 bool same_old = (s_sub[s] & 0x7FFFFFFFu) == (s_sub[q] & 0x7FFFFFFFu);
-                f = !(same_old && s_chunk[s] == s_chunk[q] && s_aux[s] == s_aux[q]);
-                if (same_old && !(s_sub[s] & 0x80000000u)) s_lcp[r] = D + shared_fields(g, s_chunk[q], s_chunk[s]);
+                if (same_old && chunk_round) {
+                    s_lcp[r] = s_dep[s] + shared_fields(g, s_key[q], s_key[s]);
+                    f = !(s_key[s] == s_key[q] && s_aux[s] == s_aux[q]);
+                } else if (same_old) {
+                    const uint64_t ls = s_l[s], lq = s_l[q];
+                    s_lcp[r] = s_dep[s] + (ls < lq ? ls : lq);
+                    const uint64_t k = s_key[s];
+                    f = !(group_key_continues(k, star) && k == s_key[q]);
+                }
             }
             s_scan[r] = f ? r : 0;
         }
         __syncthreads();
         for (uint32_t off = 1; off < m; off <<= 1) {
-            uint32_t v[ (M + B - 1) / B ];
+            uint32_t v[(M + B - 1) / B];
             uint32_t c = 0;
-            for (uint32_t r = threadIdx.x; r < m; r += B, ++c) v[c] = (r >= off && s_scan[r - off] > s_scan[r]) ? s_scan[r - off] : s_scan[r];
+            for (uint32_t r = threadIdx.x; r < m; r += B, ++c)
+                v[c] = (r >= off && s_scan[r - off] > s_scan[r]) ? s_scan[r - off] : s_scan[r];
             __syncthreads();
             c = 0;
             for (uint32_t r = threadIdx.x; r < m; r += B, ++c) s_scan[r] = v[c];
             __syncthreads();
         }
-        // (4) settle: singleton subgroups and ended subgroups
+        // (4) settle singletons and ended members; ties move to their new depth
         if (threadIdx.x == 0) s_unres = 0;
         __syncthreads();
         uint32_t unres = 0;
@@ -985,15 +1115,25 @@ __global__ __launch_bounds__(B) void k_groups(const S *__restrict__ t, CodeMap<S
             uint32_t s = s_perm[r];
             uint32_t ns = s_scan[r];
             bool has_nb = (r + 1 < m && s_scan[r + 1] == ns) || (r > 0 && s_scan[r - 1] == ns);
-            bool ended = s_aux[s] != (star ? 0 : kAuxCont);
-            bool settled = !has_nb || ended || (s_sub[s] & 0x80000000u);
+            bool settled = (s_sub[s] & 0x80000000u) || !has_nb;
+            // chunk rounds: members whose window ended inside the chunk are
+            // comparator-Equal (position order) with their tie
+            if (chunk_round && !settled && s_aux[s] != cont) settled = true;
+            if (!settled) {
+                if (chunk_round) {
+                    s_dep[s] = s_dep[s] + g.k;
+                } else {  // a tie: past the differing code, or past the reference's end
+                    const uint64_t k = s_key[s];
+                    const uint64_t sym = k & 0x1FFFFull;
+                    s_dep[s] = s_dep[s] + s_l[s] + (sym != 0 && sym != 0x1FFFFull ? 1 : 0);
+                }
+            }
             s_sub[s] = ns | (settled ? 0x80000000u : 0u);
             unres += settled ? 0 : 1;
         }
         if (unres) atomicAdd(&s_unres, unres);
         __syncthreads();
         if (s_unres == 0) break;
-        D += g.k;
         __syncthreads();
     }
     for (uint32_t r = threadIdx.x; r < m; r += B) {
