@@ -34,9 +34,19 @@ def build_gen() -> str:
     return out
 
 
+def build_host_test() -> str:
+    """The C++ API mirror's test program (host/test_host.cpp), linked against
+    the HIP library built above."""
+    out = os.path.join(HERE, "host", "test_host")
+    _run(["g++", "-O2", "-std=c++17", "-o", out, os.path.join("host", "test_host.cpp"), "-L" + HERE, "-lcaps_sa_hip",
+          "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/.."])
+    return out
+
+
 def build_all() -> None:
     build_gen()
     build_hip()
+    build_host_test()
 
 
 if __name__ == "__main__":

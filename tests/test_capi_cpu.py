@@ -90,3 +90,14 @@ def test_ext_mem_opts_from_env(monkeypatch):
     assert o.packed_prefix_seed == caps_sa.PackedPrefixSeedPolicy.DENSE_ALPHABET_ONLY
     monkeypatch.setenv("CAPS_SA_PACKED_PREFIX_REMAP_BYTES", "100")
     assert caps_sa.ExtMemOpts.from_env().packed_prefix_seed == caps_sa.PackedPrefixSeedPolicy.REMAP
+
+
+def test_cpp_api_mirror_header_compiles():
+    """The C++ mirror of the crate's API is self-contained C++17 over the C ABI."""
+    import subprocess
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = "#include \"caps_sa.hpp\"\nint main() { return 0; }\n"
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-x", "c++", "-I",
+                        os.path.join(root, "caps-sa_amd", "host"), "-"], input=src, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
