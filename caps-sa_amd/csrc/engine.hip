@@ -503,65 +503,70 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     while ((1ull << SP.fine_steps) < SP.stride) ++SP.fine_steps;
     stats.buckets = p;
 
-    // ---- classify (coarse interval) + histogram ----
+    // ---- classify (coarse interval) + per-block histograms ----
     if (SP.stride > kFineMax) throw CapsError(CAPS_E_INVALID, "too many buckets");
     const uint32_t pc = SP.n_coarse + 1;
     uint16_t *cid = (uint16_t *)W.bid.get(n_pos * 2);
     uint32_t *hist = (uint32_t *)W.hist.get(pc * 4);
     uint64_t *coff = (uint64_t *)W.toff.get((std::max<uint64_t>(pc, p) + 1) * 8);
     uint64_t *off = (uint64_t *)W.off.get((p + 1) * 8);
-    unsigned long long *cursor = (unsigned long long *)W.cursor.get(pc * 8);
-    HIPCHK(hipMemsetAsync(hist, 0, pc * 4, st));
     RouterDev R{};
     uint32_t rkind = p > 1 ? o.router : CAPS_ROUTER_BSEARCH;
     if (rkind > CAPS_ROUTER_MLP) throw CapsError(CAPS_E_INVALID, "unknown router");
     if (rkind != CAPS_ROUTER_BSEARCH)
         R = train_router<S, I>(W, rkind, o.flags, sorted_sample, s, SP, d_text, cm, g, L, max_ctx, d_pos, n_pos, st);
     tm.mark(PH_SAMPLE);
-    unsigned grid = grid_for(n_pos, 256, 8192);
     kt.begin(&stats.k_classify);
     const bool seq = d_pos == nullptr && n_pos == n;
-    // one resident wave of blocks (two per CU, bounded by LDS) striding over the
-    // chunks: the coarse splitter table is staged once per block, not per chunk
-    const uint64_t nchunks = ceil_div(n_pos, kSeqChunk);
-    auto gseq = [&](auto kern) { return (unsigned)std::min<uint64_t>(nchunks, resident_blocks(kern, 256)); };
+    // Every block owns one contiguous range of positions (a multiple of the
+    // key window), in both the classify pass and scatter pass 1: the classify
+    // block's histogram is then exactly the scatter block's run sizes.  About
+    // one resident wave of classify blocks; the coarse splitter table is
+    // staged once per block.
+    unsigned resident = 0;
+    if (seq)
+        resident = rkind == CAPS_ROUTER_PWL   ? resident_blocks(k_classify_seq<S, I, 1>, 256)
+                   : rkind == CAPS_ROUTER_MLP ? resident_blocks(k_classify_seq<S, I, 2>, 256)
+                                              : resident_blocks(k_classify_seq<S, I, 0>, 256);
+    else
+        resident = rkind == CAPS_ROUTER_PWL   ? resident_blocks(k_classify<S, I, 1>, 256)
+                   : rkind == CAPS_ROUTER_MLP ? resident_blocks(k_classify<S, I, 2>, 256)
+                                              : resident_blocks(k_classify<S, I, 0>, 256);
+    const uint64_t chunk =
+        std::max<uint64_t>(kSeqChunk, ceil_div(ceil_div(n_pos, (uint64_t)resident), kSeqChunk) * kSeqChunk);
+    const unsigned nblk = (unsigned)ceil_div(n_pos, chunk);
+    uint32_t *hblk = (uint32_t *)W.hblk.get((uint64_t)nblk * pc * 4);
     if (seq) {
         if (rkind == CAPS_ROUTER_PWL)
-            k_classify_seq<S, I, 1><<<gseq(k_classify_seq<S, I, 1>), 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R,
-                                                                                 cid, hist, pc);
+            k_classify_seq<S, I, 1><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, cid, hblk, pc, chunk);
         else if (rkind == CAPS_ROUTER_MLP)
-            k_classify_seq<S, I, 2><<<gseq(k_classify_seq<S, I, 2>), 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R,
-                                                                                 cid, hist, pc);
+            k_classify_seq<S, I, 2><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, cid, hblk, pc, chunk);
         else
-            k_classify_seq<S, I, 0><<<gseq(k_classify_seq<S, I, 0>), 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R,
-                                                                                 cid, hist, pc);
+            k_classify_seq<S, I, 0><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, cid, hblk, pc, chunk);
     } else if (rkind == CAPS_ROUTER_PWL)
-        k_classify<S, I, 1><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hist, pc);
+        k_classify<S, I, 1><<<nblk, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hblk, pc, chunk);
     else if (rkind == CAPS_ROUTER_MLP)
-        k_classify<S, I, 2><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hist, pc);
+        k_classify<S, I, 2><<<nblk, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hblk, pc, chunk);
     else
-        k_classify<S, I, 0><<<grid, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hist, pc);
+        k_classify<S, I, 0><<<nblk, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hblk, pc, chunk);
     LAUNCHCHK();
     kt.end();
+    k_block_offsets<<<(unsigned)ceil_div(pc, 256), 256, 0, st>>>(hblk, nblk, pc, hist);
+    LAUNCHCHK();
     k_scan_u32<<<1, 1024, 0, st>>>(hist, pc, coff);
     LAUNCHCHK();
-    HIPCHK(hipMemcpyAsync(cursor, coff, pc * 8, hipMemcpyDeviceToDevice, st));
     tm.mark(PH_CLASSIFY);
 
     // ---- two-level scatter ----
     uint64_t *keys = (uint64_t *)W.keys.get(n_pos * 8);
     KRec<I> *tmp = (KRec<I> *)W.recs.get(n_pos * sizeof(KRec<I>));
     kt.begin(&stats.k_scatter1);
-    // chunk per scatter block: a multiple of the key window, about one resident wave of blocks
-    const uint64_t schunk =
-        std::max<uint64_t>(kSeqChunk, ceil_div(ceil_div(n_pos, (uint64_t)W.n_cu * 2), kSeqChunk) * kSeqChunk);
-    const unsigned g1 = (unsigned)ceil_div(n_pos, schunk);
     if (seq)
-        k_scatter1<S, I, true><<<g1, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, cid, pc, cursor, tmp,
-                                                   schunk);
+        k_scatter1<S, I, true><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, cid, pc, coff, hblk,
+                                                     tmp, chunk);
     else
-        k_scatter1<S, I, false><<<g1, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, cid, pc, cursor, tmp,
-                                                    schunk);
+        k_scatter1<S, I, false><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, cid, pc, coff, hblk,
+                                                      tmp, chunk);
     LAUNCHCHK();
     kt.end();
     // the coarse ids are consumed: their buffer holds the per-record bucket within its interval

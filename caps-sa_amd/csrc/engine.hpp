@@ -42,7 +42,7 @@ struct Workspace {
     std::vector<Tile> h_tiles;
     Range h_range;
     // classification / scatter
-    DevBuf bid, hist, off, cursor, keys, keys2, idx2;
+    DevBuf bid, hist, hblk, off, cursor, keys, keys2, idx2;
     // tiles
     DevBuf tcnt, toff, toff2, tiles, big, counts, recs;
     // refinement

@@ -256,7 +256,7 @@ template <typename S, typename I, int RK>
 __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeMap<S> cm, KeyGeom g, DevLimits L,
                                                   uint64_t max_ctx, const I *__restrict__ pos, uint64_t n_pos,
                                                   Splitters SP, RouterDev R, uint16_t *__restrict__ cid,
-                                                  uint32_t *__restrict__ hist, uint32_t pc) {
+                                                  uint32_t *__restrict__ hblk, uint32_t pc, uint64_t chunk) {
     constexpr uint32_t NS = kPwlSeg;
     __shared__ uint32_t chist[kCoarseBucketsMax];
     for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) chist[c] = 0;
@@ -287,15 +287,17 @@ __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeM
     __syncthreads();
     unsigned long long my_sum = 0, my_max = 0, my_fb = 0, my_cnt = 0;
     constexpr int NB = RK == 2 ? 1 : 8;  // the MLP's hidden layer does not fit eight keys in registers
-    const uint64_t step_all = (uint64_t)gridDim.x * blockDim.x * NB;
-    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x * NB; base < n_pos; base += step_all) {
+    // this block's positions: [c0, c1), the same range the scatter's block owns
+    const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
+    const uint64_t c1 = c0 + chunk < n_pos ? c0 + chunk : n_pos;
+    for (uint64_t base = c0; base < c1; base += (uint64_t)blockDim.x * NB) {
         uint64_t key[NB];
         bool valid[NB];
         uint32_t b[NB];
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
             const uint64_t i = base + threadIdx.x + (uint64_t)j * blockDim.x;
-            valid[j] = i < n_pos;
+            valid[j] = i < c1;
             key[j] = 0;
             if (valid[j]) {
                 uint64_t p = pos ? (uint64_t)pos[i] : i;
@@ -318,8 +320,7 @@ __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeM
         }
     }
     __syncthreads();
-    for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x)
-        if (chist[c]) atomicAdd(&hist[c], chist[c]);
+    for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) hblk[(uint64_t)blockIdx.x * pc + c] = chist[c];
     if (RK != 0 && R.stats) {
         atomicAdd(&st_sum, my_sum);
         atomicMax(&st_max, my_max);
@@ -403,8 +404,8 @@ __device__ __forceinline__ void block_keys_seq(const S *__restrict__ t, uint64_t
 template <typename S, typename I, int RK>
 __global__ __launch_bounds__(256) void k_classify_seq(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, KeyGeom g,
                                                       DevLimits L, uint64_t max_ctx, Splitters SP, RouterDev R,
-                                                      uint16_t *__restrict__ cid, uint32_t *__restrict__ hist,
-                                                      uint32_t pc) {
+                                                      uint16_t *__restrict__ cid, uint32_t *__restrict__ hblk,
+                                                      uint32_t pc, uint64_t chunk) {
     constexpr uint32_t NS = kPwlSeg;
     __shared__ uint32_t chist[kCoarseBucketsMax];
     for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) chist[c] = 0;
@@ -434,8 +435,11 @@ __global__ __launch_bounds__(256) void k_classify_seq(const S *__restrict__ t, u
     }
     if (threadIdx.x == 0) st_sum = st_max = st_fb = st_cnt = 0;
     unsigned long long my_sum = 0, my_max = 0, my_fb = 0, my_cnt = 0;
-    for (uint64_t base = (uint64_t)blockIdx.x * kSeqChunk; base < n; base += (uint64_t)gridDim.x * kSeqChunk) {
-        uint32_t cnt = (uint32_t)(n - base < kSeqChunk ? n - base : kSeqChunk);
+    // this block's positions: [c0, c1), the same range the scatter's block owns
+    const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
+    const uint64_t c1 = c0 + chunk < n ? c0 + chunk : n;
+    for (uint64_t base = c0; base < c1; base += kSeqChunk) {
+        uint32_t cnt = (uint32_t)(c1 - base < kSeqChunk ? c1 - base : kSeqChunk);
         uint64_t key[kSeqItems];
         bool valid[kSeqItems];
         uint32_t b[kSeqItems];
@@ -464,8 +468,7 @@ __global__ __launch_bounds__(256) void k_classify_seq(const S *__restrict__ t, u
         __syncthreads();
     }
     __syncthreads();
-    for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x)
-        if (chist[c]) atomicAdd(&hist[c], chist[c]);
+    for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) hblk[(uint64_t)blockIdx.x * pc + c] = chist[c];
     if (RK != 0 && R.stats) {
         atomicAdd(&st_sum, my_sum);
         atomicMax(&st_max, my_max);
@@ -508,10 +511,27 @@ __global__ __launch_bounds__(1024) void k_scan_u32(const uint32_t *__restrict__ 
     if (tid == 1023) out[m] = s[1023];
 }
 
+// Per-block histograms (hblk[b * pc + c]) -> exclusive prefix over blocks in
+// place (block b's first slot inside interval c, relative to the interval) and
+// interval totals in hist[c].  One thread per interval.
+__global__ void k_block_offsets(uint32_t *__restrict__ hblk, uint32_t nblocks, uint32_t pc,
+                                uint32_t *__restrict__ hist) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= pc) return;
+    uint32_t run = 0;
+    for (uint32_t b = 0; b < nblocks; ++b) {
+        const uint32_t v = hblk[(uint64_t)b * pc + c];
+        hblk[(uint64_t)b * pc + c] = run;
+        run += v;
+    }
+    hist[c] = run;
+}
+
 // ---------------------------------------------------------------------------
-// Two-level scatter.  Pass 1: a block owns a chunk of positions, counts its
-// coarse intervals, reserves one run per interval with a single global atomic
-// each, and writes (key, position) records into those runs.  Pass 2: one
+// Two-level scatter.  Pass 1: a block owns the chunk of positions the
+// classify pass counted per interval (so its run in every interval is known
+// from the prefix over blocks) and writes (key, position) records into those
+// runs.  Pass 2: one
 // workgroup per coarse interval stages the interval's inner splitters in LDS,
 // finds each record's bucket by binary lifting over them (kept as a u16 for
 // the placing pass), histograms and scans the buckets, writes the bucket
@@ -528,7 +548,8 @@ template <typename S, typename I, bool SEQ>
 __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, KeyGeom g,
                                                   DevLimits L, uint64_t max_ctx, const I *__restrict__ pos,
                                                   uint64_t n_pos, const uint16_t *__restrict__ cid, uint32_t pc,
-                                                  unsigned long long *__restrict__ ccursor, KRec<I> *__restrict__ tmp,
+                                                  const uint64_t *__restrict__ coff,
+                                                  const uint32_t *__restrict__ hblk, KRec<I> *__restrict__ tmp,
                                                   uint64_t chunk) {
     __shared__ S win[SEQ ? kSeqChunk + 72 : 1];
     __shared__ uint64_t skey[SEQ ? kSeqChunk : 1];
@@ -538,23 +559,9 @@ __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint6
     const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
     if (c0 >= n_pos) return;
     const uint64_t c1 = c0 + chunk < n_pos ? c0 + chunk : n_pos;
-    for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) lcnt[c] = 0;
-    __syncthreads();
-    for (uint64_t i0 = c0; i0 < c1; i0 += 8 * blockDim.x) {
-        uint32_t cc[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            uint64_t i = i0 + u * blockDim.x + threadIdx.x;
-            cc[u] = i < c1 ? cid[i] : 0xFFFFFFFFu;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (cc[u] != 0xFFFFFFFFu) atomicAdd(&lcnt[cc[u]], 1u);
-    }
-    __syncthreads();
+    // this block's runs: the classify pass counted its range per interval
     for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) {
-        uint32_t m = lcnt[c];
-        cbase[c] = m ? atomicAdd(&ccursor[c], (unsigned long long)m) : 0ull;
+        cbase[c] = coff[c] + hblk[(uint64_t)blockIdx.x * pc + c];
         lcnt[c] = 0;
     }
     __syncthreads();
