@@ -86,8 +86,8 @@ __global__ void k_pick_splitters(const uint64_t *__restrict__ sorted, uint64_t s
 // coarse interval into its bucket by a search over the interval's stride-1
 // inner splitters, staged in LDS.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kCoarseMax = 2047;         // coarse splitters (LDS table of the classify pass)
-constexpr uint32_t kCoarseBucketsMax = 2048;  // coarse intervals = kCoarseMax + 1
+constexpr uint32_t kCoarseMax = 1023;         // coarse splitters (LDS table of the classify pass)
+constexpr uint32_t kCoarseBucketsMax = 1024;  // coarse intervals = kCoarseMax + 1
 constexpr uint32_t kFineMax = 4096;           // buckets per coarse interval (the stride)
 
 struct Splitters {
