@@ -200,16 +200,22 @@ __device__ __forceinline__ bool rec_less(uint64_t ka, uint32_t la, uint64_t kb, 
 // packed together; li unused); otherwise (key, li) pairs compared as above.
 template <int B, int ITEMS, bool PACKED>
 __device__ __forceinline__ void block_merge_sort(uint64_t (&k)[ITEMS], uint32_t (&li)[ITEMS], uint64_t *sk,
-                                                 uint16_t *sl) {
+                                                 uint16_t *sl, uint32_t len) {
     constexpr uint32_t T = B * ITEMS;
     auto px = [](uint32_t i) { return i + i / ITEMS; };
     auto less = [](uint64_t ka, uint32_t la, uint64_t kb, uint32_t lb) {
         return PACKED ? ka < kb : rec_less(ka, la, kb, lb);
     };
     const int tid = threadIdx.x;
+    // Records with index >= len are pads, above every real record.  Merge
+    // sort keeps index ranges together (the merged run [a0, a0 + 2w) holds
+    // exactly the records of indices in that range), so a run's real records
+    // come first and their count is known: outputs past it are pads, and a
+    // second run made only of pads leaves the first one in place.
+    const bool all_pads = (uint32_t)tid * ITEMS >= len;
     // register sort: odd-even transposition network over ITEMS records
 #pragma unroll
-    for (int round = 0; round < ITEMS; ++round) {
+    for (int round = 0; round < ITEMS && !all_pads; ++round) {
 #pragma unroll
         for (int j = round & 1; j + 1 < ITEMS; j += 2) {
             if (less(k[j + 1], li[j + 1], k[j], li[j])) {
@@ -237,6 +243,21 @@ __device__ __forceinline__ void block_merge_sort(uint64_t (&k)[ITEMS], uint32_t 
         const uint32_t per = (2 * w) / ITEMS;  // threads per merged run
         const uint32_t a0 = (tid / per) * 2 * w, b0 = a0 + w;
         const uint32_t diag = (tid % per) * ITEMS;
+        const uint32_t realc = len > a0 ? (len - a0 < 2 * w ? len - a0 : 2 * w) : 0;
+        if (diag >= realc || b0 >= len) {  // pads only, or the second run is pads
+#pragma unroll
+            for (int j = 0; j < ITEMS; ++j) {
+                const uint32_t q = diag + j;
+                if (q < realc) {
+                    k[j] = sk[px(a0 + q)];
+                    if (!PACKED) li[j] = sl[px(a0 + q)];
+                } else {
+                    k[j] = ~0ull;
+                    if (!PACKED) li[j] = 0xFFFFu;
+                }
+            }
+            continue;
+        }
         uint32_t lo = diag > w ? diag - w : 0, hi = diag < w ? diag : w;
         while (lo < hi) {
             uint32_t mid = (lo + hi) >> 1;
@@ -326,7 +347,7 @@ __global__ __launch_bounds__(256) void k_tile_merge(uint64_t *__restrict__ keys,
             const uint32_t i = tid * ITEMS + j;
             k[j] = i < len ? ((sk[px(i)] & mask) << LB) | i : ~0ull;
         }
-        block_merge_sort<B, ITEMS, true>(k, li, sk, sl);
+        block_merge_sort<B, ITEMS, true>(k, li, sk, sl, len);
         const uint64_t top = first & ~mask;
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
@@ -339,7 +360,7 @@ __global__ __launch_bounds__(256) void k_tile_merge(uint64_t *__restrict__ keys,
             k[j] = sk[px(tid * ITEMS + j)];
             li[j] = sl[px(tid * ITEMS + j)];
         }
-        block_merge_sort<B, ITEMS, false>(k, li, sk, sl);
+        block_merge_sort<B, ITEMS, false>(k, li, sk, sl, len);
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
             sk[px(tid * ITEMS + j)] = k[j];
