@@ -330,6 +330,62 @@ __global__ __launch_bounds__(256) void k_pairs_packed(const S *__restrict__ t, c
     }
 }
 
+// Fast-forward for a slot of W lanes whose members share a long stretch
+// (a repeat copy, a run of one symbol): every open member loads the next J
+// whole code words (2^lgf codes each) at depth D that lie inside its window
+// and compares them with those of its subgroup's first member (lane `sub`
+// of the slot).  Returns the number of symbols, uniform across the slot, on
+// which every open member agrees with its subgroup's first member, all
+// inside every open window: rounds over them would split nothing, settle
+// nothing and end no window.  Open and settled members never share a
+// subgroup, so a first member of an open member is open.  Loads touch only
+// words holding an in-window symbol.  Must be called by all W lanes of the
+// slot at once, with at least one open lane.
+template <int W, int J>
+__device__ __forceinline__ uint64_t ff_skip(const uint64_t *__restrict__ pk, uint32_t lgf, uint32_t bits,
+                                            uint64_t pos, uint64_t D, uint64_t cap, bool open, uint32_t sub) {
+    const int lane = threadIdx.x & 63;
+    uint32_t jw = 0;
+    uint64_t pw[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) pw[j] = 0;
+    if (open) {
+        const uint64_t rem = cap > D ? cap - D : 0;
+        const uint64_t fw = rem >> lgf;
+        jw = fw < (uint64_t)J ? (uint32_t)fw : (uint32_t)J;
+        const uint64_t q = pos + D;
+        const uint64_t w0 = q >> lgf;
+        const uint32_t rb = (uint32_t)(q & ((1ull << lgf) - 1)) * bits;
+        // word j of the stretch is raw[j] (and raw[j+1] when unaligned); raw[jw]
+        // then holds the stretch's last symbol, which is inside the window
+        const uint32_t nl = jw ? jw + (rb ? 1u : 0u) : 0u;
+        uint64_t raw[J + 1];
+#pragma unroll
+        for (int i = 0; i <= J; ++i) raw[i] = (uint32_t)i < nl ? pk[w0 + i] : 0ull;
+#pragma unroll
+        for (int j = 0; j < J; ++j) pw[j] = rb ? (raw[j] << rb) | (raw[j + 1] >> (64 - rb)) : raw[j];
+    }
+    const int ld = (lane & ~(W - 1)) + (int)(sub & (W - 1));
+    const uint32_t jl = (uint32_t)__shfl((int)jw, ld);
+    uint32_t f = open ? (jw < jl ? jw : jl) : (uint32_t)J;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const uint64_t lw = shfl64(pw[j], ld);
+        if (open && (uint32_t)j < f && lw != pw[j]) f = (uint32_t)j;
+    }
+#pragma unroll
+    for (int o = 1; o < W; o <<= 1) {
+        const uint32_t v = (uint32_t)__shfl_xor((int)f, o);
+        f = v < f ? v : f;
+    }
+    return (uint64_t)f << lgf;
+}
+
+// Rounds a group of the small classes runs before its slots start probing
+// for shared stretches with ff_skip, and the code words one probe covers.
+constexpr uint32_t kFFAfter = 2;
+constexpr int kFFWords = 8;
+
 // ---------------------------------------------------------------------------
 // Groups of 3..64 members, one W-lane slot per group (grid launch).  The
 // round structure is the one described for k_wgroups_wq below; the wavefront
@@ -358,7 +414,9 @@ __global__ __launch_bounds__(256) void k_wgroups_grid(const S *__restrict__ t, C
         cap = lim < max_ctx ? lim : max_ctx;
     }
     uint64_t D = depth0;
-    for (;;) {
+    for (uint32_t round = 0;; ++round) {
+        if (cm.packed && round >= kFFAfter && __any(!settled))
+            D += ff_skip<W, kFFWords>(cm.packed, cm.lgf, g.bits, pos, D, cap, !settled, sub);
         if (!settled) {
             chunk = key_at(t, cm, g, pos, D, cap);
             uint64_t rem = cap > D ? cap - D : 0;
@@ -501,7 +559,7 @@ __global__ __launch_bounds__(256) void k_wgroups_wq(const S *__restrict__ t, Cod
     // current group
     bool busy = false;
     uint64_t start = 0, D = 0;
-    uint32_t m = 0;
+    uint32_t m = 0, rounds = 0;
     uint64_t pos = 0, lim = 0, cap = 0, chunk = ~0ull, aux = ~0ull, h = 0;
     uint32_t sub = 0xFFFFFFFFu;
     bool settled = true;
@@ -513,6 +571,7 @@ __global__ __launch_bounds__(256) void k_wgroups_wq(const S *__restrict__ t, Cod
             busy = true;
             v2 = false;
             D = depth0;
+            rounds = 0;
             h = 0;
             chunk = ~0ull;
             aux = ~0ull;
@@ -545,6 +604,8 @@ __global__ __launch_bounds__(256) void k_wgroups_wq(const S *__restrict__ t, Cod
         }
         if (!__any(busy || v1 || v2)) break;
         const bool act = busy && r < m;
+        if (cm.packed && busy && rounds >= kFFAfter)
+            D += ff_skip<W, kFFWords>(cm.packed, cm.lgf, g.bits, pos, D, cap, !settled, sub);
         if (!settled) {
             chunk = key_at(t, cm, g, pos, D, cap);
             uint64_t rem = cap > D ? cap - D : 0;
@@ -620,6 +681,7 @@ __global__ __launch_bounds__(256) void k_wgroups_wq(const S *__restrict__ t, Cod
             settled = true;
         } else {
             D += g.k;
+            ++rounds;
         }
     }
 }

@@ -57,3 +57,21 @@ def test_batch_device_entry(cuda):
         want = O.Ctx(t).build_in_memory() + start
         assert np.array_equal(out[start:e], want)
         start = e
+
+
+def test_batch_shared_stretches(cuda):
+    """Texts whose tie groups share long stretches, alone and across texts:
+    every text is built from copies of a few common units, so groups span
+    texts and part where a text ends (a window end inside a stretch)."""
+    rng = np.random.default_rng(42)
+    units = [rng.integers(0, 4, 600).astype(np.uint8) for _ in range(3)]
+    texts = []
+    for i in range(48):
+        parts = []
+        for _ in range(int(rng.integers(1, 6))):
+            parts.append(rng.integers(0, 4, int(rng.integers(1, 30))).astype(np.uint8))
+            u = units[int(rng.integers(0, 3))]
+            parts.append(u[: len(u) - int(rng.integers(0, 80))])
+        texts.append(np.concatenate(parts))
+    outs = caps_sa.build_batch(texts)
+    _check_all(texts, outs)

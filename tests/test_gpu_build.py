@@ -240,3 +240,52 @@ def test_autotune_then_build_is_exact():
     for t in report["tried"] + [choice]:
         sa, _ = dev.build_device(d_text, len(text), 1, tile_keys=t["tile_keys"], flags=t["flags"])
         assert np.array_equal(sa.cpu().numpy().view(np.uint32).astype(np.uint64), want), t
+
+
+def _copies_text(counts, stretch, seed, sigma=4, dtype=np.uint8, run=None):
+    """Units of `stretch` symbols, each copied counts[i] times with copies cut
+    to slightly different lengths and separated by short random context, so
+    tie groups of every small size share long stretches and part at
+    different depths.  `run` makes every unit a run of that one symbol."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for c in counts:
+        unit = np.full(stretch, run) if run is not None else rng.integers(0, sigma, stretch)
+        for _ in range(c):
+            parts.append(rng.integers(0, sigma, int(rng.integers(1, 40))))
+            parts.append(unit[: stretch - int(rng.integers(0, 60))])
+    return np.concatenate(parts).astype(dtype)
+
+
+def test_shared_stretches_small_groups():
+    """Groups of 3..64 members that share hundreds of symbols (the
+    refinement's fast-forward over whole code words), in every small class,
+    with windows (segment ends, max_context) ending inside the stretches."""
+    counts = [3, 4, 5, 8, 9, 16, 17, 32, 33, 64]
+    t = _copies_text(counts, 700, seed=71)
+    check(t, label="copies u8")
+    check(_copies_text(counts, 900, seed=72, sigma=6, run=5), label="runs u8")
+    for mx in (50, 333):
+        check(t, max_ctx=mx, label=f"copies max_ctx={mx}")
+    rng = np.random.default_rng(73)
+    ends = np.unique(np.concatenate([np.sort(rng.integers(1, len(t), 60)), [len(t)]]))
+    check(t, ends=ends, label="copies segmented", with_lcp=False)
+    check(t, ends=ends, order=1, label="copies star", with_lcp=False)
+    pos = np.flatnonzero(rng.integers(0, 4, len(t)) > 0).astype(np.uint64)
+    check(t, positions=pos, idx=np.uint64, label="copies positions u64")
+    check(_copies_text(counts, 500, seed=74, sigma=300, dtype=np.uint16), label="copies u16")
+
+
+def test_shared_stretches_persistent():
+    """The same texts with every small class on the work-queue kernels."""
+    import torch
+    from caps_sa import device as dev
+
+    for text in (_copies_text([3, 4, 5, 8, 9, 16, 17, 32, 33, 64], 700, seed=75),
+                 _copies_text([3, 7, 12, 40], 600, seed=76, sigma=300, dtype=np.uint16)):
+        w = text.dtype.itemsize
+        d_text = dev.text_to_device(text)
+        sa, lcp = dev.build_device(d_text, len(text), w, want_lcp=True, flags=caps_sa.FLAG_PERSISTENT_REFINE)
+        want, want_lcp = O.Ctx(text).build_in_memory(threads=4, want_lcp=True)
+        assert np.array_equal(sa.cpu().numpy().view(np.uint32).astype(np.uint64), want)
+        assert np.array_equal(lcp.cpu().numpy().view(np.uint32).astype(np.uint64), want_lcp)
