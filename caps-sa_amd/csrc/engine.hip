@@ -282,8 +282,9 @@ static void sort_keys_single_range(Workspace &W, uint64_t *keys, uint32_t *ids, 
     HIPCHK(hipMemcpyAsync(dr, &W.h_range, sizeof(Range), hipMemcpyHostToDevice, st));
     uint64_t *ks = keys, *kd = keys2;
     uint32_t *is = ids, *id = ids2;
+    static_assert(T >= kKeyMergeTile / 2 && (T & (T - 1)) == 0, "key merge runs: powers of two >= half a tile");
     for (uint64_t run = T; run < count; run *= 2) {
-        dim3 grid((unsigned)ceil_div(ceil_div(count, 8), 256), 1);
+        dim3 grid((unsigned)ceil_div(count, kKeyMergeTile), 1);
         k_key_merge<uint32_t><<<grid, 256, 0, st>>>(dr, run, ks, is, kd, id);
         LAUNCHCHK();
         std::swap(ks, kd);
@@ -625,9 +626,10 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         uint64_t *ks = keys, *kd = k2;
         I *is = d_out, *id = i2;
         int levels = 0;
+        static_assert(T >= kKeyMergeTile / 2 && (T & (T - 1)) == 0, "key merge runs: powers of two >= half a tile");
         for (uint64_t run = T; run < maxlen; run *= 2, ++levels) {
             for (unsigned y0 = 0; y0 < n_big; y0 += 65535) {
-                dim3 gr((unsigned)ceil_div(ceil_div(maxlen, 8), 256), std::min(65535u, n_big - y0));
+                dim3 gr((unsigned)ceil_div(maxlen, kKeyMergeTile), std::min(65535u, n_big - y0));
                 k_key_merge<I><<<gr, 256, 0, st>>>(big + y0, run, ks, is, kd, id);
                 LAUNCHCHK();
             }

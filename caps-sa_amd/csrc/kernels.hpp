@@ -798,44 +798,88 @@ __global__ void k_tile_emit(const uint64_t *__restrict__ off, uint32_t p, uint32
 // Merge-path key merge of sorted runs of length `run` inside ranges
 // (oversized buckets and the splitter sample).  Ties take the left run.
 // ---------------------------------------------------------------------------
+// Merge-path split of diagonal d between sorted runs A (nA) and B (nB):
+// the number of A elements among the first d outputs, ties to A.
+template <typename K>
+__device__ __forceinline__ uint64_t merge_split(const K *A, uint64_t nA, const K *B, uint64_t nB, uint64_t d) {
+    uint64_t lo = d > nB ? d - nB : 0, hi = d < nA ? d : nA;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (A[mid] <= B[d - mid - 1]) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// One merge level: every block writes kKeyMergeTile consecutive outputs of
+// one pair of runs.  Two threads find the block's merge-path splits in global
+// memory; the block stages its A and B segments in LDS with coalesced loads,
+// each thread merges 8 outputs from LDS, and the outputs go back through LDS
+// for coalesced stores.  Requires run to be a power of two >= half a tile, so
+// a block's outputs never span two pairs of runs.
+constexpr uint32_t kKeyMergeTile = 2048;
+
 template <typename I>
 __global__ __launch_bounds__(256) void k_key_merge(const Range *__restrict__ ranges, uint64_t run,
                                                    const uint64_t *__restrict__ ks, const I *__restrict__ is,
                                                    uint64_t *__restrict__ kd, I *__restrict__ id) {
-    constexpr uint32_t ITEMS = 8;
-    Range r = ranges[blockIdx.y];
-    uint64_t o = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * ITEMS;
-    if (o >= r.len) return;
-    uint64_t base = (o / (2 * run)) * (2 * run);
-    uint64_t nA = r.len - base < run ? r.len - base : run;
-    uint64_t restB = r.len - base - nA;
-    uint64_t nB = restB < run ? restB : run;
+    constexpr uint32_t ITEMS = kKeyMergeTile / 256;
+    __shared__ uint64_t sk[kKeyMergeTile];
+    __shared__ I si[kKeyMergeTile];
+    __shared__ uint64_t split[2];
+    const Range r = ranges[blockIdx.y];
+    const uint64_t o0 = (uint64_t)blockIdx.x * kKeyMergeTile;
+    if (o0 >= r.len) return;
+    const uint32_t L = (uint32_t)(r.len - o0 < kKeyMergeTile ? r.len - o0 : kKeyMergeTile);
+    const uint64_t base = (o0 / (2 * run)) * (2 * run);
+    const uint64_t nA = r.len - base < run ? r.len - base : run;
+    const uint64_t restB = r.len - base - nA;
+    const uint64_t nB = restB < run ? restB : run;
     const uint64_t *A = ks + r.start + base;
-    const uint64_t *Bk = A + nA;
+    const uint64_t *B = A + nA;
     const I *Ai = is + r.start + base;
     const I *Bi = Ai + nA;
-    uint64_t d = o - base;
-    uint64_t lo = d > nB ? d - nB : 0, hi = d < nA ? d : nA;
-    while (lo < hi) {
-        uint64_t mid = (lo + hi) >> 1;
-        if (A[mid] <= Bk[d - mid - 1]) lo = mid + 1;
-        else hi = mid;
-    }
-    uint64_t i = lo, j = d - lo;
-    uint64_t *dk = kd + r.start + o;
-    I *di = id + r.start + o;
-    uint64_t cnt = r.len - o < ITEMS ? r.len - o : ITEMS;
-    for (uint64_t q = 0; q < cnt; ++q) {
-        bool takeA = j >= nB || (i < nA && A[i] <= Bk[j]);
-        if (takeA) {
-            dk[q] = A[i];
-            di[q] = Ai[i];
-            ++i;
+    const uint64_t d0 = o0 - base;
+    if (threadIdx.x < 2) split[threadIdx.x] = merge_split(A, nA, B, nB, d0 + (threadIdx.x ? L : 0));
+    __syncthreads();
+    const uint64_t a0 = split[0], a1 = split[1];
+    const uint32_t la = (uint32_t)(a1 - a0);
+    const uint64_t b0 = d0 - a0;
+    for (uint32_t q = threadIdx.x; q < L; q += 256) {
+        if (q < la) {
+            sk[q] = A[a0 + q];
+            si[q] = Ai[a0 + q];
         } else {
-            dk[q] = Bk[j];
-            di[q] = Bi[j];
-            ++j;
+            sk[q] = B[b0 + (q - la)];
+            si[q] = Bi[b0 + (q - la)];
         }
+    }
+    __syncthreads();
+    const uint32_t lb = L - la;
+    const uint32_t d = threadIdx.x * ITEMS < L ? threadIdx.x * ITEMS : L;
+    uint32_t i = (uint32_t)merge_split(sk, la, sk + la, lb, d), j = d - i;
+    uint64_t ok[ITEMS];
+    I oi[ITEMS];
+#pragma unroll
+    for (uint32_t q = 0; q < ITEMS; ++q) {
+        const bool takeA = j >= lb || (i < la && sk[i] <= sk[la + j]);
+        const uint32_t src = takeA ? i : la + j;
+        ok[q] = sk[src < L ? src : 0];
+        oi[q] = si[src < L ? src : 0];
+        if (takeA) ++i;
+        else ++j;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t q = 0; q < ITEMS; ++q)
+        if (d + q < L) {
+            sk[d + q] = ok[q];
+            si[d + q] = oi[q];
+        }
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < L; q += 256) {
+        kd[r.start + o0 + q] = sk[q];
+        id[r.start + o0 + q] = si[q];
     }
 }
 
