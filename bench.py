@@ -235,7 +235,8 @@ def run_genome(args, n: int) -> dict:
     d_out = torch.empty(len(pos), dtype=torch.int64, device="cuda:0")
 
     def step():
-        dev.build_device(d_text, n, 1, d_positions=d_pos, n_pos=len(pos), idx_bytes=8, lp=lp, d_out=d_out)
+        dev.build_device(d_text, n, 1, d_positions=d_pos, n_pos=len(pos), idx_bytes=8, lp=lp, d_out=d_out,
+                         flags=caps_sa.FLAG_POW2_SEED if args.pow2_seed else 0)
         return caps_sa.last_stats()
 
     for _ in range(args.warmup):
@@ -281,6 +282,8 @@ def main() -> int:
     ap.add_argument("--bucket-keys", type=int, default=0, help="mean records per bucket (0: library default)")
     ap.add_argument("--buckets", type=int, default=0, help="explicit bucket count (0: from --bucket-keys)")
     ap.add_argument("--texts", type=int, default=4096, help="batch workload: number of texts")
+    ap.add_argument("--pow2-seed", action="store_true",
+                    help="genome workload: seed keys with the packed text's power-of-two field width")
     ap.add_argument("--autotune", action="store_true",
                     help="pick bucket size / tile sort / refinement style / merge width by timing builds first")
     ap.add_argument("--no-verify", action="store_true")

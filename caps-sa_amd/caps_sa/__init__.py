@@ -336,6 +336,7 @@ FLAG_PERSISTENT_REFINE = 2
 FLAG_TILE_RADIX = 4
 FLAG_MERGE_W16 = 8
 FLAG_MERGE_W64 = 16
+FLAG_POW2_SEED = 32
 TUNE_MAX = 16
 
 

@@ -102,13 +102,21 @@ struct KeyGeom {
 template <typename S>
 struct CodeMap;
 // Symbol codes.  `packed`, when set, holds the whole text as codes of
-// g.bits (a power of two) each, MSB-first in u64 words (2^lgf codes a word,
-// zero words past the end): a key is then two aligned loads and a shift.
+// 64 >> lgf bits (a power of two) each, MSB-first in u64 words (2^lgf codes
+// a word, zero words past the end): a key of that field width is then two
+// aligned loads and a shift.  `seed`, when set, holds the text in the seed
+// keys' field width b: sk = 64 / b codes per word, MSB-first, the low
+// 64 - sk*b bits zero (the packed text itself when b is a power of two);
+// word q / sk holds code q, and q / sk = umulhi(q, srec) with
+// srec = ceil(2^64 / sk), exact for q < 2^40.
 template <>
 struct CodeMap<uint8_t> {
     const uint8_t *code = nullptr;  // nullptr = identity
     const uint64_t *packed = nullptr;
     uint32_t lgf = 0;
+    uint32_t sk = 0;
+    const uint64_t *seed = nullptr;
+    uint64_t srec = 0;
     __device__ __forceinline__ uint32_t operator()(uint8_t c) const { return code ? code[c] : c; }
 };
 template <>
@@ -116,6 +124,9 @@ struct CodeMap<uint16_t> {
     const uint16_t *code = nullptr;  // nullptr = identity
     const uint64_t *packed = nullptr;
     uint32_t lgf = 0;
+    uint32_t sk = 0;
+    const uint64_t *seed = nullptr;
+    uint64_t srec = 0;
     __device__ __forceinline__ uint32_t operator()(uint16_t c) const { return code ? code[c] : c; }
 };
 
@@ -127,13 +138,8 @@ __device__ __forceinline__ uint64_t pad_fields(const KeyGeom &g, uint32_t m) {
     return r;
 }
 
-// Key of the `k` symbols starting at depth d of suffix p, whose comparison
-// window is `cap` symbols (limit and max_context already applied).  All
-// aligned 8-byte words covering the window are loaded first (each holds at
-// least one in-range byte), then the fields are extracted with constant
-// indices, so the loads overlap instead of forming a dependent chain.
-// key_at from the packed code text (dense geometry: no bias, power-of-two
-// field width, so a word holds exactly 64 / bits codes).
+// key_at from the packed code text (dense geometry: no bias; power-of-two
+// field width, so a word holds exactly 2^lgf codes).
 template <typename CM>
 __device__ __forceinline__ uint64_t key_at_packed(const CM &cm, const KeyGeom &g, uint64_t p, uint64_t d,
                                                   uint64_t cap) {
@@ -155,10 +161,13 @@ __device__ __forceinline__ uint64_t key_at_packed(const CM &cm, const KeyGeom &g
     return fields << g.shift;
 }
 
+// Key from the text bytes: all aligned 8-byte words covering the window are
+// loaded first (each holds at least one in-range byte), then the fields are
+// extracted with constant indices, so the loads overlap instead of forming a
+// dependent chain.
 template <typename S>
-__device__ __forceinline__ uint64_t key_at(const S *__restrict__ t, CodeMap<S> cm, const KeyGeom &g, uint64_t p,
-                                           uint64_t d, uint64_t cap) {
-    if (cm.packed) return key_at_packed(cm, g, p, d, cap);
+__device__ __forceinline__ uint64_t key_at_bytes(const S *__restrict__ t, CodeMap<S> cm, const KeyGeom &g, uint64_t p,
+                                                 uint64_t d, uint64_t cap) {
     constexpr uint32_t SZ = sizeof(S);
     constexpr int NW = 9;  // 64 bytes of window + up to 7 bytes of misalignment
     uint64_t rem = cap > d ? cap - d : 0;
@@ -186,6 +195,40 @@ __device__ __forceinline__ uint64_t key_at(const S *__restrict__ t, CodeMap<S> c
     if (take == 0) key = pad_fields(g, g.k);
     else if (take < g.k) key = (key << (g.bits * (g.k - take))) | pad_fields(g, g.k - take);
     return key << g.shift;
+}
+
+// Key of the `k` symbols starting at depth d of suffix p, whose comparison
+// window is `cap` symbols (limit and max_context already applied), in the
+// packed text's field width (refinement chunks).
+template <typename S>
+__device__ __forceinline__ uint64_t key_at(const S *__restrict__ t, CodeMap<S> cm, const KeyGeom &g, uint64_t p,
+                                           uint64_t d, uint64_t cap) {
+    if (cm.packed) return key_at_packed(cm, g, p, d, cap);
+    return key_at_bytes(t, cm, g, p, d, cap);
+}
+
+// Seed key of suffix p (depth 0), in the seed geometry, whose fields may be
+// narrower than the packed text's: read from the seed-width code text
+// (word w holds codes w*sk .. w*sk+sk-1; the next word is loaded only when
+// the key's codes reach into it, so it holds an in-window code).
+template <typename S>
+__device__ __forceinline__ uint64_t key_at_seed(const S *__restrict__ t, CodeMap<S> cm, const KeyGeom &g, uint64_t p,
+                                                uint64_t cap) {
+    if (!cm.seed) return key_at_bytes(t, cm, g, p, 0, cap);
+    const uint32_t take = cap < g.k ? (uint32_t)cap : g.k;
+    if (take == 0) return pad_fields(g, g.k) << g.shift;
+    const uint32_t bits = g.bits;
+    const uint64_t w = __umul64hi(p, cm.srec);
+    const uint32_t r = (uint32_t)(p - w * cm.sk);
+    uint64_t v = cm.seed[w] << (r * bits);
+    if (r + take > cm.sk) v |= cm.seed[w + 1] >> ((cm.sk - r) * bits);
+    const uint32_t kb = g.k * bits;
+    uint64_t fields = kb >= 64 ? v : (v >> (64 - kb));
+    if (take < g.k) {
+        const uint32_t drop = (g.k - take) * bits;
+        fields = ((fields >> drop) << drop) | pad_fields(g, g.k - take);
+    }
+    return fields << g.shift;
 }
 
 // Keys of the consecutive positions base+first .. base+first+cnt-1 from a

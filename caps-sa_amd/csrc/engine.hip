@@ -125,7 +125,8 @@ DevLimits upload_limits(Workspace &W, const caps_limits_t *lim, uint64_t n, hipS
 // Key geometry from the alphabet (pack.rs:103-176 for the reference form).
 // ---------------------------------------------------------------------------
 template <typename S>
-Geometry compute_geometry(Workspace &W, const S *d_text, uint64_t n, uint32_t mode, uint32_t order, hipStream_t st) {
+Geometry compute_geometry(Workspace &W, const S *d_text, uint64_t n, uint32_t mode, uint32_t order, hipStream_t st,
+                          bool pow2_seed = false) {
     constexpr uint32_t WORDS = sizeof(S) == 1 ? 8 : 2048;
     uint32_t *bm = (uint32_t *)W.bitmap.get(WORDS * 4);
     HIPCHK(hipMemsetAsync(bm, 0, WORDS * 4, st));
@@ -163,15 +164,21 @@ Geometry compute_geometry(Workspace &W, const S *d_text, uint64_t n, uint32_t mo
         }
     }
     if (!ref) {
-        // power-of-two field width (1, 2, 4, 8, 16), as the reference's
-        // packer: codes then tile u64 words exactly
+        // seed keys: ceil(log2(distinct))-bit fields, as many symbols as fit
+        // in 64 bits; the packed code text (and the refinement's chunks) use
+        // the power of two above (1, 2, 4, 8, 16), so codes tile u64 words
+        // exactly, as in the reference's packer
         uint32_t bits = 1;
-        while ((1ull << bits) < distinct) bits <<= 1;
+        while ((1ull << bits) < distinct) ++bits;
+        uint32_t pb = 1;
+        while (pb < bits) pb <<= 1;
+        if (pow2_seed) bits = pb;
         G.g.bits = bits;
         G.g.k = 64 / bits;
         G.g.shift = 64 - G.g.k * bits;
         G.g.bias = 0;
         G.g.pad = longer_first ? (uint32_t)((1ull << bits) - 1) : 0;
+        G.pbits = pb;
         G.remap = (uint64_t)maxv >= (1ull << bits);
     }
     G.reference = ref;
@@ -448,9 +455,16 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     }
 
     // ---- alphabet and key geometry ----
-    Geometry G = compute_geometry<S>(W, d_text, n, idbits ? CAPS_KEY_DENSE : o.key_geometry, L.order, st);
-    const KeyGeom gref = G.g;  // refinement chunks: full width, no text index
-    KeyGeom g = G.g;         // seed keys
+    Geometry G = compute_geometry<S>(W, d_text, n, idbits ? CAPS_KEY_DENSE : o.key_geometry, L.order, st,
+                                     (o.flags & CAPS_FLAG_POW2_SEED) != 0);
+    KeyGeom gref = G.g;  // refinement chunks: packed-text field width, full width, no text index
+    if (!G.reference) {
+        gref.bits = G.pbits;
+        gref.k = 64 / G.pbits;
+        gref.shift = 0;
+        gref.pad = L.order == CAPS_ORDER_STAR ? (uint32_t)((1ull << G.pbits) - 1) : 0;
+    }
+    KeyGeom g = G.g;  // seed keys
     if (idbits) {
         if (idbits + G.g.bits > 64) throw CapsError(CAPS_E_INVALID, "too many texts in one batch");
         g.idbits = idbits;
@@ -460,13 +474,27 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     CodeMap<S> cm = codemap<S>(G);
     if (!G.reference) {  // packed code text for the key gathers
         uint32_t lgf = 0;
-        while ((g.bits << lgf) < 64) ++lgf;
+        while ((gref.bits << lgf) < 64) ++lgf;
         const uint64_t nwords = (n >> lgf) + 2;
         uint64_t *pk = (uint64_t *)W.packed.get(nwords * 8);
-        k_pack_codes<S><<<grid_for(nwords, 256, 8192), 256, 0, st>>>(d_text, n, cm, g.bits, lgf, pk, nwords);
+        k_pack_codes<S><<<grid_for(nwords, 256, 8192), 256, 0, st>>>(d_text, n, cm, gref.bits, lgf, pk, nwords);
         LAUNCHCHK();
         cm.packed = pk;
         cm.lgf = lgf;
+        // seed keys: the packed text itself, or a text in their narrower fields
+        if (n >= (1ull << 40)) throw CapsError(CAPS_E_INVALID, "text too long for packed seed keys");
+        const uint32_t K = 64 / g.bits;
+        if (g.bits == gref.bits) {
+            cm.seed = pk;
+        } else {
+            const uint64_t swords = n / K + 2;
+            uint64_t *sp = (uint64_t *)W.seedpk.get(swords * 8);
+            k_pack_seed<S><<<grid_for(swords, 256, 8192), 256, 0, st>>>(d_text, n, cm, g.bits, K, sp, swords);
+            LAUNCHCHK();
+            cm.seed = sp;
+        }
+        cm.sk = K;
+        cm.srec = (uint64_t)(~0ull / K) + 1;  // ceil(2^64 / K)
     }
     stats.key_bits = g.bits;
     stats.key_k = g.k;

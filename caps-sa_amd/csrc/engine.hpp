@@ -18,7 +18,8 @@ struct DevBuf {
 };
 
 struct Geometry {
-    KeyGeom g;
+    KeyGeom g;                   // seed keys
+    uint32_t pbits = 0;          // packed code text and refinement chunks: power of two >= g.bits
     const void *code = nullptr;  // device code map or nullptr (identity)
     bool remap = false;
     bool reference = false;
@@ -34,7 +35,7 @@ struct Workspace {
     std::vector<uint64_t> h_ends;
     std::vector<uint32_t> h_dir;
     // key geometry
-    DevBuf bitmap, code, packed;
+    DevBuf bitmap, code, packed, seedpk;
     std::vector<uint8_t> h_code8;
     std::vector<uint16_t> h_code16;
     // sample and splitters

@@ -47,7 +47,7 @@ __global__ __launch_bounds__(256) void k_sample_keys(const S *__restrict__ t, Co
     uint64_t p = pos ? (uint64_t)pos[j] : j;
     uint64_t seg;
     uint64_t lim = lim_seg(L, p, &seg);
-    keys[i] = with_id(g, key_at(t, cm, g, p, 0, lim < max_ctx ? lim : max_ctx), seg);
+    keys[i] = with_id(g, key_at_seed(t, cm, g, p, lim < max_ctx ? lim : max_ctx), seg);
     ids[i] = (uint32_t)i;
 }
 
@@ -66,7 +66,7 @@ __global__ __launch_bounds__(256) void k_random_keys(const S *__restrict__ t, Co
     uint64_t p = pos ? (uint64_t)pos[j] : j;
     uint64_t seg;
     uint64_t lim = lim_seg(L, p, &seg);
-    keys[i] = with_id(g, key_at(t, cm, g, p, 0, lim < max_ctx ? lim : max_ctx), seg);
+    keys[i] = with_id(g, key_at_seed(t, cm, g, p, lim < max_ctx ? lim : max_ctx), seg);
 }
 
 // splitter j-1 = sorted_sample[(j * s) / p], j = 1..p-1 (ext_mem.rs:1341).
@@ -132,6 +132,24 @@ __global__ void k_pack_codes(const S *__restrict__ t, uint64_t n, CodeMap<S> cm,
         const uint64_t j0 = w << lgf;
         uint64_t v = 0;
         for (uint32_t f = 0; f < F; ++f) {
+            const uint64_t j = j0 + f;
+            const uint32_t c = j < n ? cm(t[j]) : 0u;
+            v |= (uint64_t)c << (64 - (f + 1) * bits);
+        }
+        out[w] = v;
+    }
+}
+
+// Seed-width code text: K codes of `bits` bits per word (bits not a power of
+// two), MSB-first, the low 64 - K*bits bits zero.
+template <typename S>
+__global__ void k_pack_seed(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, uint32_t bits, uint32_t K,
+                            uint64_t *__restrict__ out, uint64_t nwords) {
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords;
+         w += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t j0 = w * K;
+        uint64_t v = 0;
+        for (uint32_t f = 0; f < K; ++f) {
             const uint64_t j = j0 + f;
             const uint32_t c = j < n ? cm(t[j]) : 0u;
             v |= (uint64_t)c << (64 - (f + 1) * bits);
@@ -303,7 +321,7 @@ __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeM
                 uint64_t p = pos ? (uint64_t)pos[i] : i;
                 uint64_t seg;
                 uint64_t lim = lim_seg(L, p, &seg);
-                key[j] = with_id(g, key_at(t, cm, g, p, 0, lim < max_ctx ? lim : max_ctx), seg);
+                key[j] = with_id(g, key_at_seed(t, cm, g, p, lim < max_ctx ? lim : max_ctx), seg);
             }
         }
         if (RK == 0) coarse_bsearch<NB>(SP, coarse, key, b);
@@ -596,7 +614,7 @@ __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint6
             unsigned long long slot = cbase[c] + atomicAdd(&lcnt[c], 1u);
             uint64_t seg;
             uint64_t lim = lim_seg(L, p, &seg);
-            tmp[slot] = KRec<I>{with_id(g, key_at(t, cm, g, p, 0, lim < max_ctx ? lim : max_ctx), seg), (I)p};
+            tmp[slot] = KRec<I>{with_id(g, key_at_seed(t, cm, g, p, lim < max_ctx ? lim : max_ctx), seg), (I)p};
         }
     }
 }

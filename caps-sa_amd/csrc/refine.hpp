@@ -96,9 +96,24 @@ __global__ __launch_bounds__(256) void k_detect_ranges(const uint64_t *__restric
             uint64_t k0 = keys[i];
             bool start = o == 0 || keys[i - 1] != k0;
             if (start && o + 1 < r.len && keys[i + 1] == k0) {
-                uint64_t j = o + 2;
-                while (j < r.len && keys[r.start + j] == k0) ++j;
-                gr = Group{i, j - o};
+                // the range is sorted, so the group ends at the first larger
+                // key: gallop, then bisect (lo: equal, hi: larger or r.len)
+                uint64_t lo = o + 1, hi = r.len;
+                for (uint64_t step = 1;; step <<= 1) {
+                    const uint64_t j = o + 1 + step;
+                    if (j >= r.len) break;
+                    if (keys[r.start + j] != k0) {
+                        hi = j;
+                        break;
+                    }
+                    lo = j;
+                }
+                while (hi - lo > 1) {
+                    const uint64_t mid = lo + ((hi - lo) >> 1);
+                    if (keys[r.start + mid] == k0) lo = mid;
+                    else hi = mid;
+                }
+                gr = Group{i, hi - o};
                 have = true;
             }
         }

@@ -91,6 +91,11 @@ typedef struct caps_opts {
  * per thread instead of 32.  The output never changes. */
 #define CAPS_FLAG_MERGE_W16 8u
 #define CAPS_FLAG_MERGE_W64 16u
+/* Dense seed keys with power-of-two fields (the packed text's width, e.g. 4
+ * bits and 16 symbols for a 6-symbol alphabet) instead of ceil(log2(sigma))
+ * bits (3 bits, 21 symbols): fewer symbols per key, more ties to refine.
+ * The output never changes. */
+#define CAPS_FLAG_POW2_SEED 32u
 
 /* Fills opts with the reference defaults (max_context unbounded, p auto). */
 void caps_opts_default(caps_opts_t *opts);

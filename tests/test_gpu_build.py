@@ -289,3 +289,40 @@ def test_shared_stretches_persistent():
         want, want_lcp = O.Ctx(text).build_in_memory(threads=4, want_lcp=True)
         assert np.array_equal(sa.cpu().numpy().view(np.uint32).astype(np.uint64), want)
         assert np.array_equal(lcp.cpu().numpy().view(np.uint32).astype(np.uint64), want_lcp)
+
+
+def test_tight_and_pow2_seed_fields_agree():
+    """Seed keys with ceil(log2(sigma))-bit fields (narrowed from the packed
+    text's power-of-two fields) and with power-of-two fields give the same
+    SA and LCP, equal to the oracle's: 3-, 5-, 7-bit u8 alphabets, 10- and
+    13-bit u16 alphabets, plain, segmented and STAR, with max_context."""
+    import torch
+    from caps_sa import device as dev
+
+    rng = np.random.default_rng(0x71)
+    cases = []
+    for sigma, dt in ((6, np.uint8), (20, np.uint8), (100, np.uint8), (1000, np.uint16), (5000, np.uint16)):
+        t = rng.integers(0, sigma, 30000).astype(dt)
+        t[5000:9000] = t[15000:19000]  # a long repeat: deep groups
+        cases.append(t)
+    for t in cases:
+        w = t.dtype.itemsize
+        n = len(t)
+        d_text = dev.text_to_device(t)
+        lengths = rng.integers(1, 400, 200)
+        lengths = lengths[np.cumsum(lengths) < n]
+        ends = np.append(np.cumsum(lengths), n).astype(np.uint64)
+        for order, lp in ((0, None), (0, caps_sa.SegmentedText(n, ends)), (1, caps_sa.StarSegmented(n, ends))):
+            for mx in (MAX, 37):
+                ctx = O.Ctx(t, None if lp is None else ends, order)
+                want = ctx.build_in_memory(max_ctx=mx, threads=4)
+                if lp is not None and order == 0:
+                    want = ctx.canonicalize(want, mx)
+                outs = []
+                for flags in (0, caps_sa.FLAG_POW2_SEED):
+                    sa, lcp = dev.build_device(d_text, n, w, lp=lp, want_lcp=True, max_context=mx, flags=flags)
+                    outs.append((sa.cpu().numpy().view(np.uint32).astype(np.uint64), lcp.cpu().numpy()))
+                label = (t.dtype, int(t.max()), order, lp is not None, mx)
+                assert np.array_equal(outs[0][0], want), label
+                assert np.array_equal(outs[1][0], want), label
+                assert np.array_equal(outs[0][1], outs[1][1]), label
