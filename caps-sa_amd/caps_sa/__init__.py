@@ -18,6 +18,7 @@ from __future__ import annotations
 import ctypes
 import enum
 import os
+import warnings
 from dataclasses import dataclass, field
 from typing import Callable, Optional, Sequence
 
@@ -55,6 +56,7 @@ class _Opts(ctypes.Structure):
         ("tile_keys", ctypes.c_uint32),
         ("want_lcp", ctypes.c_uint32),
         ("flags", ctypes.c_uint32),
+        ("batch_records", ctypes.c_uint64),
     ]
 
 
@@ -84,7 +86,7 @@ class Stats(ctypes.Structure):
             "router_err_sum", "router_err_max", "router_fallbacks", "router_count",
         )
     ] + [("key_bits", ctypes.c_uint32), ("key_k", ctypes.c_uint32), ("bucket_max", ctypes.c_uint64),
-         ("k_scatter1", ctypes.c_double), ("k_scatter2", ctypes.c_double)]
+         ("k_scatter1", ctypes.c_double), ("k_scatter2", ctypes.c_double), ("batches", ctypes.c_uint64)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -114,6 +116,14 @@ _SIGNATURES = {
     "caps_build_ext_mem_for_positions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
                                                          ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(_Limits),
                                                          ctypes.POINTER(_Opts), EMIT_FN, ctypes.c_void_p]),
+    "caps_build_ext_mem_for_filter": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                                                      ctypes.c_void_p, ctypes.POINTER(_Limits),
+                                                      ctypes.POINTER(_Opts), EMIT_FN, ctypes.c_void_p]),
+    "caps_build_device_filtered": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                                                   ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(_Limits),
+                                                   ctypes.POINTER(_Opts), ctypes.c_void_p, ctypes.c_uint64,
+                                                   ctypes.POINTER(ctypes.c_uint64), ctypes.c_void_p,
+                                                   ctypes.c_void_p]),
     "caps_build_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                          ctypes.POINTER(_Opts)]),
     "caps_build_batch_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p,
@@ -313,6 +323,7 @@ class ExtMemOpts:
     router: Router = Router.BSEARCH
     gpu_buckets: int = 0
     tile_keys: int = 0
+    batch_records: int = 0  # records per bucket batch (0: one batch while it fits in HBM)
 
     @classmethod
     def from_env(cls) -> "ExtMemOpts":
@@ -341,7 +352,7 @@ TUNE_MAX = 16
 
 
 def _c_opts(max_context=MAX_CONTEXT, subproblem_count=0, router=0, key_geometry=0, gpu_buckets=0, tile_keys=0,
-            want_lcp=0, flags=0) -> _Opts:
+            want_lcp=0, flags=0, batch_records=0) -> _Opts:
     o = _Opts()
     lib().caps_opts_default(ctypes.byref(o))
     o.max_context = min(int(max_context), MAX_CONTEXT)
@@ -352,6 +363,7 @@ def _c_opts(max_context=MAX_CONTEXT, subproblem_count=0, router=0, key_geometry=
     o.tile_keys = int(tile_keys)
     o.want_lcp = int(want_lcp)
     o.flags = int(flags)
+    o.batch_records = int(batch_records)
     return o
 
 
@@ -359,7 +371,7 @@ def _ext_c_opts(opts: Optional[ExtMemOpts]) -> _Opts:
     opts = opts or ExtMemOpts()
     geom = 1 if opts.packed_prefix_seed == PackedPrefixSeedPolicy.REMAP else 0
     return _c_opts(opts.max_context, opts.subproblem_count, int(opts.router), geom, opts.gpu_buckets,
-                   opts.tile_keys)
+                   opts.tile_keys, batch_records=opts.batch_records)
 
 
 def _as_text(text) -> np.ndarray:
@@ -473,6 +485,93 @@ class BuildError(Exception):
         super().__init__(f"{kind}: {inner}")
         self.kind = kind
         self.inner = inner
+
+
+def _emit_callback(emit, err_box):
+    def _cb(_ctx, ptr, count):
+        try:
+            arr = np.ctypeslib.as_array(ptr, shape=(count,))
+            for v in arr.tolist():
+                emit(v)
+            return 0
+        except Exception as exc:  # the callback's error aborts the build
+            err_box.append(exc)
+            return 1
+
+    return EMIT_FN(_cb)
+
+
+def _stream_result(rc, err_box):
+    if rc == CAPS_E_EMIT:
+        raise BuildError("Emit", err_box[0] if err_box else lib().caps_last_emit_code())
+    if rc != CAPS_OK:
+        raise BuildError("Io", CapsError(rc, lib().caps_last_error().decode()))
+
+
+def keep_bitmap(keep, n: int) -> np.ndarray:
+    """The kept-position bitmap the crate's FilteredSource builds by walking
+    the predicate once (ext_mem.rs:940-990): (n + 63) // 64 u64 words,
+    position q kept iff bit q % 64 of word q // 64 is set.  `keep` is a
+    boolean array of length n, or a predicate over positions: it is first
+    called once on the whole position array (a vectorised predicate), and
+    per position if that does not give n booleans."""
+    if callable(keep):
+        mask = None
+        try:
+            with warnings.catch_warnings():  # a per-position predicate may reject the array
+                warnings.simplefilter("ignore")
+                m = np.asarray(keep(np.arange(n, dtype=np.uint64)))
+            if m.shape == (n,):
+                mask = m.astype(bool)
+        except Exception:
+            mask = None
+        if mask is None:
+            mask = np.fromiter((bool(keep(q)) for q in range(n)), dtype=bool, count=n)
+    else:
+        mask = np.asarray(keep, dtype=bool)
+        if mask.shape != (n,):
+            raise ValueError("keep mask must have one entry per position")
+    words = (n + 63) // 64
+    packed = np.packbits(mask, bitorder="little")
+    buf = np.zeros(words * 8, dtype=np.uint8)
+    buf[: len(packed)] = packed
+    return buf.view(np.uint64)
+
+
+def try_build_ext_mem_for_filter_with(text, keep, lp, opts: Optional[ExtMemOpts], emit):
+    """ext_mem.rs:522-547: the kept positions come from `keep` (see
+    keep_bitmap), walked once into a bitmap; no position list is built."""
+    t = _as_text(text)
+    n = len(t)
+    bits = keep_bitmap(keep, n)
+    lim = _limits(lp, n)
+    o = _ext_c_opts(opts)
+    err_box = []
+    cb = _emit_callback(emit, err_box)
+    rc = lib().caps_build_ext_mem_for_filter(t.ctypes.data, n, t.dtype.itemsize, bits.ctypes.data,
+                                             ctypes.byref(lim), ctypes.byref(o), cb, None)
+    _stream_result(rc, err_box)
+
+
+def try_build_ext_mem_for_filter(text, keep, opts, emit):
+    """ext_mem.rs:487-499."""
+    try_build_ext_mem_for_filter_with(text, keep, None, opts, emit)
+
+
+def build_ext_mem_for_filter(text, keep, opts, emit):
+    """ext_mem.rs:472-484: errors surface as OSError (io::Error)."""
+    try:
+        try_build_ext_mem_for_filter_with(text, keep, None, opts, emit)
+    except BuildError as e:
+        raise OSError(str(e)) from e
+
+
+def build_ext_mem_for_filter_with(text, keep, lp, opts, emit):
+    """ext_mem.rs:503-519."""
+    try:
+        try_build_ext_mem_for_filter_with(text, keep, lp, opts, emit)
+    except BuildError as e:
+        raise OSError(str(e)) from e
 
 
 def _stream(entry, text, positions, lp, opts, emit):

@@ -31,7 +31,7 @@ def text_to_device(text: np.ndarray, device="cuda:0") -> torch.Tensor:
 
 def build_device(d_text: torch.Tensor, n: int, sym_bytes: int, d_positions=None, n_pos=None, idx_bytes=4,
                  lp: LimitProvider = None, max_context=MAX_CONTEXT, router=0, gpu_buckets=0, key_geometry=0,
-                 d_out=None, d_lcp=None, want_lcp=False, flags=0, tile_keys=0):
+                 d_out=None, d_lcp=None, want_lcp=False, flags=0, tile_keys=0, batch_records=0):
     """caps_build_device: all buffers already in HBM.  Returns (sa, lcp)."""
     n_pos = n if d_positions is None else (n_pos if n_pos is not None else d_positions.numel())
     if d_out is None:
@@ -40,13 +40,35 @@ def build_device(d_text: torch.Tensor, n: int, sym_bytes: int, d_positions=None,
         d_lcp = torch.empty(n_pos, dtype=_IDX[idx_bytes], device=d_text.device)
     lim = (lp or PlainText(n))._c()
     o = _c_opts(max_context, router=router, gpu_buckets=gpu_buckets, key_geometry=key_geometry,
-                want_lcp=int(d_lcp is not None), flags=flags, tile_keys=tile_keys)
+                want_lcp=int(d_lcp is not None), flags=flags, tile_keys=tile_keys, batch_records=batch_records)
     rc = lib().caps_build_device(d_text.data_ptr(), n, sym_bytes,
                                  d_positions.data_ptr() if d_positions is not None else None, n_pos, idx_bytes,
                                  ctypes.byref(lim), ctypes.byref(o), d_out.data_ptr(),
                                  d_lcp.data_ptr() if d_lcp is not None else None, _stream_ptr())
     _check(rc)
     return d_out, d_lcp
+
+
+def build_device_filtered(d_text: torch.Tensor, n: int, sym_bytes: int, d_keep: torch.Tensor, capacity: int,
+                          idx_bytes=8, lp: LimitProvider = None, max_context=MAX_CONTEXT, router=0, d_out=None,
+                          d_lcp=None, want_lcp=False, flags=0, tile_keys=0, batch_records=0):
+    """caps_build_device_filtered: text and kept-position bitmap ((n + 63) // 64
+    words, as keep_bitmap builds) in HBM.  Returns (sa, lcp, kept count); the
+    outputs hold `capacity` entries, of which the first `kept` are filled."""
+    if d_out is None:
+        d_out = torch.empty(max(capacity, 1), dtype=_IDX[idx_bytes], device=d_text.device)
+    if want_lcp and d_lcp is None:
+        d_lcp = torch.empty(max(capacity, 1), dtype=_IDX[idx_bytes], device=d_text.device)
+    lim = (lp or PlainText(n))._c()
+    o = _c_opts(max_context, router=router, want_lcp=int(d_lcp is not None), flags=flags, tile_keys=tile_keys,
+                batch_records=batch_records)
+    kept = ctypes.c_uint64(0)
+    rc = lib().caps_build_device_filtered(d_text.data_ptr(), n, sym_bytes, d_keep.data_ptr(), idx_bytes,
+                                          ctypes.byref(lim), ctypes.byref(o), d_out.data_ptr(), capacity,
+                                          ctypes.byref(kept), d_lcp.data_ptr() if d_lcp is not None else None,
+                                          _stream_ptr())
+    _check(rc)
+    return d_out, d_lcp, kept.value
 
 
 def lcp_pairs(d_text: torch.Tensor, n: int, sym_bytes: int, pairs: np.ndarray, max_context=MAX_CONTEXT):

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -324,7 +325,8 @@ static std::vector<float> mlp_init_params(uint64_t seed) {
 template <typename S, typename I>
 static RouterDev train_router(Workspace &W, uint32_t kind, uint32_t flags, const uint64_t *sorted, uint64_t s,
                               const Splitters &SP, const S *d_text, CodeMap<S> cm, KeyGeom g, DevLimits L,
-                              uint64_t max_ctx, const I *d_pos, uint64_t n_pos, hipStream_t st) {
+                              uint64_t max_ctx, const I *d_pos, uint64_t n_pos, hipStream_t st,
+                              const uint64_t *d_keep, uint64_t n) {
     // the router predicts the position among the coarse splitters (0..n_coarse)
     const uint32_t nc = SP.n_coarse;
     uint64_t *ctab = (uint64_t *)W.r_tmp.get((uint64_t)std::max<uint32_t>(nc, 1) * 8);
@@ -364,7 +366,7 @@ static RouterDev train_router(Workspace &W, uint32_t kind, uint32_t flags, const
         uint64_t *keys = (uint64_t *)W.r_x.get(m * 8);
         float *y = (float *)W.r_y.get(m * 4);
         k_random_keys<S, I><<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, m,
-                                                                         0x5EEDull, keys);
+                                                                         0x5EEDull, keys, d_keep, n);
         LAUNCHCHK();
         k_router_labels<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(keys, m, spl, (uint32_t)(p - 1), nullptr, y,
                                                                      1.f / (float)std::max<uint64_t>(p - 1, 1));
@@ -416,10 +418,52 @@ static RefinePlan refine_plan(const caps_opts_t &o) {
     return p;
 }
 
+// Receives each finished batch of the suffix array (device pointer, count,
+// the build's stream) in order, instead of the build writing into d_out.
+template <typename I>
+struct BatchSink {
+    std::function<void(const I *, uint64_t, hipStream_t)> flush;
+};
+
+// Records per bucket batch when the caller does not choose: every record in
+// one batch while the batch buffers (about 64 bytes per record: records,
+// keys, bucket ids, group lists, merge scratch) fit in 70% of the free HBM.
+static uint64_t auto_batch_records(uint64_t n_pos) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || free_b == 0) return n_pos;
+    const uint64_t fit = std::max<uint64_t>(1ull << 24, (uint64_t)(free_b * 0.7) / 64);
+    return std::min<uint64_t>(n_pos, fit);
+}
+
+// Clears the bits past n of a device bitmap and returns its kept count.
+static uint64_t device_keep_count(Workspace &W, uint64_t *d_keep, uint64_t n, hipStream_t st) {
+    unsigned long long *cnt = (unsigned long long *)W.keep_count.get(8);
+    HIPCHK(hipMemsetAsync(cnt, 0, 8, st));
+    if (n) {
+        k_keep_count<<<grid_for(ceil_div(n, 64), 256, 4096), 256, 0, st>>>(d_keep, n, cnt);
+        LAUNCHCHK();
+    }
+    unsigned long long h = 0;
+    HIPCHK(hipMemcpyAsync(&h, cnt, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return h;
+}
+
+// Position of the only kept position of a bitmap (n_pos == 1 builds).
+template <typename I>
+__global__ void k_only_kept(const uint64_t *__restrict__ keep, uint64_t n, I *__restrict__ out) {
+    const uint64_t nw = (n + 63) >> 6;
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nw; w += (uint64_t)gridDim.x * blockDim.x)
+        if (keep[w]) out[0] = (I)((w << 6) + (uint64_t)__builtin_ctzll(keep[w]));
+}
+
+// d_keep (optional): a bitmap of kept positions (bit q & 63 of word q >> 6;
+// n_pos = its population count, d_pos null).  sink (optional): batches go
+// to it instead of d_out.
 template <typename S, typename I>
 void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, const caps_limits_t *lim,
                 const caps_opts_t &o, I *d_out, I *d_lcp_out, hipStream_t st, caps_stats_t &stats,
-                uint32_t idbits = 0) {
+                uint32_t idbits = 0, const uint64_t *d_keep = nullptr, BatchSink<I> *sink = nullptr) {
     Workspace &W = workspace();
     if (!W.n_cu) {
         int dev = 0, cus = 0;
@@ -443,14 +487,17 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         HIPCHK(hipMemcpyAsync(cp, d_pos, n_pos * sizeof(I), hipMemcpyDeviceToDevice, st));
         d_pos = cp;
     }
-    const bool want_lcp = d_lcp_out != nullptr;
+    const bool want_lcp = d_lcp_out != nullptr && !sink;
     if (n_pos == 1) {
-        if (d_pos) HIPCHK(hipMemcpyAsync(d_out, d_pos, sizeof(I), hipMemcpyDeviceToDevice, st));
-        else HIPCHK(hipMemsetAsync(d_out, 0, sizeof(I), st));
+        I *o1 = sink ? (I *)W.sinkbuf.get(16) : d_out;
+        if (d_pos) HIPCHK(hipMemcpyAsync(o1, d_pos, sizeof(I), hipMemcpyDeviceToDevice, st));
+        else if (d_keep) k_only_kept<I><<<grid_for(ceil_div(n, 64), 256, 1024), 256, 0, st>>>(d_keep, n, o1);
+        else HIPCHK(hipMemsetAsync(o1, 0, sizeof(I), st));
         if (want_lcp) HIPCHK(hipMemsetAsync(d_lcp_out, 0, sizeof(I), st));
         tm.mark(PH_ALPHA);
         HIPCHK(hipStreamSynchronize(st));
         tm.collect(stats);
+        if (sink) sink->flush(o1, 1, st);
         return;
     }
 
@@ -516,7 +563,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         uint64_t *sk = (uint64_t *)W.skeys.get(s * 8), *sk2 = (uint64_t *)W.skeys2.get(s * 8);
         uint32_t *si = (uint32_t *)W.sids.get(s * 4), *si2 = (uint32_t *)W.sids2.get(s * 4);
         k_sample_keys<S, I><<<grid_for(s, 256, 1u << 30), 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, s,
-                                                                         sk, si);
+                                                                         sk, si, d_keep, n);
         LAUNCHCHK();
         uint32_t *sorted_ids;
         sort_keys_single_range<T>(W, sk, si, sk2, si2, s, st, &sorted_sample, &sorted_ids);
@@ -543,10 +590,15 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     uint32_t rkind = p > 1 ? o.router : CAPS_ROUTER_BSEARCH;
     if (rkind > CAPS_ROUTER_MLP) throw CapsError(CAPS_E_INVALID, "unknown router");
     if (rkind != CAPS_ROUTER_BSEARCH)
-        R = train_router<S, I>(W, rkind, o.flags, sorted_sample, s, SP, d_text, cm, g, L, max_ctx, d_pos, n_pos, st);
+        R = train_router<S, I>(W, rkind, o.flags, sorted_sample, s, SP, d_text, cm, g, L, max_ctx, d_pos, n_pos, st,
+                               d_keep, n);
     tm.mark(PH_SAMPLE);
     kt.begin(&stats.k_classify);
-    const bool seq = d_pos == nullptr && n_pos == n;
+    // sequential: blocks own ranges of text positions and take keys from
+    // LDS-staged text windows (every position, or the kept ones of a bitmap)
+    const bool seq = d_keep != nullptr || (d_pos == nullptr && n_pos == n);
+    const uint64_t n_iter = seq ? n : n_pos;  // positions the classify blocks walk
+    if (seq && n_iter > n_pos) cid = (uint16_t *)W.bid.get(n_iter * 2);
     // Every block owns one contiguous range of positions (a multiple of the
     // key window), in both the classify pass and scatter pass 1: the classify
     // block's histogram is then exactly the scatter block's run sizes.  About
@@ -562,16 +614,19 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
                    : rkind == CAPS_ROUTER_MLP ? resident_blocks(k_classify<S, I, 2>, 256)
                                               : resident_blocks(k_classify<S, I, 0>, 256);
     const uint64_t chunk =
-        std::max<uint64_t>(kSeqChunk, ceil_div(ceil_div(n_pos, (uint64_t)resident), kSeqChunk) * kSeqChunk);
-    const unsigned nblk = (unsigned)ceil_div(n_pos, chunk);
+        std::max<uint64_t>(kSeqChunk, ceil_div(ceil_div(n_iter, (uint64_t)resident), kSeqChunk) * kSeqChunk);
+    const unsigned nblk = (unsigned)ceil_div(n_iter, chunk);
     uint32_t *hblk = (uint32_t *)W.hblk.get((uint64_t)nblk * pc * 4);
     if (seq) {
         if (rkind == CAPS_ROUTER_PWL)
-            k_classify_seq<S, I, 1><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, cid, hblk, pc, chunk);
+            k_classify_seq<S, I, 1><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, cid, hblk, pc, chunk,
+                                                          d_keep);
         else if (rkind == CAPS_ROUTER_MLP)
-            k_classify_seq<S, I, 2><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, cid, hblk, pc, chunk);
+            k_classify_seq<S, I, 2><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, cid, hblk, pc, chunk,
+                                                          d_keep);
         else
-            k_classify_seq<S, I, 0><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, cid, hblk, pc, chunk);
+            k_classify_seq<S, I, 0><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, cid, hblk, pc, chunk,
+                                                          d_keep);
     } else if (rkind == CAPS_ROUTER_PWL)
         k_classify<S, I, 1><<<nblk, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hblk, pc, chunk);
     else if (rkind == CAPS_ROUTER_MLP)
@@ -586,110 +641,46 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     LAUNCHCHK();
     tm.mark(PH_CLASSIFY);
 
-    // ---- two-level scatter ----
-    uint64_t *keys = (uint64_t *)W.keys.get(n_pos * 8);
-    KRec<I> *tmp = (KRec<I> *)W.recs.get(n_pos * sizeof(KRec<I>));
-    kt.begin(&stats.k_scatter1);
-    if (seq)
-        k_scatter1<S, I, true><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, cid, pc, coff, hblk,
-                                                     tmp, chunk);
-    else
-        k_scatter1<S, I, false><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, cid, pc, coff, hblk,
-                                                      tmp, chunk);
-    LAUNCHCHK();
-    kt.end();
-    // the coarse ids are consumed: their buffer holds the per-record bucket within its interval
-    kt.begin(&stats.k_scatter2);
-    k_scatter2<I><<<pc, 256, SP.stride * 28, st>>>(tmp, cid, coff, SP, (uint32_t)p, off, keys, d_out);
-    LAUNCHCHK();
-    kt.end();
-    tm.mark(PH_SCATTER);
+    // ---- bucket batches: consecutive coarse intervals, at most bmax records each ----
+    // (an interval larger than bmax is a batch of its own).  Batches follow
+    // key order, so the suffix array is their concatenation.
+    std::vector<uint64_t> hcoff(pc + 1);
+    HIPCHK(hipMemcpyAsync(hcoff.data(), coff, (pc + 1) * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const uint64_t bmax = o.batch_records ? o.batch_records : auto_batch_records(n_pos);
+    std::vector<uint32_t> cuts{0};
+    for (uint32_t c = 0; c < pc; ++c)
+        if (c > cuts.back() && hcoff[c + 1] - hcoff[cuts.back()] > bmax) cuts.push_back(c);
+    cuts.push_back(pc);
+    uint64_t bcap = 0;
+    for (size_t bi = 0; bi + 1 < cuts.size(); ++bi) bcap = std::max(bcap, hcoff[cuts[bi + 1]] - hcoff[cuts[bi]]);
+    const bool single = cuts.size() == 2;
+    stats.batches = cuts.size() - 1;
+    auto first_bucket = [&](uint32_t c) -> uint64_t {  // first bucket of coarse interval c
+        return c == 0 ? 0 : std::min<uint64_t>(p, (uint64_t)(c - 1) * SP.stride + 1);
+    };
 
-    // ---- tiles: LDS radix sort per tile (by size class); whole-bucket tiles emit their tie groups ----
-    uint64_t tcap = ceil_div(n_pos, 512) + p + 1;  // tiles per class (every tile but the last of a bucket is full)
+    // batch-sized buffers
+    uint64_t *keys = (uint64_t *)W.keys.get(bcap * 8 + 8);
+    KRec<I> *tmp = (KRec<I> *)W.recs.get(bcap * sizeof(KRec<I>) + 16);
+    // the coarse ids of a single batch are consumed by pass 1, so their buffer
+    // then holds the per-record bucket within its interval; batches need theirs
+    uint16_t *fid = single ? cid : (uint16_t *)W.fid.get(bcap * 2 + 2);
+    I *sink_buf = sink ? (I *)W.sinkbuf.get(bcap * sizeof(I) + 8) : nullptr;
+    // the previous batch's last suffix (key, position), for the next batch's first LCP
+    uint64_t prev_key = 0, prev_pos = 0;
+    const uint64_t tcap = ceil_div(bcap, 512) + p + 1;  // tiles per class (every tile but the last of a bucket is full)
     Tile *tiles = (Tile *)W.tiles.get(kTileClasses * tcap * sizeof(Tile));
     Range *big = (Range *)W.big.get(p * sizeof(Range));
     // [0] oversized, [1..4) tile classes, [4] largest bucket, [8..16) group counts,
     // [16..24) refinement work queues
     unsigned int *counters = (unsigned int *)W.counts.get(128);
-    HIPCHK(hipMemsetAsync(counters, 0, 128, st));
     GroupLists gl{};
     for (int c = 0; c < kGroupClasses; ++c)
-        gl.list[c] = (Group *)W.glist[c].get((n_pos / group_class_min(c) + 1) *
+        gl.list[c] = (Group *)W.glist[c].get((bcap / group_class_min(c) + 1) *
                                              (c == 0 ? sizeof(PairRec<I>) : sizeof(Group)));
     gl.counts = counters + 8;
-    k_tile_emit<<<(unsigned)ceil_div(p, 256), 256, 0, st>>>(off, (uint32_t)p, T, tiles, tcap, counters + 1, big,
-                                                             counters, counters + 4);
-    LAUNCHCHK();
-    unsigned int hc[5];
-    HIPCHK(hipMemcpyAsync(hc, counters, 20, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    stats.bucket_max = hc[4];
-    unsigned int n_big = hc[0];
-    stats.tiles = (uint64_t)hc[1] + hc[2] + hc[3];
-    stats.oversized = n_big;
-    kt.begin(&stats.k_tile_sort);
-    if (o.flags & CAPS_FLAG_TILE_RADIX) {
-        if (hc[1]) k_tile_radix<I, 1024><<<hc[1], 256, 0, st>>>(keys, d_out, tiles, gl, 1);
-        if (hc[2]) k_tile_radix<I, 2048><<<hc[2], 256, 0, st>>>(keys, d_out, tiles + tcap, gl, 1);
-        if (hc[3]) k_tile_radix<I, 4096><<<hc[3], 256, 0, st>>>(keys, d_out, tiles + 2 * tcap, gl, 1);
-    } else {
-        if (hc[1]) k_tile_merge<I, 1024><<<hc[1], 256, 0, st>>>(keys, d_out, tiles, gl, 1);
-        if (hc[2]) k_tile_merge<I, 2048><<<hc[2], 256, 0, st>>>(keys, d_out, tiles + tcap, gl, 1);
-        if (hc[3]) k_tile_merge<I, 4096><<<hc[3], 256, 0, st>>>(keys, d_out, tiles + 2 * tcap, gl, 1);
-    }
-    LAUNCHCHK();
-    kt.end();
-    tm.mark(PH_TILE);
-
-    // ---- oversized buckets: merge the sorted tiles by key, then find their groups ----
-    if (n_big) {
-        std::vector<Range> hb(n_big);
-        HIPCHK(hipMemcpyAsync(hb.data(), big, n_big * sizeof(Range), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        uint64_t maxlen = 0;
-        for (auto &r : hb) maxlen = std::max(maxlen, r.len);
-        uint64_t *k2 = (uint64_t *)W.keys2.get(n_pos * 8);
-        I *i2 = (I *)W.idx2.get(n_pos * sizeof(I));
-        uint64_t *ks = keys, *kd = k2;
-        I *is = d_out, *id = i2;
-        int levels = 0;
-        static_assert(T >= kKeyMergeTile / 2 && (T & (T - 1)) == 0, "key merge runs: powers of two >= half a tile");
-        for (uint64_t run = T; run < maxlen; run *= 2, ++levels) {
-            for (unsigned y0 = 0; y0 < n_big; y0 += 65535) {
-                dim3 gr((unsigned)ceil_div(maxlen, kKeyMergeTile), std::min(65535u, n_big - y0));
-                k_key_merge<I><<<gr, 256, 0, st>>>(big + y0, run, ks, is, kd, id);
-                LAUNCHCHK();
-            }
-            std::swap(ks, kd);
-            std::swap(is, id);
-        }
-        if (levels & 1) {
-            for (unsigned y0 = 0; y0 < n_big; y0 += 65535) {
-                dim3 gr((unsigned)std::min<uint64_t>(ceil_div(maxlen, 256), 4096), std::min(65535u, n_big - y0));
-                k_copy_ranges<I><<<gr, 256, 0, st>>>(big + y0, k2, i2, keys, d_out);
-                LAUNCHCHK();
-            }
-        }
-        for (unsigned y0 = 0; y0 < n_big; y0 += 65535) {
-            dim3 gr((unsigned)std::min<uint64_t>(ceil_div(maxlen, 256), 1024), std::min(65535u, n_big - y0));
-            k_detect_ranges<Range, I><<<gr, 256, 0, st>>>(keys, big + y0, gl, d_out);
-            LAUNCHCHK();
-        }
-    }
-    tm.mark(PH_MERGE);
-
-    unsigned int cnt[kGroupClasses];
-    HIPCHK(hipMemcpyAsync(cnt, gl.counts, sizeof(cnt), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    stats.pairs = cnt[0];
-    stats.small_groups = cnt[1] + cnt[2] + cnt[3] + cnt[4] + cnt[5];
-    stats.mid_groups = cnt[6];
-    stats.giant_groups = cnt[7];
-    tm.mark(PH_DETECT);
-
-    // ---- refinement ----
-    I *lcp = want_lcp ? d_lcp_out : nullptr;
+    I *const lcp_all = want_lcp ? d_lcp_out : nullptr;
     const uint64_t depth0 = g.k;
     // refinement of groups of 2..64, per size class: one slot per group (grid
     // launch), or persistent slots fed from a device work queue
@@ -702,134 +693,247 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     auto chunk_for = [&](uint64_t items, uint32_t slots) {  // about four claims per wavefront
         return (uint32_t)std::max<uint64_t>(slots, std::min<uint64_t>(256, items / ((uint64_t)waves * 4)));
     };
-    if (cnt[0]) {
-        kt.begin(&stats.k_pairs);
-        const PairRec<I> *pl = (const PairRec<I> *)gl.list[0];
-        if (plan.queue[0])
-            k_pairs_wq<S, I><<<pgrid(cnt[0], 64), 256, 0, st>>>(d_text, L, max_ctx, pl, cnt[0], depth0, d_out, lcp,
-                                                                 queues + 0, chunk_for(cnt[0], 16));
-        else if (cm.packed)
-            k_pairs_packed<S, I><<<(unsigned)ceil_div(cnt[0], 64), 256, 0, st>>>(
-                d_text, cm.packed, cm.lgf, gref.bits, L, max_ctx, pl, cnt[0], depth0, d_out, lcp);
-        else
-            k_pairs4<S, I><<<(unsigned)ceil_div(cnt[0], 64), 256, 0, st>>>(d_text, L, max_ctx, pl, cnt[0], depth0,
-                                                                            d_out, lcp);
-        LAUNCHCHK();
-        kt.end();
-    }
-    kt.begin(&stats.k_groups_small);
-    auto wgroups = [&](auto wtag, int c) {
-        constexpr int Wd = decltype(wtag)::value;
-        if (!cnt[c]) return;
-        constexpr uint32_t per_block = 256 / Wd;
-        if (plan.queue[c])
-            k_wgroups_wq<S, I, Wd><<<pgrid(cnt[c], per_block), 256, 0, st>>>(
-                d_text, cm, gref, L, max_ctx, gl.list[c], cnt[c], depth0, d_out, lcp, queues + c,
-                chunk_for(cnt[c], 64 / Wd));
-        else
-            k_wgroups_grid<S, I, Wd><<<(unsigned)ceil_div(cnt[c], per_block), 256, 0, st>>>(
-                d_text, cm, gref, L, max_ctx, gl.list[c], cnt[c], depth0, d_out, lcp);
-    };
-    wgroups(std::integral_constant<int, 4>{}, 1);
-    wgroups(std::integral_constant<int, 8>{}, 2);
-    wgroups(std::integral_constant<int, 16>{}, 3);
-    wgroups(std::integral_constant<int, 32>{}, 4);
-    wgroups(std::integral_constant<int, 64>{}, 5);
-    LAUNCHCHK();
-    kt.end();
-    if (cnt[6]) {
-        kt.begin(&stats.k_groups_mid);
-        k_groups<S, I, 256, kMidMax><<<cnt[6], 256, 0, st>>>(d_text, cm, gref, L, max_ctx, gl.list[6], depth0, d_out,
-                                                              lcp, 0, 0);
-        LAUNCHCHK();
-        kt.end();
-    }
-    tm.mark(PH_REFINE);
 
-    // ---- giant groups: chunked refinement + merge-path LCP merges ----
-    // All giant groups at once: one launch sorts every chunk of every group,
-    // then one launch per merge level merges the runs of every group (a group
-    // already in one run is carried over by a copy job).  The job lists of all
-    // levels are built and uploaded together, so no level waits for the host.
-    if (cnt[7]) {
-        std::vector<Group> hg(cnt[7]);
-        HIPCHK(hipMemcpyAsync(hg.data(), gl.list[7], cnt[7] * sizeof(Group), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        I *lbuf = lcp ? lcp : (I *)W.lcp.get(n_pos * sizeof(I));
-        I *i2 = (I *)W.idx2.get(n_pos * sizeof(I));
-        I *l2 = (I *)W.lcp2.get(n_pos * sizeof(I));
-        std::vector<Group> hc;
-        uint64_t maxsize = 0;
-        for (const Group &gr : hg) {
-            maxsize = std::max<uint64_t>(maxsize, gr.size);
-            for (uint64_t c = 0; c < gr.size; c += kMidMax)
-                hc.push_back(Group{gr.start + c, std::min<uint64_t>(kMidMax, gr.size - c)});
-        }
-        int levels = 0;
-        for (uint64_t run = kMidMax; run < maxsize; run *= 2) ++levels;
-        std::vector<uint64_t> level_off(levels + 1, 0);
-        W.h_jobs.clear();
-        for (int lv = 0; lv < levels; ++lv) {
-            const uint64_t run = (uint64_t)kMidMax << lv;
-            level_off[lv] = W.h_jobs.size();
-            for (const Group &gr : hg)
-                for (uint64_t b = 0; b < gr.size; b += 2 * run) {
-                    MergeJob jb;
-                    jb.start = gr.start + b;
-                    jb.nx = std::min<uint64_t>(run, gr.size - b);
-                    jb.ny = std::min<uint64_t>(run, gr.size - b - jb.nx);
-                    W.h_jobs.push_back(jb);
-                }
-        }
-        level_off[levels] = W.h_jobs.size();
-        Group *dc = (Group *)W.chunks.get(hc.size() * sizeof(Group));
-        HIPCHK(hipMemcpyAsync(dc, hc.data(), hc.size() * sizeof(Group), hipMemcpyHostToDevice, st));
-        MergeJob *dj = (MergeJob *)W.jobs.get(std::max<size_t>(W.h_jobs.size(), 1) * sizeof(MergeJob));
-        if (!W.h_jobs.empty())
-            HIPCHK(hipMemcpyAsync(dj, W.h_jobs.data(), W.h_jobs.size() * sizeof(MergeJob), hipMemcpyHostToDevice,
-                                  st));
-        for (uint64_t c0 = 0; c0 < hc.size(); c0 += 65535u * 16) {
-            const unsigned nb = (unsigned)std::min<uint64_t>(hc.size() - c0, 65535u * 16);
-            k_groups<S, I, 256, kMidMax><<<nb, 256, 0, st>>>(d_text, cm, gref, L, max_ctx, dc + c0, depth0, d_out, lbuf,
-                                                             1, 0);
-            LAUNCHCHK();
-        }
-        I *is = d_out, *ls = lbuf, *id = i2, *ld = l2;
-        // merge-path partition width: outputs per thread (a tuning knob)
-        const int mw = (o.flags & CAPS_FLAG_MERGE_W16) ? 16 : (o.flags & CAPS_FLAG_MERGE_W64) ? 64 : 32;
-        auto merge_level = [&](auto wtag, dim3 grid, const MergeJob *jobs) {
-            constexpr int MW = decltype(wtag)::value;
-            k_lcp_merge<S, I, MW><<<grid, 128, 0, st>>>(d_text, L, max_ctx, depth0, jobs, is, ls, id, ld);
-        };
-        for (int lv = 0; lv < levels; ++lv) {
-            const uint64_t run = (uint64_t)kMidMax << lv;
-            const uint64_t span = std::min<uint64_t>(2 * run, maxsize);
-            for (uint64_t j0 = level_off[lv]; j0 < level_off[lv + 1]; j0 += 65535) {
-                const unsigned nj = (unsigned)std::min<uint64_t>(level_off[lv + 1] - j0, 65535);
-                dim3 gr2((unsigned)ceil_div(ceil_div(span, mw), 128), nj);
-                if (mw == 16) merge_level(std::integral_constant<int, 16>{}, gr2, dj + j0);
-                else if (mw == 64) merge_level(std::integral_constant<int, 64>{}, gr2, dj + j0);
-                else merge_level(std::integral_constant<int, 32>{}, gr2, dj + j0);
-                LAUNCHCHK();
-            }
-            std::swap(is, id);
-            std::swap(ls, ld);
-        }
-        if (levels & 1) {
-            for (uint64_t c0 = 0; c0 < hc.size(); c0 += 65535u * 16) {
-                const unsigned nb = (unsigned)std::min<uint64_t>(hc.size() - c0, 65535u * 16);
-                k_copy_chunks<I><<<nb, 256, 0, st>>>(dc + c0, i2, l2, d_out, lbuf);
-                LAUNCHCHK();
-            }
-        }
-    }
-    if (lcp) {
-        kt.begin(&stats.k_boundary_lcp);
-        k_boundary_lcp<I><<<grid_for(n_pos, 256, 16384), 256, 0, st>>>(keys, d_out, n_pos, L, max_ctx, g, lcp);
+    for (size_t bi = 0; bi + 1 < cuts.size(); ++bi) {
+        const uint32_t c_lo = cuts[bi], c_hi = cuts[bi + 1];
+        const uint64_t rbase = hcoff[c_lo], nb = hcoff[c_hi] - rbase;
+        const uint64_t b_lo = first_bucket(c_lo), b_hi = c_hi == pc ? p : first_bucket(c_hi);
+        const uint32_t pb = (uint32_t)(b_hi - b_lo);  // buckets of the batch
+        uint64_t *boff = off + b_lo;                  // their offsets, relative to rbase
+        I *out = sink ? sink_buf : d_out + rbase;
+        I *lcp = lcp_all ? lcp_all + rbase : nullptr;
+        if (nb == 0) continue;
+
+        // ---- two-level scatter ----
+        kt.begin(&stats.k_scatter1);
+        if (seq)
+            k_scatter1<S, I, true><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_iter, cid, pc, coff,
+                                                         hblk, tmp, chunk, d_keep, c_lo, c_hi, rbase);
+        else
+            k_scatter1<S, I, false><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, cid, pc, coff,
+                                                          hblk, tmp, chunk, nullptr, c_lo, c_hi, rbase);
         LAUNCHCHK();
         kt.end();
+        kt.begin(&stats.k_scatter2);
+        k_scatter2<I><<<c_hi - c_lo, 256, SP.stride * 28, st>>>(tmp, fid, coff, SP, (uint32_t)p, off, keys, out, c_lo,
+                                                                c_hi, rbase);
+        LAUNCHCHK();
+        kt.end();
+        tm.mark(PH_SCATTER);
+
+        // ---- tiles: sort per tile (by size class); whole-bucket tiles emit their tie groups ----
+        HIPCHK(hipMemsetAsync(counters, 0, 128, st));
+        k_tile_emit<<<(unsigned)ceil_div(pb, 256), 256, 0, st>>>(boff, pb, T, tiles, tcap, counters + 1, big,
+                                                                  counters, counters + 4);
+        LAUNCHCHK();
+        unsigned int hc[5];
+        HIPCHK(hipMemcpyAsync(hc, counters, 20, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        stats.bucket_max = std::max<uint64_t>(stats.bucket_max, hc[4]);
+        const unsigned int n_big = hc[0];
+        stats.tiles += (uint64_t)hc[1] + hc[2] + hc[3];
+        stats.oversized += n_big;
+        kt.begin(&stats.k_tile_sort);
+        if (o.flags & CAPS_FLAG_TILE_RADIX) {
+            if (hc[1]) k_tile_radix<I, 1024><<<hc[1], 256, 0, st>>>(keys, out, tiles, gl, 1);
+            if (hc[2]) k_tile_radix<I, 2048><<<hc[2], 256, 0, st>>>(keys, out, tiles + tcap, gl, 1);
+            if (hc[3]) k_tile_radix<I, 4096><<<hc[3], 256, 0, st>>>(keys, out, tiles + 2 * tcap, gl, 1);
+        } else {
+            if (hc[1]) k_tile_merge<I, 1024><<<hc[1], 256, 0, st>>>(keys, out, tiles, gl, 1);
+            if (hc[2]) k_tile_merge<I, 2048><<<hc[2], 256, 0, st>>>(keys, out, tiles + tcap, gl, 1);
+            if (hc[3]) k_tile_merge<I, 4096><<<hc[3], 256, 0, st>>>(keys, out, tiles + 2 * tcap, gl, 1);
+        }
+        LAUNCHCHK();
+        kt.end();
+        tm.mark(PH_TILE);
+
+        // ---- oversized buckets: merge the sorted tiles by key, then find their groups ----
+        if (n_big) {
+            std::vector<Range> hb(n_big);
+            HIPCHK(hipMemcpyAsync(hb.data(), big, n_big * sizeof(Range), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            uint64_t maxlen = 0;
+            for (auto &r : hb) maxlen = std::max(maxlen, r.len);
+            uint64_t *k2 = (uint64_t *)W.keys2.get(bcap * 8);
+            I *i2 = (I *)W.idx2.get(bcap * sizeof(I));
+            uint64_t *ks = keys, *kd = k2;
+            I *is = out, *id = i2;
+            int levels = 0;
+            static_assert(T >= kKeyMergeTile / 2 && (T & (T - 1)) == 0,
+                          "key merge runs: powers of two >= half a tile");
+            for (uint64_t run = T; run < maxlen; run *= 2, ++levels) {
+                for (unsigned y0 = 0; y0 < n_big; y0 += 65535) {
+                    dim3 gr((unsigned)ceil_div(maxlen, kKeyMergeTile), std::min(65535u, n_big - y0));
+                    k_key_merge<I><<<gr, 256, 0, st>>>(big + y0, run, ks, is, kd, id);
+                    LAUNCHCHK();
+                }
+                std::swap(ks, kd);
+                std::swap(is, id);
+            }
+            if (levels & 1) {
+                for (unsigned y0 = 0; y0 < n_big; y0 += 65535) {
+                    dim3 gr((unsigned)std::min<uint64_t>(ceil_div(maxlen, 256), 4096), std::min(65535u, n_big - y0));
+                    k_copy_ranges<I><<<gr, 256, 0, st>>>(big + y0, k2, i2, keys, out);
+                    LAUNCHCHK();
+                }
+            }
+            for (unsigned y0 = 0; y0 < n_big; y0 += 65535) {
+                dim3 gr((unsigned)std::min<uint64_t>(ceil_div(maxlen, 256), 1024), std::min(65535u, n_big - y0));
+                k_detect_ranges<Range, I><<<gr, 256, 0, st>>>(keys, big + y0, gl, out);
+                LAUNCHCHK();
+            }
+        }
+        tm.mark(PH_MERGE);
+
+        unsigned int cnt[kGroupClasses];
+        HIPCHK(hipMemcpyAsync(cnt, gl.counts, sizeof(cnt), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        stats.pairs += cnt[0];
+        stats.small_groups += cnt[1] + cnt[2] + cnt[3] + cnt[4] + cnt[5];
+        stats.mid_groups += cnt[6];
+        stats.giant_groups += cnt[7];
+        tm.mark(PH_DETECT);
+
+        // ---- refinement ----
+        if (cnt[0]) {
+            kt.begin(&stats.k_pairs);
+            const PairRec<I> *pl = (const PairRec<I> *)gl.list[0];
+            if (plan.queue[0])
+                k_pairs_wq<S, I><<<pgrid(cnt[0], 64), 256, 0, st>>>(d_text, L, max_ctx, pl, cnt[0], depth0, out, lcp,
+                                                                     queues + 0, chunk_for(cnt[0], 16));
+            else if (cm.packed)
+                k_pairs_packed<S, I><<<(unsigned)ceil_div(cnt[0], 64), 256, 0, st>>>(
+                    d_text, cm.packed, cm.lgf, gref.bits, L, max_ctx, pl, cnt[0], depth0, out, lcp);
+            else
+                k_pairs4<S, I><<<(unsigned)ceil_div(cnt[0], 64), 256, 0, st>>>(d_text, L, max_ctx, pl, cnt[0], depth0,
+                                                                                out, lcp);
+            LAUNCHCHK();
+            kt.end();
+        }
+        kt.begin(&stats.k_groups_small);
+        auto wgroups = [&](auto wtag, int c) {
+            constexpr int Wd = decltype(wtag)::value;
+            if (!cnt[c]) return;
+            constexpr uint32_t per_block = 256 / Wd;
+            if (plan.queue[c])
+                k_wgroups_wq<S, I, Wd><<<pgrid(cnt[c], per_block), 256, 0, st>>>(
+                    d_text, cm, gref, L, max_ctx, gl.list[c], cnt[c], depth0, out, lcp, queues + c,
+                    chunk_for(cnt[c], 64 / Wd));
+            else
+                k_wgroups_grid<S, I, Wd><<<(unsigned)ceil_div(cnt[c], per_block), 256, 0, st>>>(
+                    d_text, cm, gref, L, max_ctx, gl.list[c], cnt[c], depth0, out, lcp);
+        };
+        wgroups(std::integral_constant<int, 4>{}, 1);
+        wgroups(std::integral_constant<int, 8>{}, 2);
+        wgroups(std::integral_constant<int, 16>{}, 3);
+        wgroups(std::integral_constant<int, 32>{}, 4);
+        wgroups(std::integral_constant<int, 64>{}, 5);
+        LAUNCHCHK();
+        kt.end();
+        if (cnt[6]) {
+            kt.begin(&stats.k_groups_mid);
+            k_groups<S, I, 256, kMidMax><<<cnt[6], 256, 0, st>>>(d_text, cm, gref, L, max_ctx, gl.list[6], depth0,
+                                                                  out, lcp, 0, 0);
+            LAUNCHCHK();
+            kt.end();
+        }
+        tm.mark(PH_REFINE);
+
+        // ---- giant groups: chunked refinement + merge-path LCP merges ----
+        // All giant groups at once: one launch sorts every chunk of every
+        // group, then one launch per merge level merges the runs of every group
+        // (a group already in one run is carried over by a copy job).  The job
+        // lists of all levels are built and uploaded together, so no level
+        // waits for the host.
+        if (cnt[7]) {
+            std::vector<Group> hg(cnt[7]);
+            HIPCHK(hipMemcpyAsync(hg.data(), gl.list[7], cnt[7] * sizeof(Group), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            I *lbuf = lcp ? lcp : (I *)W.lcp.get(bcap * sizeof(I));
+            I *i2 = (I *)W.idx2.get(bcap * sizeof(I));
+            I *l2 = (I *)W.lcp2.get(bcap * sizeof(I));
+            std::vector<Group> hch;
+            uint64_t maxsize = 0;
+            for (const Group &gr : hg) {
+                maxsize = std::max<uint64_t>(maxsize, gr.size);
+                for (uint64_t c = 0; c < gr.size; c += kMidMax)
+                    hch.push_back(Group{gr.start + c, std::min<uint64_t>(kMidMax, gr.size - c)});
+            }
+            int levels = 0;
+            for (uint64_t run = kMidMax; run < maxsize; run *= 2) ++levels;
+            std::vector<uint64_t> level_off(levels + 1, 0);
+            W.h_jobs.clear();
+            for (int lv = 0; lv < levels; ++lv) {
+                const uint64_t run = (uint64_t)kMidMax << lv;
+                level_off[lv] = W.h_jobs.size();
+                for (const Group &gr : hg)
+                    for (uint64_t b = 0; b < gr.size; b += 2 * run) {
+                        MergeJob jb;
+                        jb.start = gr.start + b;
+                        jb.nx = std::min<uint64_t>(run, gr.size - b);
+                        jb.ny = std::min<uint64_t>(run, gr.size - b - jb.nx);
+                        W.h_jobs.push_back(jb);
+                    }
+            }
+            level_off[levels] = W.h_jobs.size();
+            Group *dc = (Group *)W.chunks.get(hch.size() * sizeof(Group));
+            HIPCHK(hipMemcpyAsync(dc, hch.data(), hch.size() * sizeof(Group), hipMemcpyHostToDevice, st));
+            MergeJob *dj = (MergeJob *)W.jobs.get(std::max<size_t>(W.h_jobs.size(), 1) * sizeof(MergeJob));
+            if (!W.h_jobs.empty())
+                HIPCHK(hipMemcpyAsync(dj, W.h_jobs.data(), W.h_jobs.size() * sizeof(MergeJob), hipMemcpyHostToDevice,
+                                      st));
+            for (uint64_t c0 = 0; c0 < hch.size(); c0 += 65535u * 16) {
+                const unsigned nbk = (unsigned)std::min<uint64_t>(hch.size() - c0, 65535u * 16);
+                k_groups<S, I, 256, kMidMax><<<nbk, 256, 0, st>>>(d_text, cm, gref, L, max_ctx, dc + c0, depth0, out,
+                                                                  lbuf, 1, 0);
+                LAUNCHCHK();
+            }
+            I *is = out, *ls = lbuf, *id = i2, *ld = l2;
+            // merge-path partition width: outputs per thread (a tuning knob)
+            const int mw = (o.flags & CAPS_FLAG_MERGE_W16) ? 16 : (o.flags & CAPS_FLAG_MERGE_W64) ? 64 : 32;
+            auto merge_level = [&](auto wtag, dim3 grid, const MergeJob *jobs) {
+                constexpr int MW = decltype(wtag)::value;
+                k_lcp_merge<S, I, MW><<<grid, 128, 0, st>>>(d_text, L, max_ctx, depth0, jobs, is, ls, id, ld);
+            };
+            for (int lv = 0; lv < levels; ++lv) {
+                const uint64_t run = (uint64_t)kMidMax << lv;
+                const uint64_t span = std::min<uint64_t>(2 * run, maxsize);
+                for (uint64_t j0 = level_off[lv]; j0 < level_off[lv + 1]; j0 += 65535) {
+                    const unsigned nj = (unsigned)std::min<uint64_t>(level_off[lv + 1] - j0, 65535);
+                    dim3 gr2((unsigned)ceil_div(ceil_div(span, mw), 128), nj);
+                    if (mw == 16) merge_level(std::integral_constant<int, 16>{}, gr2, dj + j0);
+                    else if (mw == 64) merge_level(std::integral_constant<int, 64>{}, gr2, dj + j0);
+                    else merge_level(std::integral_constant<int, 32>{}, gr2, dj + j0);
+                    LAUNCHCHK();
+                }
+                std::swap(is, id);
+                std::swap(ls, ld);
+            }
+            if (levels & 1) {
+                for (uint64_t c0 = 0; c0 < hch.size(); c0 += 65535u * 16) {
+                    const unsigned nbk = (unsigned)std::min<uint64_t>(hch.size() - c0, 65535u * 16);
+                    k_copy_chunks<I><<<nbk, 256, 0, st>>>(dc + c0, i2, l2, out, lbuf);
+                    LAUNCHCHK();
+                }
+            }
+        }
+        if (lcp) {
+            kt.begin(&stats.k_boundary_lcp);
+            k_boundary_lcp<I><<<grid_for(nb, 256, 16384), 256, 0, st>>>(keys, out, nb, L, max_ctx, g, lcp,
+                                                                         bi ? 1u : 0u, prev_key, prev_pos);
+            LAUNCHCHK();
+            kt.end();
+            if (!single) {
+                I last = 0;
+                HIPCHK(hipMemcpyAsync(&prev_key, keys + nb - 1, 8, hipMemcpyDeviceToHost, st));
+                HIPCHK(hipMemcpyAsync(&last, out + nb - 1, sizeof(I), hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+                prev_pos = (uint64_t)last;
+            }
+        }
+        tm.mark(PH_GIANT);
+        if (sink) sink->flush(out, nb, st);
     }
-    tm.mark(PH_GIANT);
     if (R.stats) {
         unsigned long long rs[4];
         HIPCHK(hipMemcpyAsync(rs, R.stats, 32, hipMemcpyDeviceToHost, st));
@@ -973,61 +1077,111 @@ int caps_build_in_memory(const void *text, uint64_t n, uint32_t sym_bytes, uint3
                                               out_lcp);
 }
 
+}  // extern "C"
+
+namespace caps {
+
+// Uploads a host bitmap of kept positions ((n + 63) / 64 words), clears the
+// bits past n and returns the device copy and the kept count.
+static const uint64_t *upload_keep(Workspace &W, const uint64_t *keep_words, uint64_t n, uint64_t *count,
+                                   hipStream_t st) {
+    const uint64_t nw = ceil_div(n, 64);
+    uint64_t *d = (uint64_t *)W.keep.get(nw * 8 + 8);
+    if (nw) HIPCHK(hipMemcpyAsync(d, keep_words, nw * 8, hipMemcpyHostToDevice, st));
+    *count = device_keep_count(W, d, n, st);
+    return d;
+}
+
+// A streaming build: batches of the suffix array go from HBM through a
+// pinned staging buffer to `emit`, in lexicographic order, on the caller's
+// thread (ext_mem.rs:1933-1937); the first nonzero emit aborts the build.
+template <typename S, typename I>
+static void stream_typed(const void *text, uint64_t n, const uint64_t *positions, uint64_t n_pos,
+                         const uint64_t *keep_words, const caps_limits_t *limits, const caps_opts_t &o,
+                         caps_emit_fn emit, void *ctx) {
+    Workspace &W = workspace();
+    hipStream_t st = nullptr;
+    const S *dt = (const S *)upload_text(W, text, n, sizeof(S), st);
+    const I *dp = nullptr;
+    const uint64_t *dk = nullptr;
+    if (keep_words) {
+        dk = upload_keep(W, keep_words, n, &n_pos, st);
+    } else if (positions) {
+        I *d = (I *)W.hpos.get(n_pos * sizeof(I) + 8);
+        if (sizeof(I) == 8) {
+            if (n_pos) HIPCHK(hipMemcpyAsync(d, positions, n_pos * 8, hipMemcpyHostToDevice, st));
+        } else {
+            std::vector<I> narrow(n_pos);
+            for (uint64_t i = 0; i < n_pos; ++i) narrow[i] = (I)positions[i];
+            if (n_pos) HIPCHK(hipMemcpyAsync(d, narrow.data(), n_pos * sizeof(I), hipMemcpyHostToDevice, st));
+            HIPCHK(hipStreamSynchronize(st));
+        }
+        dp = d;
+    }
+    constexpr uint64_t kStage = 1ull << 22;  // positions per emit call
+    if (W.h_stage_bytes < kStage * 16) {
+        if (W.h_stage) (void)hipHostFree(W.h_stage);
+        W.h_stage = nullptr;
+        W.h_stage_bytes = 0;
+        HIPCHK(hipHostMalloc(&W.h_stage, kStage * 16, hipHostMallocDefault));
+        W.h_stage_bytes = kStage * 16;
+    }
+    I *stage = (I *)W.h_stage;
+    uint64_t *e64 = (uint64_t *)((char *)W.h_stage + kStage * 8);
+    BatchSink<I> sink;
+    sink.flush = [&](const I *d, uint64_t cnt, hipStream_t s) {
+        for (uint64_t off = 0; off < cnt; off += kStage) {
+            const uint64_t m = std::min(kStage, cnt - off);
+            HIPCHK(hipMemcpyAsync(stage, d + off, m * sizeof(I), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            const uint64_t *src = (const uint64_t *)stage;
+            if (sizeof(I) != 8) {
+                for (uint64_t i = 0; i < m; ++i) e64[i] = (uint64_t)stage[i];
+                src = e64;
+            }
+            const int r = emit(ctx, src, m);
+            if (r != 0) {
+                g_emit_code = r;
+                throw CapsError(CAPS_E_EMIT, "emit callback aborted the build");
+            }
+        }
+    };
+    build_impl<S, I>(dt, n, dp, n_pos, limits, o, nullptr, nullptr, st, g_stats, 0, dk, &sink);
+}
+
 static int stream_build(const void *text, uint64_t n, uint32_t sym_bytes, const uint64_t *positions, uint64_t n_pos,
-                        const caps_limits_t *limits, const caps_opts_t *opts, caps_emit_fn emit, void *ctx) {
+                        const uint64_t *keep_words, const caps_limits_t *limits, const caps_opts_t *opts,
+                        caps_emit_fn emit, void *ctx) {
     g_emit_code = 0;
-    std::vector<uint64_t> sa64;
-    int rc = guarded([&] {
+    return guarded([&] {
         check_common(text, n, sym_bytes, limits);
         if (!emit) throw CapsError(CAPS_E_INVALID, "emit callback is NULL");
+        if (keep_words == nullptr && n && !text) throw CapsError(CAPS_E_INVALID, "text is NULL");
         caps_opts_t o = opts_or_default(opts);
-        sa64.resize(n_pos);
+        if (positions)
+            for (uint64_t i = 0; i < n_pos; ++i)
+                if (positions[i] >= n) throw CapsError(CAPS_E_INVALID, "position out of range");
         // index width follows the reference: u32 iff text.len() <= u32::MAX + 1
-        bool narrow = n <= (1ull << 32);
-        if (narrow) {
-            std::vector<uint32_t> pos32, out32(n_pos);
-            if (positions) {
-                pos32.resize(n_pos);
-                for (uint64_t i = 0; i < n_pos; ++i) {
-                    if (positions[i] >= n) throw CapsError(CAPS_E_INVALID, "position out of range");
-                    pos32[i] = (uint32_t)positions[i];
-                }
-            }
-            if (sym_bytes == 1)
-                build_host<uint8_t, uint32_t>(text, n, positions ? pos32.data() : nullptr, n_pos, limits, o,
-                                              out32.data(), nullptr);
-            else
-                build_host<uint16_t, uint32_t>(text, n, positions ? pos32.data() : nullptr, n_pos, limits, o,
-                                               out32.data(), nullptr);
-            for (uint64_t i = 0; i < n_pos; ++i) sa64[i] = out32[i];
-        } else {
-            if (positions)
-                for (uint64_t i = 0; i < n_pos; ++i)
-                    if (positions[i] >= n) throw CapsError(CAPS_E_INVALID, "position out of range");
-            if (sym_bytes == 1)
-                build_host<uint8_t, uint64_t>(text, n, positions, n_pos, limits, o, sa64.data(), nullptr);
-            else
-                build_host<uint16_t, uint64_t>(text, n, positions, n_pos, limits, o, sa64.data(), nullptr);
-        }
+        const bool narrow = n <= (1ull << 32);
+        if (narrow && sym_bytes == 1)
+            stream_typed<uint8_t, uint32_t>(text, n, positions, n_pos, keep_words, limits, o, emit, ctx);
+        else if (narrow)
+            stream_typed<uint16_t, uint32_t>(text, n, positions, n_pos, keep_words, limits, o, emit, ctx);
+        else if (sym_bytes == 1)
+            stream_typed<uint8_t, uint64_t>(text, n, positions, n_pos, keep_words, limits, o, emit, ctx);
+        else
+            stream_typed<uint16_t, uint64_t>(text, n, positions, n_pos, keep_words, limits, o, emit, ctx);
     });
-    if (rc != CAPS_OK) return rc;
-    // emit in order, in batches, on the caller's thread (ext_mem.rs:1933-1937)
-    const uint64_t B = 1 << 20;
-    for (uint64_t i = 0; i < n_pos; i += B) {
-        int r = emit(ctx, sa64.data() + i, std::min(B, n_pos - i));
-        if (r != 0) {
-            g_emit_code = r;
-            g_err = "emit callback aborted the build";
-            return CAPS_E_EMIT;
-        }
-    }
-    return CAPS_OK;
 }
+
+}  // namespace caps
+
+extern "C" {
 
 int caps_build_in_memory_sample_sort(const void *text, uint64_t n, uint32_t sym_bytes, const uint64_t *positions,
                                      uint64_t n_pos, const caps_limits_t *limits, const caps_opts_t *opts,
                                      caps_emit_fn emit, void *ctx) {
-    return stream_build(text, n, sym_bytes, positions, positions ? n_pos : n, limits, opts, emit, ctx);
+    return stream_build(text, n, sym_bytes, positions, positions ? n_pos : n, nullptr, limits, opts, emit, ctx);
 }
 
 int caps_build_ext_mem_for_positions(const void *text, uint64_t n, uint32_t sym_bytes, const uint64_t *positions,
@@ -1037,7 +1191,52 @@ int caps_build_ext_mem_for_positions(const void *text, uint64_t n, uint32_t sym_
         g_err = "positions is NULL";
         return CAPS_E_INVALID;
     }
-    return stream_build(text, n, sym_bytes, positions, n_pos, limits, opts, emit, ctx);
+    return stream_build(text, n, sym_bytes, positions, n_pos, nullptr, limits, opts, emit, ctx);
+}
+
+int caps_build_ext_mem_for_filter(const void *text, uint64_t n, uint32_t sym_bytes, const uint64_t *keep_words,
+                                  const caps_limits_t *limits, const caps_opts_t *opts, caps_emit_fn emit,
+                                  void *ctx) {
+    if (!keep_words && n) {
+        g_err = "keep bitmap is NULL";
+        return CAPS_E_INVALID;
+    }
+    return stream_build(text, n, sym_bytes, nullptr, 0, keep_words, limits, opts, emit, ctx);
+}
+
+int caps_build_device_filtered(const void *d_text, uint64_t n, uint32_t sym_bytes, const uint64_t *d_keep_words,
+                               uint32_t idx_bytes, const caps_limits_t *limits, const caps_opts_t *opts,
+                               void *d_out_sa, uint64_t out_capacity, uint64_t *n_out, void *d_out_lcp,
+                               void *stream) {
+    return guarded([&] {
+        check_common(d_text, n, sym_bytes, limits);
+        if (!d_keep_words && n) throw CapsError(CAPS_E_INVALID, "keep bitmap is NULL");
+        if (idx_bytes != 4 && idx_bytes != 8) throw CapsError(CAPS_E_UNSUPPORTED, "index width must be 4 or 8");
+        if (idx_bytes == 4 && n > (1ull << 32)) throw CapsError(CAPS_E_INVALID, "u32 index cannot address text");
+        caps_opts_t o = opts_or_default(opts);
+        hipStream_t st = (hipStream_t)stream;
+        Workspace &W = workspace();
+        // the caller's bitmap is left as it is: bits past n are cleared in a copy
+        const uint64_t nw = ceil_div(n, 64);
+        uint64_t *dk = (uint64_t *)W.keep.get(nw * 8 + 8);
+        if (nw) HIPCHK(hipMemcpyAsync(dk, d_keep_words, nw * 8, hipMemcpyDeviceToDevice, st));
+        const uint64_t kept = device_keep_count(W, dk, n, st);
+        if (n_out) *n_out = kept;
+        if (kept > out_capacity) throw CapsError(CAPS_E_INVALID, "output capacity below the kept count");
+        if (kept && !d_out_sa) throw CapsError(CAPS_E_INVALID, "d_out_sa is NULL");
+        if (sym_bytes == 1 && idx_bytes == 4)
+            build_impl<uint8_t, uint32_t>((const uint8_t *)d_text, n, nullptr, kept, limits, o, (uint32_t *)d_out_sa,
+                                          (uint32_t *)d_out_lcp, st, g_stats, 0, dk);
+        else if (sym_bytes == 1)
+            build_impl<uint8_t, uint64_t>((const uint8_t *)d_text, n, nullptr, kept, limits, o, (uint64_t *)d_out_sa,
+                                          (uint64_t *)d_out_lcp, st, g_stats, 0, dk);
+        else if (idx_bytes == 4)
+            build_impl<uint16_t, uint32_t>((const uint16_t *)d_text, n, nullptr, kept, limits, o,
+                                           (uint32_t *)d_out_sa, (uint32_t *)d_out_lcp, st, g_stats, 0, dk);
+        else
+            build_impl<uint16_t, uint64_t>((const uint16_t *)d_text, n, nullptr, kept, limits, o,
+                                           (uint64_t *)d_out_sa, (uint64_t *)d_out_lcp, st, g_stats, 0, dk);
+    });
 }
 
 int caps_build_device(const void *d_text, uint64_t n, uint32_t sym_bytes, const void *d_positions, uint64_t n_pos,

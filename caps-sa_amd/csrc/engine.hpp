@@ -44,11 +44,17 @@ struct Workspace {
     Range h_range;
     // classification / scatter
     DevBuf bid, hist, hblk, off, cursor, keys, keys2, idx2;
+    // bucket batches: per-record bucket ids, a batch of output; kept-position
+    // bitmap, its count
+    DevBuf fid, sinkbuf, keep, keep_count;
     // tiles
     DevBuf tcnt, toff, toff2, tiles, big, counts, recs;
     // refinement
     DevBuf glist[kGroupClasses], lcp, lcp2, first, jobs, chunks;
     std::vector<MergeJob> h_jobs;
+    // pinned staging for streamed batches (kept until the process ends)
+    void *h_stage = nullptr;
+    size_t h_stage_bytes = 0;
     // standalone merges
     DevBuf m_a, m_al, m_b, m_bl;
     // router

@@ -35,16 +35,57 @@ __global__ __launch_bounds__(256) void k_alphabet(const S *__restrict__ t, uint6
 }
 
 // ---------------------------------------------------------------------------
+// Kept-position bitmaps (bit q & 63 of word q >> 6 set: position q kept).
+// ---------------------------------------------------------------------------
+// First kept position at or after j, looking at most 32 words ahead; j
+// itself when none is found there (a sample position only steers bucket
+// sizes, never the output).  Bits at and past n are clear.
+__device__ __forceinline__ uint64_t next_kept(const uint64_t *__restrict__ keep, uint64_t j, uint64_t n) {
+    uint64_t w = j >> 6;
+    uint64_t m = keep[w] & (~0ull << (j & 63));
+    const uint64_t nw = (n + 63) >> 6;
+    for (int t = 0; t < 32; ++t) {
+        if (m) return (w << 6) + (uint64_t)__builtin_ctzll(m);
+        if (++w >= nw) break;
+        m = keep[w];
+    }
+    return j;
+}
+
+// Clears the bits at and past n (the last word), then counts the kept
+// positions into *total.
+__global__ __launch_bounds__(256) void k_keep_count(uint64_t *__restrict__ keep, uint64_t n,
+                                                    unsigned long long *__restrict__ total) {
+    const uint64_t nw = (n + 63) >> 6;
+    unsigned long long c = 0;
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nw; w += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t v = keep[w];
+        if (w == nw - 1 && (n & 63)) {
+            v &= (1ull << (n & 63)) - 1;
+            keep[w] = v;
+        }
+        c += (unsigned long long)__popcll(v);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += (unsigned long long)__shfl_xor((long long)c, o);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(total, c);
+}
+
+// ---------------------------------------------------------------------------
 // Keys of evenly spaced sample positions (ext_mem.rs:1291-1305 midpoints).
+// With a kept-position bitmap the midpoints are taken over the text and
+// moved to the next kept position.
 // ---------------------------------------------------------------------------
 template <typename S, typename I>
 __global__ __launch_bounds__(256) void k_sample_keys(const S *__restrict__ t, CodeMap<S> cm, KeyGeom g, DevLimits L,
                                                      uint64_t max_ctx, const I *__restrict__ pos, uint64_t n_pos,
-                                                     uint64_t s, uint64_t *__restrict__ keys, uint32_t *__restrict__ ids) {
+                                                     uint64_t s, uint64_t *__restrict__ keys, uint32_t *__restrict__ ids,
+                                                     const uint64_t *__restrict__ keep, uint64_t n) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s) return;
-    uint64_t j = ((2 * i + 1) * n_pos) / (2 * s);
-    uint64_t p = pos ? (uint64_t)pos[j] : j;
+    const uint64_t span = keep ? n : n_pos;
+    uint64_t j = ((2 * i + 1) * span) / (2 * s);
+    uint64_t p = keep ? next_kept(keep, j, n) : pos ? (uint64_t)pos[j] : j;
     uint64_t seg;
     uint64_t lim = lim_seg(L, p, &seg);
     keys[i] = with_id(g, key_at_seed(t, cm, g, p, lim < max_ctx ? lim : max_ctx), seg);
@@ -55,15 +96,17 @@ __global__ __launch_bounds__(256) void k_sample_keys(const S *__restrict__ t, Co
 template <typename S, typename I>
 __global__ __launch_bounds__(256) void k_random_keys(const S *__restrict__ t, CodeMap<S> cm, KeyGeom g, DevLimits L,
                                                      uint64_t max_ctx, const I *__restrict__ pos, uint64_t n_pos,
-                                                     uint64_t m, uint64_t seed, uint64_t *__restrict__ keys) {
+                                                     uint64_t m, uint64_t seed, uint64_t *__restrict__ keys,
+                                                     const uint64_t *__restrict__ keep, uint64_t n) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;  // SplitMix64 finaliser
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     z ^= z >> 31;
-    uint64_t j = (uint64_t)(((unsigned __int128)z * n_pos) >> 64);
-    uint64_t p = pos ? (uint64_t)pos[j] : j;
+    const uint64_t span = keep ? n : n_pos;
+    uint64_t j = (uint64_t)(((unsigned __int128)z * span) >> 64);
+    uint64_t p = keep ? next_kept(keep, j, n) : pos ? (uint64_t)pos[j] : j;
     uint64_t seg;
     uint64_t lim = lim_seg(L, p, &seg);
     keys[i] = with_id(g, key_at_seed(t, cm, g, p, lim < max_ctx ? lim : max_ctx), seg);
@@ -423,7 +466,7 @@ template <typename S, typename I, int RK>
 __global__ __launch_bounds__(256) void k_classify_seq(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, KeyGeom g,
                                                       DevLimits L, uint64_t max_ctx, Splitters SP, RouterDev R,
                                                       uint16_t *__restrict__ cid, uint32_t *__restrict__ hblk,
-                                                      uint32_t pc, uint64_t chunk) {
+                                                      uint32_t pc, uint64_t chunk, const uint64_t *__restrict__ keep) {
     constexpr uint32_t NS = kPwlSeg;
     __shared__ uint32_t chist[kCoarseBucketsMax];
     for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) chist[c] = 0;
@@ -463,8 +506,12 @@ __global__ __launch_bounds__(256) void k_classify_seq(const S *__restrict__ t, u
         uint32_t b[kSeqItems];
         thread_keys_seq<S>(t, n, cm, g, L, max_ctx, base, cnt, win, key);
         const uint32_t first = threadIdx.x * kSeqItems;
+        // kept positions (bit q of the bitmap; all when keep is null): the
+        // thread's 16 positions start at a multiple of 16, inside one word
+        uint32_t kb = 0xFFFFu;
+        if (keep && first < cnt) kb = (uint32_t)(keep[(base + first) >> 6] >> ((base + first) & 63)) & 0xFFFFu;
 #pragma unroll
-        for (uint32_t j = 0; j < kSeqItems; ++j) valid[j] = first + j < cnt;
+        for (uint32_t j = 0; j < kSeqItems; ++j) valid[j] = first + j < cnt && ((kb >> j) & 1u);
         if (RK == 0) coarse_bsearch<kSeqItems>(SP, coarse, key, b);
         else
             coarse_routed<RK, kSeqItems>(SP, coarse, R, rknots, rradix, ra, rb, rinv, rlo, rhi, nib, w2, key, valid,
@@ -568,7 +615,8 @@ __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint6
                                                   uint64_t n_pos, const uint16_t *__restrict__ cid, uint32_t pc,
                                                   const uint64_t *__restrict__ coff,
                                                   const uint32_t *__restrict__ hblk, KRec<I> *__restrict__ tmp,
-                                                  uint64_t chunk) {
+                                                  uint64_t chunk, const uint64_t *__restrict__ keep, uint32_t c_lo,
+                                                  uint32_t c_hi, uint64_t rbase) {
     __shared__ S win[SEQ ? kSeqChunk + 72 : 1];
     __shared__ uint64_t skey[SEQ ? kSeqChunk : 1];
     __shared__ unsigned long long cbase[kCoarseBucketsMax];
@@ -577,9 +625,11 @@ __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint6
     const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
     if (c0 >= n_pos) return;
     const uint64_t c1 = c0 + chunk < n_pos ? c0 + chunk : n_pos;
-    // this block's runs: the classify pass counted its range per interval
+    // this block's runs: the classify pass counted its range per interval;
+    // only intervals [c_lo, c_hi) (this batch) are written, at offsets
+    // relative to the batch's first record rbase
     for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) {
-        cbase[c] = coff[c] + hblk[(uint64_t)blockIdx.x * pc + c];
+        cbase[c] = coff[c] - rbase + hblk[(uint64_t)blockIdx.x * pc + c];
         lcnt[c] = 0;
     }
     __syncthreads();
@@ -590,27 +640,28 @@ __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint6
             constexpr int U = kSeqChunk / 256;
             uint32_t cc[U];
             unsigned long long slot[U];
+            bool take[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 uint32_t i = u * 256 + threadIdx.x;
-                cc[u] = i < cnt ? cid[base + i] : 0u;
+                take[u] = i < cnt && (!keep || ((keep[(base + i) >> 6] >> ((base + i) & 63)) & 1ull));
+                cc[u] = take[u] ? cid[base + i] : 0u;
+                take[u] = take[u] && cc[u] >= c_lo && cc[u] < c_hi;
             }
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                uint32_t i = u * 256 + threadIdx.x;
-                slot[u] = i < cnt ? cbase[cc[u]] + atomicAdd(&lcnt[cc[u]], 1u) : 0ull;
-            }
+            for (int u = 0; u < U; ++u) slot[u] = take[u] ? cbase[cc[u]] + atomicAdd(&lcnt[cc[u]], 1u) : 0ull;
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 uint32_t i = u * 256 + threadIdx.x;
-                if (i < cnt) tmp[slot[u]] = KRec<I>{skey[i], (I)(base + i)};
+                if (take[u]) tmp[slot[u]] = KRec<I>{skey[i], (I)(base + i)};
             }
             __syncthreads();
         }
     } else {
         for (uint64_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
-            uint64_t p = pos ? (uint64_t)pos[i] : i;
             uint32_t c = cid[i];
+            if (c < c_lo || c >= c_hi) continue;
+            uint64_t p = pos ? (uint64_t)pos[i] : i;
             unsigned long long slot = cbase[c] + atomicAdd(&lcnt[c], 1u);
             uint64_t seg;
             uint64_t lim = lim_seg(L, p, &seg);
@@ -654,7 +705,7 @@ template <typename I>
 __global__ __launch_bounds__(256) void k_scatter2(const KRec<I> *__restrict__ tmp, uint16_t *__restrict__ fid,
                                                   const uint64_t *__restrict__ coff, Splitters SP, uint32_t p,
                                                   uint64_t *__restrict__ off, uint64_t *__restrict__ keys,
-                                                  I *__restrict__ idx) {
+                                                  I *__restrict__ idx, uint32_t c_lo, uint32_t c_hi, uint64_t rbase) {
     // LDS sized by the launch for SP.stride buckets: inner splitters, global
     // cursors, tile offsets, counts
     extern __shared__ unsigned long long scatter2_lds[];
@@ -666,8 +717,9 @@ __global__ __launch_bounds__(256) void k_scatter2(const KRec<I> *__restrict__ tm
     unsigned long long *gcur = scatter2_lds + SP.stride;
     unsigned long long *lbase = scatter2_lds + 2 * SP.stride;
     uint32_t *cnt = (uint32_t *)(scatter2_lds + 3 * SP.stride);
-    const uint32_t c = blockIdx.x;
-    const uint64_t s = coff[c], e = coff[c + 1];
+    // interval c of the batch [c_lo, c_hi); record offsets relative to rbase
+    const uint32_t c = c_lo + blockIdx.x;
+    const uint64_t s = coff[c] - rbase, e = coff[c + 1] - rbase;
     // buckets of interval c: b0 .. b0 + nb - 1; its inner splitters spl[b0 .. b0 + nb - 1)
     const uint32_t b0 = c == 0 ? 0u : (c - 1) * SP.stride + 1;
     const uint32_t nb = c == 0 ? 1u : (SP.stride < p - b0 ? SP.stride : p - b0);
@@ -711,7 +763,7 @@ __global__ __launch_bounds__(256) void k_scatter2(const KRec<I> *__restrict__ tm
         off[b0 + j] = gcur[j];
         cnt[j] = 0;
     }
-    if (b0 + nb == p && threadIdx.x == 0) off[p] = e;
+    if (c + 1 == c_hi && threadIdx.x == 0) off[b0 + nb] = e;  // the batch's end
     __syncthreads();
     // placing pass, one tile of kScatterTile records at a time: count the
     // tile's buckets, scan, stage the records in LDS grouped by bucket, then
@@ -920,20 +972,26 @@ __global__ __launch_bounds__(256) void k_copy_ranges(const Range *__restrict__ r
 // LCPs across key boundaries, once refinement has fixed which member of a
 // group sits next to the boundary: min(shared fields, both windows)
 // (pack.rs:307-317 with max_context).
+// has_prev: this is a later batch, whose first suffix follows the previous
+// batch's last one (key prev_key, position prev_pos; the keys differ).  The
+// previous suffix is passed by value: a per-lane choice between a pointer
+// load and an argument leaves no load through an optional pointer.
 template <typename I>
 __global__ __launch_bounds__(256) void k_boundary_lcp(const uint64_t *__restrict__ keys, const I *__restrict__ idx,
                                                       uint64_t N, DevLimits L, uint64_t max_ctx, KeyGeom g,
-                                                      I *__restrict__ lcp) {
+                                                      I *__restrict__ lcp, uint32_t has_prev, uint64_t prev_key,
+                                                      uint64_t prev_pos) {
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
-        if (i == 0) {
+        if (i == 0 && !has_prev) {
             lcp[0] = 0;
             continue;
         }
-        uint64_t k0 = keys[i - 1], k1 = keys[i];
+        const uint64_t k0 = i ? keys[i - 1] : prev_key, k1 = keys[i];
         if (k0 == k1) continue;
+        const uint64_t p0 = i ? (uint64_t)idx[i - 1] : prev_pos;
         uint64_t v = shared_fields(g, k0, k1);
-        uint64_t ca = cap_at(L, idx[i - 1], max_ctx), cb = cap_at(L, idx[i], max_ctx);
+        uint64_t ca = cap_at(L, p0, max_ctx), cb = cap_at(L, idx[i], max_ctx);
         v = v < ca ? v : ca;
         v = v < cb ? v : cb;
         lcp[i] = (I)v;

@@ -76,6 +76,10 @@ typedef struct caps_opts {
                                   (0 = 1280; 64..4096); ignored with gpu_buckets       */
     uint32_t want_lcp;         /* fill the LCP array where an entry point takes one     */
     uint32_t flags;            /* CAPS_FLAG_* bits, 0 by default                        */
+    uint64_t batch_records;    /* records per bucket batch: the build sorts consecutive
+                                  coarse intervals of at most this many suffixes at a
+                                  time (0 = auto: one batch while its buffers fit in
+                                  HBM).  The output never changes.                     */
 } caps_opts_t;
 /* Skip router training (the learned router runs with its initial parameters;
  * used to show what training gains).  The output never changes. */
@@ -145,6 +149,17 @@ int caps_build_ext_mem_for_positions(const void *text, uint64_t n, uint32_t sym_
                                      uint64_t n_pos, const caps_limits_t *limits, const caps_opts_t *opts,
                                      caps_emit_fn emit, void *ctx);
 
+/* (try_)build_ext_mem_for_filter(_with) (ext_mem.rs:472-547).  The crate walks
+ * the caller's keep predicate once into a bitmap of kept positions with
+ * per-block popcounts (FilteredSource, ext_mem.rs:927-1092) and never calls it
+ * again; here the caller passes that bitmap: (n + 63) / 64 words, position q
+ * kept iff bit q & 63 of word q >> 6 is set (bits past n are ignored).  No
+ * position list is materialised: the GPU walks the bitmap beside the text
+ * windows.  Emits every kept position, in order, as above. */
+int caps_build_ext_mem_for_filter(const void *text, uint64_t n, uint32_t sym_bytes, const uint64_t *keep_words,
+                                  const caps_limits_t *limits, const caps_opts_t *opts, caps_emit_fn emit,
+                                  void *ctx);
+
 /* ---- device-resident builder (inputs and outputs already in HBM) ---------- */
 
 /* Same as caps_build_in_memory_for_positions with device pointers.
@@ -152,6 +167,15 @@ int caps_build_ext_mem_for_positions(const void *text, uint64_t n, uint32_t sym_
 int caps_build_device(const void *d_text, uint64_t n, uint32_t sym_bytes, const void *d_positions, uint64_t n_pos,
                       uint32_t idx_bytes, const caps_limits_t *limits, const caps_opts_t *opts, void *d_out_sa,
                       void *d_out_lcp, void *stream);
+
+/* caps_build_ext_mem_for_filter with device pointers: d_keep_words is the
+ * kept-position bitmap in HBM (read only).  *n_out receives the kept count;
+ * d_out_sa (and d_out_lcp when not NULL) must hold out_capacity >= that many
+ * entries, else CAPS_E_INVALID. */
+int caps_build_device_filtered(const void *d_text, uint64_t n, uint32_t sym_bytes, const uint64_t *d_keep_words,
+                               uint32_t idx_bytes, const caps_limits_t *limits, const caps_opts_t *opts,
+                               void *d_out_sa, uint64_t out_capacity, uint64_t *n_out, void *d_out_lcp,
+                               void *stream);
 
 /* ---- batched builds: many independent texts in one pipeline -------------- */
 
@@ -190,6 +214,7 @@ typedef struct caps_stats {
     uint32_t key_bits, key_k;
     uint64_t bucket_max; /* records in the largest bucket (imbalance = bucket_max * buckets / n) */
     double k_scatter1, k_scatter2; /* the two scatter passes (their sum is k_scatter) */
+    uint64_t batches;              /* bucket batches the build ran                      */
 } caps_stats_t;
 int caps_last_stats(caps_stats_t *out);
 

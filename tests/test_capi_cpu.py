@@ -33,7 +33,7 @@ def test_python_binding_covers_every_declared_symbol():
 
 def test_struct_layouts():
     assert ctypes.sizeof(caps_sa._Limits) == 24
-    assert ctypes.sizeof(caps_sa._Opts) == 40
+    assert ctypes.sizeof(caps_sa._Opts) == 48
 
 
 def test_opts_default():
@@ -101,3 +101,37 @@ def test_cpp_api_mirror_header_compiles():
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-x", "c++", "-I",
                         os.path.join(root, "caps-sa_amd", "host"), "-"], input=src, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_header_struct_sizes_match_the_bindings():
+    """caps_opts_t / caps_stats_t as the C compiler lays them out equal the
+    ctypes mirrors."""
+    import subprocess
+    import tempfile
+
+    src = ("#include <stdio.h>\n#include \"caps_sa.h\"\n"
+           "int main(void) { printf(\"%zu %zu %zu\\n\", sizeof(caps_opts_t), sizeof(caps_stats_t), "
+           "sizeof(caps_limits_t)); return 0; }\n")
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "sizes")
+        r = subprocess.run(["gcc", "-x", "c", "-I", os.path.join(ROOT, "include"), "-o", exe, "-"], input=src,
+                           capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        out = subprocess.run([exe], capture_output=True, text=True).stdout.split()
+    assert [int(v) for v in out] == [ctypes.sizeof(caps_sa._Opts), ctypes.sizeof(caps_sa.Stats),
+                                     ctypes.sizeof(caps_sa._Limits)]
+
+
+def test_keep_bitmap_layout():
+    """Position q kept iff bit q % 64 of word q // 64 (the crate's
+    FilteredSource bitmap, ext_mem.rs:940-990), from a mask, a vectorised
+    predicate or a per-position predicate."""
+    rng = np.random.default_rng(3)
+    for n in (0, 1, 63, 64, 65, 1000):
+        mask = rng.random(n) < 0.4
+        for keep in (mask, lambda q: mask[q.astype(np.int64)], lambda q: bool(mask[int(q)])):
+            bits = caps_sa.keep_bitmap(keep, n)
+            assert bits.dtype == np.uint64 and len(bits) == (n + 63) // 64
+            back = [(int(bits[q >> 6]) >> (q & 63)) & 1 for q in range(n)]
+            assert back == mask.astype(int).tolist()
+            assert all(int(bits[-1]) >> b == 0 for b in range(n % 64, 64) if n % 64) if n else True
