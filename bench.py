@@ -38,6 +38,11 @@ WORKLOADS = {
                           "one GPU's eighth of the 6.56e9-symbol genome: 175,000 lognormal segments (median 2,500, "
                           "sigma 1.6) of i.i.d. {0..3} ending in spacer 4, ~5.8% N runs (symbol 5, geometric mean "
                           "10,000); positions {p : text[p] < 4}, SegmentedText ShorterFirst, u64 index"),
+    "genome-6.56G": ("genome-full", 6_560_000_000,
+                     "the whole 6.56e9-symbol genome on one GPU: 1,400,000 lognormal segments (median 2,500, sigma "
+                     "1.6) of i.i.d. {0..3} ending in spacer 4, ~5.8% N runs (symbol 5, geometric mean 10,000); kept "
+                     "positions {p : text[p] < 4} as a bitmap (build_ext_mem_for_filter), SegmentedText "
+                     "ShorterFirst, u64 index, bucket batches"),
     "batch-4096": ("batch", 0, "4,096 texts, lognormal lengths (median 65,536, sigma 1.5, clipped to [1,024, "
                               "16,777,216]); even texts u8 i.i.d. {0..3}, odd texts u16 Zipf(1.1) over 50,000 symbols"),
 }
@@ -216,9 +221,11 @@ def run_batch(args, count: int) -> dict:
             "kernels": avg, "last": stats[-1], "per_width": per_width, "autotune": tuned}
 
 
-def run_genome(args, n: int) -> dict:
-    """Segmented + filtered build at genome scale on one GPU: positions and
-    suffix array in HBM (u64), SegmentedText limits."""
+def run_genome(args, n: int, filtered: bool = False) -> dict:
+    """Segmented + filtered build at genome scale on one GPU: suffix array in
+    HBM (u64), SegmentedText limits.  The kept positions are a u64 position
+    list in HBM, or (filtered) a bitmap walked beside the text, as the
+    crate's build_ext_mem_for_filter does."""
     import numpy as np
     import torch
 
@@ -227,41 +234,67 @@ def run_genome(args, n: int) -> dict:
     from caps_sa import gen
 
     n_seg = max(1, round(n * 1.40e6 / 6.56e9))
-    text, ends, pos = gen.genome(n, n_seg, seed=args.seed)
-    print(f"[bench] genome share: n={n} segments={n_seg} positions={len(pos)}", file=sys.stderr, flush=True)
+    t_gen = time.perf_counter()
+    print(f"[bench] generating genome n={n} segments={n_seg}", file=sys.stderr, flush=True)
+    text, ends, pos = gen.genome(n, n_seg, seed=args.seed, positions=not filtered)
     lp = caps_sa.SegmentedText(n, ends)
     d_text = dev.text_to_device(text)
-    d_pos = torch.from_numpy(pos.view(np.int64)).to("cuda:0")
-    d_out = torch.empty(len(pos), dtype=torch.int64, device="cuda:0")
+    flags = caps_sa.FLAG_POW2_SEED if args.pow2_seed else 0
+    if filtered:
+        keep = gen.keep_below(text, 4)
+        n_pos = int(np.bitwise_count(keep).sum())
+        d_keep = torch.from_numpy(keep.view(np.int64)).to("cuda:0")
+        d_out = torch.empty(n_pos, dtype=torch.int64, device="cuda:0")
 
-    def step():
-        dev.build_device(d_text, n, 1, d_positions=d_pos, n_pos=len(pos), idx_bytes=8, lp=lp, d_out=d_out,
-                         flags=caps_sa.FLAG_POW2_SEED if args.pow2_seed else 0)
-        return caps_sa.last_stats()
+        def step():
+            dev.build_device_filtered(d_text, n, 1, d_keep, n_pos, idx_bytes=8, lp=lp, d_out=d_out, flags=flags,
+                                      batch_records=args.batch_records)
+            return caps_sa.last_stats()
+    else:
+        n_pos = len(pos)
+        d_pos = torch.from_numpy(pos.view(np.int64)).to("cuda:0")
+        d_out = torch.empty(n_pos, dtype=torch.int64, device="cuda:0")
 
-    for _ in range(args.warmup):
-        step()
+        def step():
+            dev.build_device(d_text, n, 1, d_positions=d_pos, n_pos=n_pos, idx_bytes=8, lp=lp, d_out=d_out,
+                             flags=flags, batch_records=args.batch_records)
+            return caps_sa.last_stats()
+    print(f"[bench] genome: n={n} segments={n_seg} positions={n_pos} "
+          f"(inputs ready in {time.perf_counter() - t_gen:.1f} s)", file=sys.stderr, flush=True)
+
+    for i in range(args.warmup):
+        st = step()
+        print(f"[bench] warmup {i}: {st['ms_total']:.1f} ms, {st['batches']} batch(es)", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    stats = [step() for _ in range(args.steps)]
+    stats = []
+    for i in range(args.steps):
+        stats.append(step())
+        print(f"[bench] step {i}: {stats[-1]['ms_total']:.1f} ms", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     ms_step = (time.perf_counter() - t0) * 1e3 / max(args.steps, 1)
     verified = None
     if not args.no_verify:
         import oracle as O
 
+        t_v = time.perf_counter()
         sa = d_out.cpu().numpy().view(np.uint64)
-        bad = O.Ctx(text, ends, O.ORDER_SHORTER_FIRST).verify(sa, positions=pos, strict=True, canon=True,
-                                                               threads=args.cpu_threads)
+        ctx = O.Ctx(text, ends, O.ORDER_SHORTER_FIRST)
+        if filtered:
+            bad = ctx.verify_keep(sa, keep, strict=True, canon=True, threads=args.cpu_threads)
+        else:
+            bad = ctx.verify(sa, positions=pos, strict=True, canon=True, threads=args.cpu_threads)
         verified = bool(bad == -1)
-        print(f"[bench] verify: {'ok' if verified else f'FAILED at {bad}'}", file=sys.stderr, flush=True)
+        print(f"[bench] verify: {'ok' if verified else f'FAILED at {bad}'} ({time.perf_counter() - t_v:.1f} s)",
+              file=sys.stderr, flush=True)
     kern = ["k_classify", "k_scatter1", "k_scatter2", "k_scatter", "k_tile_sort", "k_pairs", "k_groups_small",
             "k_groups_mid", "k_boundary_lcp"]
     avg = {k: sum(st[k] for st in stats) / len(stats) for k in kern}
     phases = ("ms_alphabet", "ms_sample", "ms_classify", "ms_scatter", "ms_tile_sort", "ms_merge", "ms_detect",
               "ms_refine", "ms_giant")
     last = stats[-1]
-    return {"ms_step": ms_step, "n_pos": len(pos), "n": n, "segments": n_seg, "verified": verified,
+    return {"ms_step": ms_step, "n_pos": n_pos, "n": n, "segments": n_seg, "verified": verified,
+            "batches": last["batches"],
             "kernels": avg, "phases": {ph: round(sum(st[ph] for st in stats) / len(stats), 3) for ph in phases},
             "groups": {k: last[k] for k in ("pairs", "small_groups", "mid_groups", "giant_groups", "oversized",
                                              "buckets", "bucket_max")}, "key_k": last["key_k"]}
@@ -282,6 +315,8 @@ def main() -> int:
     ap.add_argument("--bucket-keys", type=int, default=0, help="mean records per bucket (0: library default)")
     ap.add_argument("--buckets", type=int, default=0, help="explicit bucket count (0: from --bucket-keys)")
     ap.add_argument("--texts", type=int, default=4096, help="batch workload: number of texts")
+    ap.add_argument("--batch-records", type=int, default=0,
+                    help="genome workloads: records per bucket batch (0: one batch while it fits in HBM)")
     ap.add_argument("--pow2-seed", action="store_true",
                     help="genome workload: seed keys with the packed text's power-of-two field width")
     ap.add_argument("--autotune", action="store_true",
@@ -345,8 +380,8 @@ def main() -> int:
         }
         print(json.dumps(line))
         return 0 if r["verified"] in (None, True) else 3
-    if kind == "genome":
-        r = run_genome(args, args.n or n_default)
+    if kind in ("genome", "genome-full"):
+        r = run_genome(args, args.n or n_default, filtered=kind == "genome-full")
         n_pos = r["n_pos"]
         value = n_pos / (r["ms_step"] / 1e3) / 1e6
         avg = r["kernels"]
@@ -364,7 +399,8 @@ def main() -> int:
             "warmup": args.warmup, "ms_per_step": round(r["ms_step"], 3), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
             "config": {"workload": args.workload, "n": r["n"], "segments": r["segments"], "positions": n_pos,
-                       "text": desc, "index": "u64", "key_symbols": r["key_k"], "seed": args.seed},
+                       "text": desc, "index": "u64", "key_symbols": r["key_k"], "bucket_batches": r["batches"],
+                       "seed": args.seed},
             "verified_bit_exact": r["verified"], "phases_ms": r["phases"],
             "kernels_ms": {k: round(v, 3) for k, v in avg.items()}, "groups": r["groups"],
             "roofline": rooflines[dom], "rooflines": rooflines, "cpu_baseline": None,

@@ -66,10 +66,24 @@ def lognormal_lengths(count: int, median: float, sigma: float, lo: int, hi: int,
 
 
 def genome(n: int, n_seg: int, median: float = 2500.0, sigma: float = 1.6, n_frac: float = 0.058,
-           n_mean: float = 10000.0, seed: int = 1):
-    """Segmented genome-like text: returns (text, ends, positions with text[p] < 4)."""
+           n_mean: float = 10000.0, seed: int = 1, positions: bool = True):
+    """Segmented genome-like text: returns (text, ends, positions with text[p] < 4);
+    positions is None when not asked for (a genome-scale list is 8 bytes a position)."""
     text = np.empty(n, dtype=np.uint8)
     ends = np.empty(n_seg, dtype=np.uint64)
     _lib().caps_gen_genome(text.ctypes.data, ends.ctypes.data, n, n_seg, median, sigma, n_frac, n_mean, seed)
-    pos = np.flatnonzero(text < 4).astype(np.uint64)
+    pos = np.flatnonzero(text < 4).astype(np.uint64) if positions else None
     return text, ends, pos
+
+
+def keep_below(text: np.ndarray, limit: int, chunk: int = 1 << 26) -> np.ndarray:
+    """Kept-position bitmap of {q : text[q] < limit}, built chunk by chunk
+    ((n + 63) // 64 u64 words, bit q % 64 of word q // 64)."""
+    n = len(text)
+    words = np.zeros((n + 63) // 64, dtype=np.uint64)
+    as_bytes = words.view(np.uint8)
+    for s in range(0, n, chunk):  # chunk is a multiple of 64: whole words
+        e = min(n, s + chunk)
+        packed = np.packbits(text[s:e] < limit, bitorder="little")
+        as_bytes[s // 8: s // 8 + len(packed)] = packed
+    return words

@@ -280,13 +280,41 @@ __device__ __forceinline__ uint64_t with_id(const KeyGeom &g, uint64_t key, uint
     return g.idbits ? (key | (id << (64 - g.idbits))) : key;
 }
 
-// Scalar LCP of text[p+start..] and text[q+start..], stopping at `cap`.
+// The 8 bytes starting at byte address x (little-endian), from the aligned
+// words that hold them; a word is loaded only when it holds a byte below
+// `end` (x < end), so no load leaves the window.  Bytes at and past end read
+// as zero or as whatever the loaded word holds: callers mask them.
+__device__ __forceinline__ uint64_t load8_upto(uint64_t x, uint64_t end) {
+    const uint64_t a8 = x & ~7ull;
+    const uint32_t sh = (uint32_t)(x & 7) * 8;
+    const uint64_t w0 = *(const uint64_t *)a8;
+    if (!sh) return w0;
+    const uint64_t w1 = a8 + 8 < end ? *(const uint64_t *)(a8 + 8) : 0ull;
+    return (w0 >> sh) | (w1 << (64 - sh));
+}
+
+// Scalar LCP of text[p+start..] and text[q+start..], stopping at `cap`
+// (lcp.rs:233-246 as eight bytes per step: the first differing byte of the
+// two 8-byte stretches is the lowest set byte of their XOR).
 template <typename S>
 __device__ __forceinline__ uint64_t thread_lcp(const S *__restrict__ t, uint64_t p, uint64_t q, uint64_t start,
                                                uint64_t cap) {
+    constexpr uint64_t SZ = sizeof(S);
+    const uint64_t tb = (uint64_t)(uintptr_t)t;
+    const uint64_t endp = tb + (p + cap) * SZ, endq = tb + (q + cap) * SZ;
     uint64_t o = start;
-    while (o < cap && t[p + o] == t[q + o]) ++o;
-    return o;
+    while (o < cap) {
+        const uint64_t wa = load8_upto(tb + (p + o) * SZ, endp), wb = load8_upto(tb + (q + o) * SZ, endq);
+        uint64_t x = wa ^ wb;
+        const uint64_t rem = (cap - o) * SZ;  // window bytes left
+        if (rem < 8) x |= ~0ull << (rem * 8);  // bytes past the window differ
+        if (x) {
+            const uint64_t r = o + ((uint64_t)__builtin_ctzll(x) >> 3) / SZ;
+            return r < cap ? r : cap;
+        }
+        o += 8 / SZ;
+    }
+    return cap;
 }
 
 // 64-lane wavefront LCP (the SIMD LCP of lcp.rs:292-360 as a wave-wide

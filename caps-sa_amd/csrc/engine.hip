@@ -426,12 +426,20 @@ struct BatchSink {
 };
 
 // Records per bucket batch when the caller does not choose: every record in
-// one batch while the batch buffers (about 64 bytes per record: records,
-// keys, bucket ids, group lists, merge scratch) fit in 70% of the free HBM.
-static uint64_t auto_batch_records(uint64_t n_pos) {
+// one batch while the batch buffers fit in 75% of the free HBM.  Per record,
+// at most: 16-byte record, 8-byte key, 2-byte bucket id, about 20 bytes of
+// group lists (sized for the worst case of every class), 8-byte output
+// staging, and 32 bytes of merge scratch (keys, positions, two LCP arrays)
+// when oversized buckets or giant groups need them: 112 bytes with margin.
+// Buffers the workspace already holds for batches count as free: they are
+// reused (grown, never shrunk) by this build.
+static uint64_t auto_batch_records(const Workspace &W, uint64_t n_pos) {
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || free_b == 0) return n_pos;
-    const uint64_t fit = std::max<uint64_t>(1ull << 24, (uint64_t)(free_b * 0.7) / 64);
+    size_t held = W.recs.cap + W.keys.cap + W.fid.cap + W.sinkbuf.cap + W.keys2.cap + W.idx2.cap + W.lcp.cap +
+                  W.lcp2.cap + W.tiles.cap;
+    for (int c = 0; c < kGroupClasses; ++c) held += W.glist[c].cap;
+    const uint64_t fit = std::max<uint64_t>(1ull << 24, (uint64_t)((free_b + held) * 0.75) / 112);
     return std::min<uint64_t>(n_pos, fit);
 }
 
@@ -554,7 +562,8 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     // bucket sizes by about a quarter around the mean)
     const uint64_t target = o.tile_keys ? std::max<uint32_t>(64, std::min<uint32_t>(o.tile_keys, T)) : kBucketTarget;
     uint64_t p = o.gpu_buckets ? o.gpu_buckets : ceil_div(n_pos, target);
-    p = std::max<uint64_t>(1, std::min<uint64_t>(p, std::min<uint64_t>(n_pos, 1u << 22)));
+    // at most kCoarseMax coarse intervals of kFineMax buckets each
+    p = std::max<uint64_t>(1, std::min<uint64_t>(p, std::min<uint64_t>(n_pos, (uint64_t)kCoarseMax * kFineMax + 1)));
     const uint64_t oversample = 16;
     uint64_t s = p > 1 ? std::min<uint64_t>(n_pos, p * oversample) : 0;
     uint64_t *spl = (uint64_t *)W.spl.get(std::max<uint64_t>(p, 1) * 8);
@@ -647,7 +656,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     std::vector<uint64_t> hcoff(pc + 1);
     HIPCHK(hipMemcpyAsync(hcoff.data(), coff, (pc + 1) * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    const uint64_t bmax = o.batch_records ? o.batch_records : auto_batch_records(n_pos);
+    const uint64_t bmax = o.batch_records ? o.batch_records : auto_batch_records(W, n_pos);
     std::vector<uint32_t> cuts{0};
     for (uint32_t c = 0; c < pc; ++c)
         if (c > cuts.back() && hcoff[c + 1] - hcoff[cuts.back()] > bmax) cuts.push_back(c);

@@ -741,6 +741,50 @@ int64_t orc_verify_sa(const orc_text *T, const orc_limits *L, const uint64_t *sa
     return first;
 }
 
+/* The same checks against a kept-position bitmap (keep_words: bit q & 63 of
+ * word q >> 6) instead of a position list: every entry kept and distinct,
+ * as many entries as kept positions, then adjacent order.  The permutation
+ * part runs in parallel with atomic bitmap updates. */
+int64_t orc_verify_sa_keep(const orc_text *T, const orc_limits *L, const uint64_t *sa, uint64_t n,
+                           const uint64_t *keep_words, uint64_t max_ctx, int strict, int canon, int nthreads) {
+    uint64_t tn = T->n;
+    uint64_t words = (tn + 63) / 64;
+    uint64_t kept = 0;
+    for (uint64_t w = 0; w < words; w++) {
+        uint64_t v = keep_words[w];
+        if (w == words - 1 && (tn & 63)) v &= (1ull << (tn & 63)) - 1;
+        kept += (uint64_t)__builtin_popcountll(v);
+    }
+    if (kept != n) return (int64_t)n;
+    uint64_t *seen = (uint64_t *)calloc(words + 1, 8);
+    if (!seen) return (int64_t)n;
+    int bad = 0;
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1) reduction(| : bad)
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t v = sa[i];
+        if (v >= tn) {
+            bad = 1;
+            continue;
+        }
+        uint64_t w = v >> 6, m = 1ull << (v & 63);
+        if (!(keep_words[w] & m)) bad = 1;
+        if (__atomic_fetch_or(&seen[w], m, __ATOMIC_RELAXED) & m) bad = 1;
+    }
+    free(seen);
+    if (bad) return (int64_t)n;
+    int64_t first = -1;
+#pragma omp parallel for schedule(dynamic, 65536) num_threads(nthreads > 0 ? nthreads : 1)
+    for (uint64_t i = 1; i < n; i++) {
+        int c = orc_suffix_cmp_with(T, L, sa[i - 1], sa[i], max_ctx);
+        if (canon && c == 0) c = cmp_u64(sa[i - 1], sa[i]);
+        if (c > 0 || (strict && c == 0)) {
+#pragma omp critical
+            if (first < 0 || (int64_t)i < first) first = (int64_t)i;
+        }
+    }
+    return first;
+}
+
 static int u64_cmp(const void *a, const void *b) {
     uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
     return (x > y) - (x < y);

@@ -136,6 +136,8 @@ uint64_t orc_naive_lcp(const orc_text *T, const orc_limits *L, uint64_t a, uint6
  * non-decreasing (strict=0) or increasing (strict=1) under suffix_cmp_with,
  * ascending position among Equal when canon=1.  Returns -1 if ok, else the
  * first failing index (n for a permutation failure). */
+int64_t orc_verify_sa_keep(const orc_text *T, const orc_limits *L, const uint64_t *sa, uint64_t n,
+                           const uint64_t *keep_words, uint64_t max_ctx, int strict, int canon, int nthreads);
 int64_t orc_verify_sa(const orc_text *T, const orc_limits *L, const uint64_t *sa, uint64_t n,
                       const uint64_t *positions, uint64_t max_ctx, int strict, int canon, int nthreads);
 /* Reorders each run of comparator-Equal neighbours by ascending position. */

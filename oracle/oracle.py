@@ -74,6 +74,7 @@ def lib():
             "orc_kasai_lcp": (i32, [T, vp, vp]),
             "orc_naive_lcp": (u64, [T, L, u64, u64, u64]),
             "orc_verify_sa": (ctypes.c_int64, [T, L, vp, u64, vp, u64, i32, i32, i32]),
+            "orc_verify_sa_keep": (ctypes.c_int64, [T, L, vp, u64, vp, u64, i32, i32, i32]),
             "orc_canonicalize": (None, [T, L, vp, u64, u64]),
         }
         for name, (res, args) in sig.items():
@@ -192,6 +193,13 @@ class Ctx:
         return lib().orc_verify_sa(ctypes.byref(self.T), ctypes.byref(self.L), sa.ctypes.data, len(sa),
                                    pos.ctypes.data if pos is not None else None, max_ctx, int(strict), int(canon),
                                    threads)
+
+    def verify_keep(self, sa, keep_words, max_ctx=MAX_CTX, strict=True, canon=True, threads=8):
+        """verify() against a kept-position bitmap instead of a position list."""
+        sa = _u64(sa)
+        kw = _u64(keep_words)
+        return lib().orc_verify_sa_keep(ctypes.byref(self.T), ctypes.byref(self.L), sa.ctypes.data, len(sa),
+                                        kw.ctypes.data, max_ctx, int(strict), int(canon), threads)
 
     def canonicalize(self, sa, max_ctx=MAX_CTX):
         sa = _u64(sa).copy()

@@ -265,3 +265,22 @@ def test_checks_catch_a_planted_bug(monkeypatch):
     swapped[[5, 6]] = swapped[[6, 5]]
     assert ctx.verify(swapped) != -1
     assert not np.array_equal(swapped, ctx.brute_sa())
+
+
+def test_verify_against_keep_bitmap():
+    """The bitmap verifier accepts the oracle's SA of the kept positions and
+    rejects a swapped pair, a dropped entry and an entry outside the bitmap."""
+    from caps_sa import gen
+
+    text, ends, pos = gen.genome(60000, 25, median=2500.0, n_frac=0.058, n_mean=300.0, seed=5)
+    ctx = O.Ctx(text, ends, O.ORDER_SHORTER_FIRST)
+    sa = ctx.canonicalize(ctx.build_in_memory(pos, threads=4))
+    keep = gen.keep_below(text, 4)
+    assert ctx.verify_keep(sa, keep, threads=4) == -1
+    bad = sa.copy()
+    bad[10], bad[11] = bad[11], bad[10]
+    assert ctx.verify_keep(bad, keep, threads=4) == 11
+    assert ctx.verify_keep(sa[:-1], keep, threads=4) == len(sa) - 1
+    outside = sa.copy()
+    outside[0] = int(np.flatnonzero(text >= 4)[0])
+    assert ctx.verify_keep(outside, keep, threads=4) == len(sa)
