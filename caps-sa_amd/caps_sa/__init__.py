@@ -348,6 +348,7 @@ FLAG_TILE_RADIX = 4
 FLAG_MERGE_W16 = 8
 FLAG_MERGE_W64 = 16
 FLAG_POW2_SEED = 32
+FLAG_SCATTER_PER_BATCH = 64
 TUNE_MAX = 16
 
 

@@ -425,22 +425,41 @@ struct BatchSink {
     std::function<void(const I *, uint64_t, hipStream_t)> flush;
 };
 
-// Records per bucket batch when the caller does not choose: every record in
-// one batch while the batch buffers fit in 75% of the free HBM.  Per record,
-// at most: 16-byte record, 8-byte key, 2-byte bucket id, about 20 bytes of
-// group lists (sized for the worst case of every class), 8-byte output
-// staging, and 32 bytes of merge scratch (keys, positions, two LCP arrays)
-// when oversized buckets or giant groups need them: 112 bytes with margin.
-// Buffers the workspace already holds for batches count as free: they are
-// reused (grown, never shrunk) by this build.
-static uint64_t auto_batch_records(const Workspace &W, uint64_t n_pos) {
+// Bucket-batch plan.  Per record a batch needs at most: 16-byte record,
+// 8-byte key, 2-byte bucket id, about 20 bytes of group lists (sized for the
+// worst case of every class), 8-byte output staging and 32 bytes of merge
+// scratch (keys, positions, two LCP arrays) when oversized buckets or giant
+// groups need them: 112 bytes with margin; 96 without the record when the
+// records of all batches are written at once (pass 1 runs once instead of
+// once per batch).  75% of the free HBM is used; buffers the workspace
+// already holds count as free, since this build reuses them.
+struct BatchPlan {
+    uint64_t bmax;     // records per batch
+    bool all_records;  // scatter pass 1 writes every record once
+};
+static BatchPlan plan_batches(const Workspace &W, uint64_t n_pos, uint64_t requested, size_t rec_bytes,
+                              bool per_batch_pass1) {
     size_t free_b = 0, total_b = 0;
-    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || free_b == 0) return n_pos;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || free_b == 0) return {requested ? requested : n_pos, false};
     size_t held = W.recs.cap + W.keys.cap + W.fid.cap + W.sinkbuf.cap + W.keys2.cap + W.idx2.cap + W.lcp.cap +
                   W.lcp2.cap + W.tiles.cap;
     for (int c = 0; c < kGroupClasses; ++c) held += W.glist[c].cap;
-    const uint64_t fit = std::max<uint64_t>(1ull << 24, (uint64_t)((free_b + held) * 0.75) / 112);
-    return std::min<uint64_t>(n_pos, fit);
+    const double avail = (double)(free_b + held) * 0.75;
+    if (!requested && (double)n_pos * 112 <= avail) return {n_pos, false};  // one batch
+    const double rest = avail - (double)n_pos * rec_bytes;  // beside all records
+    const uint64_t b_each = requested ? requested : std::max<uint64_t>(1ull << 24, (uint64_t)(avail / 112));
+    if (!per_batch_pass1 && rest > 0) {
+        if (requested) {
+            if ((double)requested * 96 <= rest) return {requested, true};
+        } else {
+            // one pass 1 for all records, unless it costs more than one extra
+            // batch: pass 2 runs one workgroup per coarse interval, so a batch
+            // of few intervals leaves compute units idle
+            const uint64_t b = (uint64_t)(rest / 96);
+            if (b && ceil_div(n_pos, b) <= ceil_div(n_pos, b_each) + 1) return {std::min<uint64_t>(b, n_pos), true};
+        }
+    }
+    return {std::min<uint64_t>(b_each, n_pos), false};
 }
 
 // Clears the bits past n of a device bitmap and returns its kept count.
@@ -615,9 +634,9 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     // staged once per block.
     unsigned resident = 0;
     if (seq)
-        resident = rkind == CAPS_ROUTER_PWL   ? resident_blocks(k_classify_seq<S, I, 1>, 256)
-                   : rkind == CAPS_ROUTER_MLP ? resident_blocks(k_classify_seq<S, I, 2>, 256)
-                                              : resident_blocks(k_classify_seq<S, I, 0>, 256);
+        resident = rkind == CAPS_ROUTER_PWL   ? resident_blocks(k_classify_seq<S, I, 1, false>, 256)
+                   : rkind == CAPS_ROUTER_MLP ? resident_blocks(k_classify_seq<S, I, 2, false>, 256)
+                                              : resident_blocks(k_classify_seq<S, I, 0, false>, 256);
     else
         resident = rkind == CAPS_ROUTER_PWL   ? resident_blocks(k_classify<S, I, 1>, 256)
                    : rkind == CAPS_ROUTER_MLP ? resident_blocks(k_classify<S, I, 2>, 256)
@@ -627,15 +646,23 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     const unsigned nblk = (unsigned)ceil_div(n_iter, chunk);
     uint32_t *hblk = (uint32_t *)W.hblk.get((uint64_t)nblk * pc * 4);
     if (seq) {
+        auto classify_seq = [&](auto rk, auto kp) {
+            constexpr int RK = decltype(rk)::value;
+            constexpr bool KP = decltype(kp)::value;
+            k_classify_seq<S, I, RK, KP><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, cid, hblk, pc,
+                                                               chunk, d_keep);
+        };
+        using kno = std::false_type;
+        using kyes = std::true_type;
         if (rkind == CAPS_ROUTER_PWL)
-            k_classify_seq<S, I, 1><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, cid, hblk, pc, chunk,
-                                                          d_keep);
+            d_keep ? classify_seq(std::integral_constant<int, 1>{}, kyes{})
+                   : classify_seq(std::integral_constant<int, 1>{}, kno{});
         else if (rkind == CAPS_ROUTER_MLP)
-            k_classify_seq<S, I, 2><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, cid, hblk, pc, chunk,
-                                                          d_keep);
+            d_keep ? classify_seq(std::integral_constant<int, 2>{}, kyes{})
+                   : classify_seq(std::integral_constant<int, 2>{}, kno{});
         else
-            k_classify_seq<S, I, 0><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, cid, hblk, pc, chunk,
-                                                          d_keep);
+            d_keep ? classify_seq(std::integral_constant<int, 0>{}, kyes{})
+                   : classify_seq(std::integral_constant<int, 0>{}, kno{});
     } else if (rkind == CAPS_ROUTER_PWL)
         k_classify<S, I, 1><<<nblk, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hblk, pc, chunk);
     else if (rkind == CAPS_ROUTER_MLP)
@@ -656,7 +683,9 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     std::vector<uint64_t> hcoff(pc + 1);
     HIPCHK(hipMemcpyAsync(hcoff.data(), coff, (pc + 1) * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    const uint64_t bmax = o.batch_records ? o.batch_records : auto_batch_records(W, n_pos);
+    const BatchPlan plan_b = plan_batches(W, n_pos, o.batch_records, sizeof(KRec<I>),
+                                          (o.flags & CAPS_FLAG_SCATTER_PER_BATCH) != 0);
+    const uint64_t bmax = plan_b.bmax;
     std::vector<uint32_t> cuts{0};
     for (uint32_t c = 0; c < pc; ++c)
         if (c > cuts.back() && hcoff[c + 1] - hcoff[cuts.back()] > bmax) cuts.push_back(c);
@@ -664,6 +693,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     uint64_t bcap = 0;
     for (size_t bi = 0; bi + 1 < cuts.size(); ++bi) bcap = std::max(bcap, hcoff[cuts[bi + 1]] - hcoff[cuts[bi]]);
     const bool single = cuts.size() == 2;
+    const bool all_records = !single && plan_b.all_records;  // pass 1 once, into a full-size record buffer
     stats.batches = cuts.size() - 1;
     auto first_bucket = [&](uint32_t c) -> uint64_t {  // first bucket of coarse interval c
         return c == 0 ? 0 : std::min<uint64_t>(p, (uint64_t)(c - 1) * SP.stride + 1);
@@ -671,7 +701,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
 
     // batch-sized buffers
     uint64_t *keys = (uint64_t *)W.keys.get(bcap * 8 + 8);
-    KRec<I> *tmp = (KRec<I> *)W.recs.get(bcap * sizeof(KRec<I>) + 16);
+    KRec<I> *tmp = (KRec<I> *)W.recs.get((all_records ? n_pos : bcap) * sizeof(KRec<I>) + 16);
     // the coarse ids of a single batch are consumed by pass 1, so their buffer
     // then holds the per-record bucket within its interval; batches need theirs
     uint16_t *fid = single ? cid : (uint16_t *)W.fid.get(bcap * 2 + 2);
@@ -703,6 +733,23 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         return (uint32_t)std::max<uint64_t>(slots, std::min<uint64_t>(256, items / ((uint64_t)waves * 4)));
     };
 
+    // scatter pass 1 for coarse intervals [c0, c1) into tmp, at offsets relative to rb
+    auto pass1 = [&](uint32_t c0, uint32_t c1, uint64_t rb) {
+        kt.begin(&stats.k_scatter1);
+        if (seq && d_keep)
+            k_scatter1<S, I, true, true><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_iter, cid, pc,
+                                                               coff, hblk, tmp, chunk, d_keep, c0, c1, rb);
+        else if (seq)
+            k_scatter1<S, I, true, false><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_iter, cid, pc,
+                                                                coff, hblk, tmp, chunk, nullptr, c0, c1, rb);
+        else
+            k_scatter1<S, I, false, false><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, cid, pc,
+                                                                 coff, hblk, tmp, chunk, nullptr, c0, c1, rb);
+        LAUNCHCHK();
+        kt.end();
+    };
+    if (all_records) pass1(0, pc, 0);
+
     for (size_t bi = 0; bi + 1 < cuts.size(); ++bi) {
         const uint32_t c_lo = cuts[bi], c_hi = cuts[bi + 1];
         const uint64_t rbase = hcoff[c_lo], nb = hcoff[c_hi] - rbase;
@@ -714,18 +761,18 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         if (nb == 0) continue;
 
         // ---- two-level scatter ----
-        kt.begin(&stats.k_scatter1);
-        if (seq)
-            k_scatter1<S, I, true><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_iter, cid, pc, coff,
-                                                         hblk, tmp, chunk, d_keep, c_lo, c_hi, rbase);
-        else
-            k_scatter1<S, I, false><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, cid, pc, coff,
-                                                          hblk, tmp, chunk, nullptr, c_lo, c_hi, rbase);
-        LAUNCHCHK();
-        kt.end();
+        if (!all_records) pass1(c_lo, c_hi, rbase);
         kt.begin(&stats.k_scatter2);
-        k_scatter2<I><<<c_hi - c_lo, 256, SP.stride * 28, st>>>(tmp, fid, coff, SP, (uint32_t)p, off, keys, out, c_lo,
-                                                                c_hi, rbase);
+        // 1,024 threads when LDS allows one block per compute unit (wide
+        // intervals) or the batch has too few intervals to fill the device
+        const bool wide = SP.stride * 28 > 48 * 1024 || (c_hi - c_lo) < 2 * W.n_cu;
+        KRec<I> *trec = all_records ? tmp + rbase : tmp;
+        if (wide)
+            k_scatter2<I, 1024><<<c_hi - c_lo, 1024, SP.stride * 28, st>>>(trec, fid, coff, SP, (uint32_t)p, off,
+                                                                          keys, out, c_lo, c_hi, rbase);
+        else
+            k_scatter2<I, 256><<<c_hi - c_lo, 256, SP.stride * 28, st>>>(trec, fid, coff, SP, (uint32_t)p, off,
+                                                                        keys, out, c_lo, c_hi, rbase);
         LAUNCHCHK();
         kt.end();
         tm.mark(PH_SCATTER);

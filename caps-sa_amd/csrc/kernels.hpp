@@ -462,7 +462,7 @@ __device__ __forceinline__ void block_keys_seq(const S *__restrict__ t, uint64_t
     __syncthreads();
 }
 
-template <typename S, typename I, int RK>
+template <typename S, typename I, int RK, bool KEEP>
 __global__ __launch_bounds__(256) void k_classify_seq(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, KeyGeom g,
                                                       DevLimits L, uint64_t max_ctx, Splitters SP, RouterDev R,
                                                       uint16_t *__restrict__ cid, uint32_t *__restrict__ hblk,
@@ -509,7 +509,7 @@ __global__ __launch_bounds__(256) void k_classify_seq(const S *__restrict__ t, u
         // kept positions (bit q of the bitmap; all when keep is null): the
         // thread's 16 positions start at a multiple of 16, inside one word
         uint32_t kb = 0xFFFFu;
-        if (keep && first < cnt) kb = (uint32_t)(keep[(base + first) >> 6] >> ((base + first) & 63)) & 0xFFFFu;
+        if (KEEP && first < cnt) kb = (uint32_t)(keep[(base + first) >> 6] >> ((base + first) & 63)) & 0xFFFFu;
 #pragma unroll
         for (uint32_t j = 0; j < kSeqItems; ++j) valid[j] = first + j < cnt && ((kb >> j) & 1u);
         if (RK == 0) coarse_bsearch<kSeqItems>(SP, coarse, key, b);
@@ -609,7 +609,7 @@ struct KRec {
     I idx;
 };
 
-template <typename S, typename I, bool SEQ>
+template <typename S, typename I, bool SEQ, bool KEEP>
 __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, KeyGeom g,
                                                   DevLimits L, uint64_t max_ctx, const I *__restrict__ pos,
                                                   uint64_t n_pos, const uint16_t *__restrict__ cid, uint32_t pc,
@@ -638,22 +638,23 @@ __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint6
             uint32_t cnt = (uint32_t)(c1 - base < kSeqChunk ? c1 - base : kSeqChunk);
             block_keys_seq<S>(t, n, cm, g, L, max_ctx, base, cnt, win, skey);
             constexpr int U = kSeqChunk / 256;
+            constexpr uint32_t kSkip = 0xFFFFFFFFu;  // not a record of this batch
             uint32_t cc[U];
             unsigned long long slot[U];
-            bool take[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                uint32_t i = u * 256 + threadIdx.x;
-                take[u] = i < cnt && (!keep || ((keep[(base + i) >> 6] >> ((base + i) & 63)) & 1ull));
-                cc[u] = take[u] ? cid[base + i] : 0u;
-                take[u] = take[u] && cc[u] >= c_lo && cc[u] < c_hi;
+                const uint32_t i = u * 256 + threadIdx.x;
+                bool tk = i < cnt;
+                if (KEEP) tk = tk && ((keep[(base + i) >> 6] >> ((base + i) & 63)) & 1ull);
+                const uint32_t c = tk ? (uint32_t)cid[base + i] : kSkip;
+                cc[u] = c - c_lo < c_hi - c_lo ? c : kSkip;
             }
 #pragma unroll
-            for (int u = 0; u < U; ++u) slot[u] = take[u] ? cbase[cc[u]] + atomicAdd(&lcnt[cc[u]], 1u) : 0ull;
+            for (int u = 0; u < U; ++u) slot[u] = cc[u] != kSkip ? cbase[cc[u]] + atomicAdd(&lcnt[cc[u]], 1u) : 0ull;
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                uint32_t i = u * 256 + threadIdx.x;
-                if (take[u]) tmp[slot[u]] = KRec<I>{skey[i], (I)(base + i)};
+                const uint32_t i = u * 256 + threadIdx.x;
+                if (cc[u] != kSkip) tmp[slot[u]] = KRec<I>{skey[i], (I)(base + i)};
             }
             __syncthreads();
         }
@@ -670,9 +671,9 @@ __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint6
     }
 }
 
-// Exclusive scan of cnt[0..nb) into out[0..nb) (+ base) across a 256-thread
-// block; each thread scans a contiguous slice.  Returns the total in every
-// thread.  part: 256 u64 of LDS.
+// Exclusive scan of cnt[0..nb) into out[0..nb) (+ base) across the block;
+// each thread scans a contiguous slice.  Returns the total in every thread.
+// part: blockDim.x u64 of LDS.
 __device__ __forceinline__ unsigned long long block_scan_slices(const uint32_t *cnt, uint32_t nb,
                                                                 unsigned long long base,
                                                                 unsigned long long *out,
@@ -683,7 +684,7 @@ __device__ __forceinline__ unsigned long long block_scan_slices(const uint32_t *
     for (uint32_t j = j0; j < j1; ++j) sum += cnt[j];
     part[threadIdx.x] = sum;
     __syncthreads();
-    for (uint32_t o = 1; o < 256; o <<= 1) {
+    for (uint32_t o = 1; o < blockDim.x; o <<= 1) {
         unsigned long long v = threadIdx.x >= o ? part[threadIdx.x - o] : 0ull;
         __syncthreads();
         part[threadIdx.x] += v;
@@ -694,22 +695,24 @@ __device__ __forceinline__ unsigned long long block_scan_slices(const uint32_t *
         out[j] = run;
         run += cnt[j];
     }
-    unsigned long long total = part[255];
+    unsigned long long total = part[blockDim.x - 1];
     __syncthreads();
     return total;
 }
 
 constexpr uint32_t kScatterTile = 2048;  // records staged per placing step of pass 2
 
-template <typename I>
-__global__ __launch_bounds__(256) void k_scatter2(const KRec<I> *__restrict__ tmp, uint16_t *__restrict__ fid,
+// NT threads per block: 256, or 1,024 when the LDS of a wide interval (or a
+// batch of few intervals) leaves one block per compute unit.
+template <typename I, int NT>
+__global__ __launch_bounds__(NT) void k_scatter2(const KRec<I> *__restrict__ tmp, uint16_t *__restrict__ fid,
                                                   const uint64_t *__restrict__ coff, Splitters SP, uint32_t p,
                                                   uint64_t *__restrict__ off, uint64_t *__restrict__ keys,
                                                   I *__restrict__ idx, uint32_t c_lo, uint32_t c_hi, uint64_t rbase) {
     // LDS sized by the launch for SP.stride buckets: inner splitters, global
     // cursors, tile offsets, counts
     extern __shared__ unsigned long long scatter2_lds[];
-    __shared__ unsigned long long part[256];
+    __shared__ unsigned long long part[NT];
     __shared__ uint64_t tkey[kScatterTile];
     __shared__ I tidx[kScatterTile];
     __shared__ uint16_t tf[kScatterTile];
@@ -770,11 +773,11 @@ __global__ __launch_bounds__(256) void k_scatter2(const KRec<I> *__restrict__ tm
     // write each bucket's run of the tile with consecutive threads
     for (uint64_t t0 = s; t0 < e; t0 += kScatterTile) {
         const uint32_t tl = (uint32_t)(e - t0 < kScatterTile ? e - t0 : kScatterTile);
-        constexpr int U = kScatterTile / 256;
+        constexpr int U = kScatterTile / NT;
         uint32_t f[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            uint32_t q = u * 256 + threadIdx.x;
+            uint32_t q = u * NT + threadIdx.x;
             f[u] = q < tl ? fid[t0 + q] : 0xFFFFFFFFu;
             if (q < tl) atomicAdd(&cnt[f[u]], 1u);
         }
@@ -784,7 +787,7 @@ __global__ __launch_bounds__(256) void k_scatter2(const KRec<I> *__restrict__ tm
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            uint32_t q = u * 256 + threadIdx.x;
+            uint32_t q = u * NT + threadIdx.x;
             if (q < tl) {
                 const KRec<I> r = tmp[t0 + q];
                 const uint32_t slot = (uint32_t)lbase[f[u]] + atomicAdd(&cnt[f[u]], 1u);
@@ -794,7 +797,7 @@ __global__ __launch_bounds__(256) void k_scatter2(const KRec<I> *__restrict__ tm
             }
         }
         __syncthreads();
-        for (uint32_t q = threadIdx.x; q < tl; q += 256) {
+        for (uint32_t q = threadIdx.x; q < tl; q += NT) {
             const uint32_t fb = tf[q];
             const uint64_t dst = gcur[fb] + (q - lbase[fb]);
             keys[dst] = tkey[q];

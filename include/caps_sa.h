@@ -100,6 +100,10 @@ typedef struct caps_opts {
  * bits (3 bits, 21 symbols): fewer symbols per key, more ties to refine.
  * The output never changes. */
 #define CAPS_FLAG_POW2_SEED 32u
+/* Bucket batches: run scatter pass 1 once per batch (records of that batch
+ * only) even when the records of all batches fit in HBM at once.  The output
+ * never changes. */
+#define CAPS_FLAG_SCATTER_PER_BATCH 64u
 
 /* Fills opts with the reference defaults (max_context unbounded, p auto). */
 void caps_opts_default(caps_opts_t *opts);

@@ -37,12 +37,14 @@ def test_batched_builds_match_one_batch(cuda):
         one, one_lcp, b1 = _sa_lcp(d_text, n, w)
         assert b1 == 1
         assert np.array_equal(one, want) and np.array_equal(one_lcp, want_lcp)
-        for rec in (n // 3, 20000, 4096):
-            sa, lcp, nb = _sa_lcp(d_text, n, w, batch_records=rec)
-            print(f"case n={n} w={w} batch_records={rec} batches={nb}", flush=True)
-            assert nb > 1
-            assert np.array_equal(sa, want), rec
-            assert np.array_equal(lcp, want_lcp), rec
+        # records of all batches written by one pass 1, or pass 1 per batch
+        for flags in (0, caps_sa.FLAG_SCATTER_PER_BATCH):
+            for rec in (n // 3, 20000, 4096):
+                print(f"case n={n} w={w} batch_records={rec} flags={flags}", flush=True)
+                sa, lcp, nb = _sa_lcp(d_text, n, w, batch_records=rec, flags=flags)
+                assert nb > 1
+                assert np.array_equal(sa, want), (rec, flags)
+                assert np.array_equal(lcp, want_lcp), (rec, flags)
 
 
 def test_batched_segmented_positions(cuda):
@@ -59,11 +61,11 @@ def test_batched_segmented_positions(cuda):
         want = ctx.build_in_memory(pos, threads=4)
         if order == 0:
             want = ctx.canonicalize(want, caps_sa.MAX_CONTEXT)
-        for rec in (0, 30000):
+        for rec, flags in ((0, 0), (30000, 0), (30000, caps_sa.FLAG_SCATTER_PER_BATCH)):
             sa, _ = dev.build_device(d_text, n, 1, d_positions=d_pos, n_pos=len(pos), idx_bytes=8, lp=lp,
-                                     batch_records=rec)
+                                     batch_records=rec, flags=flags)
             got = sa.cpu().numpy().view(np.uint64)
-            assert np.array_equal(got, want), (order, rec)
+            assert np.array_equal(got, want), (order, rec, flags)
 
 
 def test_filter_entry_matches_positions_entry(cuda):
@@ -109,11 +111,11 @@ def test_device_filtered_matches_positions(cuda):
     d_pos = torch.from_numpy(pos.view(np.int64)).to("cuda:0")
     want, want_lcp = dev.build_device(d_text, n, 1, d_positions=d_pos, n_pos=len(pos), idx_bytes=8, lp=lp,
                                       want_lcp=True)
-    for rec in (0, 20000):
+    for rec, flags in ((0, 0), (20000, 0), (20000, caps_sa.FLAG_SCATTER_PER_BATCH)):
         sa, lcp, kept = dev.build_device_filtered(d_text, n, 1, d_keep, len(pos), idx_bytes=8, lp=lp,
-                                                  want_lcp=True, batch_records=rec)
+                                                  want_lcp=True, batch_records=rec, flags=flags)
         assert kept == len(pos)
-        assert torch.equal(sa[:kept], want) and torch.equal(lcp[:kept], want_lcp), rec
+        assert torch.equal(sa[:kept], want) and torch.equal(lcp[:kept], want_lcp), (rec, flags)
     with pytest.raises(caps_sa.CapsError):
         dev.build_device_filtered(d_text, n, 1, d_keep, len(pos) - 1, idx_bytes=8, lp=lp)
 
