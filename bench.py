@@ -239,7 +239,7 @@ def run_genome(args, n: int, filtered: bool = False) -> dict:
     text, ends, pos = gen.genome(n, n_seg, seed=args.seed, positions=not filtered)
     lp = caps_sa.SegmentedText(n, ends)
     d_text = dev.text_to_device(text)
-    flags = caps_sa.FLAG_POW2_SEED if args.pow2_seed else 0
+    flags = (caps_sa.FLAG_POW2_SEED if args.pow2_seed else 0) | args.flags
     if filtered:
         keep = gen.keep_below(text, 4)
         n_pos = int(np.bitwise_count(keep).sum())
@@ -317,6 +317,7 @@ def main() -> int:
     ap.add_argument("--texts", type=int, default=4096, help="batch workload: number of texts")
     ap.add_argument("--batch-records", type=int, default=0,
                     help="genome workloads: records per bucket batch (0: one batch while it fits in HBM)")
+    ap.add_argument("--flags", type=int, default=0, help="genome workloads: extra CAPS_FLAG_* bits")
     ap.add_argument("--pow2-seed", action="store_true",
                     help="genome workload: seed keys with the packed text's power-of-two field width")
     ap.add_argument("--autotune", action="store_true",

@@ -945,8 +945,14 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
                 LAUNCHCHK();
             }
             I *is = out, *ls = lbuf, *id = i2, *ld = l2;
-            // merge-path partition width: outputs per thread (a tuning knob)
-            const int mw = (o.flags & CAPS_FLAG_MERGE_W16) ? 16 : (o.flags & CAPS_FLAG_MERGE_W64) ? 64 : 32;
+            // merge-path partition width: outputs per thread (a tuning knob).
+            // Unset, groups of 128K+ members take 64: each partition search
+            // compares whole suffixes, and members of huge groups share long
+            // stretches, so fewer, longer partitions pay there.
+            const int mw = (o.flags & CAPS_FLAG_MERGE_W16)   ? 16
+                           : (o.flags & CAPS_FLAG_MERGE_W64) ? 64
+                           : maxsize >= (1ull << 17)         ? 64
+                                                             : 32;
             auto merge_level = [&](auto wtag, dim3 grid, const MergeJob *jobs) {
                 constexpr int MW = decltype(wtag)::value;
                 k_lcp_merge<S, I, MW><<<grid, 128, 0, st>>>(d_text, L, max_ctx, depth0, jobs, is, ls, id, ld);

@@ -104,6 +104,7 @@ struct ExtMemOpts {
     uint64_t max_context = std::numeric_limits<uint64_t>::max();
     uint64_t subproblem_count = 0;
     uint32_t router = CAPS_ROUTER_BSEARCH;
+    uint64_t batch_records = 0;  // GPU knob: records per bucket batch (0: auto)
     ExtMemOpts &with_max_context(uint64_t m) {
         max_context = m;
         return *this;
@@ -114,6 +115,10 @@ struct ExtMemOpts {
     }
     ExtMemOpts &with_router(uint32_t r) {
         router = r;
+        return *this;
+    }
+    ExtMemOpts &with_batch_records(uint64_t b) {
+        batch_records = b;
         return *this;
     }
 };
@@ -153,6 +158,7 @@ inline caps_opts_t c_opts(const ExtMemOpts &e) {
     caps_opts_t o = c_opts(e.max_context);
     o.subproblem_count = e.subproblem_count;
     o.router = e.router;
+    o.batch_records = e.batch_records;
     return o;
 }
 inline void check(int rc) {
@@ -305,6 +311,43 @@ void build_ext_mem_for_positions_with(const std::vector<S> &text, const std::vec
             return std::nullopt;
         });
     if (r) throw *r;
+}
+
+// build_ext_mem_for_filter(_with) (ext_mem.rs:472-547): the keep predicate is
+// walked once into a bitmap of kept positions, as the crate's FilteredSource
+// does (ext_mem.rs:940-990), and never called again; no position list is built.
+template <class E, class S, class L, class Pred>
+std::optional<BuildError<E>> try_build_ext_mem_for_filter_with(const std::vector<S> &text, Pred keep, const L &lp,
+                                                               const ExtMemOpts &opts,
+                                                               std::function<std::optional<E>(uint64_t)> emit) {
+    const uint64_t n = text.size();
+    std::vector<uint64_t> bits((n + 63) / 64, 0);
+    for (uint64_t q = 0; q < n; ++q)
+        if (keep(q)) bits[q >> 6] |= 1ull << (q & 63);
+    caps_limits_t lim = lp.c_limits();
+    caps_opts_t o = detail::c_opts(opts);
+    detail::EmitCtx<E> ctx{&emit, std::nullopt};
+    int rc = caps_build_ext_mem_for_filter(text.data(), n, detail::sym_bytes<S>(), bits.data(), &lim, &o,
+                                           &detail::trampoline<E>, &ctx);
+    if (rc == CAPS_OK) return std::nullopt;
+    if (rc == CAPS_E_EMIT) return BuildError<E>(BuildError<E>::Kind::Emit, "emit failed", std::move(ctx.err));
+    if (rc == CAPS_E_INVALID) throw std::invalid_argument(caps_last_error());
+    return BuildError<E>(BuildError<E>::Kind::Io, caps_last_error());
+}
+template <class S, class L, class Pred>
+void build_ext_mem_for_filter_with(const std::vector<S> &text, Pred keep, const L &lp, const ExtMemOpts &opts,
+                                   std::function<void(uint64_t)> emit) {
+    auto r = try_build_ext_mem_for_filter_with<std::string>(text, keep, lp, opts,
+                                                            [&](uint64_t p) -> std::optional<std::string> {
+                                                                emit(p);
+                                                                return std::nullopt;
+                                                            });
+    if (r) throw *r;
+}
+template <class S, class Pred>
+void build_ext_mem_for_filter(const std::vector<S> &text, Pred keep, const ExtMemOpts &opts,
+                              std::function<void(uint64_t)> emit) {
+    build_ext_mem_for_filter_with(text, keep, PlainText(text.size()), opts, emit);
 }
 
 // ---- batched builds (no counterpart in the crate) ------------------------------

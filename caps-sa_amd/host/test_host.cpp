@@ -171,6 +171,19 @@ int main() {
         std::sort(s.begin(), s.end());
         CHECK(ok && s == pos);
     }
+    // filter builder: the kept positions of a predicate, in bucket batches, equal
+    // to the positions builder over the same positions (ext_mem.rs:472-547)
+    {
+        std::vector<uint8_t> t(40000);
+        for (auto &c : t) c = (uint8_t)(lcg() % 6);
+        auto keep = [&](uint64_t q) { return t[q] < 4; };
+        std::vector<uint64_t> pos, a, b;
+        for (uint64_t q = 0; q < t.size(); ++q)
+            if (keep(q)) pos.push_back(q);
+        build_ext_mem_for_positions_with(t, pos, PlainText(t.size()), ExtMemOpts{}, [&](uint64_t p) { a.push_back(p); });
+        build_ext_mem_for_filter(t, keep, ExtMemOpts{}.with_batch_records(5000), [&](uint64_t p) { b.push_back(p); });
+        CHECK(a.size() == pos.size() && a == b);
+    }
     // batched builds: every text's own suffix array
     {
         std::vector<std::vector<uint8_t>> texts;

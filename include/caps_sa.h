@@ -92,7 +92,8 @@ typedef struct caps_opts {
  * merge sort.  The output never changes. */
 #define CAPS_FLAG_TILE_RADIX 4u
 /* Merge-path partition width of the giant-group LCP merges: 16 or 64 outputs
- * per thread instead of 32.  The output never changes. */
+ * per thread instead of the default (32, or 64 when the largest giant group
+ * has 131,072+ members).  The output never changes. */
 #define CAPS_FLAG_MERGE_W16 8u
 #define CAPS_FLAG_MERGE_W64 16u
 /* Dense seed keys with power-of-two fields (the packed text's width, e.g. 4
