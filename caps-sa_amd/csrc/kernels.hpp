@@ -448,20 +448,6 @@ __device__ __forceinline__ void thread_keys_seq(const S *__restrict__ t, uint64_
     for (uint32_t i = 0; i < kSeqItems; ++i) out[i] = first + i < cnt ? with_id(g, keys[i], segs[i]) : 0ull;
 }
 
-// The same keys staged in LDS (skey[i] = key of position base + i).
-template <typename S>
-__device__ __forceinline__ void block_keys_seq(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, const KeyGeom &g,
-                                               const DevLimits &L, uint64_t max_ctx, uint64_t base, uint32_t cnt,
-                                               S *win, uint64_t *skey) {
-    uint64_t keys[kSeqItems];
-    thread_keys_seq<S>(t, n, cm, g, L, max_ctx, base, cnt, win, keys);
-    const uint32_t first = threadIdx.x * kSeqItems;
-#pragma unroll
-    for (uint32_t i = 0; i < kSeqItems; ++i)
-        if (first + i < cnt) skey[first + i] = keys[i];
-    __syncthreads();
-}
-
 template <typename S, typename I, int RK, bool KEEP>
 __global__ __launch_bounds__(256) void k_classify_seq(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, KeyGeom g,
                                                       DevLimits L, uint64_t max_ctx, Splitters SP, RouterDev R,
@@ -618,7 +604,6 @@ __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint6
                                                   uint64_t chunk, const uint64_t *__restrict__ keep, uint32_t c_lo,
                                                   uint32_t c_hi, uint64_t rbase) {
     __shared__ S win[SEQ ? kSeqChunk + 72 : 1];
-    __shared__ uint64_t skey[SEQ ? kSeqChunk : 1];
     __shared__ unsigned long long cbase[kCoarseBucketsMax];
     __shared__ uint32_t lcnt[kCoarseBucketsMax];
     // chunk: a multiple of kSeqChunk, so key windows never straddle two blocks
@@ -634,29 +619,36 @@ __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint6
     }
     __syncthreads();
     if (SEQ) {
+        // the classify pass's layout: a thread owns 16 consecutive positions,
+        // keeps their keys in registers and reads their 16 interval ids with
+        // two 16-byte loads (32-byte aligned: base and first are multiples of 16)
+        constexpr uint32_t kSkip = 0xFFFFFFFFu;  // not a record of this batch
         for (uint64_t base = c0; base < c1; base += kSeqChunk) {
-            uint32_t cnt = (uint32_t)(c1 - base < kSeqChunk ? c1 - base : kSeqChunk);
-            block_keys_seq<S>(t, n, cm, g, L, max_ctx, base, cnt, win, skey);
-            constexpr int U = kSeqChunk / 256;
-            constexpr uint32_t kSkip = 0xFFFFFFFFu;  // not a record of this batch
-            uint32_t cc[U];
-            unsigned long long slot[U];
+            const uint32_t cnt = (uint32_t)(c1 - base < kSeqChunk ? c1 - base : kSeqChunk);
+            uint64_t key[kSeqItems];
+            thread_keys_seq<S>(t, n, cm, g, L, max_ctx, base, cnt, win, key);
+            const uint32_t first = threadIdx.x * kSeqItems;
+            uint32_t cc[kSeqItems];
+            if (first + kSeqItems <= cnt) {
+                const uint4 *src = (const uint4 *)(cid + base + first);
+                const uint4 v0 = src[0], v1 = src[1];
+                const uint32_t w[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint32_t i = u * 256 + threadIdx.x;
-                bool tk = i < cnt;
-                if (KEEP) tk = tk && ((keep[(base + i) >> 6] >> ((base + i) & 63)) & 1ull);
-                const uint32_t c = tk ? (uint32_t)cid[base + i] : kSkip;
-                cc[u] = c - c_lo < c_hi - c_lo ? c : kSkip;
+                for (uint32_t j = 0; j < kSeqItems; ++j) cc[j] = (w[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
+            } else {
+#pragma unroll
+                for (uint32_t j = 0; j < kSeqItems; ++j) cc[j] = first + j < cnt ? (uint32_t)cid[base + first + j] : kSkip;
             }
+            uint32_t kb = 0xFFFFu;
+            if (KEEP && first < cnt) kb = (uint32_t)(keep[(base + first) >> 6] >> ((base + first) & 63)) & 0xFFFFu;
 #pragma unroll
-            for (int u = 0; u < U; ++u) slot[u] = cc[u] != kSkip ? cbase[cc[u]] + atomicAdd(&lcnt[cc[u]], 1u) : 0ull;
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint32_t i = u * 256 + threadIdx.x;
-                if (cc[u] != kSkip) tmp[slot[u]] = KRec<I>{skey[i], (I)(base + i)};
+            for (uint32_t j = 0; j < kSeqItems; ++j) {
+                const uint32_t c = cc[j];
+                if (((kb >> j) & 1u) && c != kSkip && c - c_lo < c_hi - c_lo) {
+                    const unsigned long long slot = cbase[c] + atomicAdd(&lcnt[c], 1u);
+                    tmp[slot] = KRec<I>{key[j], (I)(base + first + j)};
+                }
             }
-            __syncthreads();
         }
     } else {
         for (uint64_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
