@@ -198,6 +198,14 @@ __device__ __forceinline__ bool rec_less(uint64_t ka, uint32_t la, uint64_t kb, 
 // holds records t*ITEMS .. t*ITEMS+ITEMS-1), through the padded LDS arrays
 // sk/sl.  PACKED: the records are unique u64 values (key bits and tile index
 // packed together; li unused); otherwise (key, li) pairs compared as above.
+// Orders this wavefront's LDS accesses: every earlier LDS write of the
+// wavefront is complete and visible to its lanes before any later LDS access.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int B, int ITEMS, bool PACKED>
 __device__ __forceinline__ void block_merge_sort(uint64_t (&k)[ITEMS], uint32_t (&li)[ITEMS], uint64_t *sk,
                                                  uint16_t *sl, uint32_t len) {
@@ -230,16 +238,23 @@ __device__ __forceinline__ void block_merge_sort(uint64_t (&k)[ITEMS], uint32_t 
             }
         }
     }
-    // merge levels: runs of w records -> runs of 2w
+    // merge levels: runs of w records -> runs of 2w.  While a merged run
+    // (2w records, 2w / ITEMS threads) lies inside one wavefront, only that
+    // wavefront reads the LDS it wrote: a wavefront fence replaces the
+    // workgroup barrier, and the wavefronts of the block run independently.
+    constexpr uint32_t kWaveSpan = 64 * ITEMS;
 #pragma unroll 1
     for (uint32_t w = ITEMS; w < T; w *= 2) {
-        __syncthreads();
+        const bool in_wave = 2 * w <= kWaveSpan;
+        if (in_wave) wave_lds_sync();
+        else __syncthreads();
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
             sk[px(tid * ITEMS + j)] = k[j];
             if (!PACKED) sl[px(tid * ITEMS + j)] = (uint16_t)li[j];
         }
-        __syncthreads();
+        if (in_wave) wave_lds_sync();
+        else __syncthreads();
         const uint32_t per = (2 * w) / ITEMS;  // threads per merged run
         const uint32_t a0 = (tid / per) * 2 * w, b0 = a0 + w;
         const uint32_t diag = (tid % per) * ITEMS;
