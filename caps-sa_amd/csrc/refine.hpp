@@ -346,8 +346,8 @@ __global__ __launch_bounds__(256) void k_pairs_packed(const S *__restrict__ t, c
 }
 
 // Fast-forward for a slot of W lanes whose members share a long stretch
-// (a repeat copy, a run of one symbol): every open member loads the next J
-// whole code words (2^lgf codes each) at depth D that lie inside its window
+// (a repeat copy, a run of one symbol): every open member loads the next lim
+// (<= J) whole code words (2^lgf codes each) at depth D that lie inside its window
 // and compares them with those of its subgroup's first member (lane `sub`
 // of the slot).  Returns the number of symbols, uniform across the slot, on
 // which every open member agrees with its subgroup's first member, all
@@ -358,7 +358,8 @@ __global__ __launch_bounds__(256) void k_pairs_packed(const S *__restrict__ t, c
 // slot at once, with at least one open lane.
 template <int W, int J>
 __device__ __forceinline__ uint64_t ff_skip(const uint64_t *__restrict__ pk, uint32_t lgf, uint32_t bits,
-                                            uint64_t pos, uint64_t D, uint64_t cap, bool open, uint32_t sub) {
+                                            uint64_t pos, uint64_t D, uint64_t cap, bool open, uint32_t sub,
+                                            uint32_t lim) {
     const int lane = threadIdx.x & 63;
     uint32_t jw = 0;
     uint64_t pw[J];
@@ -367,7 +368,7 @@ __device__ __forceinline__ uint64_t ff_skip(const uint64_t *__restrict__ pk, uin
     if (open) {
         const uint64_t rem = cap > D ? cap - D : 0;
         const uint64_t fw = rem >> lgf;
-        jw = fw < (uint64_t)J ? (uint32_t)fw : (uint32_t)J;
+        jw = fw < (uint64_t)lim ? (uint32_t)fw : lim;
         const uint64_t q = pos + D;
         const uint64_t w0 = q >> lgf;
         const uint32_t rb = (uint32_t)(q & ((1ull << lgf) - 1)) * bits;
@@ -396,10 +397,33 @@ __device__ __forceinline__ uint64_t ff_skip(const uint64_t *__restrict__ pk, uin
     return (uint64_t)f << lgf;
 }
 
-// Rounds a group of the small classes runs before its slots start probing
-// for shared stretches with ff_skip, and the code words one probe covers.
-constexpr uint32_t kFFAfter = 2;
-constexpr int kFFWords = 8;
+// Probing schedule of the small classes: after the first round a slot
+// probes kFFFirst words; while a probe matches in full (every open member
+// agrees with its subgroup's first member on every word) it probes again,
+// kFFWords words at a time, up to kFFProbes probes per round.  Repeat copies
+// with scattered differences part within the first short probe; runs of one
+// symbol are crossed kFFWords words per probe without the round's sort.
+constexpr uint32_t kFFAfter = 1;
+constexpr uint32_t kFFFirst = 4;
+constexpr int kFFWords = 4;
+constexpr int kFFProbes = 16;
+
+template <int W>
+__device__ __forceinline__ uint64_t ff_advance(const uint64_t *__restrict__ pk, uint32_t lgf, uint32_t bits,
+                                               uint64_t pos, uint64_t D, uint64_t cap, bool open, uint32_t sub) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t smask = (W == 64) ? ~0ull : (((1ull << W) - 1) << (lane & ~(W - 1)));
+    if ((__ballot(open) & smask) == 0) return 0;  // a settled slot
+    uint64_t adv = 0;
+    uint32_t lim = kFFFirst;
+    for (int it = 0; it < kFFProbes; ++it) {  // uniform across the slot
+        const uint64_t sk = ff_skip<W, kFFWords>(pk, lgf, bits, pos, D + adv, cap, open, sub, lim);
+        adv += sk;
+        if (sk != ((uint64_t)lim << lgf)) break;
+        lim = kFFWords;
+    }
+    return adv;
+}
 
 // ---------------------------------------------------------------------------
 // Groups of 3..64 members, one W-lane slot per group (grid launch).  The
@@ -431,7 +455,7 @@ __global__ __launch_bounds__(256) void k_wgroups_grid(const S *__restrict__ t, C
     uint64_t D = depth0;
     for (uint32_t round = 0;; ++round) {
         if (cm.packed && round >= kFFAfter && __any(!settled))
-            D += ff_skip<W, kFFWords>(cm.packed, cm.lgf, g.bits, pos, D, cap, !settled, sub);
+            D += ff_advance<W>(cm.packed, cm.lgf, g.bits, pos, D, cap, !settled, sub);
         if (!settled) {
             chunk = key_at(t, cm, g, pos, D, cap);
             uint64_t rem = cap > D ? cap - D : 0;
@@ -620,7 +644,7 @@ __global__ __launch_bounds__(256) void k_wgroups_wq(const S *__restrict__ t, Cod
         if (!__any(busy || v1 || v2)) break;
         const bool act = busy && r < m;
         if (cm.packed && busy && rounds >= kFFAfter)
-            D += ff_skip<W, kFFWords>(cm.packed, cm.lgf, g.bits, pos, D, cap, !settled, sub);
+            D += ff_advance<W>(cm.packed, cm.lgf, g.bits, pos, D, cap, !settled, sub);
         if (!settled) {
             chunk = key_at(t, cm, g, pos, D, cap);
             uint64_t rem = cap > D ? cap - D : 0;
