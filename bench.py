@@ -67,8 +67,8 @@ def algorithmic_bytes_per_suffix(kernel: str, I: int) -> float:
 
 
 # rocprofv3 kernel names of those kernels on the default workload.
-ROCPROF_NAMES = {"k_classify": "k_classify_seq<u8, u32, 0>", "k_scatter1": "k_scatter1<u8, u32, true>",
-                 "k_scatter2": "k_scatter2<u32>"}
+ROCPROF_NAMES = {"k_classify": "k_classify_seq<u8, u32, 0, false>", "k_scatter1": "k_scatter1<u8, u32, true, false>",
+                 "k_scatter2": "k_scatter2<u32, 256>"}
 
 
 def measured_traffic(workload: str, kernel: str):
