@@ -358,6 +358,51 @@ __device__ __forceinline__ int thread_cmp(const S *__restrict__ t, const DevLimi
     return boundary_then_pos(L, a, la, b, lb);
 }
 
+// LCP from the packed code text (2^lgf codes of 64 >> lgf bits per word):
+// a word of codes per side and step (two aligned loads and a funnel shift),
+// the first differing code from the leading zeros of their XOR.  Codes past
+// the window count as a difference at its end.  Every load stays inside the
+// packed buffer, which holds two words past the last code.
+template <typename CM>
+__device__ __forceinline__ uint64_t packed_lcp(const CM &cm, uint64_t p, uint64_t q, uint64_t start, uint64_t cap) {
+    const uint32_t lgf = cm.lgf, bits = 64u >> lgf;
+    const uint64_t F = 1ull << lgf;
+    auto word = [&](uint64_t x) {
+        const uint64_t w = x >> lgf;
+        const uint32_t rb = (uint32_t)(x & (F - 1)) * bits;
+        uint64_t v = cm.packed[w];
+        if (rb) v = (v << rb) | (cm.packed[w + 1] >> (64 - rb));
+        return v;
+    };
+    uint64_t o = start;
+    while (o < cap) {
+        uint64_t x = word(p + o) ^ word(q + o);
+        const uint64_t rem = cap - o;
+        if (rem < F) x |= ~0ull >> (rem * bits);  // codes past the window differ
+        if (x) {
+            const uint64_t r = o + (uint64_t)__builtin_clzll(x) / bits;
+            return r < cap ? r : cap;
+        }
+        o += F;
+    }
+    return cap;
+}
+
+// thread_cmp with the LCP scan over the packed codes (same result).
+template <typename S>
+__device__ __forceinline__ int thread_cmp_packed(const S *__restrict__ t, const CodeMap<S> &cm, const DevLimits &L,
+                                                 uint64_t max_ctx, uint64_t a, uint64_t b, uint64_t known,
+                                                 uint64_t *lcp) {
+    uint64_t la = lim_at(L, a), lb = lim_at(L, b);
+    uint64_t ca = la < max_ctx ? la : max_ctx, cb = lb < max_ctx ? lb : max_ctx;
+    uint64_t cap = ca < cb ? ca : cb;
+    uint64_t s = known < cap ? known : cap;
+    uint64_t c = packed_lcp(cm, a, b, s, cap);
+    *lcp = c;
+    if (c < cap) return t[a + c] < t[b + c] ? -1 : 1;
+    return boundary_then_pos(L, a, la, b, lb);
+}
+
 template <typename S>
 __device__ __forceinline__ int wave_cmp(const S *__restrict__ t, const DevLimits &L, uint64_t max_ctx, uint64_t a,
                                         uint64_t b, uint64_t known, uint64_t *lcp) {

@@ -955,7 +955,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
                                                              : 32;
             auto merge_level = [&](auto wtag, dim3 grid, const MergeJob *jobs) {
                 constexpr int MW = decltype(wtag)::value;
-                k_lcp_merge<S, I, MW><<<grid, 128, 0, st>>>(d_text, L, max_ctx, depth0, jobs, is, ls, id, ld);
+                k_lcp_merge<S, I, MW><<<grid, 128, 0, st>>>(d_text, cm, L, max_ctx, depth0, jobs, is, ls, id, ld);
             };
             for (int lv = 0; lv < levels; ++lv) {
                 const uint64_t run = (uint64_t)kMidMax << lv;
@@ -1511,7 +1511,7 @@ static void merge_jobs_host(const void *text, uint64_t n, const caps_limits_t *l
         HIPCHK(hipMemcpyAsync(dj, W.h_jobs.data(), W.h_jobs.size() * sizeof(MergeJob), hipMemcpyHostToDevice, st));
         constexpr int MW = 32;
         dim3 gr((unsigned)ceil_div(ceil_div(maxlen, MW), 128), (unsigned)W.h_jobs.size());
-        k_lcp_merge<S, uint64_t, MW><<<gr, 128, 0, st>>>(dt, L, max_ctx, 0, dj, s_sa, s_l, d_sa, d_l);
+        k_lcp_merge<S, uint64_t, MW><<<gr, 128, 0, st>>>(dt, CodeMap<S>{}, L, max_ctx, 0, dj, s_sa, s_l, d_sa, d_l);
         LAUNCHCHK();
         HIPCHK(hipStreamSynchronize(st));
         std::swap(s_sa, d_sa);

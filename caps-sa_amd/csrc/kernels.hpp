@@ -24,10 +24,25 @@ __global__ __launch_bounds__(256) void k_alphabet(const S *__restrict__ t, uint6
     for (uint32_t w = threadIdx.x; w < WORDS; w += blockDim.x) bm[w] = 0;
     __syncthreads();
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        uint32_t c = t[i];
+    auto note = [&](uint32_t c) {
         uint32_t bit = 1u << (c & 31);
         if (!(bm[c >> 5] & bit)) atomicOr(&bm[c >> 5], bit);
+    };
+    if (sizeof(S) == 1 && ((uintptr_t)t & 15) == 0) {
+        // 16 bytes per load over the whole 16-byte words, then the tail
+        const uint64_t n16 = n >> 4;
+        const uint4 *t16 = (const uint4 *)t;
+        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) {
+            const uint4 v = t16[i];
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) note((w[q] >> (8 * b)) & 0xFFu);
+        }
+        for (uint64_t i = (n16 << 4) + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) note(t[i]);
+    } else {
+        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) note(t[i]);
     }
     __syncthreads();
     for (uint32_t w = threadIdx.x; w < WORDS; w += blockDim.x)
@@ -170,14 +185,29 @@ template <typename S>
 __global__ void k_pack_codes(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, uint32_t bits, uint32_t lgf,
                              uint64_t *__restrict__ out, uint64_t nwords) {
     const uint32_t F = 1u << lgf;
+    // u8 text at a 16-byte aligned address, 8+ codes a word: the word's F
+    // bytes come in 8-byte loads (aligned: j0 is a multiple of F)
+    const bool vec = sizeof(S) == 1 && F >= 8 && ((uintptr_t)t & 15) == 0;
     for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords;
          w += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t j0 = w << lgf;
         uint64_t v = 0;
-        for (uint32_t f = 0; f < F; ++f) {
-            const uint64_t j = j0 + f;
-            const uint32_t c = j < n ? cm(t[j]) : 0u;
-            v |= (uint64_t)c << (64 - (f + 1) * bits);
+        if (vec && j0 + F <= n) {
+            const uint64_t *src = (const uint64_t *)(t + j0);
+            for (uint32_t q = 0; q < F / 8; ++q) {
+                const uint64_t x = src[q];
+#pragma unroll
+                for (uint32_t b = 0; b < 8; ++b) {
+                    const uint32_t c = cm((S)((x >> (8 * b)) & 0xFFu));
+                    v |= (uint64_t)c << (64 - (q * 8 + b + 1) * bits);
+                }
+            }
+        } else {
+            for (uint32_t f = 0; f < F; ++f) {
+                const uint64_t j = j0 + f;
+                const uint32_t c = j < n ? cm(t[j]) : 0u;
+                v |= (uint64_t)c << (64 - (f + 1) * bits);
+            }
         }
         out[w] = v;
     }
@@ -1303,10 +1333,15 @@ struct MergeJob {
 };
 
 template <typename S, typename I, int W>
-__global__ __launch_bounds__(128) void k_lcp_merge(const S *__restrict__ t, DevLimits L, uint64_t max_ctx,
-                                                   uint64_t known, const MergeJob *__restrict__ jobs,
-                                                   const I *__restrict__ si, const I *__restrict__ sl,
-                                                   I *__restrict__ di, I *__restrict__ dl) {
+__global__ __launch_bounds__(128) void k_lcp_merge(const S *__restrict__ t, CodeMap<S> cm, DevLimits L,
+                                                   uint64_t max_ctx, uint64_t known,
+                                                   const MergeJob *__restrict__ jobs, const I *__restrict__ si,
+                                                   const I *__restrict__ sl, I *__restrict__ di,
+                                                   I *__restrict__ dl) {
+    // suffix compares scan the packed codes when the build has them
+    auto cmp = [&](uint64_t a, uint64_t b, uint64_t kn, uint64_t *l) {
+        return cm.packed ? thread_cmp_packed(t, cm, L, max_ctx, a, b, kn, l) : thread_cmp(t, L, max_ctx, a, b, kn, l);
+    };
     MergeJob jb = jobs[blockIdx.y];
     uint64_t rlen = jb.nx + jb.ny;
     uint64_t d = ((uint64_t)blockIdx.x * 128 + threadIdx.x) * W;
@@ -1329,7 +1364,7 @@ __global__ __launch_bounds__(128) void k_lcp_merge(const S *__restrict__ t, DevL
     while (lo < hi) {
         uint64_t mid = (lo + hi) >> 1;
         uint64_t l;
-        if (thread_cmp(t, L, max_ctx, (uint64_t)X[mid], (uint64_t)Y[d - mid - 1], known, &l) < 0) lo = mid + 1;
+        if (cmp((uint64_t)X[mid], (uint64_t)Y[d - mid - 1], known, &l) < 0) lo = mid + 1;
         else hi = mid;
     }
     uint64_t i = lo, j = d - lo;
@@ -1343,7 +1378,7 @@ __global__ __launch_bounds__(128) void k_lcp_merge(const S *__restrict__ t, DevL
         else if (j == 0) lastX = true;
         else {
             uint64_t l;
-            lastX = thread_cmp(t, L, max_ctx, (uint64_t)X[i - 1], (uint64_t)Y[j - 1], known, &l) > 0;
+            lastX = cmp((uint64_t)X[i - 1], (uint64_t)Y[j - 1], known, &l) > 0;
         }
         if (!lastX) {
             A = Y, Bs = X, LA = LY, LB = LX;
@@ -1351,7 +1386,7 @@ __global__ __launch_bounds__(128) void k_lcp_merge(const S *__restrict__ t, DevL
         }
         if (iB < nB) {
             uint64_t l;
-            thread_cmp(t, L, max_ctx, (uint64_t)A[iA - 1], (uint64_t)Bs[iB], known, &l);
+            cmp((uint64_t)A[iA - 1], (uint64_t)Bs[iB], known, &l);
             m = l;
         }
     }
@@ -1367,7 +1402,7 @@ __global__ __launch_bounds__(128) void k_lcp_merge(const S *__restrict__ t, DevL
                 outA = false, lo_out = m, new_m = la;
             } else {
                 uint64_t l;
-                int c = thread_cmp(t, L, max_ctx, (uint64_t)A[iA], (uint64_t)Bs[iB], m > known ? m : known, &l);
+                int c = cmp((uint64_t)A[iA], (uint64_t)Bs[iB], m > known ? m : known, &l);
                 outA = c < 0, lo_out = m, new_m = l;
             }
         } else if (hasA) {
