@@ -133,3 +133,50 @@ def test_emit_error_stops_a_batched_stream(cuda):
         caps_sa.try_build_in_memory_sample_sort(t, caps_sa.ExtMemOpts(batch_records=10000), emit)
     assert e.value.kind == "Emit" and isinstance(e.value.inner, ValueError)
     assert len(seen) == 5
+
+
+def test_filter_star_and_u16_positions_batches(cuda):
+    """The filter entry under STAR segment order, and a u16 text with a
+    position list sorted in bucket batches, both against the positions entry
+    and the oracle."""
+    import torch
+    from caps_sa import device as dev
+
+    text, ends, pos = gen.genome(120000, 60, median=2500.0, n_frac=0.058, n_mean=400.0, seed=70)
+    n = len(text)
+    lp = caps_sa.StarSegmented(n, ends)
+    for rec in (0, 9000):
+        a, b = [], []
+        opts = caps_sa.ExtMemOpts(batch_records=rec)
+        caps_sa.build_ext_mem_for_positions_with(text, pos, lp, opts, a.append)
+        caps_sa.build_ext_mem_for_filter_with(text, text < 4, lp, opts, b.append)
+        want = O.Ctx(text, ends, 1).build_in_memory(pos, threads=4)
+        assert a == b and np.array_equal(np.array(a, dtype=np.uint64), want), rec
+    t16 = gen.zipf_u16(60000, 2000, 1.1, seed=71)
+    keep = np.random.default_rng(72).random(len(t16)) < 0.6
+    p16 = np.flatnonzero(keep).astype(np.uint64)
+    d_text = dev.text_to_device(t16)
+    d_pos = torch.from_numpy(p16.view(np.int64)).to("cuda:0")
+    want = O.Ctx(t16).build_in_memory(p16, threads=4)
+    for rec in (0, 5000):
+        sa, _ = dev.build_device(d_text, len(t16), 2, d_positions=d_pos, n_pos=len(p16), idx_bytes=8,
+                                 batch_records=rec)
+        assert np.array_equal(sa.cpu().numpy().view(np.uint64), want), rec
+
+
+def test_filtered_builds_with_learned_routers(cuda):
+    """The learned routers sample kept positions from the bitmap (router
+    training and splitter sample); the suffix array does not change."""
+    import torch
+    from caps_sa import device as dev
+
+    text, ends, pos = gen.genome(400000, 150, median=2500.0, n_frac=0.058, n_mean=500.0, seed=73)
+    n = len(text)
+    lp = caps_sa.SegmentedText(n, ends)
+    d_text = dev.text_to_device(text)
+    d_keep = torch.from_numpy(caps_sa.keep_bitmap(text < 4, n).view(np.int64)).to("cuda:0")
+    base, _, kept = dev.build_device_filtered(d_text, n, 1, d_keep, len(pos), idx_bytes=8, lp=lp)
+    for router in (1, 2):
+        sa, _, k2 = dev.build_device_filtered(d_text, n, 1, d_keep, len(pos), idx_bytes=8, lp=lp, router=router,
+                                              batch_records=60000)
+        assert k2 == kept and torch.equal(sa[:k2], base[:kept]), router
