@@ -2,6 +2,8 @@
 
   libcaps_sa_hip.so  hipcc, gfx950: kernels + C ABI (csrc/engine.hip)
   libcaps_gen.so     gcc: seeded workload generators (gen/gen.c)
+  host/test_host     g++: test program of the C++ API mirror
+  host/caps_sa       g++: command-line builder (host/caps_sa_cli.cpp)
 """
 from __future__ import annotations
 
@@ -43,10 +45,20 @@ def build_host_test() -> str:
     return out
 
 
+def build_cli() -> str:
+    """The command-line builder (host/caps_sa_cli.cpp), linked against the
+    HIP library built above."""
+    out = os.path.join(HERE, "host", "caps_sa")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-o", out, os.path.join("host", "caps_sa_cli.cpp"), "-L" + HERE,
+          "-lcaps_sa_hip", "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/.."])
+    return out
+
+
 def build_all() -> None:
     build_gen()
     build_hip()
     build_host_test()
+    build_cli()
 
 
 if __name__ == "__main__":

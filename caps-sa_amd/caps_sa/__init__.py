@@ -631,6 +631,34 @@ def build_in_memory_sample_sort_with(text, lp, opts, emit):
         raise OSError(str(e)) from e
 
 
+def try_build_ext_mem_with(text, lp, opts: Optional[ExtMemOpts], emit):
+    """ext_mem.rs:324-356: every position of the text.  On the GPU this is the
+    same build as the in-memory sample sort over all positions (the crate's
+    two paths differ only in where buckets are kept)."""
+    _stream(lib().caps_build_in_memory_sample_sort, text, None, lp, opts, emit)
+
+
+def try_build_ext_mem(text, opts: Optional[ExtMemOpts], emit):
+    """ext_mem.rs:298-307."""
+    try_build_ext_mem_with(text, None, opts, emit)
+
+
+def build_ext_mem(text, opts: Optional[ExtMemOpts], emit):
+    """ext_mem.rs:289-295: errors surface as OSError (io::Error)."""
+    try:
+        try_build_ext_mem(text, opts, emit)
+    except BuildError as e:
+        raise OSError(str(e)) from e
+
+
+def build_ext_mem_with(text, lp, opts, emit):
+    """ext_mem.rs:315-321."""
+    try:
+        try_build_ext_mem_with(text, lp, opts, emit)
+    except BuildError as e:
+        raise OSError(str(e)) from e
+
+
 def try_build_in_memory_sample_sort_for_positions_with(text, positions, lp, opts, emit):
     """ext_mem.rs:842-871."""
     _stream(lib().caps_build_in_memory_sample_sort, text, positions, lp, opts, emit)

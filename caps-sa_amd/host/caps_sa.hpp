@@ -313,6 +313,33 @@ void build_ext_mem_for_positions_with(const std::vector<S> &text, const std::vec
     if (r) throw *r;
 }
 
+// build_ext_mem(_with) (ext_mem.rs:289-356): every position of the text.  On
+// the GPU this is the same build as build_in_memory_sample_sort over all
+// positions (the crate's two paths differ only in where buckets are kept).
+template <class E, class S, class L>
+std::optional<BuildError<E>> try_build_ext_mem_with(const std::vector<S> &text, const L &lp, const ExtMemOpts &opts,
+                                                    std::function<std::optional<E>(uint64_t)> emit) {
+    return try_build_in_memory_sample_sort_for_positions_with<E>(text, nullptr, lp, opts, std::move(emit));
+}
+template <class E, class S>
+std::optional<BuildError<E>> try_build_ext_mem(const std::vector<S> &text, const ExtMemOpts &opts,
+                                               std::function<std::optional<E>(uint64_t)> emit) {
+    return try_build_ext_mem_with<E>(text, PlainText(text.size()), opts, std::move(emit));
+}
+template <class S, class L>
+void build_ext_mem_with(const std::vector<S> &text, const L &lp, const ExtMemOpts &opts,
+                        std::function<void(uint64_t)> emit) {
+    auto r = try_build_ext_mem_with<std::string>(text, lp, opts, [&](uint64_t p) -> std::optional<std::string> {
+        emit(p);
+        return std::nullopt;
+    });
+    if (r) throw *r;
+}
+template <class S>
+void build_ext_mem(const std::vector<S> &text, const ExtMemOpts &opts, std::function<void(uint64_t)> emit) {
+    build_ext_mem_with(text, PlainText(text.size()), opts, std::move(emit));
+}
+
 // build_ext_mem_for_filter(_with) (ext_mem.rs:472-547): the keep predicate is
 // walked once into a bitmap of kept positions, as the crate's FilteredSource
 // does (ext_mem.rs:940-990), and never called again; no position list is built.
