@@ -859,7 +859,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
                                                                      queues + 0, chunk_for(cnt[0], 16));
             else if (cm.packed)
                 k_pairs_packed<S, I><<<(unsigned)ceil_div(cnt[0], 64), 256, 0, st>>>(
-                    d_text, cm.packed, cm.lgf, gref.bits, L, max_ctx, pl, cnt[0], depth0, out, lcp);
+                    cm.packed, cm.lgf, gref.bits, L, max_ctx, pl, cnt[0], depth0, out, lcp);
             else
                 k_pairs4<S, I><<<(unsigned)ceil_div(cnt[0], 64), 256, 0, st>>>(d_text, L, max_ctx, pl, cnt[0], depth0,
                                                                                 out, lcp);

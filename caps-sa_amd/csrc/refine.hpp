@@ -243,7 +243,8 @@ constexpr uint64_t kAuxCont = ~0ull;
 // compares 128 bits of codes (128 / bits symbols) of both suffixes per step,
 // loaded as three aligned words and shifted; the first step uses the pair's
 // first lane only.  The first differing bit gives the first differing code,
-// so the LCP (in symbols) is exact; codes order like symbols.  A word is
+// so the LCP (in symbols) is exact; codes order like symbols, so the two
+// windows also give the pair's order without touching the text.  A word is
 // loaded only if it holds a symbol inside the comparison window.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void load128_codes(const uint64_t *__restrict__ pk, uint32_t lgf, uint32_t bits,
@@ -267,7 +268,7 @@ __device__ __forceinline__ uint32_t first_diff_code(uint64_t ahi, uint64_t alo, 
 }
 
 template <typename S, typename I>
-__global__ __launch_bounds__(256) void k_pairs_packed(const S *__restrict__ t, const uint64_t *__restrict__ pk,
+__global__ __launch_bounds__(256) void k_pairs_packed(const uint64_t *__restrict__ pk,
                                                       uint32_t lgf, uint32_t bits, DevLimits L, uint64_t max_ctx,
                                                       const PairRec<I> *__restrict__ pairs, uint32_t n_pairs,
                                                       uint64_t known, I *__restrict__ idx, I *__restrict__ lcp) {
@@ -289,6 +290,10 @@ __global__ __launch_bounds__(256) void k_pairs_packed(const S *__restrict__ t, c
     uint64_t o = known < cap ? known : cap;  // symbols known equal
     bool done = !valid || o >= cap;
     uint64_t res = cap;
+    // a's suffix orders after b's (meaningful when res < cap): the 128-bit
+    // windows that hold the first differing code compare like it, so the
+    // order comes from the packed words already loaded, not from the text
+    bool gt = false;
     if (!done && sl == 0) {
         uint64_t ahi, alo, bhi, blo;
         load128_codes(pk, lgf, bits, a + o, a + cap, ahi, alo);
@@ -298,6 +303,7 @@ __global__ __launch_bounds__(256) void k_pairs_packed(const S *__restrict__ t, c
         if (in_win < fc) fc = (uint32_t)in_win;
         if (fc < FL) {
             res = o + fc;
+            gt = ahi != bhi ? ahi > bhi : alo > blo;
             done = true;
         }
     }
@@ -306,6 +312,7 @@ __global__ __launch_bounds__(256) void k_pairs_packed(const S *__restrict__ t, c
     if (!done) o += FL;
     while (__any(!done)) {
         uint32_t fc = FL;
+        bool g = false;
         if (!done) {
             const uint64_t rel = o + (uint64_t)sl * FL;
             if (rel < cap) {
@@ -313,6 +320,7 @@ __global__ __launch_bounds__(256) void k_pairs_packed(const S *__restrict__ t, c
                 load128_codes(pk, lgf, bits, a + rel, a + cap, ahi, alo);
                 load128_codes(pk, lgf, bits, b + rel, b + cap, bhi, blo);
                 fc = first_diff_code(ahi, alo, bhi, blo, bits);
+                g = ahi != bhi ? ahi > bhi : alo > blo;
             } else {
                 fc = 0;
             }
@@ -323,10 +331,12 @@ __global__ __launch_bounds__(256) void k_pairs_packed(const S *__restrict__ t, c
         const uint32_t gm = (uint32_t)(mask >> (grp * 4)) & 0xFu;
         const int fl = gm ? __ffs(gm) - 1 : 0;
         const uint32_t fq = __shfl(fc, (int)(grp * 4 + fl));
+        const bool gq = __shfl(g ? 1 : 0, (int)(grp * 4 + fl)) != 0;
         if (!done) {
             if (gm) {
                 const uint64_t r = o + (uint64_t)fl * FL + fq;
                 res = r < cap ? r : cap;
+                gt = gq;
                 done = true;
             } else {
                 o += 4ull * FL;
@@ -335,7 +345,7 @@ __global__ __launch_bounds__(256) void k_pairs_packed(const S *__restrict__ t, c
     }
     if (valid && sl == 0) {
         int c;
-        if (res < cap) c = t[a + res] < t[b + res] ? -1 : 1;
+        if (res < cap) c = gt ? 1 : -1;
         else c = boundary_then_pos(L, a, la, b, lb);
         if (c > 0) {
             idx[s] = (I)b;
