@@ -362,9 +362,12 @@ __device__ __forceinline__ int thread_cmp(const S *__restrict__ t, const DevLimi
 // a word of codes per side and step (two aligned loads and a funnel shift),
 // the first differing code from the leading zeros of their XOR.  Codes past
 // the window count as a difference at its end.  Every load stays inside the
-// packed buffer, which holds two words past the last code.
+// packed buffer, which holds two words past the last code.  When the result
+// is below `cap`, *gt tells whether p's code there is above q's: the words
+// agree above their first differing code, so they compare like it.
 template <typename CM>
-__device__ __forceinline__ uint64_t packed_lcp(const CM &cm, uint64_t p, uint64_t q, uint64_t start, uint64_t cap) {
+__device__ __forceinline__ uint64_t packed_lcp(const CM &cm, uint64_t p, uint64_t q, uint64_t start, uint64_t cap,
+                                               bool *gt) {
     const uint32_t lgf = cm.lgf, bits = 64u >> lgf;
     const uint64_t F = 1ull << lgf;
     auto word = [&](uint64_t x) {
@@ -376,11 +379,13 @@ __device__ __forceinline__ uint64_t packed_lcp(const CM &cm, uint64_t p, uint64_
     };
     uint64_t o = start;
     while (o < cap) {
-        uint64_t x = word(p + o) ^ word(q + o);
+        const uint64_t wp = word(p + o), wq = word(q + o);
+        uint64_t x = wp ^ wq;
         const uint64_t rem = cap - o;
         if (rem < F) x |= ~0ull >> (rem * bits);  // codes past the window differ
         if (x) {
             const uint64_t r = o + (uint64_t)__builtin_clzll(x) / bits;
+            *gt = wp > wq;
             return r < cap ? r : cap;
         }
         o += F;
@@ -388,18 +393,19 @@ __device__ __forceinline__ uint64_t packed_lcp(const CM &cm, uint64_t p, uint64_
     return cap;
 }
 
-// thread_cmp with the LCP scan over the packed codes (same result).
+// thread_cmp with the LCP scan over the packed codes (same result); the
+// order at the first differing code comes from the packed words as well.
 template <typename S>
-__device__ __forceinline__ int thread_cmp_packed(const S *__restrict__ t, const CodeMap<S> &cm, const DevLimits &L,
-                                                 uint64_t max_ctx, uint64_t a, uint64_t b, uint64_t known,
-                                                 uint64_t *lcp) {
+__device__ __forceinline__ int thread_cmp_packed(const CodeMap<S> &cm, const DevLimits &L, uint64_t max_ctx,
+                                                 uint64_t a, uint64_t b, uint64_t known, uint64_t *lcp) {
     uint64_t la = lim_at(L, a), lb = lim_at(L, b);
     uint64_t ca = la < max_ctx ? la : max_ctx, cb = lb < max_ctx ? lb : max_ctx;
     uint64_t cap = ca < cb ? ca : cb;
     uint64_t s = known < cap ? known : cap;
-    uint64_t c = packed_lcp(cm, a, b, s, cap);
+    bool gt = false;
+    uint64_t c = packed_lcp(cm, a, b, s, cap, &gt);
     *lcp = c;
-    if (c < cap) return t[a + c] < t[b + c] ? -1 : 1;
+    if (c < cap) return gt ? 1 : -1;
     return boundary_then_pos(L, a, la, b, lb);
 }
 

@@ -618,12 +618,21 @@ __global__ void k_block_offsets(uint32_t *__restrict__ hblk, uint32_t nblocks, u
 // the placing pass), histograms and scans the buckets, writes the bucket
 // offsets, and places every record with LDS cursors.
 // ---------------------------------------------------------------------------
-// A record between the two scatter passes: one 16-byte store per record.
+// A record between the two scatter passes: key and index, 16 bytes with a
+// u64 index; with a u32 index packed to 12 bytes (4-byte aligned), so both
+// passes move a quarter fewer bytes.
 template <typename I>
 struct KRec {
     uint64_t key;
     I idx;
 };
+template <>
+struct __attribute__((packed, aligned(4))) KRec<uint32_t> {
+    uint64_t key;
+    uint32_t idx;
+};
+static_assert(sizeof(KRec<uint32_t>) == 12 && alignof(KRec<uint32_t>) == 4, "packed u32 record");
+static_assert(sizeof(KRec<uint64_t>) == 16, "u64 record");
 
 template <typename S, typename I, bool SEQ, bool KEEP>
 __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, KeyGeom g,
@@ -1340,7 +1349,7 @@ __global__ __launch_bounds__(128) void k_lcp_merge(const S *__restrict__ t, Code
                                                    I *__restrict__ dl) {
     // suffix compares scan the packed codes when the build has them
     auto cmp = [&](uint64_t a, uint64_t b, uint64_t kn, uint64_t *l) {
-        return cm.packed ? thread_cmp_packed(t, cm, L, max_ctx, a, b, kn, l) : thread_cmp(t, L, max_ctx, a, b, kn, l);
+        return cm.packed ? thread_cmp_packed(cm, L, max_ctx, a, b, kn, l) : thread_cmp(t, L, max_ctx, a, b, kn, l);
     };
     MergeJob jb = jobs[blockIdx.y];
     uint64_t rlen = jb.nx + jb.ny;
