@@ -273,16 +273,21 @@ __device__ __forceinline__ void block_merge_sort(uint64_t (&k)[ITEMS], uint32_t 
             }
             continue;
         }
-        uint32_t lo = diag > w ? diag - w : 0, hi = diag < w ? diag : w;
-        while (lo < hi) {
-            uint32_t mid = (lo + hi) >> 1;
-            uint32_t bi = b0 + diag - 1 - mid;
-            const bool bl = PACKED ? sk[px(bi)] < sk[px(a0 + mid)]
-                                   : rec_less(sk[px(bi)], sl[px(bi)], sk[px(a0 + mid)], sl[px(a0 + mid)]);
-            if (!bl) lo = mid + 1;
-            else hi = mid;
+        // merge path: the first x in [lo, hi) with B[diag-1-x] < A[x] (hi if
+        // none), by bit descent over a range of at most w + 1 candidates: the
+        // loop runs the same log2(w) + 1 steps on every lane
+        const uint32_t lo = diag > w ? diag - w : 0, hi = diag < w ? diag : w;
+        uint32_t x = lo;
+        for (uint32_t st = w; st; st >>= 1) {
+            const uint32_t c = x + st;  // candidate count of A records before the split
+            if (c <= hi) {
+                const uint32_t bi = b0 + diag - c, ai = a0 + c - 1;
+                const bool bl = PACKED ? sk[px(bi)] < sk[px(ai)]
+                                       : rec_less(sk[px(bi)], sl[px(bi)], sk[px(ai)], sl[px(ai)]);
+                if (!bl) x = c;
+            }
         }
-        uint32_t a = a0 + lo, b = b0 + (diag - lo);
+        uint32_t a = a0 + x, b = b0 + (diag - x);
         const uint32_t ae = a0 + w, be = b0 + w;
         uint64_t ka = a < ae ? sk[px(a)] : ~0ull, kb = b < be ? sk[px(b)] : ~0ull;
         uint32_t la = 0, lb = 0;
@@ -290,21 +295,24 @@ __device__ __forceinline__ void block_merge_sort(uint64_t (&k)[ITEMS], uint32_t 
             la = a < ae ? sl[px(a)] : 0xFFFFFFFFu;
             lb = b < be ? sl[px(b)] : 0xFFFFFFFFu;
         }
+        // serial merge without branches: each step refills only the side it
+        // took, with one LDS read
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
             const bool take_a = b >= be || (a < ae && !less(kb, lb, ka, la));
-            if (take_a) {
-                k[j] = ka;
-                if (!PACKED) li[j] = la;
-                ++a;
-                ka = a < ae ? sk[px(a)] : ~0ull;
-                if (!PACKED) la = a < ae ? sl[px(a)] : 0xFFFFFFFFu;
-            } else {
-                k[j] = kb;
-                if (!PACKED) li[j] = lb;
-                ++b;
-                kb = b < be ? sk[px(b)] : ~0ull;
-                if (!PACKED) lb = b < be ? sl[px(b)] : 0xFFFFFFFFu;
+            k[j] = take_a ? ka : kb;
+            if (!PACKED) li[j] = take_a ? la : lb;
+            a += take_a ? 1u : 0u;
+            b += take_a ? 0u : 1u;
+            const uint32_t nx = take_a ? a : b;
+            const bool in = nx < (take_a ? ae : be);
+            const uint64_t nk = in ? sk[px(nx)] : ~0ull;
+            ka = take_a ? nk : ka;
+            kb = take_a ? kb : nk;
+            if (!PACKED) {
+                const uint32_t nl = in ? sl[px(nx)] : 0xFFFFFFFFu;
+                la = take_a ? nl : la;
+                lb = take_a ? lb : nl;
             }
         }
     }
