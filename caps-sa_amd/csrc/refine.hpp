@@ -49,6 +49,53 @@ __device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// Lane moves inside a slot of W lanes (W a power of two, slots aligned).
+// A slot of W <= 16 lanes lies inside one row of 16, so DPP (a modifier on a
+// VALU move, no round trip through the LDS crossbar) does them: quad
+// permutations for xor 1 and xor 2, row shifts for lane -1, +1 and -o.  Other
+// moves, and wider slots, use ds_bpermute.  seg_prev / seg_next / seg_up
+// return the lane's own value where the source lane is outside the row;
+// callers use them only where the source lies inside the slot.
+template <int W>
+__device__ __forceinline__ uint32_t seg_xor(uint32_t v, int stride) {
+    if (W <= 16 && stride == 1) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
+    if (W <= 16 && stride == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);
+    return (uint32_t)__shfl_xor((int)v, stride);
+}
+template <int W>
+__device__ __forceinline__ uint32_t seg_prev(uint32_t v) {
+    if (W <= 16) return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x111, 0xF, 0xF, false);
+    const int lane = threadIdx.x & 63;
+    return (uint32_t)__shfl((int)v, lane - 1 < 0 ? 0 : lane - 1);
+}
+template <int W>
+__device__ __forceinline__ uint32_t seg_next(uint32_t v) {
+    if (W <= 16) return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x101, 0xF, 0xF, false);
+    const int lane = threadIdx.x & 63;
+    return (uint32_t)__shfl((int)v, lane + 1 < 64 ? lane + 1 : lane);
+}
+template <int W>
+__device__ __forceinline__ uint32_t seg_up(uint32_t v, int o) {
+    if (W <= 16) {
+        switch (o) {
+            case 1: return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x111, 0xF, 0xF, false);
+            case 2: return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x112, 0xF, 0xF, false);
+            case 4: return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x114, 0xF, 0xF, false);
+            case 8: return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x118, 0xF, 0xF, false);
+            default: break;
+        }
+    }
+    return (uint32_t)__shfl_up((int)v, o, W);
+}
+template <int W>
+__device__ __forceinline__ uint64_t seg_xor64(uint64_t v, int stride) {
+    return ((uint64_t)seg_xor<W>((uint32_t)(v >> 32), stride) << 32) | seg_xor<W>((uint32_t)v, stride);
+}
+template <int W>
+__device__ __forceinline__ uint64_t seg_prev64(uint64_t v) {
+    return ((uint64_t)seg_prev<W>((uint32_t)(v >> 32)) << 32) | seg_prev<W>((uint32_t)v);
+}
+
 // Two-member groups are listed with their positions, so the pair kernel reads
 // the text without first gathering the positions from the suffix array.
 template <typename I>
@@ -477,13 +524,13 @@ __global__ __launch_bounds__(256) void k_wgroups_grid(const S *__restrict__ t, C
         for (int size = 2; size <= W; size <<= 1) {
 #pragma unroll
             for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                uint32_t p_sub = __shfl_xor(sub, stride);
-                uint64_t p_chunk = shfl_xor64(chunk, stride);
-                uint64_t p_aux = shfl_xor64(aux, stride);
-                uint64_t p_pos = shfl_xor64(pos, stride);
-                uint64_t p_lim = shfl_xor64(lim, stride);
-                uint64_t p_cap = shfl_xor64(cap, stride);
-                int p_settled = __shfl_xor((int)settled, stride);
+                uint32_t p_sub = seg_xor<W>(sub, stride);
+                uint64_t p_chunk = seg_xor64<W>(chunk, stride);
+                uint64_t p_aux = seg_xor64<W>(aux, stride);
+                uint64_t p_pos = seg_xor64<W>(pos, stride);
+                uint64_t p_lim = seg_xor64<W>(lim, stride);
+                uint64_t p_cap = seg_xor64<W>(cap, stride);
+                int p_settled = (int)seg_xor<W>((uint32_t)settled, stride);
                 bool less;  // me < partner
                 if (sub != p_sub) less = sub < p_sub;
                 else if (chunk != p_chunk) less = chunk < p_chunk;
@@ -499,10 +546,9 @@ __global__ __launch_bounds__(256) void k_wgroups_grid(const S *__restrict__ t, C
             }
         }
         // neighbour r-1 after the sort
-        int prev = lane - 1 < 0 ? 0 : lane - 1;
-        uint32_t q_sub = __shfl(sub, prev);
-        uint64_t q_chunk = shfl64(chunk, prev);
-        uint64_t q_aux = shfl64(aux, prev);
+        uint32_t q_sub = seg_prev<W>(sub);
+        uint64_t q_chunk = seg_prev64<W>(chunk);
+        uint64_t q_aux = seg_prev64<W>(aux);
         bool same_old = r > 0 && r < m && sub == q_sub;
         bool f = !(same_old && chunk == q_chunk && aux == q_aux);
         if (same_old && !settled) h = D + shared_fields(g, q_chunk, chunk);
@@ -510,11 +556,11 @@ __global__ __launch_bounds__(256) void k_wgroups_grid(const S *__restrict__ t, C
         uint32_t ns = f ? r : 0;
 #pragma unroll
         for (int o = 1; o < W; o <<= 1) {
-            uint32_t v = __shfl_up(ns, o, W);
+            uint32_t v = seg_up<W>(ns, o);
             if ((int)r >= o) ns = v > ns ? v : ns;
         }
-        uint32_t ns_prev = __shfl(ns, prev);
-        uint32_t ns_next = __shfl(ns, lane + 1 < 64 ? lane + 1 : lane);
+        uint32_t ns_prev = seg_prev<W>(ns);
+        uint32_t ns_next = seg_next<W>(ns);
         bool nb = (r > 0 && r < m && ns_prev == ns) || (r + 1 < m && ns_next == ns);
         bool ended = aux != cont;
         if (act) {
@@ -524,7 +570,7 @@ __global__ __launch_bounds__(256) void k_wgroups_grid(const S *__restrict__ t, C
         if (!__any(!settled)) break;
         D += g.k;
     }
-    uint64_t q_cap = shfl64(cap, lane - 1 < 0 ? 0 : lane - 1);
+    uint64_t q_cap = seg_prev64<W>(cap);
     if (act) {
         idx[gr.start + r] = (I)pos;
         if (lcp && r > 0) {
@@ -666,13 +712,13 @@ __global__ __launch_bounds__(256) void k_wgroups_wq(const S *__restrict__ t, Cod
         for (int size = 2; size <= W; size <<= 1) {
 #pragma unroll
             for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                uint32_t p_sub = __shfl_xor(sub, stride);
-                uint64_t p_chunk = shfl_xor64(chunk, stride);
-                uint64_t p_aux = shfl_xor64(aux, stride);
-                uint64_t p_pos = shfl_xor64(pos, stride);
-                uint64_t p_lim = shfl_xor64(lim, stride);
-                uint64_t p_cap = shfl_xor64(cap, stride);
-                int p_settled = __shfl_xor((int)settled, stride);
+                uint32_t p_sub = seg_xor<W>(sub, stride);
+                uint64_t p_chunk = seg_xor64<W>(chunk, stride);
+                uint64_t p_aux = seg_xor64<W>(aux, stride);
+                uint64_t p_pos = seg_xor64<W>(pos, stride);
+                uint64_t p_lim = seg_xor64<W>(lim, stride);
+                uint64_t p_cap = seg_xor64<W>(cap, stride);
+                int p_settled = (int)seg_xor<W>((uint32_t)settled, stride);
                 bool less;  // me < partner
                 if (sub != p_sub) less = sub < p_sub;
                 else if (chunk != p_chunk) less = chunk < p_chunk;
@@ -688,11 +734,10 @@ __global__ __launch_bounds__(256) void k_wgroups_wq(const S *__restrict__ t, Cod
             }
         }
         // neighbour r-1 after the sort
-        const int prev = lane - 1 < 0 ? 0 : lane - 1;
-        uint32_t q_sub = __shfl(sub, prev);
-        uint64_t q_chunk = shfl64(chunk, prev);
-        uint64_t q_aux = shfl64(aux, prev);
-        uint64_t q_cap = shfl64(cap, prev);
+        uint32_t q_sub = seg_prev<W>(sub);
+        uint64_t q_chunk = seg_prev64<W>(chunk);
+        uint64_t q_aux = seg_prev64<W>(aux);
+        uint64_t q_cap = seg_prev64<W>(cap);
         bool same_old = r > 0 && r < m && sub == q_sub;
         bool f = !(same_old && chunk == q_chunk && aux == q_aux);
         if (act && same_old && !settled) h = D + shared_fields(g, q_chunk, chunk);
@@ -700,11 +745,11 @@ __global__ __launch_bounds__(256) void k_wgroups_wq(const S *__restrict__ t, Cod
         uint32_t ns = f ? r : 0;
 #pragma unroll
         for (int o = 1; o < W; o <<= 1) {
-            uint32_t v = __shfl_up(ns, o, W);
+            uint32_t v = seg_up<W>(ns, o);
             if ((int)r >= o) ns = v > ns ? v : ns;
         }
-        uint32_t ns_prev = __shfl(ns, prev);
-        uint32_t ns_next = __shfl(ns, lane + 1 < 64 ? lane + 1 : lane);
+        uint32_t ns_prev = seg_prev<W>(ns);
+        uint32_t ns_next = seg_next<W>(ns);
         bool nb = (r > 0 && r < m && ns_prev == ns) || (r + 1 < m && ns_next == ns);
         bool ended = aux != cont;
         if (act) {
