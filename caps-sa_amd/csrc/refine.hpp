@@ -448,7 +448,7 @@ __device__ __forceinline__ uint64_t ff_skip(const uint64_t *__restrict__ pk, uin
     }
 #pragma unroll
     for (int o = 1; o < W; o <<= 1) {
-        const uint32_t v = (uint32_t)__shfl_xor((int)f, o);
+        const uint32_t v = seg_xor<W>(f, o);
         f = v < f ? v : f;
     }
     return (uint64_t)f << lgf;
