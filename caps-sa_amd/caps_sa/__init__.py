@@ -16,6 +16,7 @@ Text is a numpy array of ``uint8`` or ``uint16`` (``bytes`` is accepted as
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 import enum
 import os
 import warnings
@@ -313,9 +314,72 @@ class Opts:
     max_context: int = MAX_CONTEXT
 
 
+@dataclass(frozen=True)
+class GeometricMemoizationConfig:
+    """lcp_memo.rs:64-145: tuning of the crate's geometric LCP memo, each
+    value non-zero (NonZeroUsize there: ValueError here).  Memoization never
+    changes the output; the GPU build accepts the policy and does not use it
+    (its long-LCP scans read packed code words and fast-forward over shared
+    stretches instead)."""
+
+    probe_symbols: int = 256
+    min_lcp_symbols: int = 1024
+    activate_after_entries: int = 64
+    max_entries_per_partition: int = 4096
+
+    def __post_init__(self):
+        for name in ("probe_symbols", "min_lcp_symbols", "activate_after_entries", "max_entries_per_partition"):
+            v = getattr(self, name)
+            if not isinstance(v, int) or v <= 0:
+                raise ValueError(f"{name} must be a positive integer")
+
+    def with_probe_symbols(self, v: int) -> "GeometricMemoizationConfig":
+        return dataclasses.replace(self, probe_symbols=v)
+
+    def with_min_lcp_symbols(self, v: int) -> "GeometricMemoizationConfig":
+        return dataclasses.replace(self, min_lcp_symbols=v)
+
+    def with_activate_after_entries(self, v: int) -> "GeometricMemoizationConfig":
+        return dataclasses.replace(self, activate_after_entries=v)
+
+    def with_max_entries_per_partition(self, v: int) -> "GeometricMemoizationConfig":
+        return dataclasses.replace(self, max_entries_per_partition=v)
+
+
+class LcpMemoizationPolicy:
+    """lcp_memo.rs:16-40: Disabled is None, Geometric is a configuration."""
+
+    DISABLED = None
+
+    @staticmethod
+    def geometric(config: Optional[GeometricMemoizationConfig] = None) -> GeometricMemoizationConfig:
+        return config if config is not None else GeometricMemoizationConfig()
+
+
+def _env_usize(name: str) -> Optional[int]:
+    """read_env_usize (ext_mem.rs:241-243): usize::from_str of the value, an
+    optional '+' then ASCII digits, else None."""
+    v = os.environ.get(name)
+    if v is None:
+        return None
+    body = v[1:] if v.startswith("+") else v
+    if not body or not all("0" <= ch <= "9" for ch in body):
+        return None
+    x = int(body)
+    return x if x < (1 << 64) else None
+
+
+def _env_bool(name: str) -> bool:
+    """read_env_bool (ext_mem.rs:249-253)."""
+    return os.environ.get(name) in ("1", "true", "yes", "on")
+
+
 @dataclass
 class ExtMemOpts:
-    """ext_mem.rs:58-239 (the fields with a GPU meaning) plus GPU knobs."""
+    """ext_mem.rs:58-239 (the fields with a GPU meaning) plus GPU knobs.
+    lcp_memoization (None: LcpMemoizationPolicy::Disabled, else the geometric
+    configuration) and collect_lcp_memoization_stats are accepted for
+    callers of the crate and leave the GPU build unchanged."""
 
     max_context: int = MAX_CONTEXT
     subproblem_count: int = 0
@@ -324,21 +388,42 @@ class ExtMemOpts:
     gpu_buckets: int = 0
     tile_keys: int = 0
     batch_records: int = 0  # records per bucket batch (0: one batch while it fits in HBM)
+    lcp_memoization: Optional[GeometricMemoizationConfig] = None
+    collect_lcp_memoization_stats: bool = False
 
     @classmethod
     def from_env(cls) -> "ExtMemOpts":
-        """ext_mem.rs:154-196, for the variables with a GPU meaning."""
+        """ext_mem.rs:154-196, for the variables with a GPU meaning and the
+        memoization policy (its tuning variables count only when
+        CAPS_SA_GEOMETRIC_MEMO enables it; zero or invalid values keep the
+        defaults)."""
         o = cls()
-        v = os.environ.get("CAPS_SA_SUBPROBLEMS")
-        if v and v.isdigit():
-            o.subproblem_count = int(v)
-        v = os.environ.get("CAPS_SA_MAX_CONTEXT")
-        if v and v.isdigit():
-            o.max_context = int(v)
-        if os.environ.get("CAPS_SA_PACKED_PREFIX_REMAP_BYTES", "").isdigit():
+        v = _env_usize("CAPS_SA_SUBPROBLEMS")
+        if v is not None:
+            o.subproblem_count = v
+        v = _env_usize("CAPS_SA_MAX_CONTEXT")
+        if v is not None:
+            o.max_context = v
+        if _env_usize("CAPS_SA_PACKED_PREFIX_REMAP_BYTES") is not None:
             o.packed_prefix_seed = PackedPrefixSeedPolicy.REMAP
-        elif os.environ.get("CAPS_SA_PACKED_PREFIX_SEED", "") in ("1", "true", "yes", "on"):
+        elif _env_bool("CAPS_SA_PACKED_PREFIX_SEED"):
             o.packed_prefix_seed = PackedPrefixSeedPolicy.DENSE_ALPHABET_ONLY
+        if _env_bool("CAPS_SA_GEOMETRIC_MEMO"):
+            c = GeometricMemoizationConfig()
+            v = _env_usize("CAPS_SA_MEMO_PROBE")
+            if v:
+                c = c.with_probe_symbols(v)
+            v = _env_usize("CAPS_SA_MEMO_MIN_LCP")
+            if v:
+                c = c.with_min_lcp_symbols(v)
+            v = _env_usize("CAPS_SA_MEMO_ACTIVATE_ENTRIES")
+            if v:
+                c = c.with_activate_after_entries(v)
+            v = _env_usize("CAPS_SA_MEMO_CAPACITY")
+            if v:
+                c = c.with_max_entries_per_partition(v)
+            o.lcp_memoization = c
+        o.collect_lcp_memoization_stats = _env_bool("CAPS_SA_MEMO_STATS")
         return o
 
 

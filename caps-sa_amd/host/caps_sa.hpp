@@ -15,6 +15,7 @@
 // Header-only; link against libcaps_sa_hip.so.
 #pragma once
 #include <cstdint>
+#include <cstdlib>
 #include <functional>
 #include <limits>
 #include <optional>
@@ -98,13 +99,93 @@ struct Opts {
     }
 };
 
+// GeometricMemoizationConfig (lcp_memo.rs:64-145): tuning of the crate's
+// geometric LCP memo; every value non-zero (std::invalid_argument otherwise).
+// Memoization never changes the output; the GPU build accepts the policy and
+// does not use it (its long-LCP scans read packed code words and fast-forward
+// over shared stretches instead).
+struct GeometricMemoizationConfig {
+    uint64_t probe_symbols = 256;
+    uint64_t min_lcp_symbols = 1024;
+    uint64_t activate_after_entries = 64;
+    uint64_t max_entries_per_partition = 4096;
+    GeometricMemoizationConfig with_probe_symbols(uint64_t v) const { return set(&GeometricMemoizationConfig::probe_symbols, v); }
+    GeometricMemoizationConfig with_min_lcp_symbols(uint64_t v) const {
+        return set(&GeometricMemoizationConfig::min_lcp_symbols, v);
+    }
+    GeometricMemoizationConfig with_activate_after_entries(uint64_t v) const {
+        return set(&GeometricMemoizationConfig::activate_after_entries, v);
+    }
+    GeometricMemoizationConfig with_max_entries_per_partition(uint64_t v) const {
+        return set(&GeometricMemoizationConfig::max_entries_per_partition, v);
+    }
+
+  private:
+    GeometricMemoizationConfig set(uint64_t GeometricMemoizationConfig::*f, uint64_t v) const {
+        if (v == 0) throw std::invalid_argument("memoization settings must be non-zero");
+        GeometricMemoizationConfig c = *this;
+        c.*f = v;
+        return c;
+    }
+};
+
+namespace detail {
+// read_env_usize / read_env_bool (ext_mem.rs:241-253)
+inline std::optional<uint64_t> env_usize(const char *name) {
+    const char *v = std::getenv(name);
+    if (!v) return std::nullopt;
+    if (*v == '+') ++v;
+    if (!*v) return std::nullopt;
+    uint64_t x = 0;
+    for (; *v; ++v) {
+        if (*v < '0' || *v > '9') return std::nullopt;
+        const uint64_t d = (uint64_t)(*v - '0');
+        if (x > (std::numeric_limits<uint64_t>::max() - d) / 10) return std::nullopt;
+        x = x * 10 + d;
+    }
+    return x;
+}
+inline bool env_bool(const char *name) {
+    const char *v = std::getenv(name);
+    if (!v) return false;
+    const std::string s(v);
+    return s == "1" || s == "true" || s == "yes" || s == "on";
+}
+}  // namespace detail
+
 // ExtMemOpts (ext_mem.rs:58-239): the fields with a meaning on the GPU, with
-// the crate's builder methods.  The output never depends on them.
+// the crate's builder methods, plus the memoization policy (std::nullopt:
+// LcpMemoizationPolicy::Disabled), accepted and unused.  The output never
+// depends on them.
 struct ExtMemOpts {
     uint64_t max_context = std::numeric_limits<uint64_t>::max();
     uint64_t subproblem_count = 0;
     uint32_t router = CAPS_ROUTER_BSEARCH;
     uint64_t batch_records = 0;  // GPU knob: records per bucket batch (0: auto)
+    std::optional<GeometricMemoizationConfig> lcp_memoization;
+    bool collect_lcp_memoization_stats = false;
+    // ExtMemOpts::from_env (ext_mem.rs:154-196) for the variables with a GPU
+    // meaning and the memoization policy.
+    static ExtMemOpts from_env() {
+        ExtMemOpts o;
+        if (auto v = detail::env_usize("CAPS_SA_SUBPROBLEMS")) o.subproblem_count = *v;
+        if (auto v = detail::env_usize("CAPS_SA_MAX_CONTEXT")) o.max_context = *v;
+        if (detail::env_bool("CAPS_SA_GEOMETRIC_MEMO")) {
+            GeometricMemoizationConfig c;
+            if (auto v = detail::env_usize("CAPS_SA_MEMO_PROBE"); v && *v) c = c.with_probe_symbols(*v);
+            if (auto v = detail::env_usize("CAPS_SA_MEMO_MIN_LCP"); v && *v) c = c.with_min_lcp_symbols(*v);
+            if (auto v = detail::env_usize("CAPS_SA_MEMO_ACTIVATE_ENTRIES"); v && *v)
+                c = c.with_activate_after_entries(*v);
+            if (auto v = detail::env_usize("CAPS_SA_MEMO_CAPACITY"); v && *v) c = c.with_max_entries_per_partition(*v);
+            o.lcp_memoization = c;
+        }
+        o.collect_lcp_memoization_stats = detail::env_bool("CAPS_SA_MEMO_STATS");
+        return o;
+    }
+    ExtMemOpts &with_lcp_memoization(std::optional<GeometricMemoizationConfig> p) {
+        lcp_memoization = p;
+        return *this;
+    }
     ExtMemOpts &with_max_context(uint64_t m) {
         max_context = m;
         return *this;

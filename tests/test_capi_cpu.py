@@ -135,3 +135,65 @@ def test_keep_bitmap_layout():
             back = [(int(bits[q >> 6]) >> (q & 63)) & 1 for q in range(n)]
             assert back == mask.astype(int).tolist()
             assert all(int(bits[-1]) >> b == 0 for b in range(n % 64, 64) if n % 64) if n else True
+
+
+def test_memoization_policy_from_env(monkeypatch):
+    """ext_mem.rs:154-196 and lcp_memo.rs:11-14, 64-145: the geometric memo
+    is off by default; CAPS_SA_GEOMETRIC_MEMO turns it on with the defaults
+    (256, 1024, 64, 4096); its tuning variables count only then, and zero or
+    malformed values keep the defaults."""
+    for var in ("CAPS_SA_GEOMETRIC_MEMO", "CAPS_SA_MEMO_PROBE", "CAPS_SA_MEMO_MIN_LCP",
+                "CAPS_SA_MEMO_ACTIVATE_ENTRIES", "CAPS_SA_MEMO_CAPACITY", "CAPS_SA_MEMO_STATS"):
+        monkeypatch.delenv(var, raising=False)
+    o = caps_sa.ExtMemOpts.from_env()
+    assert o.lcp_memoization is None and not o.collect_lcp_memoization_stats
+    monkeypatch.setenv("CAPS_SA_MEMO_PROBE", "9")  # ignored while the memo is off
+    assert caps_sa.ExtMemOpts.from_env().lcp_memoization is None
+    monkeypatch.setenv("CAPS_SA_GEOMETRIC_MEMO", "yes")
+    monkeypatch.setenv("CAPS_SA_MEMO_MIN_LCP", "0")
+    monkeypatch.setenv("CAPS_SA_MEMO_ACTIVATE_ENTRIES", "12x")
+    monkeypatch.setenv("CAPS_SA_MEMO_CAPACITY", "+77")
+    monkeypatch.setenv("CAPS_SA_MEMO_STATS", "on")
+    o = caps_sa.ExtMemOpts.from_env()
+    assert o.lcp_memoization == caps_sa.GeometricMemoizationConfig(9, 1024, 64, 77)
+    assert o.collect_lcp_memoization_stats
+    monkeypatch.setenv("CAPS_SA_GEOMETRIC_MEMO", "TRUE")  # only 1/true/yes/on
+    assert caps_sa.ExtMemOpts.from_env().lcp_memoization is None
+    d = caps_sa.LcpMemoizationPolicy.geometric()
+    assert (d.probe_symbols, d.min_lcp_symbols, d.activate_after_entries, d.max_entries_per_partition) == \
+        (256, 1024, 64, 4096)
+    assert d.with_probe_symbols(5).probe_symbols == 5 and d.probe_symbols == 256
+    with pytest.raises(ValueError):
+        d.with_max_entries_per_partition(0)
+
+
+def test_cpp_mirror_memoization_from_env():
+    """The C++ mirror parses the same variables the same way."""
+    import subprocess
+    import tempfile
+
+    src = r'''
+#include <cstdio>
+#include "caps_sa.hpp"
+int main() {
+    auto o = caps_sa::ExtMemOpts::from_env();
+    if (!o.lcp_memoization) { std::printf("off %d\n", (int)o.collect_lcp_memoization_stats); return 0; }
+    const auto &c = *o.lcp_memoization;
+    std::printf("%llu %llu %llu %llu %d\n", (unsigned long long)c.probe_symbols, (unsigned long long)c.min_lcp_symbols,
+                (unsigned long long)c.activate_after_entries, (unsigned long long)c.max_entries_per_partition,
+                (int)o.collect_lcp_memoization_stats);
+    try { c.with_probe_symbols(0); } catch (const std::invalid_argument &) { std::printf("zero rejected\n"); }
+    return 0;
+}
+'''
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "memo")
+        r = subprocess.run(["g++", "-std=c++17", "-Wall", "-x", "c++", "-I", os.path.join(ROOT, "caps-sa_amd", "host"),
+                            "-o", exe, "-"], input=src, capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        base = {k: v for k, v in os.environ.items() if not k.startswith("CAPS_SA_")}
+        run = lambda env: subprocess.run([exe], env={**base, **env}, capture_output=True, text=True).stdout.split("\n")
+        assert run({"CAPS_SA_MEMO_PROBE": "9"})[0] == "off 0"
+        out = run({"CAPS_SA_GEOMETRIC_MEMO": "1", "CAPS_SA_MEMO_PROBE": "9", "CAPS_SA_MEMO_MIN_LCP": "0",
+                   "CAPS_SA_MEMO_ACTIVATE_ENTRIES": "12x", "CAPS_SA_MEMO_CAPACITY": "+77", "CAPS_SA_MEMO_STATS": "on"})
+        assert out[0] == "9 1024 64 77 1" and out[1] == "zero rejected"

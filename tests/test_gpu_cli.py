@@ -76,3 +76,8 @@ def test_build_ext_mem_every_position():
     got = []
     caps_sa.build_ext_mem_with(text, caps_sa.PlainText(len(text)), caps_sa.ExtMemOpts(subproblem_count=5), got.append)
     assert np.array_equal(np.array(got, dtype=np.uint64), want)
+    # the geometric memo policy is accepted and leaves the output unchanged
+    got = []
+    caps_sa.build_ext_mem(text, caps_sa.ExtMemOpts(lcp_memoization=caps_sa.LcpMemoizationPolicy.geometric(),
+                                                   collect_lcp_memoization_stats=True), got.append)
+    assert np.array_equal(np.array(got, dtype=np.uint64), want)
