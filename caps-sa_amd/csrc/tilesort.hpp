@@ -396,32 +396,40 @@ __global__ __launch_bounds__(256) void k_tile_merge(uint64_t *__restrict__ keys,
         idx[tl.start + i] = si[sl[px(i)]];
     }
     if (!emit || !tl.whole || len < 2) return;
-    // tie groups of the bucket: two passes (count, then place)
+    // tie groups of the bucket, in one pass over the sorted tile: each group
+    // start takes a slot of its size class in LDS and keeps (size, class,
+    // slot) in a register (size <= T <= 4096: 13 bits; 8 classes: 3 bits;
+    // slot < T / 2: 11 bits); the block then reserves each class's slots in
+    // the global lists once and writes the groups
+    static_assert(T <= 4096 && kGroupClasses <= 8, "group record packing");
     if (tid < kGroupClasses) gcnt[tid] = 0;
     __syncthreads();
-    for (uint32_t r = tid; r < len; r += B) {
-        uint64_t kr = sk[px(r)];
-        if ((r == 0 || sk[px(r - 1)] != kr) && r + 1 < len && sk[px(r + 1)] == kr) {
-            uint32_t e = r + 2;
-            while (e < len && sk[px(e)] == kr) ++e;
-            atomicAdd(&gcnt[group_class(e - r)], 1u);
+    uint32_t info[ITEMS];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const uint32_t r = tid + i * B;
+        info[i] = 0;
+        if (r < len) {
+            const uint64_t kr = sk[px(r)];
+            if ((r == 0 || sk[px(r - 1)] != kr) && r + 1 < len && sk[px(r + 1)] == kr) {
+                uint32_t e = r + 2;
+                while (e < len && sk[px(e)] == kr) ++e;
+                const uint32_t c = (uint32_t)group_class(e - r);
+                info[i] = (e - r) << 14 | c << 11 | atomicAdd(&gcnt[c], 1u);
+            }
         }
     }
     __syncthreads();
-    if (tid < kGroupClasses) {
-        gbase[tid] = gcnt[tid] ? atomicAdd(&gl.counts[tid], gcnt[tid]) : 0u;
-        gcnt[tid] = 0;
-    }
+    if (tid < kGroupClasses) gbase[tid] = gcnt[tid] ? atomicAdd(&gl.counts[tid], gcnt[tid]) : 0u;
     __syncthreads();
-    for (uint32_t r = tid; r < len; r += B) {
-        uint64_t kr = sk[px(r)];
-        if ((r == 0 || sk[px(r - 1)] != kr) && r + 1 < len && sk[px(r + 1)] == kr) {
-            uint32_t e = r + 2;
-            while (e < len && sk[px(e)] == kr) ++e;
-            int c = group_class(e - r);
-            uint32_t slot = gbase[c] + atomicAdd(&gcnt[c], 1u);
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        if (info[i]) {
+            const uint32_t r = tid + i * B;
+            const uint32_t size = info[i] >> 14, c = (info[i] >> 11) & 7u;
+            const uint32_t slot = gbase[c] + (info[i] & 0x7FFu);
             if (c == 0) ((PairRec<I> *)gl.list[0])[slot] = PairRec<I>{tl.start + r, si[sl[px(r)]], si[sl[px(r + 1)]]};
-            else gl.list[c][slot] = Group{tl.start + r, e - r};
+            else gl.list[c][slot] = Group{tl.start + r, size};
         }
     }
 }
