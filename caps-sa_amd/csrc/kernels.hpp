@@ -478,8 +478,11 @@ __device__ __forceinline__ void thread_keys_seq(const S *__restrict__ t, uint64_
     for (uint32_t i = 0; i < kSeqItems; ++i) out[i] = first + i < cnt ? with_id(g, keys[i], segs[i]) : 0ull;
 }
 
+// The binary-search variant is held to 128 VGPRs (four wavefronts per SIMD
+// instead of three; the allocator spills two dwords): 1.85 -> 1.60 ms on the
+// 268M text.  The router variants keep their own allocation.
 template <typename S, typename I, int RK, bool KEEP>
-__global__ __launch_bounds__(256) void k_classify_seq(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, KeyGeom g,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RK == 0 ? 4 : 1))) void k_classify_seq(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, KeyGeom g,
                                                       DevLimits L, uint64_t max_ctx, Splitters SP, RouterDev R,
                                                       uint16_t *__restrict__ cid, uint32_t *__restrict__ hblk,
                                                       uint32_t pc, uint64_t chunk, const uint64_t *__restrict__ keep) {
