@@ -87,7 +87,8 @@ class Stats(ctypes.Structure):
             "router_err_sum", "router_err_max", "router_fallbacks", "router_count",
         )
     ] + [("key_bits", ctypes.c_uint32), ("key_k", ctypes.c_uint32), ("bucket_max", ctypes.c_uint64),
-         ("k_scatter1", ctypes.c_double), ("k_scatter2", ctypes.c_double), ("batches", ctypes.c_uint64)]
+         ("k_scatter1", ctypes.c_double), ("k_scatter2", ctypes.c_double), ("batches", ctypes.c_uint64),
+         ("coarse_buckets", ctypes.c_uint64), ("coarse_max", ctypes.c_uint64)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}

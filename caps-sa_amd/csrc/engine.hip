@@ -66,9 +66,9 @@ static inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b
 // Blocks of `kernel` that fit on the device at once (one resident wave of
 // blocks), for grid-stride kernels that stage tables once per block.
 template <typename K>
-static unsigned resident_blocks(K kernel, unsigned block) {
+static unsigned resident_blocks(K kernel, unsigned block, size_t dyn_lds = 0) {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)block, 0) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)block, dyn_lds) != hipSuccess || per_cu < 1)
         per_cu = 1;
     return (unsigned)per_cu * std::max<uint32_t>(workspace().n_cu, 1);
 }
@@ -330,7 +330,7 @@ static RouterDev train_router(Workspace &W, uint32_t kind, uint32_t flags, const
     // the router predicts the position among the coarse splitters (0..n_coarse)
     const uint32_t nc = SP.n_coarse;
     uint64_t *ctab = (uint64_t *)W.r_tmp.get((uint64_t)std::max<uint32_t>(nc, 1) * 8);
-    k_coarse_table<<<(unsigned)ceil_div(std::max<uint32_t>(nc, 1), 256), 256, 0, st>>>(SP.spl, SP.stride, nc, ctab);
+    k_coarse_table<<<(unsigned)ceil_div(std::max<uint32_t>(nc, 1), 256), 256, 0, st>>>(SP, ctab);
     LAUNCHCHK();
     const uint64_t *spl = ctab;
     const uint64_t p = (uint64_t)nc + 1;
@@ -580,11 +580,25 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     // bucket fits the 2048-record tile class (sample-based splitters spread
     // bucket sizes by about a quarter around the mean)
     const uint64_t target = o.tile_keys ? std::max<uint32_t>(64, std::min<uint32_t>(o.tile_keys, T)) : kBucketTarget;
-    uint64_t p = o.gpu_buckets ? o.gpu_buckets : ceil_div(n_pos, target);
-    // at most kCoarseMax coarse intervals of kFineMax buckets each
-    p = std::max<uint64_t>(1, std::min<uint64_t>(p, std::min<uint64_t>(n_pos, (uint64_t)kCoarseMax * kFineMax + 1)));
+    // p buckets in C coarse intervals of `stride` buckets.  C is the number of
+    // classification buckets (the classify pass decides among them, by binary
+    // search or the router): gpu_buckets when set, else 1,024 (more when the
+    // intervals would exceed kFineMax buckets).  The fine buckets aim at
+    // `target` records each.
+    if (o.gpu_buckets > kCoarseBucketsMax) throw CapsError(CAPS_E_INVALID, "gpu_buckets above 4096");
+    const uint64_t want = std::max<uint64_t>(1, ceil_div(n_pos, target));
+    uint64_t C = o.gpu_buckets ? o.gpu_buckets
+                               : std::min<uint64_t>(kCoarseBucketsMax,
+                                                    std::max<uint64_t>(std::min<uint64_t>(want, 1024),
+                                                                       ceil_div(want, kFineMax)));
+    C = std::max<uint64_t>(1, std::min<uint64_t>(C, n_pos));
+    uint64_t stride = std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(want, C), n_pos / C));
+    stride = std::min<uint64_t>(stride, kFineMax);
+    uint64_t p = C * stride;
+    // sample: 16 keys per bucket and at least 1% of the positions (the
+    // router's training sample), at most all of them
     const uint64_t oversample = 16;
-    uint64_t s = p > 1 ? std::min<uint64_t>(n_pos, p * oversample) : 0;
+    uint64_t s = p > 1 ? std::min<uint64_t>(n_pos, std::max<uint64_t>(p * oversample, n_pos / 100)) : 0;
     uint64_t *spl = (uint64_t *)W.spl.get(std::max<uint64_t>(p, 1) * 8);
     uint64_t *sorted_sample = nullptr;
     if (p > 1) {
@@ -601,8 +615,8 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     Splitters SP{};
     SP.spl = spl;
     SP.n_spl = (uint32_t)(p - 1);
-    SP.stride = (uint32_t)std::max<uint64_t>(1, ceil_div(p - 1, kCoarseMax));
-    SP.n_coarse = (uint32_t)ceil_div(p - 1, SP.stride);
+    SP.stride = (uint32_t)stride;
+    SP.n_coarse = (uint32_t)(C - 1);
     while ((1ull << SP.coarse_steps) <= SP.n_coarse) ++SP.coarse_steps;
     while ((1ull << SP.fine_steps) < SP.stride) ++SP.fine_steps;
     stats.buckets = p;
@@ -611,7 +625,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     if (SP.stride > kFineMax) throw CapsError(CAPS_E_INVALID, "too many buckets");
     const uint32_t pc = SP.n_coarse + 1;
     uint16_t *cid = (uint16_t *)W.bid.get(n_pos * 2);
-    uint32_t *hist = (uint32_t *)W.hist.get(pc * 4);
+    uint64_t *hist = (uint64_t *)W.hist.get(pc * 8);
     uint64_t *coff = (uint64_t *)W.toff.get((std::max<uint64_t>(pc, p) + 1) * 8);
     uint64_t *off = (uint64_t *)W.off.get((p + 1) * 8);
     RouterDev R{};
@@ -633,14 +647,15 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     // one resident wave of classify blocks; the coarse splitter table is
     // staged once per block.
     unsigned resident = 0;
+    const size_t cls_lds = classify_lds_bytes(SP.n_coarse);
     if (seq)
-        resident = rkind == CAPS_ROUTER_PWL   ? resident_blocks(k_classify_seq<S, I, 1, false>, 256)
-                   : rkind == CAPS_ROUTER_MLP ? resident_blocks(k_classify_seq<S, I, 2, false>, 256)
-                                              : resident_blocks(k_classify_seq<S, I, 0, false>, 256);
+        resident = rkind == CAPS_ROUTER_PWL   ? resident_blocks(k_classify_seq<S, I, 1, false>, 256, cls_lds)
+                   : rkind == CAPS_ROUTER_MLP ? resident_blocks(k_classify_seq<S, I, 2, false>, 256, cls_lds)
+                                              : resident_blocks(k_classify_seq<S, I, 0, false>, 256, cls_lds);
     else
-        resident = rkind == CAPS_ROUTER_PWL   ? resident_blocks(k_classify<S, I, 1>, 256)
-                   : rkind == CAPS_ROUTER_MLP ? resident_blocks(k_classify<S, I, 2>, 256)
-                                              : resident_blocks(k_classify<S, I, 0>, 256);
+        resident = rkind == CAPS_ROUTER_PWL   ? resident_blocks(k_classify<S, I, 1>, 256, cls_lds)
+                   : rkind == CAPS_ROUTER_MLP ? resident_blocks(k_classify<S, I, 2>, 256, cls_lds)
+                                              : resident_blocks(k_classify<S, I, 0>, 256, cls_lds);
     const uint64_t chunk =
         std::max<uint64_t>(kSeqChunk, ceil_div(ceil_div(n_iter, (uint64_t)resident), kSeqChunk) * kSeqChunk);
     const unsigned nblk = (unsigned)ceil_div(n_iter, chunk);
@@ -649,7 +664,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         auto classify_seq = [&](auto rk, auto kp) {
             constexpr int RK = decltype(rk)::value;
             constexpr bool KP = decltype(kp)::value;
-            k_classify_seq<S, I, RK, KP><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, cid, hblk, pc,
+            k_classify_seq<S, I, RK, KP><<<nblk, 256, cls_lds, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, cid, hblk, pc,
                                                                chunk, d_keep);
         };
         using kno = std::false_type;
@@ -664,16 +679,16 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
             d_keep ? classify_seq(std::integral_constant<int, 0>{}, kyes{})
                    : classify_seq(std::integral_constant<int, 0>{}, kno{});
     } else if (rkind == CAPS_ROUTER_PWL)
-        k_classify<S, I, 1><<<nblk, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hblk, pc, chunk);
+        k_classify<S, I, 1><<<nblk, 256, cls_lds, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hblk, pc, chunk);
     else if (rkind == CAPS_ROUTER_MLP)
-        k_classify<S, I, 2><<<nblk, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hblk, pc, chunk);
+        k_classify<S, I, 2><<<nblk, 256, cls_lds, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hblk, pc, chunk);
     else
-        k_classify<S, I, 0><<<nblk, 256, 0, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hblk, pc, chunk);
+        k_classify<S, I, 0><<<nblk, 256, cls_lds, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hblk, pc, chunk);
     LAUNCHCHK();
     kt.end();
     k_block_offsets<<<(unsigned)ceil_div(pc, 256), 256, 0, st>>>(hblk, nblk, pc, hist);
     LAUNCHCHK();
-    k_scan_u32<<<1, 1024, 0, st>>>(hist, pc, coff);
+    k_scan_u64<<<1, 1024, 0, st>>>(hist, pc, coff);
     LAUNCHCHK();
     tm.mark(PH_CLASSIFY);
 
@@ -683,6 +698,14 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     std::vector<uint64_t> hcoff(pc + 1);
     HIPCHK(hipMemcpyAsync(hcoff.data(), coff, (pc + 1) * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    // per-interval counts and in-interval offsets are 32-bit in the scatter
+    // passes (block prefixes, LDS bucket counts)
+    uint64_t cmax = 0;
+    for (uint32_t c = 0; c < pc; ++c) cmax = std::max<uint64_t>(cmax, hcoff[c + 1] - hcoff[c]);
+    if (cmax >= (1ull << 32))
+        throw CapsError(CAPS_E_UNSUPPORTED, "a classification bucket holds 2^32 or more suffixes");
+    stats.coarse_buckets = pc;
+    stats.coarse_max = cmax;
     const BatchPlan plan_b = plan_batches(W, n_pos, o.batch_records, sizeof(KRec<I>),
                                           (o.flags & CAPS_FLAG_SCATTER_PER_BATCH) != 0);
     const uint64_t bmax = plan_b.bmax;
@@ -696,7 +719,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     const bool all_records = !single && plan_b.all_records;  // pass 1 once, into a full-size record buffer
     stats.batches = cuts.size() - 1;
     auto first_bucket = [&](uint32_t c) -> uint64_t {  // first bucket of coarse interval c
-        return c == 0 ? 0 : std::min<uint64_t>(p, (uint64_t)(c - 1) * SP.stride + 1);
+        return std::min<uint64_t>(p, (uint64_t)c * SP.stride);
     };
 
     // batch-sized buffers
@@ -737,13 +760,13 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     auto pass1 = [&](uint32_t c0, uint32_t c1, uint64_t rb) {
         kt.begin(&stats.k_scatter1);
         if (seq && d_keep)
-            k_scatter1<S, I, true, true><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_iter, cid, pc,
+            k_scatter1<S, I, true, true><<<nblk, 256, (size_t)pc * 12, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_iter, cid, pc,
                                                                coff, hblk, tmp, chunk, d_keep, c0, c1, rb);
         else if (seq)
-            k_scatter1<S, I, true, false><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_iter, cid, pc,
+            k_scatter1<S, I, true, false><<<nblk, 256, (size_t)pc * 12, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_iter, cid, pc,
                                                                 coff, hblk, tmp, chunk, nullptr, c0, c1, rb);
         else
-            k_scatter1<S, I, false, false><<<nblk, 256, 0, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, cid, pc,
+            k_scatter1<S, I, false, false><<<nblk, 256, (size_t)pc * 12, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, cid, pc,
                                                                  coff, hblk, tmp, chunk, nullptr, c0, c1, rb);
         LAUNCHCHK();
         kt.end();

@@ -137,46 +137,40 @@ __global__ void k_pick_splitters(const uint64_t *__restrict__ sorted, uint64_t s
 
 // ---------------------------------------------------------------------------
 // Bucket classification.  Bucket of a key = number of splitters <= key, so
-// equal keys always share a bucket.  It is found in two steps, which are also
-// the two levels of the scatter: the classify pass finds the key's coarse
-// interval (the number of coarse splitters <= key, coarse splitter j being
-// splitter j*stride), and the second scatter pass places each record of one
-// coarse interval into its bucket by a search over the interval's stride-1
-// inner splitters, staged in LDS.
+// equal keys always share a bucket.  The p buckets form coarse intervals of
+// `stride` consecutive buckets each (the last may be shorter): interval c
+// holds buckets [c*stride, c*stride + stride), and coarse splitter c is
+// splitter (c+1)*stride - 1, the last splitter below interval c+1.  The
+// classify pass finds a key's coarse interval (the number of coarse
+// splitters <= key), by binary search or the learned router; the second
+// scatter pass places each record of one interval into its bucket by a
+// search over the interval's stride-1 inner splitters
+// spl[c*stride .. c*stride + stride - 2], staged in LDS.  Both levels are
+// equal-depth: every bucket and every interval covers the same share of the
+// sorted sample.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kCoarseMax = 1023;         // coarse splitters (LDS table of the classify pass)
-constexpr uint32_t kCoarseBucketsMax = 1024;  // coarse intervals = kCoarseMax + 1
+constexpr uint32_t kCoarseMax = 4095;         // coarse splitters (LDS table of the classify pass)
+constexpr uint32_t kCoarseBucketsMax = 4096;  // coarse intervals = kCoarseMax + 1
 constexpr uint32_t kFineMax = 4096;           // buckets per coarse interval (the stride)
 
 struct Splitters {
     const uint64_t *spl;    // p-1 sorted keys
     uint32_t n_spl;         // p-1
-    uint32_t stride;        // coarse stride
-    uint32_t n_coarse;      // ceil(n_spl / stride)
+    uint32_t stride;        // buckets per coarse interval
+    uint32_t n_coarse;      // coarse splitters: ceil(p / stride) - 1
     uint32_t coarse_steps;  // ceil(log2(n_coarse + 1)): binary-lifting steps over the coarse table
     uint32_t fine_steps;    // ceil(log2(stride)): steps inside one coarse interval
 };
 
-// Number of splitters <= key.
-__device__ __forceinline__ uint32_t upper_bound_coarse(const Splitters &S, const uint64_t *coarse, uint64_t key) {
-    // coarse[c] = spl[c*stride]; find the number of coarse entries <= key.
-    uint32_t lo = 0, hi = S.n_coarse;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (coarse[mid] <= key) lo = mid + 1;
-        else hi = mid;
-    }
-    // all splitters before index (lo-1)*stride+1 .. are known; refine in [base, base+stride)
-    if (lo == 0) return 0;
-    uint32_t b = (lo - 1) * S.stride + 1;
-    uint32_t e = b + S.stride - 1;
-    if (e > S.n_spl) e = S.n_spl;
-    while (b < e) {
-        uint32_t mid = (b + e) >> 1;
-        if (S.spl[mid] <= key) b = mid + 1;
-        else e = mid;
-    }
-    return b;
+// Coarse splitter c (c < n_coarse).
+__device__ __forceinline__ uint64_t coarse_splitter(const Splitters &SP, uint32_t c) {
+    return SP.spl[(uint64_t)(c + 1) * SP.stride - 1];
+}
+
+// LDS of the classify kernels, sized by the launch: the coarse table, then
+// the per-interval histogram.
+__host__ __device__ constexpr uint32_t classify_lds_bytes(uint32_t n_coarse) {
+    return ((n_coarse + 1) & ~1u) * 8 + (n_coarse + 1) * 4;
 }
 
 // Packed code text: word w holds the codes of symbols w*F .. w*F+F-1
@@ -231,12 +225,10 @@ __global__ void k_pack_seed(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, 
     }
 }
 
-// The coarse splitter table: every stride-th splitter (what the learned router
-// is trained to index).
-__global__ void k_coarse_table(const uint64_t *__restrict__ spl, uint32_t stride, uint32_t n_coarse,
-                               uint64_t *__restrict__ ctab) {
+// The coarse splitter table (what the learned router is trained to index).
+__global__ void k_coarse_table(Splitters SP, uint64_t *__restrict__ ctab) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < n_coarse) ctab[c] = spl[(uint64_t)c * stride];
+    if (c < SP.n_coarse) ctab[c] = coarse_splitter(SP, c);
 }
 
 // Exact position given a predicted one in a sorted table `tab` of n keys (the
@@ -349,16 +341,17 @@ __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeM
                                                   Splitters SP, RouterDev R, uint16_t *__restrict__ cid,
                                                   uint32_t *__restrict__ hblk, uint32_t pc, uint64_t chunk) {
     constexpr uint32_t NS = kPwlSeg;
-    __shared__ uint32_t chist[kCoarseBucketsMax];
+    extern __shared__ uint64_t classify_lds[];
+    uint64_t *coarse = classify_lds;
+    uint32_t *chist = (uint32_t *)(classify_lds + ((SP.n_coarse + 1) & ~1u));
     for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) chist[c] = 0;
-    __shared__ uint64_t coarse[kCoarseMax];
     __shared__ uint64_t rknots[RK == 1 ? NS + 1 : 1];
     __shared__ float ra[RK == 1 ? NS : 1], rb[RK == 1 ? NS : 1], rinv[RK == 1 ? NS : 1];
     __shared__ uint32_t rlo[RK == 1 ? NS : 1], rhi[RK == 1 ? NS : 1];
     __shared__ uint16_t rradix[RK == 1 ? (1u << kRouterRadixBits) + 1 : 1];
     __shared__ float nib[RK == 2 ? 16 * 16 * kMlpHid : 1], w2[RK == 2 ? kMlpHid + 1 : 1];
     __shared__ unsigned long long st_sum, st_max, st_fb, st_cnt;
-    for (uint32_t c = threadIdx.x; c < SP.n_coarse; c += blockDim.x) coarse[c] = SP.spl[(uint64_t)c * SP.stride];
+    for (uint32_t c = threadIdx.x; c < SP.n_coarse; c += blockDim.x) coarse[c] = coarse_splitter(SP, c);
     if (RK == 1) {
         for (uint32_t c = threadIdx.x; c < R.ns; c += blockDim.x) {
             rknots[c] = R.knots[c];
@@ -487,17 +480,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RK == 0 ? 4
                                                       uint16_t *__restrict__ cid, uint32_t *__restrict__ hblk,
                                                       uint32_t pc, uint64_t chunk, const uint64_t *__restrict__ keep) {
     constexpr uint32_t NS = kPwlSeg;
-    __shared__ uint32_t chist[kCoarseBucketsMax];
+    extern __shared__ uint64_t classify_lds[];
+    uint64_t *coarse = classify_lds;
+    uint32_t *chist = (uint32_t *)(classify_lds + ((SP.n_coarse + 1) & ~1u));
     for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) chist[c] = 0;
     __shared__ S win[kSeqChunk + 72];
-    __shared__ uint64_t coarse[kCoarseMax];
     __shared__ uint64_t rknots[RK == 1 ? NS + 1 : 1];
     __shared__ float ra[RK == 1 ? NS : 1], rb[RK == 1 ? NS : 1], rinv[RK == 1 ? NS : 1];
     __shared__ uint32_t rlo[RK == 1 ? NS : 1], rhi[RK == 1 ? NS : 1];
     __shared__ uint16_t rradix[RK == 1 ? (1u << kRouterRadixBits) + 1 : 1];
     __shared__ float nib[RK == 2 ? 16 * 16 * kMlpHid : 1], w2[RK == 2 ? kMlpHid + 1 : 1];
     __shared__ unsigned long long st_sum, st_max, st_fb, st_cnt;
-    for (uint32_t c = threadIdx.x; c < SP.n_coarse; c += blockDim.x) coarse[c] = SP.spl[(uint64_t)c * SP.stride];
+    for (uint32_t c = threadIdx.x; c < SP.n_coarse; c += blockDim.x) coarse[c] = coarse_splitter(SP, c);
     if (RK == 1) {
         for (uint32_t c = threadIdx.x; c < R.ns; c += blockDim.x) {
             rknots[c] = R.knots[c];
@@ -568,8 +562,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RK == 0 ? 4
     }
 }
 
-// Exclusive scan of m u32 counts into m+1 u64 offsets (one workgroup).
-__global__ __launch_bounds__(1024) void k_scan_u32(const uint32_t *__restrict__ in, uint64_t m,
+// Exclusive scan of m u64 counts into m+1 u64 offsets (one workgroup).
+__global__ __launch_bounds__(1024) void k_scan_u64(const uint64_t *__restrict__ in, uint64_t m,
                                                    uint64_t *__restrict__ out) {
     __shared__ uint64_t s[1024];
     uint32_t tid = threadIdx.x;
@@ -597,15 +591,17 @@ __global__ __launch_bounds__(1024) void k_scan_u32(const uint32_t *__restrict__ 
 
 // Per-block histograms (hblk[b * pc + c]) -> exclusive prefix over blocks in
 // place (block b's first slot inside interval c, relative to the interval) and
-// interval totals in hist[c].  One thread per interval.
+// interval totals in hist[c], summed in 64 bits: an interval of 2^32 or more
+// records (which the host then refuses) cannot wrap its total.  One thread
+// per interval.
 __global__ void k_block_offsets(uint32_t *__restrict__ hblk, uint32_t nblocks, uint32_t pc,
-                                uint32_t *__restrict__ hist) {
+                                uint64_t *__restrict__ hist) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= pc) return;
-    uint32_t run = 0;
+    uint64_t run = 0;
     for (uint32_t b = 0; b < nblocks; ++b) {
         const uint32_t v = hblk[(uint64_t)b * pc + c];
-        hblk[(uint64_t)b * pc + c] = run;
+        hblk[(uint64_t)b * pc + c] = (uint32_t)run;
         run += v;
     }
     hist[c] = run;
@@ -646,8 +642,10 @@ __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint6
                                                   uint64_t chunk, const uint64_t *__restrict__ keep, uint32_t c_lo,
                                                   uint32_t c_hi, uint64_t rbase) {
     __shared__ S win[SEQ ? kSeqChunk + 72 : 1];
-    __shared__ unsigned long long cbase[kCoarseBucketsMax];
-    __shared__ uint32_t lcnt[kCoarseBucketsMax];
+    // LDS sized by the launch (pc * 12 bytes): run cursors, then counts
+    extern __shared__ unsigned long long scatter1_lds[];
+    unsigned long long *cbase = scatter1_lds;
+    uint32_t *lcnt = (uint32_t *)(scatter1_lds + pc);
     // chunk: a multiple of kSeqChunk, so key windows never straddle two blocks
     const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
     if (c0 >= n_pos) return;
@@ -758,8 +756,8 @@ __global__ __launch_bounds__(NT) void k_scatter2(const KRec<I> *__restrict__ tmp
     const uint32_t c = c_lo + blockIdx.x;
     const uint64_t s = coff[c] - rbase, e = coff[c + 1] - rbase;
     // buckets of interval c: b0 .. b0 + nb - 1; its inner splitters spl[b0 .. b0 + nb - 1)
-    const uint32_t b0 = c == 0 ? 0u : (c - 1) * SP.stride + 1;
-    const uint32_t nb = c == 0 ? 1u : (SP.stride < p - b0 ? SP.stride : p - b0);
+    const uint32_t b0 = c * SP.stride;
+    const uint32_t nb = SP.stride < p - b0 ? SP.stride : p - b0;
     const uint32_t ninner = nb - 1;
     uint32_t fsteps = 0;
     while ((1u << fsteps) <= ninner) ++fsteps;

@@ -71,9 +71,12 @@ typedef struct caps_opts {
                                   depends on it; kept for the API)                     */
     uint32_t router;           /* CAPS_ROUTER_*                                         */
     uint32_t key_geometry;     /* CAPS_KEY_*                                            */
-    uint32_t gpu_buckets;      /* 0 = auto (one bucket per tile_keys/2 suffixes)        */
+    uint32_t gpu_buckets;      /* classification buckets the classify pass (binary
+                                  search or router) decides among, 1..4096; each is
+                                  split into LDS-sized buckets by the second scatter
+                                  pass.  0 = auto (1,024, more for huge inputs)         */
     uint32_t tile_keys;        /* mean records per bucket the splitters aim at
-                                  (0 = 1280; 64..4096); ignored with gpu_buckets       */
+                                  (0 = 1280; 64..4096)                                 */
     uint32_t want_lcp;         /* fill the LCP array where an entry point takes one     */
     uint32_t flags;            /* CAPS_FLAG_* bits, 0 by default                        */
     uint64_t batch_records;    /* records per bucket batch: the build sorts consecutive
@@ -220,6 +223,8 @@ typedef struct caps_stats {
     uint64_t bucket_max; /* records in the largest bucket (imbalance = bucket_max * buckets / n) */
     double k_scatter1, k_scatter2; /* the two scatter passes (their sum is k_scatter) */
     uint64_t batches;              /* bucket batches the build ran                      */
+    uint64_t coarse_buckets;       /* classification buckets (coarse intervals)         */
+    uint64_t coarse_max;           /* records in the largest classification bucket      */
 } caps_stats_t;
 int caps_last_stats(caps_stats_t *out);
 
