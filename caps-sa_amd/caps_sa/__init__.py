@@ -88,7 +88,7 @@ class Stats(ctypes.Structure):
         )
     ] + [("key_bits", ctypes.c_uint32), ("key_k", ctypes.c_uint32), ("bucket_max", ctypes.c_uint64),
          ("k_scatter1", ctypes.c_double), ("k_scatter2", ctypes.c_double), ("batches", ctypes.c_uint64),
-         ("coarse_buckets", ctypes.c_uint64), ("coarse_max", ctypes.c_uint64)]
+         ("coarse_buckets", ctypes.c_uint64), ("coarse_max", ctypes.c_uint64), ("scratch_bytes", ctypes.c_uint64)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -142,6 +142,10 @@ _SIGNATURES = {
                                           ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(_Limits),
                                           ctypes.POINTER(_Opts), ctypes.c_void_p, ctypes.c_void_p,
                                           ctypes.c_void_p]),
+    "caps_build_device_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p,
+                                                  ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(_Limits),
+                                                  ctypes.POINTER(_Opts), ctypes.c_void_p, ctypes.c_void_p,
+                                                  ctypes.c_void_p]),
     "caps_lcp_pairs_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p,
                                               ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p]),
     "caps_suffix_cmp_pairs_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,

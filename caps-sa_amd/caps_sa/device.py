@@ -49,6 +49,25 @@ def build_device(d_text: torch.Tensor, n: int, sym_bytes: int, d_positions=None,
     return d_out, d_lcp
 
 
+def build_device_noalloc(d_text: torch.Tensor, n: int, sym_bytes: int, d_positions=None, n_pos=None, idx_bytes=4,
+                         lp: LimitProvider = None, max_context=MAX_CONTEXT, router=0, d_out=None, d_lcp=None,
+                         flags=0):
+    """caps_build_device_noalloc: build_device with every scratch buffer in
+    the library's fixed device array (no device allocation by the library;
+    the outputs are the caller's).  Returns (sa, lcp)."""
+    n_pos = n if d_positions is None else (n_pos if n_pos is not None else d_positions.numel())
+    if d_out is None:
+        d_out = torch.empty(max(n_pos, 1), dtype=_IDX[idx_bytes], device=d_text.device)
+    lim = (lp or PlainText(n))._c()
+    o = _c_opts(max_context, router=router, want_lcp=int(d_lcp is not None), flags=flags)
+    rc = lib().caps_build_device_noalloc(d_text.data_ptr(), n, sym_bytes,
+                                         d_positions.data_ptr() if d_positions is not None else None, n_pos,
+                                         idx_bytes, ctypes.byref(lim), ctypes.byref(o), d_out.data_ptr(),
+                                         d_lcp.data_ptr() if d_lcp is not None else None, _stream_ptr())
+    _check(rc)
+    return d_out, d_lcp
+
+
 def build_device_filtered(d_text: torch.Tensor, n: int, sym_bytes: int, d_keep: torch.Tensor, capacity: int,
                           idx_bytes=8, lp: LimitProvider = None, max_context=MAX_CONTEXT, router=0, d_out=None,
                           d_lcp=None, want_lcp=False, flags=0, tile_keys=0, batch_records=0):

@@ -40,6 +40,37 @@ struct CapsError : std::runtime_error {
 
 #define LAUNCHCHK() HIPCHK(hipGetLastError())
 
+// ---------------------------------------------------------------------------
+// No-alloc builds: every device buffer of the build is carved from one
+// static device array per GPU instead of hipMalloc.  Its device address is
+// looked up once per device with hipGetSymbolAddress (the array's name in
+// host code is only a host-side handle, never a device address).
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = CAPS_NOALLOC_SCRATCH_MIB << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+struct Arena {
+    unsigned char *base = nullptr;
+    size_t cap = 0, used = 0;
+    std::vector<std::pair<const DevBuf *, std::pair<void *, size_t>>> slots;  // per buffer: grow-only, as DevBuf
+    void *take(const DevBuf *owner, size_t bytes) {
+        for (auto &sl : slots)
+            if (sl.first == owner && sl.second.second >= bytes) return sl.second.first;
+        const size_t at = (used + 255) & ~(size_t)255;
+        if (at + bytes > cap)
+            throw CapsError(CAPS_E_INVALID, "the build needs more than the " + std::to_string(cap >> 20) +
+                                                " MiB no-alloc scratch");
+        used = at + bytes;
+        void *ptr = base + at;
+        bool found = false;
+        for (auto &sl : slots)
+            if (sl.first == owner) sl.second = {ptr, bytes}, found = true;
+        if (!found) slots.push_back({owner, {ptr, bytes}});
+        return ptr;
+    }
+};
+static Arena *g_arena = nullptr;  // set for the duration of a no-alloc build (under g_mu)
+
 void DevBuf::release() {
     if (p) (void)hipFree(p);
     p = nullptr;
@@ -48,6 +79,7 @@ void DevBuf::release() {
 
 void *DevBuf::get(size_t bytes) {
     if (bytes == 0) bytes = 256;
+    if (g_arena) return g_arena->take(this, bytes);
     if (bytes > cap) {
         release();
         HIPCHK(hipMalloc(&p, bytes));
@@ -439,6 +471,7 @@ struct BatchPlan {
 };
 static BatchPlan plan_batches(const Workspace &W, uint64_t n_pos, uint64_t requested, size_t rec_bytes,
                               bool per_batch_pass1) {
+    if (g_arena) return {n_pos, false};  // no-alloc builds: one batch, sized by the scratch
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || free_b == 0) return {requested ? requested : n_pos, false};
     size_t held = W.recs.cap + W.keys.cap + W.fid.cap + W.sinkbuf.cap + W.keys2.cap + W.idx2.cap + W.lcp.cap +
@@ -1338,6 +1371,51 @@ int caps_build_device(const void *d_text, uint64_t n, uint32_t sym_bytes, const 
         else if (sym_bytes == 1) run_device<uint8_t, uint64_t>(d_text, n, d_positions, n_pos, limits, o, d_out_sa, d_out_lcp, st);
         else if (idx_bytes == 4) run_device<uint16_t, uint32_t>(d_text, n, d_positions, n_pos, limits, o, d_out_sa, d_out_lcp, st);
         else run_device<uint16_t, uint64_t>(d_text, n, d_positions, n_pos, limits, o, d_out_sa, d_out_lcp, st);
+    });
+}
+
+// Device address of the no-alloc scratch array on the current device, looked
+// up once per device (each GPU holds its own copy of the array).
+static unsigned char *noalloc_scratch() {
+    static unsigned char *addr[64] = {};
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) throw CapsError(CAPS_E_UNSUPPORTED, "device ordinal above 63");
+    if (!addr[dev]) {
+        void *ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw CapsError(CAPS_E_DEVICE, "no device address for the no-alloc scratch");
+        addr[dev] = (unsigned char *)ptr;
+    }
+    return addr[dev];
+}
+
+int caps_build_device_noalloc(const void *d_text, uint64_t n, uint32_t sym_bytes, const void *d_positions,
+                              uint64_t n_pos, uint32_t idx_bytes, const caps_limits_t *limits,
+                              const caps_opts_t *opts, void *d_out_sa, void *d_out_lcp, void *stream) {
+    return guarded([&] {
+        check_common(d_text, n, sym_bytes, limits);
+        if (idx_bytes != 4 && idx_bytes != 8) throw CapsError(CAPS_E_UNSUPPORTED, "index width must be 4 or 8");
+        if (!d_positions) n_pos = n;
+        if (n > CAPS_NOALLOC_MAX_SYMBOLS || n_pos > CAPS_NOALLOC_MAX_SYMBOLS)
+            throw CapsError(CAPS_E_INVALID, "no-alloc builds take at most 2^20 symbols and positions");
+        if (limits && limits->kind == CAPS_LIMITS_SEGMENTED && limits->n_ends > CAPS_NOALLOC_MAX_ENDS)
+            throw CapsError(CAPS_E_INVALID, "no-alloc builds take at most 2^16 segment ends");
+        caps_opts_t o = opts_or_default(opts);
+        o.batch_records = 0;
+        Arena a;
+        a.base = noalloc_scratch();
+        a.cap = kNoAllocScratchBytes;
+        struct Off {
+            ~Off() { g_arena = nullptr; }
+        } off;
+        g_arena = &a;
+        hipStream_t st = (hipStream_t)stream;
+        if (sym_bytes == 1 && idx_bytes == 4) run_device<uint8_t, uint32_t>(d_text, n, d_positions, n_pos, limits, o, d_out_sa, d_out_lcp, st);
+        else if (sym_bytes == 1) run_device<uint8_t, uint64_t>(d_text, n, d_positions, n_pos, limits, o, d_out_sa, d_out_lcp, st);
+        else if (idx_bytes == 4) run_device<uint16_t, uint32_t>(d_text, n, d_positions, n_pos, limits, o, d_out_sa, d_out_lcp, st);
+        else run_device<uint16_t, uint64_t>(d_text, n, d_positions, n_pos, limits, o, d_out_sa, d_out_lcp, st);
+        g_stats.scratch_bytes = a.used;
     });
 }
 

@@ -185,6 +185,28 @@ int caps_build_device_filtered(const void *d_text, uint64_t n, uint32_t sym_byte
                                void *d_out_sa, uint64_t out_capacity, uint64_t *n_out, void *d_out_lcp,
                                void *stream);
 
+/* ---- no-alloc builder (no counterpart in the reference) ------------------- */
+
+/* For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Same
+ * arguments and results as caps_build_device, but every device buffer the
+ * build needs beyond its inputs and outputs (packed text, keys, records,
+ * splitters, group lists, merge scratch, counters) is carved from one static
+ * device array of CAPS_NOALLOC_SCRATCH_MIB MiB, which each GPU the library
+ * runs on holds until the process ends.  No hipMalloc, hipFree or allocating
+ * library routine is called.  n (and n_pos) above CAPS_NOALLOC_MAX_SYMBOLS,
+ * or more than CAPS_NOALLOC_MAX_ENDS segment ends, return CAPS_E_INVALID.
+ * Every call on a GPU shares that GPU's array, which keeps the last call's
+ * contents (the build clears what must start at zero): calls that overlap on
+ * one GPU race.  The library serializes its calls within a process and each
+ * call waits for its stream before returning. */
+#define CAPS_NOALLOC_SCRATCH_MIB 128u
+#define CAPS_NOALLOC_MAX_SYMBOLS (1ull << 20)
+#define CAPS_NOALLOC_MAX_ENDS (1ull << 16)
+int caps_build_device_noalloc(const void *d_text, uint64_t n, uint32_t sym_bytes, const void *d_positions,
+                              uint64_t n_pos, uint32_t idx_bytes, const caps_limits_t *limits,
+                              const caps_opts_t *opts, void *d_out_sa, void *d_out_lcp, void *stream);
+
 /* ---- batched builds: many independent texts in one pipeline -------------- */
 
 /* One text of a batch (host pointers).  out_sa receives n entries of
@@ -225,6 +247,7 @@ typedef struct caps_stats {
     uint64_t batches;              /* bucket batches the build ran                      */
     uint64_t coarse_buckets;       /* classification buckets (coarse intervals)         */
     uint64_t coarse_max;           /* records in the largest classification bucket      */
+    uint64_t scratch_bytes;        /* no-alloc builds: bytes of the scratch array used  */
 } caps_stats_t;
 int caps_last_stats(caps_stats_t *out);
 
