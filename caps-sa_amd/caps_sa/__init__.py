@@ -58,7 +58,20 @@ class _Opts(ctypes.Structure):
         ("want_lcp", ctypes.c_uint32),
         ("flags", ctypes.c_uint32),
         ("batch_records", ctypes.c_uint64),
+        ("packed_prefix_seed", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("remap_budget", ctypes.c_uint64),
     ]
+
+
+class SeedParams(ctypes.Structure):
+    """caps_seed_params_t: the reference's packed-seed decision."""
+
+    _fields_ = [("reason", ctypes.c_uint32), ("bits", ctypes.c_uint32), ("k", ctypes.c_uint32),
+                ("alphabet", ctypes.c_uint32), ("remap", ctypes.c_uint32)]
+
+
+SEED_OK, SEED_DISABLED, SEED_MAX_CONTEXT, SEED_NOT_U8, SEED_GAPPED_ALPHABET, SEED_REMAP_BUDGET, SEED_NO_SENTINEL = range(7)
 
 
 class TuneReport(ctypes.Structure):
@@ -142,6 +155,9 @@ _SIGNATURES = {
                                           ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(_Limits),
                                           ctypes.POINTER(_Opts), ctypes.c_void_p, ctypes.c_void_p,
                                           ctypes.c_void_p]),
+    "caps_seed_params_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                                                ctypes.POINTER(_Limits), ctypes.POINTER(_Opts),
+                                                ctypes.POINTER(SeedParams), ctypes.c_void_p]),
     "caps_build_device_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p,
                                                   ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(_Limits),
                                                   ctypes.POINTER(_Opts), ctypes.c_void_p, ctypes.c_void_p,
@@ -297,9 +313,10 @@ class StarSegmented(SegmentedText):
 # Options
 # ---------------------------------------------------------------------------
 class PackedPrefixSeedPolicy(enum.Enum):
-    """pack.rs:37-69.  On the GPU every build seeds with packed keys; the
-    policy picks the key geometry (Disabled/DenseAlphabetOnly: dense fields;
-    Remap: the reference geometry where eligible).  Output never changes."""
+    """pack.rs:37-69.  On the GPU every build seeds with packed keys: the
+    reference's geometry where the reference would seed
+    (seed_params_device; the remap budget is ExtMemOpts.remap_budget),
+    dense fields otherwise.  Output never changes."""
 
     DISABLED = 0
     DENSE_ALPHABET_ONLY = 1
@@ -389,6 +406,7 @@ class ExtMemOpts:
     max_context: int = MAX_CONTEXT
     subproblem_count: int = 0
     packed_prefix_seed: PackedPrefixSeedPolicy = PackedPrefixSeedPolicy.DISABLED
+    remap_budget: int = 0  # PackedPrefixSeedPolicy::Remap { max_extra_bytes }
     router: Router = Router.BSEARCH
     gpu_buckets: int = 0
     tile_keys: int = 0
@@ -409,8 +427,10 @@ class ExtMemOpts:
         v = _env_usize("CAPS_SA_MAX_CONTEXT")
         if v is not None:
             o.max_context = v
-        if _env_usize("CAPS_SA_PACKED_PREFIX_REMAP_BYTES") is not None:
+        v = _env_usize("CAPS_SA_PACKED_PREFIX_REMAP_BYTES")
+        if v is not None:
             o.packed_prefix_seed = PackedPrefixSeedPolicy.REMAP
+            o.remap_budget = v
         elif _env_bool("CAPS_SA_PACKED_PREFIX_SEED"):
             o.packed_prefix_seed = PackedPrefixSeedPolicy.DENSE_ALPHABET_ONLY
         if _env_bool("CAPS_SA_GEOMETRIC_MEMO"):
@@ -443,7 +463,7 @@ TUNE_MAX = 16
 
 
 def _c_opts(max_context=MAX_CONTEXT, subproblem_count=0, router=0, key_geometry=0, gpu_buckets=0, tile_keys=0,
-            want_lcp=0, flags=0, batch_records=0) -> _Opts:
+            want_lcp=0, flags=0, batch_records=0, packed_prefix_seed=0, remap_budget=0) -> _Opts:
     o = _Opts()
     lib().caps_opts_default(ctypes.byref(o))
     o.max_context = min(int(max_context), MAX_CONTEXT)
@@ -455,14 +475,17 @@ def _c_opts(max_context=MAX_CONTEXT, subproblem_count=0, router=0, key_geometry=
     o.want_lcp = int(want_lcp)
     o.flags = int(flags)
     o.batch_records = int(batch_records)
+    o.packed_prefix_seed = int(packed_prefix_seed)
+    o.remap_budget = int(remap_budget)
     return o
 
 
 def _ext_c_opts(opts: Optional[ExtMemOpts]) -> _Opts:
     opts = opts or ExtMemOpts()
-    geom = 1 if opts.packed_prefix_seed == PackedPrefixSeedPolicy.REMAP else 0
+    geom = 0 if opts.packed_prefix_seed == PackedPrefixSeedPolicy.DISABLED else 1
     return _c_opts(opts.max_context, opts.subproblem_count, int(opts.router), geom, opts.gpu_buckets,
-                   opts.tile_keys, batch_records=opts.batch_records)
+                   opts.tile_keys, batch_records=opts.batch_records,
+                   packed_prefix_seed=opts.packed_prefix_seed.value, remap_budget=opts.remap_budget)
 
 
 def _as_text(text) -> np.ndarray:

@@ -122,6 +122,20 @@ def seed_keys(d_text: torch.Tensor, n: int, sym_bytes: int, positions: np.ndarra
     return keys.cpu().numpy().view(np.uint64)[:count], (geo[0], geo[1])
 
 
+def seed_params(d_text: torch.Tensor, n: int, sym_bytes: int, policy=0, remap_budget=0, max_context=MAX_CONTEXT,
+                lp=None):
+    """caps_seed_params_device: the reference's packed-seed decision
+    (ext_mem.rs:1495-1513, pack.rs:217-234) as a dict."""
+    from . import SeedParams
+
+    out = SeedParams()
+    lim = (lp or PlainText(n))._c()
+    o = _c_opts(max_context, packed_prefix_seed=policy, remap_budget=remap_budget)
+    _check(lib().caps_seed_params_device(d_text.data_ptr(), n, sym_bytes, ctypes.byref(lim), ctypes.byref(o),
+                                         ctypes.byref(out), _stream_ptr()))
+    return {name: getattr(out, name) for name, _ in out._fields_}
+
+
 MLP_IN, MLP_HID = 64, 32
 MLP_PARAMS = MLP_IN * MLP_HID + MLP_HID + MLP_HID + 1
 

@@ -155,11 +155,54 @@ DevLimits upload_limits(Workspace &W, const caps_limits_t *lim, uint64_t n, hipS
 }
 
 // ---------------------------------------------------------------------------
-// Key geometry from the alphabet (pack.rs:103-176 for the reference form).
+// Whether the reference seeds a build with packed keys: the checks of
+// packed_prefix_seed_params (ext_mem.rs:1495-1513) and seed_params /
+// Packer::new / has_sentinel (pack.rs:103-176, 217-234), in the reference's
+// order.  The cheap checks come first; the alphabet scan only after them.
 // ---------------------------------------------------------------------------
+struct SeedDecision {
+    uint32_t reason = CAPS_SEED_OK, bits = 0, k = 0, alphabet = 0, remap = 0;
+};
+
+static SeedDecision seed_checks_before_scan(uint32_t policy, uint64_t max_ctx, uint32_t sym_bytes) {
+    SeedDecision d;
+    if (policy == CAPS_SEED_POLICY_DISABLED) d.reason = CAPS_SEED_DISABLED;
+    else if (max_ctx != ~0ull) d.reason = CAPS_SEED_MAX_CONTEXT;
+    else if (sym_bytes != 1) d.reason = CAPS_SEED_NOT_U8;
+    // boundary_rank(): every limits descriptor declares one (ShorterFirst or
+    // LongerFirst), so the provider check always passes here
+    return d;
+}
+
+static void seed_checks_after_scan(SeedDecision &d, uint32_t policy, uint64_t budget, uint64_t n, uint32_t distinct,
+                                   bool identity) {
+    d.alphabet = distinct;
+    if (!identity) {  // the ranked copy: Remap only, within its byte budget
+        if (policy != CAPS_SEED_POLICY_REMAP) {
+            d.reason = CAPS_SEED_GAPPED_ALPHABET;
+            return;
+        }
+        if (n > budget) {
+            d.reason = CAPS_SEED_REMAP_BUDGET;
+            return;
+        }
+        d.remap = 1;
+    }
+    const uint32_t bits = distinct <= 1 ? 1 : distinct <= 3 ? 2 : distinct <= 15 ? 4 : 8;
+    if ((uint64_t)distinct >= (1ull << bits)) {  // no sentinel code beside the alphabet
+        d.reason = CAPS_SEED_NO_SENTINEL;
+        d.remap = 0;
+        return;
+    }
+    d.bits = bits;
+    d.k = 64 / bits;
+}
+
+// Distinct values of the text (one device scan into a presence bitmap) and
+// whether they are already the dense range 0..distinct.
 template <typename S>
-Geometry compute_geometry(Workspace &W, const S *d_text, uint64_t n, uint32_t mode, uint32_t order, hipStream_t st,
-                          bool pow2_seed = false) {
+static void scan_alphabet(Workspace &W, const S *d_text, uint64_t n, hipStream_t st, std::vector<uint32_t> &code,
+                          uint32_t &distinct, uint32_t &maxv, bool &identity) {
     constexpr uint32_t WORDS = sizeof(S) == 1 ? 8 : 2048;
     uint32_t *bm = (uint32_t *)W.bitmap.get(WORDS * 4);
     HIPCHK(hipMemsetAsync(bm, 0, WORDS * 4, st));
@@ -170,9 +213,9 @@ Geometry compute_geometry(Workspace &W, const S *d_text, uint64_t n, uint32_t mo
     std::vector<uint32_t> h(WORDS);
     HIPCHK(hipMemcpyAsync(h.data(), bm, WORDS * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    uint32_t distinct = 0, maxv = 0;
-    bool identity = true;
-    std::vector<uint32_t> code(WORDS * 32, 0);
+    distinct = 0, maxv = 0;
+    identity = true;
+    code.assign(WORDS * 32, 0);
     for (uint32_t v = 0; v < WORDS * 32; ++v)
         if (h[v >> 5] >> (v & 31) & 1) {
             code[v] = distinct;
@@ -180,16 +223,32 @@ Geometry compute_geometry(Workspace &W, const S *d_text, uint64_t n, uint32_t mo
             ++distinct;
             maxv = v;
         }
+}
+
+// ---------------------------------------------------------------------------
+// Key geometry from the alphabet (pack.rs:103-176 for the reference form).
+// mode CAPS_KEY_REFERENCE: the reference's packed keys when the seed checks
+// above pass for (policy, budget, max_ctx), dense keys otherwise.
+// ---------------------------------------------------------------------------
+template <typename S>
+Geometry compute_geometry(Workspace &W, const S *d_text, uint64_t n, uint32_t mode, uint32_t order, hipStream_t st,
+                          bool pow2_seed = false, uint32_t policy = CAPS_SEED_POLICY_REMAP, uint64_t budget = ~0ull,
+                          uint64_t max_ctx = ~0ull) {
+    std::vector<uint32_t> code;
+    uint32_t distinct = 0, maxv = 0;
+    bool identity = true;
+    scan_alphabet<S>(W, d_text, n, st, code, distinct, maxv, identity);
     Geometry G{};
     G.distinct = distinct;
     bool longer_first = order == CAPS_ORDER_STAR;
     bool ref = false;
-    if (mode == CAPS_KEY_REFERENCE && sizeof(S) == 1) {
-        uint32_t bits = distinct <= 1 ? 1 : distinct <= 3 ? 2 : distinct <= 15 ? 4 : 8;
-        if ((uint64_t)distinct < (1ull << bits)) {  // has_sentinel (pack.rs:174-176)
+    if (mode == CAPS_KEY_REFERENCE) {
+        SeedDecision d = seed_checks_before_scan(policy, max_ctx, sizeof(S));
+        if (d.reason == CAPS_SEED_OK) seed_checks_after_scan(d, policy, budget, n, distinct, identity);
+        if (d.reason == CAPS_SEED_OK) {
             ref = true;
-            G.g.bits = bits;
-            G.g.k = 64 / bits;
+            G.g.bits = d.bits;
+            G.g.k = d.k;
             G.g.shift = 0;
             G.g.bias = longer_first ? 0 : 1;
             G.g.pad = longer_first ? distinct : 0;
@@ -542,6 +601,16 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         tm.collect(stats);
         return;
     }
+    if (d_pos) {  // every position must address the text: refuse the build before any key gather
+        unsigned int *bad = (unsigned int *)W.poscheck.get(4);
+        unsigned int hbad = 0;
+        HIPCHK(hipMemsetAsync(bad, 0, 4, st));
+        k_positions_check<I><<<grid_for(n_pos, 256, 8192), 256, 0, st>>>(d_pos, n_pos, n, bad);
+        LAUNCHCHK();
+        HIPCHK(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (hbad) throw CapsError(CAPS_E_INVALID, "position out of range");
+    }
     if (d_pos && (const void *)d_pos == (const void *)d_out) {
         I *cp = (I *)W.pos_copy.get(n_pos * sizeof(I));
         HIPCHK(hipMemcpyAsync(cp, d_pos, n_pos * sizeof(I), hipMemcpyDeviceToDevice, st));
@@ -563,7 +632,8 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
 
     // ---- alphabet and key geometry ----
     Geometry G = compute_geometry<S>(W, d_text, n, idbits ? CAPS_KEY_DENSE : o.key_geometry, L.order, st,
-                                     (o.flags & CAPS_FLAG_POW2_SEED) != 0);
+                                     (o.flags & CAPS_FLAG_POW2_SEED) != 0, o.packed_prefix_seed, o.remap_budget,
+                                     o.max_context);
     KeyGeom gref = G.g;  // refinement chunks: packed-text field width, full width, no text index
     if (!G.reference) {
         gref.bits = G.pbits;
@@ -1181,6 +1251,12 @@ int caps_build_in_memory_for_positions(const void *text, uint64_t n, uint32_t sy
         if (!out_sa && n_pos) throw CapsError(CAPS_E_INVALID, "out_sa is NULL");
         if (idx_bytes != 4 && idx_bytes != 8) throw CapsError(CAPS_E_UNSUPPORTED, "index width must be 4 or 8");
         if (idx_bytes == 4 && n > (1ull << 32)) throw CapsError(CAPS_E_INVALID, "u32 index cannot address text");
+        if (!positions && n_pos != n) throw CapsError(CAPS_E_INVALID, "positions is NULL but n_pos != n");
+        if (positions)
+            for (uint64_t i = 0; i < n_pos; ++i) {
+                const uint64_t q = idx_bytes == 4 ? ((const uint32_t *)positions)[i] : ((const uint64_t *)positions)[i];
+                if (q >= n) throw CapsError(CAPS_E_INVALID, "position out of range");
+            }
         caps_opts_t o = opts_or_default(opts);
         if (sym_bytes == 1 && idx_bytes == 4) build_host<uint8_t, uint32_t>(text, n, positions, n_pos, limits, o, out_sa, out_lcp);
         else if (sym_bytes == 1) build_host<uint8_t, uint64_t>(text, n, positions, n_pos, limits, o, out_sa, out_lcp);
@@ -1547,6 +1623,27 @@ int caps_suffix_cmp_pairs_device(const void *d_text, uint64_t n, uint32_t sym_by
         else k_cmp_pairs<uint16_t><<<gr, 256, 0, st>>>((const uint16_t *)d_text, L, d_pairs, count, max_ctx, d_out);
         LAUNCHCHK();
         HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
+int caps_seed_params_device(const void *d_text, uint64_t n, uint32_t sym_bytes, const caps_limits_t *limits,
+                            const caps_opts_t *opts, caps_seed_params_t *out, void *stream) {
+    return guarded([&] {
+        if (!out) throw CapsError(CAPS_E_INVALID, "out is NULL");
+        check_common(d_text, n, sym_bytes, limits);
+        const caps_opts_t o = opts_or_default(opts);
+        if (o.packed_prefix_seed > CAPS_SEED_POLICY_REMAP) throw CapsError(CAPS_E_INVALID, "unknown seed policy");
+        SeedDecision d = seed_checks_before_scan(o.packed_prefix_seed, o.max_context, sym_bytes);
+        if (d.reason == CAPS_SEED_OK) {  // u8 text: scan its alphabet once
+            Workspace &W = workspace();
+            std::vector<uint32_t> code;
+            uint32_t distinct = 0, maxv = 0;
+            bool identity = true;
+            scan_alphabet<uint8_t>(W, (const uint8_t *)d_text, n, (hipStream_t)stream, code, distinct, maxv,
+                                   identity);
+            seed_checks_after_scan(d, o.packed_prefix_seed, o.remap_budget, n, distinct, identity);
+        }
+        out->reason = d.reason, out->bits = d.bits, out->k = d.k, out->alphabet = d.alphabet, out->remap = d.remap;
     });
 }
 

@@ -127,6 +127,18 @@ __global__ __launch_bounds__(256) void k_random_keys(const S *__restrict__ t, Co
     keys[i] = with_id(g, key_at_seed(t, cm, g, p, lim < max_ctx ? lim : max_ctx), seg);
 }
 
+// Sets *bad when any of the positions is n or more (one atomic per
+// wavefront that finds one).
+template <typename I>
+__global__ __launch_bounds__(256) void k_positions_check(const I *__restrict__ pos, uint64_t n_pos, uint64_t n,
+                                                         unsigned int *__restrict__ bad) {
+    bool out = false;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pos;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        out |= (uint64_t)pos[i] >= n;
+    if (__ballot(out) && (threadIdx.x & 63) == 0) atomicOr(bad, 1u);
+}
+
 // splitter j-1 = sorted_sample[(j * s) / p], j = 1..p-1 (ext_mem.rs:1341).
 __global__ void k_pick_splitters(const uint64_t *__restrict__ sorted, uint64_t s, uint32_t p,
                                  uint64_t *__restrict__ spl) {

@@ -162,6 +162,10 @@ struct ExtMemOpts {
     uint64_t subproblem_count = 0;
     uint32_t router = CAPS_ROUTER_BSEARCH;
     uint64_t batch_records = 0;  // GPU knob: records per bucket batch (0: auto)
+    // PackedPrefixSeedPolicy (pack.rs:37-69): CAPS_SEED_POLICY_*, with the
+    // Remap byte budget
+    uint32_t packed_prefix_seed = CAPS_SEED_POLICY_DISABLED;
+    uint64_t remap_budget = 0;
     std::optional<GeometricMemoizationConfig> lcp_memoization;
     bool collect_lcp_memoization_stats = false;
     // ExtMemOpts::from_env (ext_mem.rs:154-196) for the variables with a GPU
@@ -170,6 +174,12 @@ struct ExtMemOpts {
         ExtMemOpts o;
         if (auto v = detail::env_usize("CAPS_SA_SUBPROBLEMS")) o.subproblem_count = *v;
         if (auto v = detail::env_usize("CAPS_SA_MAX_CONTEXT")) o.max_context = *v;
+        if (auto v = detail::env_usize("CAPS_SA_PACKED_PREFIX_REMAP_BYTES")) {
+            o.packed_prefix_seed = CAPS_SEED_POLICY_REMAP;
+            o.remap_budget = *v;
+        } else if (detail::env_bool("CAPS_SA_PACKED_PREFIX_SEED")) {
+            o.packed_prefix_seed = CAPS_SEED_POLICY_DENSE_ALPHABET_ONLY;
+        }
         if (detail::env_bool("CAPS_SA_GEOMETRIC_MEMO")) {
             GeometricMemoizationConfig c;
             if (auto v = detail::env_usize("CAPS_SA_MEMO_PROBE"); v && *v) c = c.with_probe_symbols(*v);
@@ -240,6 +250,9 @@ inline caps_opts_t c_opts(const ExtMemOpts &e) {
     o.subproblem_count = e.subproblem_count;
     o.router = e.router;
     o.batch_records = e.batch_records;
+    o.packed_prefix_seed = e.packed_prefix_seed;
+    o.remap_budget = e.remap_budget;
+    o.key_geometry = e.packed_prefix_seed == CAPS_SEED_POLICY_DISABLED ? CAPS_KEY_DENSE : CAPS_KEY_REFERENCE;
     return o;
 }
 inline void check(int rc) {

@@ -83,7 +83,18 @@ typedef struct caps_opts {
                                   coarse intervals of at most this many suffixes at a
                                   time (0 = auto: one batch while its buffers fit in
                                   HBM).  The output never changes.                     */
+    uint32_t packed_prefix_seed; /* ExtMemOpts::packed_prefix_seed (pack.rs:37-69):
+                                  CAPS_SEED_POLICY_*; with key_geometry
+                                  CAPS_KEY_REFERENCE the build seeds with the
+                                  reference's packed keys exactly when the
+                                  reference would (caps_seed_params_device), else
+                                  with dense keys.  The output never changes.     */
+    uint32_t reserved;
+    uint64_t remap_budget;     /* PackedPrefixSeedPolicy::Remap { max_extra_bytes }  */
 } caps_opts_t;
+#define CAPS_SEED_POLICY_DISABLED 0u
+#define CAPS_SEED_POLICY_DENSE_ALPHABET_ONLY 1u
+#define CAPS_SEED_POLICY_REMAP 2u
 /* Skip router training (the learned router runs with its initial parameters;
  * used to show what training gains).  The output never changes. */
 #define CAPS_FLAG_ROUTER_UNTRAINED 1u
@@ -130,7 +141,9 @@ int caps_build_in_memory(const void *text, uint64_t n, uint32_t sym_bytes, uint3
 
 /* build_in_memory_for_positions(_with_opts|_with) (sample_sort.rs:143-206):
  * sorts `positions` (n_pos entries of idx_bytes) by their suffixes.  out_sa
- * may alias positions (the reference reuses the moved-in Vec). */
+ * may alias positions (the reference reuses the moved-in Vec).  A position of
+ * n or more, or positions == NULL with n_pos != n, returns CAPS_E_INVALID
+ * before any device work (the reference panics on the out-of-range index). */
 int caps_build_in_memory_for_positions(const void *text, uint64_t n, uint32_t sym_bytes, const void *positions,
                                        uint64_t n_pos, uint32_t idx_bytes, const caps_limits_t *limits,
                                        const caps_opts_t *opts, void *out_sa, void *out_lcp);
@@ -171,7 +184,8 @@ int caps_build_ext_mem_for_filter(const void *text, uint64_t n, uint32_t sym_byt
 /* ---- device-resident builder (inputs and outputs already in HBM) ---------- */
 
 /* Same as caps_build_in_memory_for_positions with device pointers.
- * d_positions == NULL means 0..n-1.  d_out_lcp nullable. */
+ * d_positions == NULL means 0..n-1.  d_out_lcp nullable.  The positions are
+ * checked on the device first: one of n or more returns CAPS_E_INVALID. */
 int caps_build_device(const void *d_text, uint64_t n, uint32_t sym_bytes, const void *d_positions, uint64_t n_pos,
                       uint32_t idx_bytes, const caps_limits_t *limits, const caps_opts_t *opts, void *d_out_sa,
                       void *d_out_lcp, void *stream);
@@ -287,6 +301,31 @@ int caps_lcp_pairs_device(const void *d_text, uint64_t n, uint32_t sym_bytes, co
 int caps_suffix_cmp_pairs_device(const void *d_text, uint64_t n, uint32_t sym_bytes, const caps_limits_t *limits,
                                  const uint64_t *d_pairs, uint64_t count, uint64_t max_ctx, int32_t *d_out,
                                  void *stream);
+
+/* packed_prefix_seed_params (ext_mem.rs:1495-1513) then seed_params
+ * (pack.rs:217-234): whether the reference seeds a build of this text with
+ * packed keys, checked in the reference's order, stopping at the first check
+ * that fails: the policy is Disabled; max_context is bounded; the symbol type
+ * is not u8; (boundary_rank: every descriptor here declares one); the
+ * alphabet (one scan of the device text) is not the identity map and the
+ * policy is DenseAlphabetOnly, or Remap over its byte budget; no sentinel
+ * code fits beside the alphabet (pack.rs:174-176).  out receives the reason
+ * (CAPS_SEED_*) and, when eligible, the geometry. */
+#define CAPS_SEED_OK 0u
+#define CAPS_SEED_DISABLED 1u
+#define CAPS_SEED_MAX_CONTEXT 2u
+#define CAPS_SEED_NOT_U8 3u
+#define CAPS_SEED_GAPPED_ALPHABET 4u
+#define CAPS_SEED_REMAP_BUDGET 5u
+#define CAPS_SEED_NO_SENTINEL 6u
+typedef struct caps_seed_params {
+    uint32_t reason;   /* CAPS_SEED_*                                   */
+    uint32_t bits, k;  /* field width and symbols per key when eligible */
+    uint32_t alphabet; /* distinct symbols (0 when the scan was skipped) */
+    uint32_t remap;    /* 1 when the ranked copy of the text is needed  */
+} caps_seed_params_t;
+int caps_seed_params_device(const void *d_text, uint64_t n, uint32_t sym_bytes, const caps_limits_t *limits,
+                            const caps_opts_t *opts, caps_seed_params_t *out, void *stream);
 
 /* Packer::key_at_bounded (pack.rs:187-203) for `count` positions, in the
  * geometry `key_geometry` (u8 text).  rank: 0 shorter first, 1 longer first.

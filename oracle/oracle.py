@@ -171,6 +171,21 @@ class Ctx:
             return rc
         return np.concatenate(out) if out else np.zeros(0, dtype=np.uint64)
 
+    def seed_subarray(self, packer, rank, positions):
+        """pack.rs:241-321 on a u8 text: (sorted positions, adjacent LCPs)."""
+        sa = _u64(positions).copy()
+        lcp = np.zeros(len(sa), dtype=np.uint64)
+        lib().orc_seed_subarray(ctypes.byref(self.T), ctypes.byref(self.L), ctypes.byref(packer.P), rank,
+                                sa.ctypes.data if len(sa) else None, lcp.ctypes.data if len(sa) else None, len(sa))
+        return sa, lcp
+
+    def upper_bound_by_pivot(self, recs, pivot, max_ctx=MAX_CTX):
+        """ext_mem.rs:1705-1734: first index of a sorted run whose suffix is
+        strictly greater than the pivot suffix."""
+        recs = _u64(recs)
+        return lib().orc_upper_bound_by_pivot(ctypes.byref(self.T), ctypes.byref(self.L),
+                                              recs.ctypes.data if len(recs) else None, len(recs), pivot, max_ctx)
+
     def brute_sa(self, positions=None, max_ctx=MAX_CTX):
         sa = _u64(np.arange(self.n, dtype=np.uint64) if positions is None else positions).copy()
         lib().orc_brute_sa(ctypes.byref(self.T), ctypes.byref(self.L), sa.ctypes.data, len(sa), max_ctx)

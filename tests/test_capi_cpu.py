@@ -33,7 +33,7 @@ def test_python_binding_covers_every_declared_symbol():
 
 def test_struct_layouts():
     assert ctypes.sizeof(caps_sa._Limits) == 24
-    assert ctypes.sizeof(caps_sa._Opts) == 48
+    assert ctypes.sizeof(caps_sa._Opts) == 64
 
 
 def test_opts_default():
@@ -55,6 +55,26 @@ def test_argument_validation_before_device():
     assert _call_invalid(idx_bytes=2) == caps_sa.CAPS_E_UNSUPPORTED
     assert _call_invalid(text=None) == caps_sa.CAPS_E_INVALID
     assert "NULL" in caps_sa.lib().caps_last_error().decode()
+
+
+def test_positions_are_checked_before_device():
+    """Host positions of n or more, or NULL positions with n_pos != n, are
+    refused with CAPS_E_INVALID before any device work (so also without a
+    GPU); the reference panics on such an index."""
+    t = np.frombuffer(b"abcdabcd", dtype=np.uint8)
+    out = np.zeros(8, dtype=np.uint64)
+    lim = caps_sa.PlainText(8)._c()
+    o = caps_sa._c_opts()
+    L = caps_sa.lib()
+    for idx_bytes, dt in ((4, np.uint32), (8, np.uint64)):
+        pos = np.array([0, 3, 8], dtype=dt)
+        rc = L.caps_build_in_memory_for_positions(t.ctypes.data, 8, 1, pos.ctypes.data, 3, idx_bytes,
+                                                  ctypes.byref(lim), ctypes.byref(o), out.ctypes.data, None)
+        assert rc == caps_sa.CAPS_E_INVALID
+        assert "out of range" in L.caps_last_error().decode()
+    rc = L.caps_build_in_memory_for_positions(t.ctypes.data, 8, 1, None, 5, 4, ctypes.byref(lim), ctypes.byref(o),
+                                              out.ctypes.data, None)
+    assert rc == caps_sa.CAPS_E_INVALID
 
 
 def test_signed_symbols_are_unsupported():

@@ -86,6 +86,53 @@ def test_reference_geometry_ineligible(cuda):
     assert e.value.code == caps_sa.CAPS_E_UNSUPPORTED
 
 
+def test_seed_eligibility_chain_matches_oracle(cuda):
+    """caps_seed_params_device against the oracle's seed_params
+    (pack.rs:217-234) on the pack.rs:484-528 texts, with the order of
+    ext_mem.rs:1495-1513 pinned as ext_mem.rs:2534-2567 pins it: an earlier
+    failing check wins over a later one."""
+    D = lambda t: dev.text_to_device(np.asarray(t, dtype=np.uint8))
+    gapped = [0, 2, 0, 2, 0, 2, 0, 2]
+    cases = [
+        (list(range(255)), 1, 0),  # 255 values: 8-bit fields, k = 8
+        (list(range(256)), 1, 0),  # 256 values: no sentinel code
+        (gapped, 1, 0),  # DenseAlphabetOnly declines a gapped alphabet
+        (gapped, 2, 7),  # Remap over its budget
+        (gapped, 2, 8),  # an exactly budgeted Remap
+        ([0, 1, 2, 3] * 100, 1, 0),
+        ([0, 1, 2] * 50, 2, 0),  # identity alphabet: no copy, any budget
+        ([7, 9, 200] * 40, 2, 1000),
+    ]
+    for text, policy, budget in cases:
+        got = dev.seed_params(D(text), len(text), 1, policy=policy, remap_budget=budget)
+        pk = O.OraclePacker(np.asarray(text, dtype=np.uint8), policy=policy, remap_budget=budget)
+        assert (got["reason"] == caps_sa.SEED_OK) == pk.ok, (text[:4], policy, budget, got)
+        if pk.ok:
+            assert (got["bits"], got["k"]) == (pk.bits, pk.k)
+        assert got["alphabet"] == len(set(text))
+    assert dev.seed_params(D(list(range(256))), 256, 1, policy=1)["reason"] == caps_sa.SEED_NO_SENTINEL
+    assert dev.seed_params(D(gapped), 8, 1, policy=1)["reason"] == caps_sa.SEED_GAPPED_ALPHABET
+    assert dev.seed_params(D(gapped), 8, 1, policy=2, remap_budget=7)["reason"] == caps_sa.SEED_REMAP_BUDGET
+    # order: Disabled before max_context before the symbol type before the scan
+    t4 = D([0, 1, 2, 3])
+    r = lambda **kw: dev.seed_params(t4, 4, 1, **kw)["reason"]
+    assert r(policy=0) == caps_sa.SEED_DISABLED
+    assert r(policy=0, max_context=3) == caps_sa.SEED_DISABLED
+    assert r(policy=1, max_context=3) == caps_sa.SEED_MAX_CONTEXT
+    assert r(policy=1) == caps_sa.SEED_OK
+    t16 = dev.text_to_device(np.array([0, 1, 2, 3], dtype=np.uint16))
+    assert dev.seed_params(t16, 4, 2, policy=1)["reason"] == caps_sa.SEED_NOT_U8
+    assert dev.seed_params(t16, 4, 2, policy=1, max_context=3)["reason"] == caps_sa.SEED_MAX_CONTEXT
+    # a build under every policy is the same suffix array
+    t = gen.markov_repeats(20000, seed=3, mean_len=100.0)
+    want = O.Ctx(t).build_in_memory()
+    for pol in caps_sa.PackedPrefixSeedPolicy:
+        opts = caps_sa.ExtMemOpts(packed_prefix_seed=pol, remap_budget=1 << 20)
+        got = []
+        caps_sa.build_in_memory_sample_sort(t, opts, got.append)
+        assert np.array_equal(np.array(got, dtype=np.uint64), want), pol
+
+
 def test_dense_keys_order_property(cuda):
     """Dense (weakly order-preserving) keys: key_a < key_b must imply
     suffix_a < suffix_b under the comparator, for both boundary orders."""

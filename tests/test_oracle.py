@@ -242,16 +242,6 @@ def test_packer_kats():
     assert p.shared_fields(1 << 63, 0) == 0
 
 
-def test_packer_k32_depths():
-    # pack.rs:531-563: 3 symbols -> 2-bit fields, k = 32; seeds exact at k-1, k, k+1
-    k = 32
-    segs = [[0] * (k - 1) + [1], [0] * (k - 1) + [2], [1] * k + [0], [1] * k + [2], [2] * (k + 1) + [0],
-            [2] * (k + 1) + [1], [0] * (k + 2), [0] * (k + 2)]
-    text = np.array(sum(segs, []), dtype=np.uint8)
-    p = O.OraclePacker(text)
-    assert p.k == k and p.bits == 2
-
-
 def test_checks_catch_a_planted_bug(monkeypatch):
     """A plausible mistake (an off-by-one in the LCP cap) must fail the
     checks used above: brute force and the LCP postcondition."""
@@ -284,3 +274,176 @@ def test_verify_against_keep_bitmap():
     outside = sa.copy()
     outside[0] = int(np.flatnonzero(text >= 4)[0])
     assert ctx.verify_keep(outside, keep, threads=4) == len(sa)
+
+
+# ---- seed_subarray (pack.rs:241-321) and upper_bound_by_pivot
+# (ext_mem.rs:1705-1734), pinned by a pure-Python comparator ---------------
+
+def _lcg(state):
+    state = (state * 6364136223846793005 + 1442695040888963407) & ((1 << 64) - 1)
+    return state, state >> 33
+
+
+def _lcg_text(n, mod, seed, spacer_every=0, spacer_run=0):
+    out, st = [], seed
+    for i in range(n):
+        if spacer_every and i % spacer_every < spacer_run:
+            out.append(5)
+        else:
+            st, v = _lcg(st)
+            out.append(v % mod)
+    return np.array(out, dtype=np.uint8)
+
+
+def _spacer_ends(text):
+    """One past each maximal non-spacer run, closing at the text length
+    (pack.rs:567-584)."""
+    ends, in_run = [], False
+    for i, b in enumerate(text.tolist()):
+        if b == 5:
+            if in_run:
+                ends.append(i)
+                in_run = False
+        else:
+            in_run = True
+    if not ends or ends[-1] != len(text):
+        ends.append(len(text))
+    return ends
+
+
+class _Brute:
+    """Suffix order by slicing (sample_sort.rs:643-647), limits from the ends
+    by bisection, ties by the default boundary order (limits.rs:66-75) or
+    STAR's lim_b.cmp(lim_a).then(pos) (pack.rs:337-346)."""
+
+    def __init__(self, text, ends, order):
+        import bisect
+
+        self.t = text.tolist()
+        self.ends = list(ends) if ends is not None else [len(text)]
+        self.order = order
+        self._bis = bisect.bisect_right
+
+    def lim(self, p):
+        return self.ends[self._bis(self.ends, p)] - p
+
+    def lcp(self, a, b):
+        cap = min(self.lim(a), self.lim(b))
+        i = 0
+        while i < cap and self.t[a + i] == self.t[b + i]:
+            i += 1
+        return i
+
+    def cmp(self, a, b):
+        la, lb = self.lim(a), self.lim(b)
+        h = self.lcp(a, b)
+        if h < min(la, lb):
+            return -1 if self.t[a + h] < self.t[b + h] else 1
+        if self.order == O.ORDER_SHORTER_FIRST:
+            return (la > lb) - (la < lb)
+        return ((lb > la) - (lb < la)) or ((a > b) - (a < b))
+
+
+def _check_seed(text, ends, order, positions, policy=1, budget=0):
+    ctx = O.Ctx(text, ends, order)
+    pk = O.OraclePacker(text, policy=policy, remap_budget=budget)
+    assert pk.ok
+    rank = 0 if order == O.ORDER_SHORTER_FIRST else 1
+    sa, lcp = ctx.seed_subarray(pk, rank, positions)
+    br = _Brute(text, ends, order)
+    assert sorted(sa.tolist()) == sorted(int(x) for x in positions)
+    assert lcp[0] == 0
+    for i in range(1, len(sa)):
+        a, b = int(sa[i - 1]), int(sa[i])
+        c = br.cmp(a, b)
+        assert c < 0 if order == O.ORDER_STAR else c <= 0, (i, a, b)
+        assert int(lcp[i]) == br.lcp(a, b), (i, a, b)
+    return sa, lcp, pk, br
+
+
+def test_seed_subarray_plain_lcg_text_equals_brute_force():
+    # pack.rs:433-439: 4096 LCG symbols over {0..3}, every position
+    text = _lcg_text(4096, 4, 12345)
+    sa, lcp, _, _ = _check_seed(text, None, O.ORDER_SHORTER_FIRST, np.arange(len(text)))
+    ctx = O.Ctx(text)
+    assert np.array_equal(sa, ctx.brute_sa())
+    assert np.array_equal(lcp, ctx.kasai(sa))
+
+
+def test_seed_subarray_segmented_lcg_texts_both_orders():
+    # pack.rs:441-481: spacer runs every 617 / 411 symbols, ACGT positions
+    for seed, every, run, order in ((999, 617, 9, O.ORDER_SHORTER_FIRST), (4242, 411, 7, O.ORDER_STAR)):
+        text = _lcg_text(8192, 5, seed, every, run)
+        ends = _spacer_ends(text)
+        acgt = np.flatnonzero(text < 4)
+        _check_seed(text, ends, order, acgt)
+
+
+def test_seed_subarray_remapped_gapped_alphabet():
+    # pack.rs:484-513: an exactly budgeted remap orders the gapped text
+    text = np.array([0, 2, 0, 2, 0, 2, 0, 2], dtype=np.uint8)
+    sa, _, _, _ = _check_seed(text, None, O.ORDER_SHORTER_FIRST, np.arange(8), policy=2, budget=8)
+    assert sa.tolist() == O.Ctx(text).brute_sa().tolist()
+
+
+def test_packer_k32_depths():
+    """pack.rs:531-563: pairs sharing k-1, k and k+1 symbols and a pair
+    identical through a simultaneous segment end; 3 symbols -> 2-bit fields,
+    k = 32.  Keys of each pair share exactly min(true LCP, k) fields, key
+    order is suffix order on the first k symbols, and the seed's adjacent
+    LCPs are the true (limit-capped) ones under both boundary orders."""
+    k = 32
+    segs = [[0] * (k - 1) + [1], [0] * (k - 1) + [2], [1] * k + [0], [1] * k + [2], [2] * (k + 1) + [0],
+            [2] * (k + 1) + [1], [0] * (k + 2), [0] * (k + 2)]
+    text = np.array(sum(segs, []), dtype=np.uint8)
+    ends = np.cumsum([len(s) for s in segs]).tolist()
+    starts = [0] + ends[:-1]
+    p = O.OraclePacker(text)
+    assert p.k == k and p.bits == 2
+    for order, rank in ((O.ORDER_SHORTER_FIRST, 0), (O.ORDER_STAR, 1)):
+        br = _Brute(text, ends, order)
+        keys = [p.key(s, br.lim(s), rank) for s in starts]
+        for j in range(0, len(starts), 2):
+            a, b = starts[j], starts[j + 1]
+            true_lcp = br.lcp(a, b)
+            assert true_lcp == [k - 1, k, k + 1, k + 2][j // 2]
+            assert p.shared_fields(keys[j], keys[j + 1]) == min(true_lcp, k)
+            if true_lcp < k:  # the keys decide the order on their own
+                assert (keys[j] < keys[j + 1]) == (br.cmp(a, b) < 0)
+        sa, lcp, _, _ = _check_seed(text, ends, order, np.array(starts))
+        # depths k-1, k, k+1: the pairs stay adjacent with their true LCP
+        for depth in (k - 1, k, k + 1):
+            assert depth in lcp.tolist()
+
+
+def test_upper_bound_by_pivot_semantics():
+    """First index whose suffix is strictly greater than the pivot: a pivot
+    inside the run lands one past itself (a lower bound would land on it),
+    and a run of comparator-Equal suffixes is passed over whole."""
+    text = _lcg_text(3000, 3, 77)
+    ctx = O.Ctx(text)
+    br = _Brute(text, None, O.ORDER_SHORTER_FIRST)
+    run = ctx.brute_sa(np.arange(0, 3000, 3, dtype=np.uint64))
+    for idx in (0, 1, 500, len(run) - 1):
+        pivot = int(run[idx])
+        assert ctx.upper_bound_by_pivot(run, pivot) == idx + 1
+    for pivot in (1, 2, 1000, 2999):  # pivots outside the run
+        want = sum(1 for r in run.tolist() if br.cmp(r, pivot) <= 0)
+        assert ctx.upper_bound_by_pivot(run, pivot) == want
+    assert ctx.upper_bound_by_pivot(run[:0], 5) == 0
+    # segmented ShorterFirst: equal segments -> Equal suffixes at the same
+    # offset; the bound passes every one of them
+    segs = [[1, 0, 2, 1]] * 5 + [[1, 0, 2]] + [[1, 0, 2, 1, 1]]
+    t2 = np.array(sum(segs, []), dtype=np.uint8)
+    e2 = np.cumsum([len(s) for s in segs]).tolist()
+    c2 = O.Ctx(t2, e2, O.ORDER_SHORTER_FIRST)
+    b2 = _Brute(t2, e2, O.ORDER_SHORTER_FIRST)
+    run2 = c2.canonicalize(c2.build_in_memory())
+    starts = [0] + e2[:-1]
+    for pivot in starts:
+        want = sum(1 for r in run2.tolist() if b2.cmp(r, pivot) <= 0)
+        lower = sum(1 for r in run2.tolist() if b2.cmp(r, pivot) < 0)
+        got = c2.upper_bound_by_pivot(run2, pivot)
+        assert got == want
+        if pivot < 20:  # the five equal segments: Equal class of five
+            assert want - lower == 5
