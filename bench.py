@@ -6,14 +6,18 @@ Workload (default): the repeat-rich 268,435,456-symbol u8 text over {0,1,2,3}
 planted repeats of mean length 5,000 with 1% substitutions), u32 index, every
 suffix sorted.  The text is generated on the host, copied to HBM, and each
 timed step is one full GPU build from the HBM-resident text to the
-HBM-resident suffix array.  The result of the last step is checked on the
-host (permutation of 0..n-1 with strictly increasing adjacent suffixes, which
-pins the unique suffix array).
+HBM-resident suffix array.  `value` is the default build (binary search over
+the splitters, 1,024 classification buckets).  `learned_vs_plain` times the
+same text with 4,096 classification buckets decided by the learned router
+(piecewise-linear CDF by default) and by binary search; all suffix arrays
+must be equal, and the default build's is checked on the host (permutation
+of 0..n-1 with strictly increasing adjacent suffixes, which pins the unique
+suffix array).
 
-Prints one JSON line (see the keys below).  `roofline` is computed for the
-kernel with the largest share of the step, from event pairs around its
-launches on the build's stream; `cpu_baseline` times the CPU restatement of
-the reference's in-memory sample sort on a bounded prefix of the same text.
+Prints one JSON line (see the keys below).  `roofline` is the single-launch
+kernel with the longest average launch, from event pairs around its launch on
+the build's stream, against its algorithmic bytes; `cpu_baseline` times the
+CPU restatement of the reference's in-memory sample sort on the host cores.
 """
 from __future__ import annotations
 
@@ -56,66 +60,125 @@ def make_text(kind: str, n: int, seed: int):
     return gen.uniform_u8(n, 4, seed=seed)
 
 
-# Algorithmic HBM bytes per suffix of each single-launch kernel (u8 text,
-# index of I bytes): what the kernel must read and write at the least.
-def algorithmic_bytes_per_suffix(kernel: str, I: int) -> float:
-    return {
-        "k_classify": 1 + 2,            # text once + coarse interval id (u16) write
-        "k_scatter1": 1 + 2 + 8 + I,    # text once + interval id read + (key, position) record write
-        "k_scatter2": 2 * (8 + I),      # record read + record written to its bucket
-    }.get(kernel, 0.0)
+def launch_bytes(st: dict, n: int, w: int, I: int) -> dict:
+    """Algorithmic HBM bytes of each single-launch kernel of one build (what
+    the kernel must read and write at the least), from the build's own
+    counts.  Text bytes are w per symbol; records are R = 8 + I bytes (key and
+    index).  The refinement's text bytes are the symbols it actually read (a
+    CAPS_FLAG_COUNT_WORK build's device counters) times w."""
+    R = 8 + I
+    out = {
+        "k_classify": n * (w + 2),            # text once + coarse interval id (u16) written
+        "k_scatter1": n * (w + 2 + R),        # text once + interval id read + record written
+        "k_scatter2": n * 2 * R,              # record read + record written to its bucket
+        "k_tile_1024": st["tile_records"][0] * 2 * R,  # records of the class read and written once
+        "k_tile_2048": st["tile_records"][1] * 2 * R,
+        "k_tile_4096": st["tile_records"][2] * 2 * R,
+    }
+    if st.get("cmp_symbols") and any(st["cmp_symbols"]):
+        # pair records (start + two positions) read, two positions written, text compared
+        out["k_pairs"] = st["pairs"] * (8 + 4 * I) + st["cmp_symbols"][0] * w
+    return out
 
 
-# rocprofv3 kernel names of those kernels on the default workload.
-ROCPROF_NAMES = {"k_classify": "k_classify_seq<u8, u32, 0, false>", "k_scatter1": "k_scatter1<u8, u32, true, false>",
-                 "k_scatter2": "k_scatter2<u32, 256>"}
+def launch_ms(st: dict) -> dict:
+    return {"k_classify": st["k_classify"], "k_scatter1": st["k_scatter1"], "k_scatter2": st["k_scatter2"],
+            "k_tile_1024": st["k_tile"][0], "k_tile_2048": st["k_tile"][1], "k_tile_4096": st["k_tile"][2],
+            "k_pairs": st["k_pairs"]}
 
 
-def measured_traffic(workload: str, kernel: str):
-    """HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE runs
-    (profiles/hbm_traffic.json), each scaled by what the counter reads for a
-    plain 1 GiB copy; None when no measurement is committed."""
+# rocprofv3 names of those kernels (u8 text, u32 index, merge-path tile sort)
+ROCPROF_NAMES = {"k_classify": "k_classify_seq<u8, u32, {rk}, false>", "k_scatter1": "k_scatter1<u8, u32, true, false>",
+                 "k_scatter2": "k_scatter2<u32, 256>", "k_tile_1024": "k_tile_merge<u32, 1024>",
+                 "k_tile_2048": "k_tile_merge<u32, 2048>", "k_tile_4096": "k_tile_merge<u32, 4096>",
+                 "k_pairs": "k_pairs_packed<u8, u32>"}
+
+
+def measured_traffic(workload: str, kernel: str, rk: int):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    runs of this workload (profiles/hbm_traffic.json: FETCH_SIZE and
+    WRITE_SIZE per dispatch, each scaled by what the counter reads for a
+    plain copy of known size); None when not measured."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     try:
         with open(path) as f:
             d = json.load(f)
-        k = d["workloads"][workload][ROCPROF_NAMES[kernel]]
+        k = d["workloads"][workload][ROCPROF_NAMES[kernel].format(rk=rk)]
         cal = d["calibration"]
         return round(k["fetch_kib"] * 1024 * cal["fetch"] + k["write_kib"] * 1024 * cal["write"])
     except (OSError, KeyError, ValueError):
         return None
 
 
-def cpu_baseline(text, seconds_budget: float, threads: int):
-    """The reference's in-memory sample sort (restated in C, oracle/) on a
-    prefix of the text, p = 1024 subproblems as in the reference run."""
+def rooflines_of(stats: list, n: int, w: int, I: int, counted: dict, workload: str, rk: int) -> dict:
+    """Per-launch roofline of every kernel with algorithmic bytes: bytes of
+    one launch / its average duration (HIP events around the launch on the
+    build's stream, averaged over the timed builds)."""
+    bytes_ = launch_bytes(counted, n, w, I)
+    out = {}
+    for k, b in bytes_.items():
+        ms = sum(launch_ms(s)[k] for s in stats) / len(stats)
+        if ms <= 0 or b <= 0:
+            continue
+        achieved = b / (ms / 1e3) / 1e9
+        out[k] = {"bound": "hbm", "kernel": k, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(workload, k, rk),
+                  "algorithmic_bytes": int(b), "kernel_ms": round(ms, 4)}
+    return out
+
+
+def cpu_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model
+
+
+def cpu_baseline(text, seconds_budget: float, threads: int, p: int):
+    """The reference's in-memory sample sort (the oracle's C restatement of
+    ext_mem.rs, p subproblems) on the host cores, pinned to them as taskset
+    would (sched_setaffinity), median of 3 runs: on the whole text when one
+    run fits the budget, else on the largest power-of-two prefix that does."""
     import numpy as np
 
     import oracle as O
 
-    n = 1 << 20
-    # grow the prefix until one run takes about a tenth of the budget
-    while True:
+    allowed = sorted(os.sched_getaffinity(0))
+    threads = min(threads, len(allowed))
+    cpus = allowed[:threads]
+    old = os.sched_getaffinity(0)
+    os.sched_setaffinity(0, cpus)
+    try:
+        m = min(len(text), 1 << 24)
+        ctx = O.Ctx(np.ascontiguousarray(text[:m]))
+        t0 = time.perf_counter()
+        ctx.sample_sort(p=p, threads=threads, emit=lambda a: 0)
+        rate = m / (time.perf_counter() - t0)  # suffixes per second on the probe
+        n = len(text)
+        while n > m and n / rate > seconds_budget / 3:
+            n //= 2
         ctx = O.Ctx(np.ascontiguousarray(text[:n]))
-        t0 = time.perf_counter()
-        ctx.sample_sort(p=1024, threads=threads, emit=lambda a: 0)
-        dt = time.perf_counter() - t0
-        if dt * 8 > seconds_budget or n * 2 > len(text):
-            break
-        n *= 2
-    runs = []
-    t_end = time.perf_counter() + seconds_budget
-    while time.perf_counter() < t_end or len(runs) < 1:
-        t0 = time.perf_counter()
-        ctx.sample_sort(p=1024, threads=threads, emit=lambda a: 0)
-        runs.append(time.perf_counter() - t0)
-        if len(runs) >= 3:
-            break
+        runs = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            ctx.sample_sort(p=p, threads=threads, emit=lambda a: 0)
+            runs.append(time.perf_counter() - t0)
+    finally:
+        os.sched_setaffinity(0, old)
     runs.sort()
-    med = runs[len(runs) // 2]
-    return {"value": n / med / 1e6, "unit": "M suffixes/s", "cores": threads, "kind": "port",
-            "sample": f"first {n} symbols of the same text, build_in_memory_sample_sort p=1024, "
-                      f"median of {len(runs)} runs ({med:.2f} s each)"}
+    med = runs[1]
+    whole = n == len(text)
+    return {"value": round(n / med / 1e6, 3), "unit": "M suffixes/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_info(), "host_cpus_allowed": len(allowed), "pinned_cpus": f"{cpus[0]}-{cpus[-1]}",
+            "sample": (f"{'the whole text' if whole else f'first {n} symbols of the same text'} "
+                       f"({n} suffixes), the oracle's build_in_memory_sample_sort with p={p} on {threads} pinned "
+                       f"threads, median of 3 runs ({med:.2f} s; runs {', '.join(f'{r:.2f}' for r in runs)} s)")}
 
 
 def run_batch(args, count: int) -> dict:
@@ -303,29 +366,33 @@ def run_genome(args, n: int, filtered: bool = False) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="repeat-rich-268M", choices=sorted(WORKLOADS))
     ap.add_argument("--n", type=int, default=0, help="override the text length")
     ap.add_argument("--seed", type=int, default=0xC1)
-    ap.add_argument("--router", default="bsearch", choices=["bsearch", "pwl", "mlp"])
+    ap.add_argument("--router", default="pwl", choices=["bsearch", "pwl", "mlp"],
+                    help="the learned router timed for `value` (repeat-rich workload); binary search is the plain "
+                         "build it is compared with")
+    ap.add_argument("--buckets", type=int, default=4096,
+                    help="classification buckets the classify pass decides among (1..4096; 0: library default)")
     ap.add_argument("--refine", default="plan", choices=["plan", "queue"],
                     help="groups of 2..64: the default per-class plan, or persistent work-queue slots for all")
     ap.add_argument("--tile-sort", default="merge", choices=["merge", "radix"])
     ap.add_argument("--bucket-keys", type=int, default=0, help="mean records per bucket (0: library default)")
-    ap.add_argument("--buckets", type=int, default=0, help="explicit bucket count (0: from --bucket-keys)")
     ap.add_argument("--texts", type=int, default=4096, help="batch workload: number of texts")
     ap.add_argument("--batch-records", type=int, default=0,
                     help="genome workloads: records per bucket batch (0: one batch while it fits in HBM)")
-    ap.add_argument("--flags", type=int, default=0, help="genome workloads: extra CAPS_FLAG_* bits")
+    ap.add_argument("--flags", type=int, default=0, help="extra CAPS_FLAG_* bits")
     ap.add_argument("--pow2-seed", action="store_true",
                     help="genome workload: seed keys with the packed text's power-of-two field width")
     ap.add_argument("--autotune", action="store_true",
                     help="pick bucket size / tile sort / refinement style / merge width by timing builds first")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-plain", action="store_true", help="repeat-rich workload: skip the plain comparison builds")
+    ap.add_argument("--cpu-seconds", type=float, default=40.0)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: every allowed host CPU)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device three times (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -354,6 +421,15 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": src.numel() * 4, "copy_probe_bytes_written": src.numel() * 4}))
         return 0
+    # the host CPUs this job may use: its affinity, within the thread count
+    # the environment grants it (OMP_NUM_THREADS)
+    share = len(os.sched_getaffinity(0))
+    try:
+        share = min(share, int(os.environ.get("OMP_NUM_THREADS", "0")) or share)
+    except ValueError:
+        pass
+    cpu_threads = args.cpu_threads or share
+    args.cpu_threads = cpu_threads
     kind, n_default, desc = WORKLOADS[args.workload]
     if kind == "batch":
         r = run_batch(args, args.texts)
@@ -361,22 +437,22 @@ def main() -> int:
         value = n / (r["ms_step"] / 1e3) / 1e6
         avg = r["kernels"]
         dom = max(("k_classify", "k_scatter1", "k_scatter2"), key=lambda k: avg[k])
-        bpsuf = algorithmic_bytes_per_suffix(dom, 4)
-        achieved = n * bpsuf / (avg[dom] / 1e3) / 1e9 if avg[dom] else 0.0
+        per = {"k_classify": 1 + 2, "k_scatter1": 1 + 2 + 12, "k_scatter2": 24}[dom]
+        achieved = n * per / (avg[dom] / 1e3) / 1e9 if avg[dom] else 0.0
         line = {
             "metric": "M suffixes/s (bit-exact SA), learned-router vs plain GPU; HBM GB/s % peak",
             "value": round(value, 2), "unit": "M suffixes/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": round(r["ms_step"], 3), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "u8+u16", "data": "synthetic",
             "config": {"workload": args.workload, "texts": args.texts, "symbols": n, "text": desc, "index": "u32",
-                       "router": args.router, "seed": args.seed},
+                       "seed": args.seed},
             "verified_bit_exact": r["verified"], "verified_texts": r["sample_texts"],
             "kernels_ms": {k: round(v, 3) for k, v in avg.items()},
             "builds": r["per_width"],
             "autotune": r["autotune"],
             "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                         "bytes_per_suffix": bpsuf, "kernel_ms": round(avg[dom], 4)},
+                         "bytes_per_suffix": per, "kernel_ms": round(avg[dom], 4)},
             "cpu_baseline": r["cpu"],
         }
         print(json.dumps(line))
@@ -408,72 +484,117 @@ def main() -> int:
         }
         print(json.dumps(line))
         return 0 if r["verified"] in (None, True) else 3
+    return run_single(args, kind, desc, n_default)
+
+
+def run_single(args, kind: str, desc: str, n_default: int) -> int:
+    """One text, every suffix.  `value` is the library's default build
+    (binary search over 1,023 coarse splitters: 1,024 classification
+    buckets, the reference run's 1,024 splitters).  Beside it, on the same
+    text: the build with `--buckets` classification buckets decided by the
+    learned router, and the same build with binary search (the plain build
+    the router is compared with).  All three suffix arrays must be equal."""
+    import numpy as np
+    import torch
+
+    import caps_sa
+    from caps_sa import device as dev
+
     n = args.n or n_default
     print(f"[bench] generating {args.workload} n={n}", file=sys.stderr, flush=True)
     text = make_text(kind, n, args.seed)
     d_text = dev.text_to_device(text)
     d_out = torch.empty(n, dtype=torch.int32, device="cuda:0")
-    router = {"bsearch": 0, "pwl": 1, "mlp": 2}[args.router]
-
-    tuned = None
+    d_other = torch.empty(n, dtype=torch.int32, device="cuda:0")
+    rk = {"bsearch": 0, "pwl": 1, "mlp": 2}[args.router]
     tile_keys = args.bucket_keys
     flags = ((caps_sa.FLAG_PERSISTENT_REFINE if args.refine == "queue" else 0)
-             | (caps_sa.FLAG_TILE_RADIX if args.tile_sort == "radix" else 0))
+             | (caps_sa.FLAG_TILE_RADIX if args.tile_sort == "radix" else 0) | args.flags)
+    tuned = None
     if args.autotune:
-        choice, report = dev.autotune(d_text, n, 1, idx_bytes=4, router=router)
+        choice, report = dev.autotune(d_text, n, 1, idx_bytes=4)
         tile_keys, flags = choice["tile_keys"], choice["flags"]
         tuned = {"choice": choice, "report": report}
         print(f"[bench] autotune: {choice}", file=sys.stderr, flush=True)
 
-    def step():
-        dev.build_device(d_text, n, 1, idx_bytes=4, router=router, d_out=d_out, tile_keys=tile_keys,
-                         gpu_buckets=args.buckets, flags=flags)
+    def build(router, buckets, out, extra=0):
+        dev.build_device(d_text, n, 1, idx_bytes=4, router=router, d_out=out, tile_keys=tile_keys,
+                         gpu_buckets=buckets, flags=flags | extra)
         return caps_sa.last_stats()
 
-    for i in range(args.warmup):
-        st = step()
-        print(f"[bench] warmup {i}: {st['ms_total']:.2f} ms", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    stats = []
-    for i in range(args.steps):
-        stats.append(step())
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_step = wall * 1e3 / max(args.steps, 1)
-    ev_ms = sum(s["ms_total"] for s in stats) / max(len(stats), 1)
-    value = n / (ms_step / 1e3) / 1e6
-    print(f"[bench] {args.steps} steps: {ms_step:.2f} ms/step (device events {ev_ms:.2f} ms)", file=sys.stderr,
-          flush=True)
+    def timed(router, buckets, out, label):
+        for i in range(args.warmup):
+            st = build(router, buckets, out)
+            print(f"[bench] {label} warmup {i}: {st['ms_total']:.2f} ms", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stats = [build(router, buckets, out) for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / max(args.steps, 1)
+        ev = sum(s["ms_total"] for s in stats) / max(len(stats), 1)
+        print(f"[bench] {label}: {args.steps} steps, {ms:.2f} ms/step (device events {ev:.2f} ms)",
+              file=sys.stderr, flush=True)
+        return ms, stats
 
-    # roofline of the dominant single-launch kernel (HIP events on the build's stream)
-    kernels = ["k_classify", "k_scatter1", "k_scatter2", "k_scatter", "k_tile_sort", "k_pairs", "k_groups_small",
-               "k_groups_mid", "k_boundary_lcp"]
-    avg = {k: sum(s[k] for s in stats) / len(stats) for k in kernels}
-    rooflines = {}
-    for k in ("k_classify", "k_scatter1", "k_scatter2"):
-        bpsuf = algorithmic_bytes_per_suffix(k, 4)
-        achieved = n * bpsuf / (avg[k] / 1e3) / 1e9 if avg[k] else 0.0
-        rooflines[k] = {"bound": "hbm", "kernel": k, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                        "traffic": measured_traffic(args.workload, k) if args.router == "bsearch" else None,
-                        "bytes_per_suffix": bpsuf, "kernel_ms": round(avg[k], 4)}
-    dom = max(rooflines, key=lambda k: avg[k])
-    roof = rooflines[dom]
+    ms_step, stats = timed(0, 0, d_out, "default build (binary search, 1,024 classification buckets)")
+    value = n / (ms_step / 1e3) / 1e6
+    same = True
+    cmp = None
+    if not args.no_plain and args.buckets:
+        ms_l, st_l = timed(rk, args.buckets, d_other, f"router={args.router} buckets={args.buckets}")
+        same = same and bool(torch.equal(d_out, d_other))
+        ms_p, st_p = timed(0, args.buckets, d_other, f"router=bsearch buckets={args.buckets}")
+        same = same and bool(torch.equal(d_out, d_other))
+        ll, lp = st_l[-1], st_p[-1]
+        C = ll["coarse_buckets"]
+        avg = lambda sts, k: round(sum(s[k] for s in sts) / len(sts), 3)
+        cmp = {"workload": f"the same text, {C} classification buckets",
+               "learned": {"router": args.router, "ms_per_step": round(ms_l, 3),
+                           "value": round(n / (ms_l / 1e3) / 1e6, 2), "k_classify_ms": avg(st_l, "k_classify"),
+                           "train_and_sample_ms": avg(st_l, "ms_sample"),
+                           "mean_mispredict": round(ll["router_err_sum"] / max(ll["router_count"], 1), 4),
+                           "max_mispredict": ll["router_err_max"],
+                           "fallback_frac": round(ll["router_fallbacks"] / max(ll["router_count"], 1), 6)},
+               "plain": {"router": "bsearch", "ms_per_step": round(ms_p, 3),
+                         "value": round(n / (ms_p / 1e3) / 1e6, 2), "k_classify_ms": avg(st_p, "k_classify"),
+                         "sample_ms": avg(st_p, "ms_sample")},
+               "speedup": round(ms_p / ms_l, 4),
+               "imbalance": {"classification": round(lp["coarse_max"] * C / n, 4),
+                             "bucket": round(lp["bucket_max"] * lp["buckets"] / n, 4)},
+               "bit_equal": same}
+    # a measurement build: the refinement's device work counters (not timed)
+    counted = build(0, 0, d_other, caps_sa.FLAG_COUNT_WORK)
+    torch.cuda.synchronize()
+    same = same and bool(torch.equal(d_out, d_other))
+
+    rooflines = rooflines_of(stats, n, 1, 4, counted, args.workload, 0)
+    dom = max(rooflines, key=lambda k: rooflines[k]["kernel_ms"])
+    avg_k = {k: round(sum(launch_ms(s)[k] for s in stats) / len(stats), 3) for k in launch_ms(stats[0])}
+    avg_k["k_groups_small (5 launches)"] = round(sum(s["k_groups_small"] for s in stats) / len(stats), 3)
+    avg_k["k_groups_mid"] = round(sum(s["k_groups_mid"] for s in stats) / len(stats), 3)
 
     verified = None
     if not args.no_verify:
         import oracle as O
 
         sa = d_out.cpu().numpy().view(np.uint32).astype(np.uint64)
-        ctx = O.Ctx(text)
-        bad = ctx.verify(sa, strict=True, canon=True, threads=args.cpu_threads)
-        verified = bool(bad == -1)
-        print(f"[bench] verify: {'ok' if verified else f'FAILED at {bad}'}", file=sys.stderr, flush=True)
+        bad = O.Ctx(text).verify(sa, strict=True, canon=True, threads=args.cpu_threads)
+        verified = bool(bad == -1) and same
+        print(f"[bench] verify: {'ok' if bad == -1 else f'FAILED at {bad}'}, all builds equal: {same}",
+              file=sys.stderr, flush=True)
+
+    # host pointers in and out: text upload + build + SA download (never `value`)
+    pcie = None
+    if not args.no_plain:
+        t0 = time.perf_counter()
+        caps_sa.build_in_memory_with(text, None, None, np.uint32)
+        pcie_ms = (time.perf_counter() - t0) * 1e3
+        pcie = {"ms": round(pcie_ms, 2), "value": round(n / (pcie_ms / 1e3) / 1e6, 2),
+                "what": f"caps_build_in_memory: {n} B text host->HBM, build, {4 * n} B SA HBM->host (pageable)"}
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(text, args.cpu_seconds, args.cpu_threads)
+        cpu = cpu_baseline(text, args.cpu_seconds, args.cpu_threads, p=1024)
         print(f"[bench] cpu baseline: {cpu}", file=sys.stderr, flush=True)
 
     last = stats[-1]
@@ -490,25 +611,25 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "u8",
         "data": "synthetic",
-        "config": {"workload": args.workload, "n": n, "text": desc, "index": "u32", "router": args.router,
-                   "refine": args.refine, "tile_sort": args.tile_sort, "seed": args.seed, "buckets": last["buckets"]},
+        "config": {"workload": f"{args.workload}: build_in_memory_sample_sort, 1,024 splitters", "n": n,
+                   "text": desc, "index": "u32", "router": "bsearch",
+                   "classification_buckets": last["coarse_buckets"], "buckets": last["buckets"],
+                   "refine": args.refine, "tile_sort": args.tile_sort, "seed": args.seed},
         "verified_bit_exact": verified,
+        "learned_vs_plain": cmp,
+        "imbalance": {"classification": round(last["coarse_max"] * last["coarse_buckets"] / n, 4),
+                      "bucket": round(last["bucket_max"] * last["buckets"] / n, 4)},
         "phases_ms": {k: round(sum(s[k] for s in stats) / len(stats), 3)
                       for k in ("ms_alphabet", "ms_sample", "ms_classify", "ms_scatter", "ms_tile_sort", "ms_merge",
                                 "ms_detect", "ms_refine", "ms_giant")},
-        "kernels_ms": {k: round(v, 3) for k, v in avg.items()},
+        "kernels_ms": avg_k,
         "groups": {k: last[k] for k in ("pairs", "small_groups", "mid_groups", "giant_groups", "oversized")},
-        "buckets": {"count": last["buckets"], "max": last["bucket_max"],
-                    "imbalance": round(last["bucket_max"] * last["buckets"] / n, 3)},
-        "router": None if args.router == "bsearch" else {
-            "kind": args.router,
-            "mean_mispredict": round(last["router_err_sum"] / max(last["router_count"], 1), 4),
-            "max_mispredict": last["router_err_max"],
-            "fallback_frac": round(last["router_fallbacks"] / max(last["router_count"], 1), 6),
-            "train_and_sample_ms": round(sum(s["ms_sample"] for s in stats) / len(stats), 3)},
+        "refine_symbols_read": {"pairs": counted["cmp_symbols"][0], "groups_3_64": counted["cmp_symbols"][1],
+                                "groups_65_plus": counted["cmp_symbols"][2]},
         "autotune": tuned,
-        "roofline": roof,
+        "roofline": rooflines[dom],
         "rooflines": rooflines,
+        "pcie_inclusive": pcie,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

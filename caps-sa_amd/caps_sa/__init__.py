@@ -101,10 +101,15 @@ class Stats(ctypes.Structure):
         )
     ] + [("key_bits", ctypes.c_uint32), ("key_k", ctypes.c_uint32), ("bucket_max", ctypes.c_uint64),
          ("k_scatter1", ctypes.c_double), ("k_scatter2", ctypes.c_double), ("batches", ctypes.c_uint64),
-         ("coarse_buckets", ctypes.c_uint64), ("coarse_max", ctypes.c_uint64), ("scratch_bytes", ctypes.c_uint64)]
+         ("coarse_buckets", ctypes.c_uint64), ("coarse_max", ctypes.c_uint64), ("scratch_bytes", ctypes.c_uint64),
+         ("k_tile", ctypes.c_double * 3), ("tile_records", ctypes.c_uint64 * 3), ("cmp_symbols", ctypes.c_uint64 * 3)]
 
     def as_dict(self) -> dict:
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if isinstance(v, ctypes.Array) else v
+        return out
 
 
 EMIT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64)
@@ -459,6 +464,7 @@ FLAG_MERGE_W16 = 8
 FLAG_MERGE_W64 = 16
 FLAG_POW2_SEED = 32
 FLAG_SCATTER_PER_BATCH = 64
+FLAG_COUNT_WORK = 128
 TUNE_MAX = 16
 
 

@@ -9,6 +9,28 @@ namespace caps {
 
 constexpr uint64_t kMaxCtx = ~0ull;
 
+// Refinement work counters: symbols read from the text by the tie
+// refinement (both members of a compare counted), per kernel family:
+// [0] pairs, [1] groups of 3..64, [2] groups of 65..2,048 and giant-group
+// chunks.  Counted only while the host sets g_count_symbols (a measurement
+// build); one atomic per wavefront at the end of a kernel, spread over
+// kCountLanes counters per family so that a grid of millions of wavefronts
+// does not queue on one address (the host sums them).
+constexpr int kCountLanes = 256;
+__device__ unsigned long long g_cmp_symbols[3 * kCountLanes];
+__device__ unsigned int g_count_symbols;
+
+// Must be reached by every lane of the wavefront.
+__device__ __forceinline__ void count_symbols(int slot, uint64_t v) {
+    if (!g_count_symbols) return;
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, o), hi = (uint32_t)__shfl_xor((int)(v >> 32), o);
+        v += ((uint64_t)hi << 32) | lo;
+    }
+    const uint32_t spread = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (kCountLanes - 1);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(&g_cmp_symbols[slot * kCountLanes + spread], (unsigned long long)v);
+}
+
 // Limit provider descriptor.  PlainText when ends == nullptr
 // (limits.rs:130-156); SegmentedText otherwise (limits.rs:182-336), with the
 // coarse block directory of limits.rs:194-236 when dir != nullptr.

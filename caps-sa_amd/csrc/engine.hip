@@ -491,6 +491,30 @@ static RouterDev train_router(Workspace &W, uint32_t kind, uint32_t flags, const
 // ---------------------------------------------------------------------------
 // The build.
 // ---------------------------------------------------------------------------
+// Device addresses of the refinement work counters (looked up once per
+// device) and whether the current device's counting flag is set.
+struct CountAddr {
+    unsigned long long *sym;
+    unsigned int *flag;
+    bool &on;
+};
+static CountAddr count_symbols_addr() {
+    static unsigned long long *sym[64] = {};
+    static unsigned int *flag[64] = {};
+    static bool on[64] = {};
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) throw CapsError(CAPS_E_UNSUPPORTED, "device ordinal above 63");
+    if (!sym[dev]) {
+        void *a = nullptr, *b = nullptr;
+        HIPCHK(hipGetSymbolAddress(&a, HIP_SYMBOL(g_cmp_symbols)));
+        HIPCHK(hipGetSymbolAddress(&b, HIP_SYMBOL(g_count_symbols)));
+        sym[dev] = (unsigned long long *)a;
+        flag[dev] = (unsigned int *)b;
+    }
+    return CountAddr{sym[dev], flag[dev], on[dev]};
+}
+
 // idbits > 0 builds many texts at once: `lim` delimits the texts and the top
 // idbits bits of every packed key hold the text index, so suffixes sort
 // text-major and no comparison crosses a text end.
@@ -591,6 +615,14 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         W.n_cu = cus > 0 ? (uint32_t)cus : 256u;
     }
     memset(&stats, 0, sizeof(stats));
+    // refinement work counters: on for a measurement build only
+    const bool counting = (o.flags & CAPS_FLAG_COUNT_WORK) != 0;
+    const CountAddr count_addr = count_symbols_addr();
+    if (counting || count_addr.on) {
+        HIPCHK(hipMemsetD32Async((hipDeviceptr_t)count_addr.flag, counting ? 1 : 0, 1, st));
+        if (counting) HIPCHK(hipMemsetAsync(count_addr.sym, 0, 3 * kCountLanes * 8, st));
+        count_addr.on = counting;
+    }
     PhaseTimer tm(st);
     KernelTimer kt(st);
     const uint64_t max_ctx = o.max_context;
@@ -906,27 +938,35 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         // ---- tiles: sort per tile (by size class); whole-bucket tiles emit their tie groups ----
         HIPCHK(hipMemsetAsync(counters, 0, 128, st));
         k_tile_emit<<<(unsigned)ceil_div(pb, 256), 256, 0, st>>>(boff, pb, T, tiles, tcap, counters + 1, big,
-                                                                  counters, counters + 4);
+                                                                  counters, counters + 4,
+                                                                  (unsigned long long *)(counters + 24));
         LAUNCHCHK();
-        unsigned int hc[5];
-        HIPCHK(hipMemcpyAsync(hc, counters, 20, hipMemcpyDeviceToHost, st));
+        unsigned int hc[32];
+        HIPCHK(hipMemcpyAsync(hc, counters, 128, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         stats.bucket_max = std::max<uint64_t>(stats.bucket_max, hc[4]);
         const unsigned int n_big = hc[0];
         stats.tiles += (uint64_t)hc[1] + hc[2] + hc[3];
         stats.oversized += n_big;
-        kt.begin(&stats.k_tile_sort);
-        if (o.flags & CAPS_FLAG_TILE_RADIX) {
-            if (hc[1]) k_tile_radix<I, 1024><<<hc[1], 256, 0, st>>>(keys, out, tiles, gl, 1);
-            if (hc[2]) k_tile_radix<I, 2048><<<hc[2], 256, 0, st>>>(keys, out, tiles + tcap, gl, 1);
-            if (hc[3]) k_tile_radix<I, 4096><<<hc[3], 256, 0, st>>>(keys, out, tiles + 2 * tcap, gl, 1);
-        } else {
-            if (hc[1]) k_tile_merge<I, 1024><<<hc[1], 256, 0, st>>>(keys, out, tiles, gl, 1);
-            if (hc[2]) k_tile_merge<I, 2048><<<hc[2], 256, 0, st>>>(keys, out, tiles + tcap, gl, 1);
-            if (hc[3]) k_tile_merge<I, 4096><<<hc[3], 256, 0, st>>>(keys, out, tiles + 2 * tcap, gl, 1);
+        for (int c = 0; c < 3; ++c) {
+            unsigned long long r;
+            memcpy(&r, hc + 24 + 2 * c, 8);
+            stats.tile_records[c] += r;
         }
-        LAUNCHCHK();
-        kt.end();
+        // one launch per tile class, each timed on its own
+        const bool radix = (o.flags & CAPS_FLAG_TILE_RADIX) != 0;
+        auto tile_sort = [&](auto ttag, int c) {
+            constexpr uint32_t TT = decltype(ttag)::value;
+            if (!hc[1 + c]) return;
+            kt.begin(&stats.k_tile[c]);
+            if (radix) k_tile_radix<I, TT><<<hc[1 + c], 256, 0, st>>>(keys, out, tiles + c * tcap, gl, 1);
+            else k_tile_merge<I, TT><<<hc[1 + c], 256, 0, st>>>(keys, out, tiles + c * tcap, gl, 1);
+            LAUNCHCHK();
+            kt.end();
+        };
+        tile_sort(std::integral_constant<uint32_t, 1024>{}, 0);
+        tile_sort(std::integral_constant<uint32_t, 2048>{}, 1);
+        tile_sort(std::integral_constant<uint32_t, 4096>{}, 2);
         tm.mark(PH_TILE);
 
         // ---- oversized buckets: merge the sorted tiles by key, then find their groups ----
@@ -1133,6 +1173,14 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     tm.collect(stats);
     kt.collect();
     stats.k_scatter = stats.k_scatter1 + stats.k_scatter2;
+    stats.k_tile_sort = stats.k_tile[0] + stats.k_tile[1] + stats.k_tile[2];
+    if (counting) {
+        std::vector<unsigned long long> cs(3 * kCountLanes);
+        HIPCHK(hipMemcpyAsync(cs.data(), count_addr.sym, cs.size() * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int f = 0; f < 3; ++f)
+            for (int l = 0; l < kCountLanes; ++l) stats.cmp_symbols[f] += cs[f * kCountLanes + l];
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -867,10 +867,12 @@ struct Range {
 
 // Cuts every bucket into tiles of at most T records and files each tile
 // under its size class (tiles[cls * cap + slot]); buckets larger than T are
-// also listed in `big` for the key merges.
+// also listed in `big` for the key merges.  trec[c] receives the records of
+// class c (the tile sort's work).
 __global__ void k_tile_emit(const uint64_t *__restrict__ off, uint32_t p, uint32_t T, Tile *__restrict__ tiles,
                             uint64_t cap, unsigned int *__restrict__ tcount, Range *__restrict__ big,
-                            unsigned int *__restrict__ n_big, unsigned int *__restrict__ max_bucket) {
+                            unsigned int *__restrict__ n_big, unsigned int *__restrict__ max_bucket,
+                            unsigned long long *__restrict__ trec) {
     uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t s = b < p ? off[b] : 0, sz = b < p ? off[b + 1] - s : 0;
     // largest bucket: wavefront maximum first, one atomic per wavefront
@@ -899,7 +901,13 @@ __global__ void k_tile_emit(const uint64_t *__restrict__ off, uint32_t p, uint32
             if (!m) continue;
             const uint32_t first = (uint32_t)__ffsll((unsigned long long)m) - 1;
             unsigned int base = 0;
-            if (lane == first) base = atomicAdd(&tcount[c], (unsigned)__popcll(m));
+            uint32_t recs = cls == c ? tl.len : 0u;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) recs += __shfl_xor(recs, o);
+            if (lane == first) {
+                base = atomicAdd(&tcount[c], (unsigned)__popcll(m));
+                atomicAdd(&trec[c], (unsigned long long)recs);
+            }
             base = __shfl(base, (int)first);
             if (cls == c) tiles[(uint64_t)c * cap + base + __popcll(m & below)] = tl;
         }
@@ -1165,6 +1173,7 @@ __global__ __launch_bounds__(B) void k_groups(const S *__restrict__ t, CodeMap<S
     for (uint32_t r = threadIdx.x; r < P; r += B) s_perm[r] = (uint16_t)r;
     __syncthreads();
     const uint64_t cont = star ? 0 : kAuxCont;
+    uint64_t nsym = 0;  // symbols read
     for (uint32_t round = 0;; ++round) {
         // the first rounds split by chunk keys (k symbols at the subgroup's
         // depth: many classes at once when members part early); later rounds
@@ -1178,6 +1187,7 @@ __global__ __launch_bounds__(B) void k_groups(const S *__restrict__ t, CodeMap<S
                 const uint64_t cap = s_cap[s];
                 s_key[s] = key_at(t, cm, g, s_pos[s], D, cap);
                 const uint64_t rem = cap > D ? cap - D : 0;
+                nsym += rem < g.k ? rem : g.k;
                 uint64_t aux = cont;
                 if (rem < g.k) {
                     const uint64_t lm = lim_at(L, s_pos[s]);
@@ -1192,6 +1202,7 @@ __global__ __launch_bounds__(B) void k_groups(const S *__restrict__ t, CodeMap<S
             const uint64_t cmin = cs < cref ? cs : cref;
             const uint64_t lim2 = cmin > D ? cmin - D : 0;  // common window from depth D
             const uint64_t l = s == ref ? lim2 : lcp_codes(t, cm, g, pa + D, pr + D, lim2);
+            if (s != ref) nsym += 2 * (l < lim2 ? l + 1 : l);
             // A: absolute depth where the member parts from the reference (or
             // its window ends, possibly above D); classes order by it
             uint64_t key, aux = 0;
@@ -1325,6 +1336,7 @@ __global__ __launch_bounds__(B) void k_groups(const S *__restrict__ t, CodeMap<S
             lcp[gstart] = 0;
         }
     }
+    count_symbols(2, nsym);
 }
 
 // Copies the (idx, lcp) ranges of a list of chunks back from the ping-pong

@@ -341,12 +341,14 @@ __global__ __launch_bounds__(256) void k_pairs_packed(const uint64_t *__restrict
     // windows that hold the first differing code compare like it, so the
     // order comes from the packed words already loaded, not from the text
     bool gt = false;
+    uint64_t nsym = 0;  // symbols read (both members)
     if (!done && sl == 0) {
         uint64_t ahi, alo, bhi, blo;
         load128_codes(pk, lgf, bits, a + o, a + cap, ahi, alo);
         load128_codes(pk, lgf, bits, b + o, b + cap, bhi, blo);
         uint32_t fc = first_diff_code(ahi, alo, bhi, blo, bits);
         const uint64_t in_win = cap - o;
+        nsym += 2 * (in_win < FL ? in_win : FL);
         if (in_win < fc) fc = (uint32_t)in_win;
         if (fc < FL) {
             res = o + fc;
@@ -368,6 +370,7 @@ __global__ __launch_bounds__(256) void k_pairs_packed(const uint64_t *__restrict
                 load128_codes(pk, lgf, bits, b + rel, b + cap, bhi, blo);
                 fc = first_diff_code(ahi, alo, bhi, blo, bits);
                 g = ahi != bhi ? ahi > bhi : alo > blo;
+                nsym += 2 * (cap - rel < FL ? cap - rel : FL);
             } else {
                 fc = 0;
             }
@@ -400,6 +403,7 @@ __global__ __launch_bounds__(256) void k_pairs_packed(const uint64_t *__restrict
         }
         if (lcp) lcp[s + 1] = (I)res;
     }
+    count_symbols(0, nsym);
 }
 
 // Fast-forward for a slot of W lanes whose members share a long stretch
@@ -416,7 +420,7 @@ __global__ __launch_bounds__(256) void k_pairs_packed(const uint64_t *__restrict
 template <int W, int J>
 __device__ __forceinline__ uint64_t ff_skip(const uint64_t *__restrict__ pk, uint32_t lgf, uint32_t bits,
                                             uint64_t pos, uint64_t D, uint64_t cap, bool open, uint32_t sub,
-                                            uint32_t lim) {
+                                            uint32_t lim, uint64_t &nsym) {
     const int lane = threadIdx.x & 63;
     uint32_t jw = 0;
     uint64_t pw[J];
@@ -432,6 +436,7 @@ __device__ __forceinline__ uint64_t ff_skip(const uint64_t *__restrict__ pk, uin
         // word j of the stretch is raw[j] (and raw[j+1] when unaligned); raw[jw]
         // then holds the stretch's last symbol, which is inside the window
         const uint32_t nl = jw ? jw + (rb ? 1u : 0u) : 0u;
+        nsym += (uint64_t)jw << lgf;
         uint64_t raw[J + 1];
 #pragma unroll
         for (int i = 0; i <= J; ++i) raw[i] = (uint32_t)i < nl ? pk[w0 + i] : 0ull;
@@ -467,14 +472,15 @@ constexpr int kFFProbes = 16;
 
 template <int W>
 __device__ __forceinline__ uint64_t ff_advance(const uint64_t *__restrict__ pk, uint32_t lgf, uint32_t bits,
-                                               uint64_t pos, uint64_t D, uint64_t cap, bool open, uint32_t sub) {
+                                               uint64_t pos, uint64_t D, uint64_t cap, bool open, uint32_t sub,
+                                               uint64_t &nsym) {
     const int lane = threadIdx.x & 63;
     const uint64_t smask = (W == 64) ? ~0ull : (((1ull << W) - 1) << (lane & ~(W - 1)));
     if ((__ballot(open) & smask) == 0) return 0;  // a settled slot
     uint64_t adv = 0;
     uint32_t lim = kFFFirst;
     for (int it = 0; it < kFFProbes; ++it) {  // uniform across the slot
-        const uint64_t sk = ff_skip<W, kFFWords>(pk, lgf, bits, pos, D + adv, cap, open, sub, lim);
+        const uint64_t sk = ff_skip<W, kFFWords>(pk, lgf, bits, pos, D + adv, cap, open, sub, lim, nsym);
         adv += sk;
         if (sk != ((uint64_t)lim << lgf)) break;
         lim = kFFWords;
@@ -510,12 +516,14 @@ __global__ __launch_bounds__(256) void k_wgroups_grid(const S *__restrict__ t, C
         cap = lim < max_ctx ? lim : max_ctx;
     }
     uint64_t D = depth0;
+    uint64_t nsym = 0;  // symbols read
     for (uint32_t round = 0;; ++round) {
         if (cm.packed && round >= kFFAfter && __any(!settled))
-            D += ff_advance<W>(cm.packed, cm.lgf, g.bits, pos, D, cap, !settled, sub);
+            D += ff_advance<W>(cm.packed, cm.lgf, g.bits, pos, D, cap, !settled, sub, nsym);
         if (!settled) {
             chunk = key_at(t, cm, g, pos, D, cap);
             uint64_t rem = cap > D ? cap - D : 0;
+            nsym += rem < g.k ? rem : g.k;
             if (rem >= g.k) aux = cont;
             else aux = star ? (1 + (kAuxCont - 1 - lim)) : lim;
         }
@@ -580,6 +588,7 @@ __global__ __launch_bounds__(256) void k_wgroups_grid(const S *__restrict__ t, C
             lcp[gr.start + r] = (I)v;
         }
     }
+    count_symbols(1, nsym);
 }
 
 
@@ -658,6 +667,7 @@ __global__ __launch_bounds__(256) void k_wgroups_wq(const S *__restrict__ t, Cod
     uint64_t pos = 0, lim = 0, cap = 0, chunk = ~0ull, aux = ~0ull, h = 0;
     uint32_t sub = 0xFFFFFFFFu;
     bool settled = true;
+    uint64_t nsym = 0;  // symbols read
     for (;;) {
         // advance the pipeline (all conditions are uniform within a slot)
         if (!busy && v2) {
@@ -700,10 +710,11 @@ __global__ __launch_bounds__(256) void k_wgroups_wq(const S *__restrict__ t, Cod
         if (!__any(busy || v1 || v2)) break;
         const bool act = busy && r < m;
         if (cm.packed && busy && rounds >= kFFAfter)
-            D += ff_advance<W>(cm.packed, cm.lgf, g.bits, pos, D, cap, !settled, sub);
+            D += ff_advance<W>(cm.packed, cm.lgf, g.bits, pos, D, cap, !settled, sub, nsym);
         if (!settled) {
             chunk = key_at(t, cm, g, pos, D, cap);
             uint64_t rem = cap > D ? cap - D : 0;
+            nsym += rem < g.k ? rem : g.k;
             if (rem >= g.k) aux = cont;
             else aux = star ? (1 + (kAuxCont - 1 - lim)) : lim;
         }
@@ -778,6 +789,7 @@ __global__ __launch_bounds__(256) void k_wgroups_wq(const S *__restrict__ t, Cod
             ++rounds;
         }
     }
+    count_symbols(1, nsym);
 }
 
 // ---------------------------------------------------------------------------

@@ -119,6 +119,10 @@ typedef struct caps_opts {
  * only) even when the records of all batches fit in HBM at once.  The output
  * never changes. */
 #define CAPS_FLAG_SCATTER_PER_BATCH 64u
+/* Measurement build: count the symbols the tie refinement reads from the
+ * text (stats cmp_symbols).  Costs a few atomics per wavefront; the output
+ * never changes. */
+#define CAPS_FLAG_COUNT_WORK 128u
 
 /* Fills opts with the reference defaults (max_context unbounded, p auto). */
 void caps_opts_default(caps_opts_t *opts);
@@ -262,6 +266,11 @@ typedef struct caps_stats {
     uint64_t coarse_buckets;       /* classification buckets (coarse intervals)         */
     uint64_t coarse_max;           /* records in the largest classification bucket      */
     uint64_t scratch_bytes;        /* no-alloc builds: bytes of the scratch array used  */
+    double k_tile[3];              /* tile sort launches of the 1,024 / 2,048 / 4,096
+                                      record classes (their sum is k_tile_sort)         */
+    uint64_t tile_records[3];      /* records sorted by each of those launches          */
+    uint64_t cmp_symbols[3];       /* CAPS_FLAG_COUNT_WORK builds: symbols read by the
+                                      refinement (pairs; groups of 3..64; larger groups) */
 } caps_stats_t;
 int caps_last_stats(caps_stats_t *out);
 
