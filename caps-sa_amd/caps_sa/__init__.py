@@ -465,6 +465,7 @@ FLAG_MERGE_W64 = 16
 FLAG_POW2_SEED = 32
 FLAG_SCATTER_PER_BATCH = 64
 FLAG_COUNT_WORK = 128
+FLAG_SCATTER_DIRECT = 256
 TUNE_MAX = 16
 
 

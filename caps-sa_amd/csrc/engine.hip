@@ -894,7 +894,18 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     // scatter pass 1 for coarse intervals [c0, c1) into tmp, at offsets relative to rb
     auto pass1 = [&](uint32_t c0, uint32_t c1, uint64_t rb) {
         kt.begin(&stats.k_scatter1);
-        if (seq && d_keep)
+        // contiguous positions, u32 index, at most 1,024 intervals: the
+        // LDS-staged pass with coalesced record writes
+        constexpr bool kStageable = sizeof(I) == 4;
+        if (kStageable && seq && pc <= kStageIntervals && !(o.flags & CAPS_FLAG_SCATTER_DIRECT)) {
+            KRec<uint32_t> *t32 = (KRec<uint32_t> *)tmp;
+            if (d_keep)
+                k_scatter1_staged<S, true><<<nblk, 256, 0, st>>>(d_text, n_iter, cm, g, L, max_ctx, cid, pc, coff,
+                                                                 hblk, t32, chunk, d_keep, c0, c1, rb);
+            else
+                k_scatter1_staged<S, false><<<nblk, 256, 0, st>>>(d_text, n_iter, cm, g, L, max_ctx, cid, pc, coff,
+                                                                  hblk, t32, chunk, nullptr, c0, c1, rb);
+        } else if (seq && d_keep)
             k_scatter1<S, I, true, true><<<nblk, 256, (size_t)pc * 12, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_iter, cid, pc,
                                                                coff, hblk, tmp, chunk, d_keep, c0, c1, rb);
         else if (seq)

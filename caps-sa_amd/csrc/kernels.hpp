@@ -715,6 +715,102 @@ __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint6
     }
 }
 
+// Scatter pass 1 for contiguous positions (every suffix, or the kept ones
+// of a bitmap) with coalesced writes: each 4,096-position step of a block
+// counting-sorts its records by interval in LDS, then consecutive threads
+// write each interval's run of the step, so a wavefront's store covers a few
+// contiguous runs instead of 64 scattered records.  For at most
+// kStageIntervals intervals and 12-byte records (u32 index); the block owns
+// the same range of positions as in k_scatter1.
+constexpr uint32_t kStageIntervals = 1024;
+template <typename S, bool KEEP>
+__global__ __launch_bounds__(256) void k_scatter1_staged(const S *__restrict__ t, uint64_t n, CodeMap<S> cm,
+                                                         KeyGeom g, DevLimits L, uint64_t max_ctx,
+                                                         const uint16_t *__restrict__ cid, uint32_t pc,
+                                                         const uint64_t *__restrict__ coff,
+                                                         const uint32_t *__restrict__ hblk,
+                                                         KRec<uint32_t> *__restrict__ tmp, uint64_t chunk,
+                                                         const uint64_t *__restrict__ keep, uint32_t c_lo,
+                                                         uint32_t c_hi, uint64_t rbase) {
+    constexpr uint32_t E = kStageIntervals / 256;  // intervals per thread in the scan
+    constexpr uint32_t kSkip = 0xFFFFFFFFu;
+    __shared__ S win[kSeqChunk + 72];
+    __shared__ uint64_t s_key[kSeqChunk];
+    __shared__ uint32_t s_pos[kSeqChunk];
+    __shared__ uint16_t s_c[kSeqChunk];
+    __shared__ unsigned long long cbase[kStageIntervals];
+    __shared__ uint32_t icnt[kStageIntervals], ioff[kStageIntervals];
+    __shared__ uint32_t wsum[4];
+    const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
+    if (c0 >= n) return;
+    const uint64_t c1 = c0 + chunk < n ? c0 + chunk : n;
+    for (uint32_t c = threadIdx.x; c < kStageIntervals; c += blockDim.x) {
+        cbase[c] = c < pc ? coff[c] - rbase + hblk[(uint64_t)blockIdx.x * pc + c] : 0ull;
+        icnt[c] = 0;
+    }
+    for (uint64_t base = c0; base < c1; base += kSeqChunk) {
+        const uint32_t cnt = (uint32_t)(c1 - base < kSeqChunk ? c1 - base : kSeqChunk);
+        uint64_t key[kSeqItems];
+        thread_keys_seq<S>(t, n, cm, g, L, max_ctx, base, cnt, win, key);  // begins with a barrier
+        const uint32_t first = threadIdx.x * kSeqItems;
+        uint32_t cc[kSeqItems], lr[kSeqItems];
+        if (first + kSeqItems <= cnt) {
+            const uint4 *src = (const uint4 *)(cid + base + first);
+            const uint4 v0 = src[0], v1 = src[1];
+            const uint32_t w[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+            for (uint32_t j = 0; j < kSeqItems; ++j) cc[j] = (w[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
+        } else {
+#pragma unroll
+            for (uint32_t j = 0; j < kSeqItems; ++j) cc[j] = first + j < cnt ? (uint32_t)cid[base + first + j] : kSkip;
+        }
+        uint32_t kb = 0xFFFFu;
+        if (KEEP && first < cnt) kb = (uint32_t)(keep[(base + first) >> 6] >> ((base + first) & 63)) & 0xFFFFu;
+        // local rank of each record inside its interval's run of this step
+#pragma unroll
+        for (uint32_t j = 0; j < kSeqItems; ++j) {
+            const uint32_t c = cc[j];
+            const bool take = ((kb >> j) & 1u) && c != kSkip && c - c_lo < c_hi - c_lo;
+            cc[j] = take ? c : kSkip;
+            lr[j] = take ? atomicAdd(&icnt[c], 1u) : 0u;
+        }
+        __syncthreads();
+        // interval offsets of the step: exclusive scan of icnt
+        uint32_t v[E], sum = 0;
+#pragma unroll
+        for (uint32_t e = 0; e < E; ++e) {
+            v[e] = icnt[threadIdx.x * E + e];
+            sum += v[e];
+        }
+        uint32_t pre = block_excl_scan256(sum, wsum);
+        const uint32_t total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+#pragma unroll
+        for (uint32_t e = 0; e < E; ++e) {
+            ioff[threadIdx.x * E + e] = pre;
+            pre += v[e];
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t j = 0; j < kSeqItems; ++j)
+            if (cc[j] != kSkip) {
+                const uint32_t q = ioff[cc[j]] + lr[j];
+                s_key[q] = key[j];
+                s_pos[q] = (uint32_t)(base + first + j);
+                s_c[q] = (uint16_t)cc[j];
+            }
+        __syncthreads();
+        for (uint32_t q = threadIdx.x; q < total; q += blockDim.x) {
+            const uint32_t c = s_c[q];
+            tmp[cbase[c] + (q - ioff[c])] = KRec<uint32_t>{s_key[q], s_pos[q]};
+        }
+        __syncthreads();
+        for (uint32_t c = threadIdx.x; c < kStageIntervals; c += blockDim.x) {
+            cbase[c] += icnt[c];
+            icnt[c] = 0;
+        }
+    }
+}
+
 // Exclusive scan of cnt[0..nb) into out[0..nb) (+ base) across the block;
 // each thread scans a contiguous slice.  Returns the total in every thread.
 // part: blockDim.x u64 of LDS.

@@ -123,6 +123,11 @@ typedef struct caps_opts {
  * text (stats cmp_symbols).  Costs a few atomics per wavefront; the output
  * never changes. */
 #define CAPS_FLAG_COUNT_WORK 128u
+/* Scatter pass 1 writes each record straight to its interval run instead of
+ * staging every 4,096-position step in LDS first (the staged pass is used for
+ * every suffix of a text, u32 index, up to 1,024 classification buckets).
+ * The output never changes. */
+#define CAPS_FLAG_SCATTER_DIRECT 256u
 
 /* Fills opts with the reference defaults (max_context unbounded, p auto). */
 void caps_opts_default(caps_opts_t *opts);

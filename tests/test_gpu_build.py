@@ -326,3 +326,24 @@ def test_tight_and_pow2_seed_fields_agree():
                 assert np.array_equal(outs[0][0], want), label
                 assert np.array_equal(outs[1][0], want), label
                 assert np.array_equal(outs[0][1], outs[1][1]), label
+
+
+def test_scatter_pass1_variants_agree(cuda):
+    """Scatter pass 1 staged in LDS (default for every suffix, u32 index, up
+    to 1,024 classification buckets), written record by record
+    (CAPS_FLAG_SCATTER_DIRECT), and with 4,096 classification buckets: the
+    same suffix array as the oracle's."""
+    from caps_sa import device as dev
+    import torch
+
+    text = gen.markov_repeats(3_000_017, seed=21, mean_len=400.0)
+    want = O.Ctx(text).build_in_memory(threads=8)
+    d_text = dev.text_to_device(text)
+    for label, kw in (("staged", {}), ("direct", {"flags": caps_sa.FLAG_SCATTER_DIRECT}),
+                      ("4096 buckets", {"gpu_buckets": 4096}), ("u64 index", {"idx_bytes": 8})):
+        print(f"case scatter pass 1 {label}", flush=True)
+        idx = kw.pop("idx_bytes", 4)
+        sa, _ = dev.build_device(d_text, len(text), 1, idx_bytes=idx, **kw)
+        torch.cuda.synchronize()
+        got = sa.cpu().numpy().view(np.uint32 if idx == 4 else np.uint64).astype(np.uint64)
+        assert np.array_equal(got, want), label
