@@ -163,6 +163,10 @@ _SIGNATURES = {
     "caps_seed_params_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
                                                 ctypes.POINTER(_Limits), ctypes.POINTER(_Opts),
                                                 ctypes.POINTER(SeedParams), ctypes.c_void_p]),
+    "caps_router_pwl_fit_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                                                   ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                   ctypes.c_void_p]),
     "caps_build_device_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p,
                                                   ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(_Limits),
                                                   ctypes.POINTER(_Opts), ctypes.c_void_p, ctypes.c_void_p,

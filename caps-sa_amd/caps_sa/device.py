@@ -136,6 +136,20 @@ def seed_params(d_text: torch.Tensor, n: int, sym_bytes: int, policy=0, remap_bu
     return {name: getattr(out, name) for name, _ in out._fields_}
 
 
+def pwl_fit(keys: np.ndarray, labels: np.ndarray, ns: int, n_buckets: int, device="cuda:0"):
+    """caps_router_pwl_fit_device over sorted keys and their labels: dict of
+    knots, a, b, inv, lo, hi (numpy arrays)."""
+    k = torch.from_numpy(np.ascontiguousarray(keys, dtype=np.uint64).view(np.int64)).to(device)
+    lab = torch.from_numpy(np.ascontiguousarray(labels, dtype=np.uint32).view(np.int32)).to(device)
+    out = {"knots": np.zeros(ns + 1, dtype=np.uint64), "a": np.zeros(ns, dtype=np.float32),
+           "b": np.zeros(ns, dtype=np.float32), "inv": np.zeros(ns, dtype=np.float32),
+           "lo": np.zeros(ns, dtype=np.uint32), "hi": np.zeros(ns, dtype=np.uint32)}
+    _check(lib().caps_router_pwl_fit_device(k.data_ptr(), lab.data_ptr(), len(keys), ns, n_buckets,
+                                            *(out[x].ctypes.data for x in ("knots", "a", "b", "inv", "lo", "hi")),
+                                            _stream_ptr()))
+    return out
+
+
 MLP_IN, MLP_HID = 64, 32
 MLP_PARAMS = MLP_IN * MLP_HID + MLP_HID + MLP_HID + 1
 

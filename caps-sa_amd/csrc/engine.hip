@@ -1814,6 +1814,29 @@ int caps_router_mlp_forward_device(const float *d_params, const uint64_t *d_keys
     });
 }
 
+int caps_router_pwl_fit_device(const uint64_t *d_keys, const uint32_t *d_labels, uint64_t s, uint32_t ns,
+                               uint32_t n_buckets, uint64_t *knots, float *a, float *b, float *inv, uint32_t *lo,
+                               uint32_t *hi, void *stream) {
+    return guarded([&] {
+        if (s == 0 || ns == 0 || ns > s) throw CapsError(CAPS_E_INVALID, "need 1 <= ns <= s");
+        if (!knots || !a || !b || !inv || !lo || !hi) throw CapsError(CAPS_E_INVALID, "output pointer is NULL");
+        hipStream_t st = (hipStream_t)stream;
+        Workspace &W = workspace();
+        uint64_t *dk = (uint64_t *)W.r_knots.get((ns + 1) * 8);
+        float *da = (float *)W.r_a.get(ns * 4), *db = (float *)W.r_b.get(ns * 4), *di = (float *)W.r_inv.get(ns * 4);
+        uint32_t *dlo = (uint32_t *)W.r_lo.get(ns * 4), *dhi = (uint32_t *)W.r_hi.get(ns * 4);
+        k_pwl_fit<<<ns, 256, 0, st>>>(d_keys, d_labels, s, ns, n_buckets, dk, da, db, di, dlo, dhi);
+        LAUNCHCHK();
+        HIPCHK(hipMemcpyAsync(knots, dk, (ns + 1) * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(a, da, ns * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(b, db, ns * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(inv, di, ns * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(lo, dlo, ns * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(hi, dhi, ns * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
 int caps_merge_lcp(const void *text, uint64_t n, uint32_t sym_bytes, const caps_limits_t *limits, const uint64_t *x,
                    const uint64_t *lcp_x, uint64_t nx, const uint64_t *y, const uint64_t *lcp_y, uint64_t ny,
                    uint64_t max_ctx, uint64_t *z, uint64_t *lcp_z) {

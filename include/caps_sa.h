@@ -377,6 +377,19 @@ int caps_router_mlp_grad_device(const float *d_params, const uint64_t *d_keys, c
 int caps_router_mlp_forward_device(const float *d_params, const uint64_t *d_keys, uint32_t count, float *d_out,
                                    void *stream);
 
+/* The piecewise-linear router's least-squares fit, as the build runs it on
+ * its sorted sample: ns segments over the sorted keys d_keys[0..s) with
+ * their bucket labels d_labels (non-decreasing).  Segment j covers sample
+ * entries [j*s/ns, (j+1)*s/ns); its knot is the first key, x = float(key -
+ * knot) * inv with inv = float(1 / (next knot - knot)), and (a, b) is the
+ * least-squares line label = a*x + b (a clamped at 0 to keep the CDF
+ * monotone).  Outputs (host arrays of ns entries): knots (ns + 1, the last
+ * = UINT64_MAX), a, b, inv, lo / hi (the labels at the segment's two
+ * knots: exact bounds for any key between them). */
+int caps_router_pwl_fit_device(const uint64_t *d_keys, const uint32_t *d_labels, uint64_t s, uint32_t ns,
+                               uint32_t n_buckets, uint64_t *knots, float *a, float *b, float *inv, uint32_t *lo,
+                               uint32_t *hi, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

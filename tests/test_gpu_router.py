@@ -74,3 +74,40 @@ def test_training_reduces_misprediction(cuda):
         print(name, res[name], st["router_err_max"], st["router_fallbacks"])
     assert res["mlp"] < 0.5 * res["mlp-untrained"]
     assert res["pwl"] < res["mlp-untrained"]
+
+
+def test_pwl_fit_matches_least_squares(cuda):
+    """The GPU piecewise-linear fit against numpy.polyfit (degree 1) on each
+    segment's own points, in float64: slopes and intercepts agree to the
+    float32 rounding of x; a segment whose labels fall is flattened to its
+    mean (the monotone clamp); a segment of one label is flat; the knots and
+    label bounds are the segment's first sample key and labels."""
+    rng = np.random.default_rng(12)
+    s, ns, nb = 40_000, 64, 4096
+    keys = np.sort(rng.integers(0, 1 << 63, s, dtype=np.uint64) * np.uint64(2))
+    keys[1000:1600] = keys[1000]  # a run of equal keys
+    # labels: a monotone step function of the key (bucket of a splitter table)
+    spl = np.sort(rng.choice(keys, nb - 1, replace=False))
+    labels = np.searchsorted(spl, keys, side="right").astype(np.uint32)
+    got = dev.pwl_fit(keys, labels, ns, nb)
+    for j in range(ns):
+        i0, i1 = j * s // ns, (j + 1) * s // ns
+        k0 = int(keys[i0])
+        k1 = int(keys[i1]) if j + 1 < ns else int(keys[s - 1])
+        span = float(k1 - k0) if k1 > k0 else 1.0
+        inv = np.float32(1.0 / span)
+        x = ((keys[i0:i1] - np.uint64(k0)).astype(np.float32) * inv).astype(np.float64)
+        y = labels[i0:i1].astype(np.float64)
+        assert got["knots"][j] == k0
+        assert got["lo"][j] == labels[i0] and got["hi"][j] == (labels[i1] if j + 1 < ns else nb - 1)
+        assert got["inv"][j] == inv
+        if np.ptp(x) > 0:
+            a, b = np.polyfit(x, y, 1)
+            if a < 0:
+                a, b = 0.0, y.mean()
+        else:
+            a, b = 0.0, y.mean()
+        scale = max(1.0, abs(y).max())
+        assert abs(got["a"][j] - a) <= 2e-3 * scale + 1e-3 * abs(a), (j, got["a"][j], a)
+        assert abs(got["b"][j] - b) <= 2e-3 * scale, (j, got["b"][j], b)
+    assert got["knots"][ns] == np.uint64(0xFFFFFFFFFFFFFFFF)
