@@ -221,7 +221,7 @@ def run_batch(args, count: int) -> dict:
         out = []
         for gi, (w, sel, ends, d_text, d_out, m) in enumerate(groups):
             dev.build_batch_device(d_text, m, w, ends, 4, d_out, tile_keys=knobs[gi]["tile_keys"],
-                                   flags=knobs[gi]["flags"])
+                                   flags=knobs[gi]["flags"] | args.flags)
             out.append(caps_sa.last_stats())
         return out
 
@@ -444,7 +444,7 @@ def main() -> int:
             "value": round(value, 2), "unit": "M suffixes/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": round(r["ms_step"], 3), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "u8+u16", "data": "synthetic",
-            "config": {"workload": args.workload, "texts": args.texts, "symbols": n, "text": desc, "index": "u32",
+            "config": {"workload": args.workload, "texts": args.texts, "symbols": n, "text": desc, "index": "u32", "flags": args.flags,
                        "seed": args.seed},
             "verified_bit_exact": r["verified"], "verified_texts": r["sample_texts"],
             "kernels_ms": {k: round(v, 3) for k, v in avg.items()},

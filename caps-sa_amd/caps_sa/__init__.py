@@ -102,7 +102,8 @@ class Stats(ctypes.Structure):
     ] + [("key_bits", ctypes.c_uint32), ("key_k", ctypes.c_uint32), ("bucket_max", ctypes.c_uint64),
          ("k_scatter1", ctypes.c_double), ("k_scatter2", ctypes.c_double), ("batches", ctypes.c_uint64),
          ("coarse_buckets", ctypes.c_uint64), ("coarse_max", ctypes.c_uint64), ("scratch_bytes", ctypes.c_uint64),
-         ("k_tile", ctypes.c_double * 3), ("tile_records", ctypes.c_uint64 * 3), ("cmp_symbols", ctypes.c_uint64 * 3)]
+         ("k_tile", ctypes.c_double * 3), ("tile_records", ctypes.c_uint64 * 3), ("cmp_symbols", ctypes.c_uint64 * 3),
+         ("k_graph", ctypes.c_double * 3)]
 
     def as_dict(self) -> dict:
         out = {}
@@ -470,6 +471,7 @@ FLAG_POW2_SEED = 32
 FLAG_SCATTER_PER_BATCH = 64
 FLAG_COUNT_WORK = 128
 FLAG_SCATTER_DIRECT = 256
+FLAG_GRAPH = 512
 TUNE_MAX = 16
 
 

@@ -359,6 +359,59 @@ struct KernelTimer {
     }
 };
 
+// A phase of the build captured into a hipGraph (CAPS_FLAG_GRAPH): while
+// the section is open its launches go to a private capture stream (`ls`);
+// finish() ends the capture, updates the phase's instantiated graph in place
+// when its kernels are the same as last time (only grids and arguments
+// differ) or instantiates it again, and launches it on the build's stream.
+// Closed, the section launches straight on the build's stream.
+struct GraphSection {
+    Workspace &W;
+    hipStream_t st, ls;
+    int slot;
+    bool capturing = false;
+    GraphSection(Workspace &w, hipStream_t s, bool on, int sl) : W(w), st(s), ls(s), slot(sl) {
+        if (!on) return;
+        if (!W.cap_stream) HIPCHK(hipStreamCreateWithFlags(&W.cap_stream, hipStreamNonBlocking));
+        HIPCHK(hipStreamBeginCapture(W.cap_stream, hipStreamCaptureModeRelaxed));
+        capturing = true;
+        ls = W.cap_stream;
+    }
+    // returns true when a graph was launched (timed by the caller's event pair)
+    void finish(KernelTimer &kt, double *slot_ms) {
+        if (!capturing) return;
+        capturing = false;
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamEndCapture(W.cap_stream, &g));
+        hipGraphExec_t &ex = W.graph_exec[slot];
+        bool updated = false;
+        if (ex) {
+            hipGraphNode_t err_node = nullptr;
+            hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+            if (hipGraphExecUpdate(ex, g, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess)
+                updated = true;
+            else {
+                (void)hipGetLastError();
+                (void)hipGraphExecDestroy(ex);
+                ex = nullptr;
+            }
+        }
+        hipError_t e = updated ? hipSuccess : hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(e);
+        kt.begin(slot_ms);
+        HIPCHK(hipGraphLaunch(ex, st));
+        kt.end();
+    }
+    ~GraphSection() {
+        if (capturing) {  // an exception left the capture open
+            hipGraph_t g = nullptr;
+            if (hipStreamEndCapture(W.cap_stream, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+        }
+    }
+};
+
 // Sort `count` keys (with u32 payload) in place: LDS tiles + key merges.
 template <int T>
 static void sort_keys_single_range(Workspace &W, uint64_t *keys, uint32_t *ids, uint64_t *keys2, uint32_t *ids2,
@@ -649,6 +702,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         d_pos = cp;
     }
     const bool want_lcp = d_lcp_out != nullptr && !sink;
+    const bool graph = (o.flags & CAPS_FLAG_GRAPH) != 0;
     if (n_pos == 1) {
         I *o1 = sink ? (I *)W.sinkbuf.get(16) : d_out;
         if (d_pos) HIPCHK(hipMemcpyAsync(o1, d_pos, sizeof(I), hipMemcpyDeviceToDevice, st));
@@ -964,20 +1018,25 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
             memcpy(&r, hc + 24 + 2 * c, 8);
             stats.tile_records[c] += r;
         }
-        // one launch per tile class, each timed on its own
+        // one launch per tile class, each timed on its own (graph builds:
+        // the three launches are one captured graph, timed as a whole)
         const bool radix = (o.flags & CAPS_FLAG_TILE_RADIX) != 0;
-        auto tile_sort = [&](auto ttag, int c) {
-            constexpr uint32_t TT = decltype(ttag)::value;
-            if (!hc[1 + c]) return;
-            kt.begin(&stats.k_tile[c]);
-            if (radix) k_tile_radix<I, TT><<<hc[1 + c], 256, 0, st>>>(keys, out, tiles + c * tcap, gl, 1);
-            else k_tile_merge<I, TT><<<hc[1 + c], 256, 0, st>>>(keys, out, tiles + c * tcap, gl, 1);
-            LAUNCHCHK();
-            kt.end();
-        };
-        tile_sort(std::integral_constant<uint32_t, 1024>{}, 0);
-        tile_sort(std::integral_constant<uint32_t, 2048>{}, 1);
-        tile_sort(std::integral_constant<uint32_t, 4096>{}, 2);
+        {
+            GraphSection gs(W, st, graph && (hc[1] || hc[2] || hc[3]), 0);
+            auto tile_sort = [&](auto ttag, int c) {
+                constexpr uint32_t TT = decltype(ttag)::value;
+                if (!hc[1 + c]) return;
+                if (!gs.capturing) kt.begin(&stats.k_tile[c]);
+                if (radix) k_tile_radix<I, TT><<<hc[1 + c], 256, 0, gs.ls>>>(keys, out, tiles + c * tcap, gl, 1);
+                else k_tile_merge<I, TT><<<hc[1 + c], 256, 0, gs.ls>>>(keys, out, tiles + c * tcap, gl, 1);
+                LAUNCHCHK();
+                if (!gs.capturing) kt.end();
+            };
+            tile_sort(std::integral_constant<uint32_t, 1024>{}, 0);
+            tile_sort(std::integral_constant<uint32_t, 2048>{}, 1);
+            tile_sort(std::integral_constant<uint32_t, 4096>{}, 2);
+            gs.finish(kt, &stats.k_graph[0]);
+        }
         tm.mark(PH_TILE);
 
         // ---- oversized buckets: merge the sorted tiles by key, then find their groups ----
@@ -1028,47 +1087,55 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         tm.mark(PH_DETECT);
 
         // ---- refinement ----
-        if (cnt[0]) {
-            kt.begin(&stats.k_pairs);
-            const PairRec<I> *pl = (const PairRec<I> *)gl.list[0];
-            if (plan.queue[0])
-                k_pairs_wq<S, I><<<pgrid(cnt[0], 64), 256, 0, st>>>(d_text, L, max_ctx, pl, cnt[0], depth0, out, lcp,
-                                                                     queues + 0, chunk_for(cnt[0], 16));
-            else if (cm.packed)
-                k_pairs_packed<S, I><<<(unsigned)ceil_div(cnt[0], 64), 256, 0, st>>>(
-                    cm.packed, cm.lgf, gref.bits, L, max_ctx, pl, cnt[0], depth0, out, lcp);
-            else
-                k_pairs4<S, I><<<(unsigned)ceil_div(cnt[0], 64), 256, 0, st>>>(d_text, L, max_ctx, pl, cnt[0], depth0,
-                                                                                out, lcp);
+        {
+            bool any = false;
+            for (int c = 0; c < 7; ++c) any |= cnt[c] != 0;
+            GraphSection gs(W, st, graph && any, 1);
+            const hipStream_t ls = gs.ls;
+            const bool timed = !gs.capturing;
+            if (cnt[0]) {
+                if (timed) kt.begin(&stats.k_pairs);
+                const PairRec<I> *pl = (const PairRec<I> *)gl.list[0];
+                if (plan.queue[0])
+                    k_pairs_wq<S, I><<<pgrid(cnt[0], 64), 256, 0, ls>>>(d_text, L, max_ctx, pl, cnt[0], depth0, out,
+                                                                         lcp, queues + 0, chunk_for(cnt[0], 16));
+                else if (cm.packed)
+                    k_pairs_packed<S, I><<<(unsigned)ceil_div(cnt[0], 64), 256, 0, ls>>>(
+                        cm.packed, cm.lgf, gref.bits, L, max_ctx, pl, cnt[0], depth0, out, lcp);
+                else
+                    k_pairs4<S, I><<<(unsigned)ceil_div(cnt[0], 64), 256, 0, ls>>>(d_text, L, max_ctx, pl, cnt[0],
+                                                                                    depth0, out, lcp);
+                LAUNCHCHK();
+                if (timed) kt.end();
+            }
+            if (timed) kt.begin(&stats.k_groups_small);
+            auto wgroups = [&](auto wtag, int c) {
+                constexpr int Wd = decltype(wtag)::value;
+                if (!cnt[c]) return;
+                constexpr uint32_t per_block = 256 / Wd;
+                if (plan.queue[c])
+                    k_wgroups_wq<S, I, Wd><<<pgrid(cnt[c], per_block), 256, 0, ls>>>(
+                        d_text, cm, gref, L, max_ctx, gl.list[c], cnt[c], depth0, out, lcp, queues + c,
+                        chunk_for(cnt[c], 64 / Wd));
+                else
+                    k_wgroups_grid<S, I, Wd><<<(unsigned)ceil_div(cnt[c], per_block), 256, 0, ls>>>(
+                        d_text, cm, gref, L, max_ctx, gl.list[c], cnt[c], depth0, out, lcp);
+            };
+            wgroups(std::integral_constant<int, 4>{}, 1);
+            wgroups(std::integral_constant<int, 8>{}, 2);
+            wgroups(std::integral_constant<int, 16>{}, 3);
+            wgroups(std::integral_constant<int, 32>{}, 4);
+            wgroups(std::integral_constant<int, 64>{}, 5);
             LAUNCHCHK();
-            kt.end();
-        }
-        kt.begin(&stats.k_groups_small);
-        auto wgroups = [&](auto wtag, int c) {
-            constexpr int Wd = decltype(wtag)::value;
-            if (!cnt[c]) return;
-            constexpr uint32_t per_block = 256 / Wd;
-            if (plan.queue[c])
-                k_wgroups_wq<S, I, Wd><<<pgrid(cnt[c], per_block), 256, 0, st>>>(
-                    d_text, cm, gref, L, max_ctx, gl.list[c], cnt[c], depth0, out, lcp, queues + c,
-                    chunk_for(cnt[c], 64 / Wd));
-            else
-                k_wgroups_grid<S, I, Wd><<<(unsigned)ceil_div(cnt[c], per_block), 256, 0, st>>>(
-                    d_text, cm, gref, L, max_ctx, gl.list[c], cnt[c], depth0, out, lcp);
-        };
-        wgroups(std::integral_constant<int, 4>{}, 1);
-        wgroups(std::integral_constant<int, 8>{}, 2);
-        wgroups(std::integral_constant<int, 16>{}, 3);
-        wgroups(std::integral_constant<int, 32>{}, 4);
-        wgroups(std::integral_constant<int, 64>{}, 5);
-        LAUNCHCHK();
-        kt.end();
-        if (cnt[6]) {
-            kt.begin(&stats.k_groups_mid);
-            k_groups<S, I, 256, kMidMax><<<cnt[6], 256, 0, st>>>(d_text, cm, gref, L, max_ctx, gl.list[6], depth0,
-                                                                  out, lcp, 0, 0);
-            LAUNCHCHK();
-            kt.end();
+            if (timed) kt.end();
+            if (cnt[6]) {
+                if (timed) kt.begin(&stats.k_groups_mid);
+                k_groups<S, I, 256, kMidMax><<<cnt[6], 256, 0, ls>>>(d_text, cm, gref, L, max_ctx, gl.list[6],
+                                                                      depth0, out, lcp, 0, 0);
+                LAUNCHCHK();
+                if (timed) kt.end();
+            }
+            gs.finish(kt, &stats.k_graph[1]);
         }
         tm.mark(PH_REFINE);
 
@@ -1115,9 +1182,11 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
             if (!W.h_jobs.empty())
                 HIPCHK(hipMemcpyAsync(dj, W.h_jobs.data(), W.h_jobs.size() * sizeof(MergeJob), hipMemcpyHostToDevice,
                                       st));
+            GraphSection gsg(W, st, graph, 2);
+            const hipStream_t gst = gsg.ls;
             for (uint64_t c0 = 0; c0 < hch.size(); c0 += 65535u * 16) {
                 const unsigned nbk = (unsigned)std::min<uint64_t>(hch.size() - c0, 65535u * 16);
-                k_groups<S, I, 256, kMidMax><<<nbk, 256, 0, st>>>(d_text, cm, gref, L, max_ctx, dc + c0, depth0, out,
+                k_groups<S, I, 256, kMidMax><<<nbk, 256, 0, gst>>>(d_text, cm, gref, L, max_ctx, dc + c0, depth0, out,
                                                                   lbuf, 1, 0);
                 LAUNCHCHK();
             }
@@ -1132,7 +1201,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
                                                              : 32;
             auto merge_level = [&](auto wtag, dim3 grid, const MergeJob *jobs) {
                 constexpr int MW = decltype(wtag)::value;
-                k_lcp_merge<S, I, MW><<<grid, 128, 0, st>>>(d_text, cm, L, max_ctx, depth0, jobs, is, ls, id, ld);
+                k_lcp_merge<S, I, MW><<<grid, 128, 0, gst>>>(d_text, cm, L, max_ctx, depth0, jobs, is, ls, id, ld);
             };
             for (int lv = 0; lv < levels; ++lv) {
                 const uint64_t run = (uint64_t)kMidMax << lv;
@@ -1151,10 +1220,11 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
             if (levels & 1) {
                 for (uint64_t c0 = 0; c0 < hch.size(); c0 += 65535u * 16) {
                     const unsigned nbk = (unsigned)std::min<uint64_t>(hch.size() - c0, 65535u * 16);
-                    k_copy_chunks<I><<<nbk, 256, 0, st>>>(dc + c0, i2, l2, out, lbuf);
+                    k_copy_chunks<I><<<nbk, 256, 0, gst>>>(dc + c0, i2, l2, out, lbuf);
                     LAUNCHCHK();
                 }
             }
+            gsg.finish(kt, &stats.k_graph[2]);
         }
         if (lcp) {
             kt.begin(&stats.k_boundary_lcp);
@@ -1538,6 +1608,7 @@ int caps_build_device_noalloc(const void *d_text, uint64_t n, uint32_t sym_bytes
             throw CapsError(CAPS_E_INVALID, "no-alloc builds take at most 2^16 segment ends");
         caps_opts_t o = opts_or_default(opts);
         o.batch_records = 0;
+        o.flags &= ~CAPS_FLAG_GRAPH;  // graph instantiation may allocate device memory
         Arena a;
         a.base = noalloc_scratch();
         a.cap = kNoAllocScratchBytes;

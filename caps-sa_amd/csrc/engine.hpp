@@ -60,6 +60,11 @@ struct Workspace {
     // router
     DevBuf r_a, r_b, r_inv, r_lo, r_hi, r_knots, r_radix, r_labels, r_stats, r_x, r_y, r_w, r_nib, r_tmp;
     std::vector<float> h_mlp;
+    // graph builds: the capture stream and the instantiated graph of each
+    // captured phase (tile sort, refinement, giant groups), updated in place
+    // by the next build that captures the same kernels
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t graph_exec[3] = {nullptr, nullptr, nullptr};
 };
 
 Workspace &workspace();

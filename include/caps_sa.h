@@ -128,6 +128,14 @@ typedef struct caps_opts {
  * every suffix of a text, u32 index, up to 1,024 classification buckets).
  * The output never changes. */
 #define CAPS_FLAG_SCATTER_DIRECT 256u
+/* Graph builds: the tile sort, the refinement and the giant-group merges of
+ * every bucket batch are each captured into a hipGraph (on a library-owned
+ * stream, default priority) and launched on the build's stream; the
+ * instantiated graphs are kept and updated in place by the next build whose
+ * phase launches the same kernels (hipGraphExecUpdate), else instantiated
+ * again.  Their kernels take their work from the device group lists and
+ * work queues as in an eager build.  The output never changes. */
+#define CAPS_FLAG_GRAPH 512u
 
 /* Fills opts with the reference defaults (max_context unbounded, p auto). */
 void caps_opts_default(caps_opts_t *opts);
@@ -219,6 +227,7 @@ int caps_build_device_filtered(const void *d_text, uint64_t n, uint32_t sym_byte
  * runs on holds until the process ends.  No hipMalloc, hipFree or allocating
  * library routine is called.  n (and n_pos) above CAPS_NOALLOC_MAX_SYMBOLS,
  * or more than CAPS_NOALLOC_MAX_ENDS segment ends, return CAPS_E_INVALID.
+ * CAPS_FLAG_GRAPH is ignored here (instantiating a graph may allocate).
  * Every call on a GPU shares that GPU's array, which keeps the last call's
  * contents (the build clears what must start at zero): calls that overlap on
  * one GPU race.  The library serializes its calls within a process and each
@@ -276,6 +285,8 @@ typedef struct caps_stats {
     uint64_t tile_records[3];      /* records sorted by each of those launches          */
     uint64_t cmp_symbols[3];       /* CAPS_FLAG_COUNT_WORK builds: symbols read by the
                                       refinement (pairs; groups of 3..64; larger groups) */
+    double k_graph[3];             /* CAPS_FLAG_GRAPH builds: launch-to-end time of the
+                                      tile-sort, refinement and giant-group graphs      */
 } caps_stats_t;
 int caps_last_stats(caps_stats_t *out);
 
