@@ -472,6 +472,7 @@ FLAG_SCATTER_PER_BATCH = 64
 FLAG_COUNT_WORK = 128
 FLAG_SCATTER_DIRECT = 256
 FLAG_GRAPH = 512
+FLAG_SEED_ESCAPE32 = 1024
 TUNE_MAX = 16
 
 

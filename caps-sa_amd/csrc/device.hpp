@@ -117,9 +117,24 @@ __device__ __forceinline__ int boundary_then_pos(const DevLimits &L, uint64_t pa
 //     are settled by refinement.
 //   batched builds: the top `idbits` bits carry the text (segment) index, so
 //     one sort orders many texts text-major; the fields sit below them.
+//   escape geometry (esc > 0; 2-bit fields, 32 per key, no shift): codes
+//     below esc (the four nucleotides of a genome) are their own fields; a
+//     code >= esc (a spacer, an N) and every field after it inside the
+//     window become the largest field value, whatever the window does later;
+//     a window end without an escape pads as the dense geometry does.  Codes
+//     >= esc order above every field value, so a strict key inequality still
+//     implies the suffix inequality (weakly order-preserving), and equal keys
+//     do not imply equal leading symbols: their refinement starts at depth 0.
 struct KeyGeom {
     uint32_t bits, k, bias, pad, shift, idbits;
+    uint32_t esc;  // 0: no escape codes
 };
+
+// Escape geometry: the fields from the first escape on, given the escape
+// indicator key e (a field is all ones where its symbol is an escape).
+__device__ __forceinline__ uint64_t escape_tail(uint64_t e) {
+    return e ? (~0ull >> ((uint32_t)__builtin_clzll(e) & ~1u)) : 0ull;
+}
 
 template <typename S>
 struct CodeMap;
@@ -205,13 +220,22 @@ __device__ __forceinline__ uint64_t key_at_bytes(const S *__restrict__ t, CodeMa
 #pragma unroll
         for (int i = 0; i < NW; ++i) w[i] = i < (int)nw ? *(const uint64_t *)(a0 + 8 * i) : 0ull;
         uint32_t lo = (uint32_t)(b0 - a0), hi = lo + take * SZ;
+        uint64_t esc = 0;  // escape geometry: the escape indicator fields
+        const uint64_t fmax = (1ull << g.bits) - 1;
 #pragma unroll
         for (int j = 0; j < NW * 8; j += SZ) {
             if ((uint32_t)j >= lo && (uint32_t)j < hi) {
                 uint32_t sym = SZ == 1 ? (uint32_t)(w[j >> 3] >> (8 * (j & 7))) & 0xFFu
                                        : (uint32_t)(w[j >> 3] >> (8 * (j & 7))) & 0xFFFFu;
-                key = (key << g.bits) | (uint64_t)(cm((S)sym) + g.bias);
+                const uint32_t c = cm((S)sym) + g.bias;
+                const bool e = g.esc && c >= g.esc;
+                key = (key << g.bits) | (e ? fmax : (uint64_t)c);
+                esc = (esc << g.bits) | (e ? fmax : 0ull);
             }
+        }
+        if (g.esc) {  // 2-bit fields, 32 per key: fields after the first escape, past the window too
+            const uint64_t tail = take < g.k ? (uint64_t)(g.k - take) * g.bits : 0;
+            if (esc) return ((key << tail) | escape_tail(esc << tail));
         }
     }
     if (take == 0) key = pad_fields(g, g.k);
@@ -258,8 +282,58 @@ __device__ __forceinline__ uint64_t key_at_seed(const S *__restrict__ t, CodeMap
 // field per position while the whole window is inside the suffix's limit.
 // caps[i] is the comparison window of position base+first+i.
 template <typename S, int CNT>
+__device__ __forceinline__ void keys_from_window_esc(const S *win, CodeMap<S> cm, const KeyGeom &g, uint32_t first,
+                                                     const uint64_t *caps, uint64_t *out) {
+    // escape geometry: 2-bit fields, 32 per key (k * bits == 64, no shift);
+    // the raw fields and the escape indicator roll together
+    const uint32_t k = g.k;
+    uint64_t ku = 0, eu = 0;
+    bool rolling = false;
+#pragma unroll
+    for (int i = 0; i < CNT; ++i) {
+        const uint32_t q = first + i;
+        const uint64_t cap = caps[i];
+        if (cap >= k) {
+            if (rolling) {
+                const uint32_t c = cm(win[q + k - 1]);
+                const bool e = c >= g.esc;
+                ku = (ku << 2) | (e ? 3ull : (uint64_t)c);
+                eu = (eu << 2) | (e ? 3ull : 0ull);
+            } else {
+                ku = 0, eu = 0;
+                for (uint32_t f = 0; f < k; ++f) {
+                    const uint32_t c = cm(win[q + f]);
+                    const bool e = c >= g.esc;
+                    ku = (ku << 2) | (e ? 3ull : (uint64_t)c);
+                    eu = (eu << 2) | (e ? 3ull : 0ull);
+                }
+                rolling = true;
+            }
+            out[i] = ku | escape_tail(eu);
+        } else {
+            const uint32_t take = (uint32_t)cap;
+            uint64_t key = 0, e = 0;
+            for (uint32_t f = 0; f < take; ++f) {
+                const uint32_t c = cm(win[q + f]);
+                const bool x = c >= g.esc;
+                key = (key << 2) | (x ? 3ull : (uint64_t)c);
+                e = (e << 2) | (x ? 3ull : 0ull);
+            }
+            const uint32_t tail = (k - take) * 2;
+            if (e) out[i] = (take ? key << tail : 0ull) | escape_tail(take ? e << tail : 0ull);
+            else out[i] = take ? ((key << tail) | pad_fields(g, k - take)) : pad_fields(g, k);
+            rolling = false;
+        }
+    }
+}
+
+template <typename S, int CNT>
 __device__ __forceinline__ void keys_from_window(const S *win, CodeMap<S> cm, const KeyGeom &g, uint32_t first,
                                                  const uint64_t *caps, uint64_t *out) {
+    if (g.esc) {
+        keys_from_window_esc<S, CNT>(win, cm, g, first, caps, out);
+        return;
+    }
     const uint32_t k = g.k, bits = g.bits;
     const uint64_t mask = (k * bits >= 64) ? ~0ull : ((1ull << (k * bits)) - 1);
     uint64_t ku = 0;

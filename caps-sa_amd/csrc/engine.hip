@@ -233,7 +233,7 @@ static void scan_alphabet(Workspace &W, const S *d_text, uint64_t n, hipStream_t
 template <typename S>
 Geometry compute_geometry(Workspace &W, const S *d_text, uint64_t n, uint32_t mode, uint32_t order, hipStream_t st,
                           bool pow2_seed = false, uint32_t policy = CAPS_SEED_POLICY_REMAP, uint64_t budget = ~0ull,
-                          uint64_t max_ctx = ~0ull) {
+                          uint64_t max_ctx = ~0ull, bool escape32 = false) {
     std::vector<uint32_t> code;
     uint32_t distinct = 0, maxv = 0;
     bool identity = true;
@@ -265,13 +265,20 @@ Geometry compute_geometry(Workspace &W, const S *d_text, uint64_t n, uint32_t mo
         uint32_t pb = 1;
         while (pb < bits) pb <<= 1;
         if (pow2_seed) bits = pb;
+        G.remap = (uint64_t)maxv >= (1ull << bits);
+        if (escape32 && sizeof(S) == 1 && distinct >= 5 && distinct <= 16) {
+            // 32 two-bit fields, codes >= 4 escape (the packed text keeps 4-bit codes)
+            bits = 2;
+            pb = 4;
+            G.g.esc = 4;
+            G.remap = (uint64_t)maxv >= 16;
+        }
         G.g.bits = bits;
         G.g.k = 64 / bits;
         G.g.shift = 64 - G.g.k * bits;
         G.g.bias = 0;
         G.g.pad = longer_first ? (uint32_t)((1ull << bits) - 1) : 0;
         G.pbits = pb;
-        G.remap = (uint64_t)maxv >= (1ull << bits);
     }
     G.reference = ref;
     if (G.remap) {
@@ -719,13 +726,14 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     // ---- alphabet and key geometry ----
     Geometry G = compute_geometry<S>(W, d_text, n, idbits ? CAPS_KEY_DENSE : o.key_geometry, L.order, st,
                                      (o.flags & CAPS_FLAG_POW2_SEED) != 0, o.packed_prefix_seed, o.remap_budget,
-                                     o.max_context);
+                                     o.max_context, !idbits && (o.flags & CAPS_FLAG_SEED_ESCAPE32) != 0);
     KeyGeom gref = G.g;  // refinement chunks: packed-text field width, full width, no text index
     if (!G.reference) {
         gref.bits = G.pbits;
         gref.k = 64 / G.pbits;
         gref.shift = 0;
         gref.pad = L.order == CAPS_ORDER_STAR ? (uint32_t)((1ull << G.pbits) - 1) : 0;
+        gref.esc = 0;  // refinement chunks hold exact codes
     }
     KeyGeom g = G.g;  // seed keys
     if (idbits) {
@@ -747,7 +755,9 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         // seed keys: the packed text itself, or a text in their narrower fields
         if (n >= (1ull << 40)) throw CapsError(CAPS_E_INVALID, "text too long for packed seed keys");
         const uint32_t K = 64 / g.bits;
-        if (g.bits == gref.bits) {
+        if (g.esc) {
+            cm.seed = nullptr;  // escape keys come from the text bytes (their fields cannot hold every code)
+        } else if (g.bits == gref.bits) {
             cm.seed = pk;
         } else {
             const uint64_t swords = n / K + 2;
@@ -932,7 +942,9 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
                                              (c == 0 ? sizeof(PairRec<I>) : sizeof(Group)));
     gl.counts = counters + 8;
     I *const lcp_all = want_lcp ? d_lcp_out : nullptr;
-    const uint64_t depth0 = g.k;
+    // symbols every tie group's members are known to share: the key's, or none
+    // for escape keys (equal escape keys may differ from the first escape on)
+    const uint64_t depth0 = g.esc ? 0 : g.k;
     // refinement of groups of 2..64, per size class: one slot per group (grid
     // launch), or persistent slots fed from a device work queue
     const RefinePlan plan = refine_plan(o);
@@ -1229,7 +1241,8 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         if (lcp) {
             kt.begin(&stats.k_boundary_lcp);
             k_boundary_lcp<I><<<grid_for(nb, 256, 16384), 256, 0, st>>>(keys, out, nb, L, max_ctx, g, lcp,
-                                                                         bi ? 1u : 0u, prev_key, prev_pos);
+                                                                         bi ? 1u : 0u, prev_key, prev_pos,
+                                                                         cm.packed, cm.lgf);
             LAUNCHCHK();
             kt.end();
             if (!single) {
@@ -1788,14 +1801,18 @@ int caps_seed_keys_device(const void *d_text, uint64_t n, uint32_t sym_bytes, co
         uint32_t order = rank ? CAPS_ORDER_STAR : CAPS_ORDER_SHORTER_FIRST;
         unsigned gr = (unsigned)ceil_div(count ? count : 1, 256);
         if (sym_bytes == 1) {
-            Geometry G = compute_geometry<uint8_t>(W, (const uint8_t *)d_text, n, key_geometry, order, st);
+            const bool esc = key_geometry == CAPS_KEY_ESCAPE32;
+            Geometry G = compute_geometry<uint8_t>(W, (const uint8_t *)d_text, n, esc ? CAPS_KEY_DENSE : key_geometry,
+                                                   order, st, false, CAPS_SEED_POLICY_REMAP, ~0ull, ~0ull, esc);
             if (key_geometry == CAPS_KEY_REFERENCE && !G.reference)
                 throw CapsError(CAPS_E_UNSUPPORTED, "no sentinel code fits the alphabet (pack.rs:174-176)");
+            if (esc && !G.g.esc) throw CapsError(CAPS_E_UNSUPPORTED, "escape keys take 5..16 symbols");
             if (geometry_out) geometry_out[0] = G.g.bits, geometry_out[1] = G.g.k;
             if (count) k_seed_keys<uint8_t><<<gr, 256, 0, st>>>((const uint8_t *)d_text, codemap<uint8_t>(G), G.g, L, d_pos, count, d_keys);
         } else {
             if (key_geometry == CAPS_KEY_REFERENCE)
                 throw CapsError(CAPS_E_UNSUPPORTED, "the reference key geometry is u8-only (pack.rs:223-228)");
+            if (key_geometry == CAPS_KEY_ESCAPE32) throw CapsError(CAPS_E_UNSUPPORTED, "escape keys are u8-only");
             Geometry G = compute_geometry<uint16_t>(W, (const uint16_t *)d_text, n, key_geometry, order, st);
             if (geometry_out) geometry_out[0] = G.g.bits, geometry_out[1] = G.g.k;
             if (count) k_seed_keys<uint16_t><<<gr, 256, 0, st>>>((const uint16_t *)d_text, codemap<uint16_t>(G), G.g, L, d_pos, count, d_keys);

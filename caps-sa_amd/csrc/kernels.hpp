@@ -1127,11 +1127,27 @@ __global__ __launch_bounds__(256) void k_copy_ranges(const Range *__restrict__ r
 // batch's last one (key prev_key, position prev_pos; the keys differ).  The
 // previous suffix is passed by value: a per-lane choice between a pointer
 // load and an argument leaves no load through an optional pointer.
+// Escape geometry: offset of the first code >= esc among the `take` (<= 32)
+// codes of suffix p, from the 4-bit packed code text; take when none.
+__device__ __forceinline__ uint32_t first_escape(const uint64_t *__restrict__ pk, uint32_t lgf, uint64_t p,
+                                                 uint32_t take, uint32_t esc) {
+    if (take == 0) return 0;
+    uint64_t hi, lo;
+    load128_codes(pk, lgf, 4, p, p + take, hi, lo);
+    // nibble >= esc (esc = 4: either of the nibble's two high bits)
+    const uint64_t m = 0xCCCCCCCCCCCCCCCCull;
+    (void)esc;
+    const uint64_t xh = hi & m, xl = lo & m;
+    const uint32_t f = xh ? (uint32_t)__builtin_clzll(xh) / 4 : xl ? 16 + (uint32_t)__builtin_clzll(xl) / 4 : 32u;
+    return f < take ? f : take;
+}
+
 template <typename I>
 __global__ __launch_bounds__(256) void k_boundary_lcp(const uint64_t *__restrict__ keys, const I *__restrict__ idx,
                                                       uint64_t N, DevLimits L, uint64_t max_ctx, KeyGeom g,
                                                       I *__restrict__ lcp, uint32_t has_prev, uint64_t prev_key,
-                                                      uint64_t prev_pos) {
+                                                      uint64_t prev_pos, const uint64_t *__restrict__ pk,
+                                                      uint32_t lgf) {
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
         if (i == 0 && !has_prev) {
@@ -1142,9 +1158,16 @@ __global__ __launch_bounds__(256) void k_boundary_lcp(const uint64_t *__restrict
         if (k0 == k1) continue;
         const uint64_t p0 = i ? (uint64_t)idx[i - 1] : prev_pos;
         uint64_t v = shared_fields(g, k0, k1);
-        uint64_t ca = cap_at(L, p0, max_ctx), cb = cap_at(L, idx[i], max_ctx);
+        const uint64_t p1 = (uint64_t)idx[i];
+        uint64_t ca = cap_at(L, p0, max_ctx), cb = cap_at(L, p1, max_ctx);
         v = v < ca ? v : ca;
         v = v < cb ? v : cb;
+        if (g.esc && v) {  // escape fields agree with T: only the fields before an escape are symbols
+            const uint32_t ea = first_escape(pk, lgf, p0, (uint32_t)v, g.esc);
+            const uint32_t eb = first_escape(pk, lgf, p1, (uint32_t)v, g.esc);
+            v = ea < v ? ea : v;
+            v = eb < v ? eb : v;
+        }
         lcp[i] = (I)v;
     }
 }

@@ -63,6 +63,8 @@ typedef struct caps_limits {
 /* Packed key geometry. */
 #define CAPS_KEY_DENSE 0u     /* ceil(log2(sigma))-bit fields, weakly order preserving */
 #define CAPS_KEY_REFERENCE 1u /* pack.rs geometry (u8 only; dense when ineligible)    */
+#define CAPS_KEY_ESCAPE32 2u  /* caps_seed_keys_device only: the CAPS_FLAG_SEED_ESCAPE32
+                                 keys (u8 text of 5..16 symbols)                      */
 
 /* Opts (sample_sort.rs:71-85) + ExtMemOpts (ext_mem.rs:58-239) + GPU knobs. */
 typedef struct caps_opts {
@@ -136,6 +138,13 @@ typedef struct caps_opts {
  * again.  Their kernels take their work from the device group lists and
  * work queues as in an eager build.  The output never changes. */
 #define CAPS_FLAG_GRAPH 512u
+/* Genome seed keys: for u8 texts of 5..16 symbols (nucleotides 0..3 plus
+ * spacers and N), 32 two-bit fields per seed key instead of ceil(log2 sigma)
+ * bits (3 bits, 21 fields for 6 symbols).  A code >= 4 and every field after
+ * it become the largest field value; the keys stay weakly order-preserving,
+ * and tie groups are refined from depth 0.  Not for batched builds.  The
+ * output never changes. */
+#define CAPS_FLAG_SEED_ESCAPE32 1024u
 
 /* Fills opts with the reference defaults (max_context unbounded, p auto). */
 void caps_opts_default(caps_opts_t *opts);
