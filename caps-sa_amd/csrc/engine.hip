@@ -961,9 +961,11 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     auto pass1 = [&](uint32_t c0, uint32_t c1, uint64_t rb) {
         kt.begin(&stats.k_scatter1);
         // contiguous positions, u32 index, at most 1,024 intervals: the
-        // LDS-staged pass with coalesced record writes
+        // LDS-staged pass with coalesced record writes.  Not for batched
+        // builds: their keys are text-major, so a step's records fall in one
+        // or two intervals and are written contiguously already.
         constexpr bool kStageable = sizeof(I) == 4;
-        if (kStageable && seq && pc <= kStageIntervals && !(o.flags & CAPS_FLAG_SCATTER_DIRECT)) {
+        if (kStageable && seq && !idbits && pc <= kStageIntervals && !(o.flags & CAPS_FLAG_SCATTER_DIRECT)) {
             KRec<uint32_t> *t32 = (KRec<uint32_t> *)tmp;
             if (d_keep)
                 k_scatter1_staged<S, true><<<nblk, 256, 0, st>>>(d_text, n_iter, cm, g, L, max_ctx, cid, pc, coff,

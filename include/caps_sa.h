@@ -127,7 +127,7 @@ typedef struct caps_opts {
 #define CAPS_FLAG_COUNT_WORK 128u
 /* Scatter pass 1 writes each record straight to its interval run instead of
  * staging every 4,096-position step in LDS first (the staged pass is used for
- * every suffix of a text, u32 index, up to 1,024 classification buckets).
+ * every suffix of one text, u32 index, up to 1,024 classification buckets).
  * The output never changes. */
 #define CAPS_FLAG_SCATTER_DIRECT 256u
 /* Graph builds: the tile sort, the refinement and the giant-group merges of
