@@ -103,7 +103,7 @@ class Stats(ctypes.Structure):
          ("k_scatter1", ctypes.c_double), ("k_scatter2", ctypes.c_double), ("batches", ctypes.c_uint64),
          ("coarse_buckets", ctypes.c_uint64), ("coarse_max", ctypes.c_uint64), ("scratch_bytes", ctypes.c_uint64),
          ("k_tile", ctypes.c_double * 3), ("tile_records", ctypes.c_uint64 * 3), ("cmp_symbols", ctypes.c_uint64 * 3),
-         ("k_graph", ctypes.c_double * 3)]
+         ("k_graph", ctypes.c_double * 3), ("runs", ctypes.c_uint64)]
 
     def as_dict(self) -> dict:
         out = {}
@@ -473,6 +473,7 @@ FLAG_COUNT_WORK = 128
 FLAG_SCATTER_DIRECT = 256
 FLAG_GRAPH = 512
 FLAG_SEED_ESCAPE32 = 1024
+FLAG_NO_RUN_MEMO = 2048
 TUNE_MAX = 16
 
 

@@ -136,6 +136,32 @@ __device__ __forceinline__ uint64_t escape_tail(uint64_t e) {
     return e ? (~0ull >> ((uint32_t)__builtin_clzll(e) & ~1u)) : 0ull;
 }
 
+// Run memo: maximal runs of one code in the packed code text that cover at
+// least kRunMinWords aligned words, as [start, end) symbol intervals sorted
+// by start, with a directory over blocks of 2^kRunDirShift symbols (dir[b]:
+// the first run whose end lies past the block's start).  Each entry proves
+// that every suffix starting inside the run begins with end - x copies of
+// the run's code, so two suffixes inside runs of the same code agree on the
+// shorter remainder: an LCP scan skips it at once.  A subset of the runs is
+// as correct as all of them (a missing run only means no skip).
+struct RunTable {
+    const uint64_t *start = nullptr, *end = nullptr;
+    const uint32_t *dir = nullptr;  // n_dir + 1 entries
+    uint64_t n_runs = 0;
+    uint64_t n_dir = 0;
+};
+constexpr uint32_t kRunDirShift = 12;
+constexpr uint32_t kRunMinWords = 3;
+
+// Symbols from x to the end of the recorded run holding x (0 if none).
+__device__ __forceinline__ uint64_t run_remaining(const RunTable &R, uint64_t x) {
+    const uint64_t b = x >> kRunDirShift;
+    if (b >= R.n_dir) return 0;
+    uint64_t i = R.dir[b];
+    while (i < R.n_runs && R.end[i] <= x) ++i;
+    return (i < R.n_runs && R.start[i] <= x) ? R.end[i] - x : 0;
+}
+
 template <typename S>
 struct CodeMap;
 // Symbol codes.  `packed`, when set, holds the whole text as codes of
@@ -150,6 +176,7 @@ template <>
 struct CodeMap<uint8_t> {
     const uint8_t *code = nullptr;  // nullptr = identity
     const uint64_t *packed = nullptr;
+    RunTable runs;
     uint32_t lgf = 0;
     uint32_t sk = 0;
     const uint64_t *seed = nullptr;
@@ -160,6 +187,7 @@ template <>
 struct CodeMap<uint16_t> {
     const uint16_t *code = nullptr;  // nullptr = identity
     const uint64_t *packed = nullptr;
+    RunTable runs;
     uint32_t lgf = 0;
     uint32_t sk = 0;
     const uint64_t *seed = nullptr;
@@ -483,6 +511,16 @@ __device__ __forceinline__ uint64_t packed_lcp(const CM &cm, uint64_t p, uint64_
             const uint64_t r = o + (uint64_t)__builtin_clzll(x) / bits;
             *gt = wp > wq;
             return r < cap ? r : cap;
+        }
+        // a matched word of one repeated code: both suffixes may sit in
+        // recorded runs of that code, which then agree on the shorter rest
+        if (cm.runs.n_runs && wp == ((wp << bits) | (wp >> (64 - bits)))) {
+            const uint64_t rp = run_remaining(cm.runs, p + o), rq = run_remaining(cm.runs, q + o);
+            const uint64_t sk = rp < rq ? rp : rq;
+            if (sk > F) {
+                o += sk;
+                continue;
+            }
         }
         o += F;
     }

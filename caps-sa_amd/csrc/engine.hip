@@ -366,6 +366,46 @@ struct KernelTimer {
     }
 };
 
+// The run memo of the packed code text (k_find_runs, then the host sorts the
+// runs by start and k_run_dir builds the block directory).  At most kRunCap
+// runs are kept; a subset is as correct as all of them.
+constexpr uint64_t kRunCap = 1ull << 20;
+template <typename S>
+static RunTable build_run_memo(Workspace &W, const CodeMap<S> &cm, uint64_t n, hipStream_t st) {
+    RunTable R{};
+    if (!cm.packed || n == 0) return R;
+    const uint64_t nwords = (n >> cm.lgf) + 1;
+    const uint64_t cap = std::min<uint64_t>(kRunCap, nwords / kRunMinWords + 1);  // runs the text can hold
+    uint64_t *rs = (uint64_t *)W.run_s.get(cap * 8), *re = (uint64_t *)W.run_e.get(cap * 8);
+    unsigned long long *cnt = (unsigned long long *)W.run_count.get(8);
+    HIPCHK(hipMemsetAsync(cnt, 0, 8, st));
+    k_find_runs<<<grid_for(nwords, 256, 8192), 256, 0, st>>>(cm.packed, nwords, cm.lgf, n, rs, re, cnt, cap);
+    LAUNCHCHK();
+    unsigned long long total = 0;
+    HIPCHK(hipMemcpyAsync(&total, cnt, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const uint64_t m = std::min<uint64_t>(total, cap);
+    if (m == 0) return R;
+    std::vector<uint64_t> hs(m), he(m);
+    HIPCHK(hipMemcpyAsync(hs.data(), rs, m * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(he.data(), re, m * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    W.h_runs.resize(m);
+    for (uint64_t i = 0; i < m; ++i) W.h_runs[i] = {hs[i], he[i]};
+    std::sort(W.h_runs.begin(), W.h_runs.end());
+    for (uint64_t i = 0; i < m; ++i) hs[i] = W.h_runs[i].first, he[i] = W.h_runs[i].second;
+    uint64_t *ss = (uint64_t *)W.run_ss.get(m * 8), *se = (uint64_t *)W.run_se.get(m * 8);
+    HIPCHK(hipMemcpyAsync(ss, hs.data(), m * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(se, he.data(), m * 8, hipMemcpyHostToDevice, st));
+    const uint64_t n_dir = (n >> kRunDirShift) + 1;
+    uint32_t *dir = (uint32_t *)W.run_dir.get((n_dir + 1) * 4);
+    k_run_dir<<<(unsigned)ceil_div(n_dir + 1, 256), 256, 0, st>>>(se, m, n_dir, dir);
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(st));  // the host vectors are pageable staging for the copies above
+    R.start = ss, R.end = se, R.dir = dir, R.n_runs = m, R.n_dir = n_dir;
+    return R;
+}
+
 // A phase of the build captured into a hipGraph (CAPS_FLAG_GRAPH): while
 // the section is open its launches go to a private capture stream (`ls`);
 // finish() ends the capture, updates the phase's instantiated graph in place
@@ -945,6 +985,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     // symbols every tie group's members are known to share: the key's, or none
     // for escape keys (equal escape keys may differ from the first escape on)
     const uint64_t depth0 = g.esc ? 0 : g.k;
+    bool runs_built = false;
     // refinement of groups of 2..64, per size class: one slot per group (grid
     // launch), or persistent slots fed from a device work queue
     const RefinePlan plan = refine_plan(o);
@@ -1099,6 +1140,13 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         stats.mid_groups += cnt[6];
         stats.giant_groups += cnt[7];
         tm.mark(PH_DETECT);
+
+        // the run memo, once per build, for the LCP scans of larger groups
+        if ((cnt[6] || cnt[7]) && !runs_built && !(o.flags & CAPS_FLAG_NO_RUN_MEMO)) {
+            cm.runs = build_run_memo<S>(W, cm, n, st);
+            runs_built = true;
+            stats.runs = cm.runs.n_runs;
+        }
 
         // ---- refinement ----
         {

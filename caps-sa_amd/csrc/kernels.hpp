@@ -1231,9 +1231,77 @@ __device__ __forceinline__ uint64_t lcp_codes(const S *__restrict__ t, const Cod
                 return r < limit ? r : limit;
             }
         }
+        // the first word matched and repeats one code: recorded runs of that
+        // code under both suffixes agree on the shorter rest (run memo)
+        if (cm.runs.n_runs && va[0] == ((va[0] << bits) | (va[0] >> (64 - bits)))) {
+            const uint64_t ra = run_remaining(cm.runs, a + o), rb = run_remaining(cm.runs, b + o);
+            const uint64_t sk = ra < rb ? ra : rb;
+            if (sk > 2ull * F) {
+                o += sk;
+                continue;
+            }
+        }
         o += 2ull * F;
     }
     return limit;
+}
+
+// Run memo construction, pass 1: every first word of a stretch of aligned
+// words that all repeat one code (and that the previous word does not
+// continue) measures its stretch; stretches of kRunMinWords words or more are
+// extended to the exact run (codes of that value in the neighbouring words)
+// and appended, up to `cap` runs (a subset is still a correct memo).
+__global__ void k_find_runs(const uint64_t *__restrict__ pk, uint64_t nwords, uint32_t lgf, uint64_t n,
+                            uint64_t *__restrict__ rs, uint64_t *__restrict__ re, unsigned long long *__restrict__ count,
+                            uint64_t cap) {
+    const uint32_t bits = 64u >> lgf;
+    const uint64_t F = 1ull << lgf;
+    auto uniform = [&](uint64_t v) { return v == ((v << bits) | (v >> (64 - bits))); };
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords;
+         w += (uint64_t)gridDim.x * blockDim.x) {
+        if ((w << lgf) >= n) break;
+        const uint64_t v = pk[w];
+        if (!uniform(v)) continue;
+        if (w > 0 && pk[w - 1] == v) continue;  // not the stretch's first word
+        uint64_t m = 1;
+        while (((w + m) << lgf) < n && pk[w + m] == v) ++m;
+        if (m < kRunMinWords) continue;
+        const uint64_t code = v >> (64 - bits);
+        // codes of the same value just before and just after the stretch
+        uint64_t back = 0, fwd = 0;
+        if (w > 0) {
+            const uint64_t pv = pk[w - 1];
+            while (back < F && ((pv >> (back * bits)) & ((1ull << bits) - 1)) == code) ++back;
+        }
+        if (((w + m) << lgf) < n) {
+            const uint64_t nv = pk[w + m];
+            while (fwd < F && ((nv >> (64 - (fwd + 1) * bits)) & ((1ull << bits) - 1)) == code) ++fwd;
+        }
+        const uint64_t s = (w << lgf) - back;
+        uint64_t e = ((w + m) << lgf) + fwd;
+        if (e > n) e = n;
+        const unsigned long long slot = atomicAdd(count, 1ull);
+        if (slot < cap) {
+            rs[slot] = s;
+            re[slot] = e;
+        }
+    }
+}
+
+// Run memo construction, pass 2: dir[b] = the first run (sorted) whose end
+// lies past block b's start, for b in [0, n_dir].
+__global__ void k_run_dir(const uint64_t *__restrict__ re, uint64_t n_runs, uint64_t n_dir,
+                          uint32_t *__restrict__ dir) {
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > n_dir) return;
+    const uint64_t x = b << kRunDirShift;
+    uint64_t lo = 0, hi = n_runs;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (re[mid] <= x) lo = mid + 1;
+        else hi = mid;
+    }
+    dir[b] = (uint32_t)lo;
 }
 
 // Class keys that members can share and refine further: a differing code

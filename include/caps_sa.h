@@ -145,6 +145,12 @@ typedef struct caps_opts {
  * and tie groups are refined from depth 0.  Not for batched builds.  The
  * output never changes. */
 #define CAPS_FLAG_SEED_ESCAPE32 1024u
+/* Build without the run memo: by default, when a build has groups of 65 or
+ * more tied suffixes, the maximal runs of one code (48+ symbols) of the
+ * packed text are recorded once, and the LCP scans of those groups skip the
+ * common part of two runs of the same code at once.  The output never
+ * changes. */
+#define CAPS_FLAG_NO_RUN_MEMO 2048u
 
 /* Fills opts with the reference defaults (max_context unbounded, p auto). */
 void caps_opts_default(caps_opts_t *opts);
@@ -296,6 +302,7 @@ typedef struct caps_stats {
                                       refinement (pairs; groups of 3..64; larger groups) */
     double k_graph[3];             /* CAPS_FLAG_GRAPH builds: launch-to-end time of the
                                       tile-sort, refinement and giant-group graphs      */
+    uint64_t runs;                 /* runs recorded in the run memo (0: not built)      */
 } caps_stats_t;
 int caps_last_stats(caps_stats_t *out);
 
