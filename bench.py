@@ -64,8 +64,10 @@ def launch_bytes(st: dict, n: int, w: int, I: int) -> dict:
     """Algorithmic HBM bytes of each single-launch kernel of one build (what
     the kernel must read and write at the least), from the build's own
     counts.  Text bytes are w per symbol; records are R = 8 + I bytes (key and
-    index).  The refinement's text bytes are the symbols it actually read (a
-    CAPS_FLAG_COUNT_WORK build's device counters) times w."""
+    index).  The pair refinement reads the packed code text (fields of pb
+    bits, the power of two at or above the seed field width): its text bytes
+    are the symbols it actually read (a CAPS_FLAG_COUNT_WORK build's device
+    counters) times pb / 8."""
     R = 8 + I
     out = {
         "k_classify": n * (w + 2),            # text once + coarse interval id (u16) written
@@ -76,8 +78,11 @@ def launch_bytes(st: dict, n: int, w: int, I: int) -> dict:
         "k_tile_4096": st["tile_records"][2] * 2 * R,
     }
     if st.get("cmp_symbols") and any(st["cmp_symbols"]):
-        # pair records (start + two positions) read, two positions written, text compared
-        out["k_pairs"] = st["pairs"] * (8 + 4 * I) + st["cmp_symbols"][0] * w
+        # pair records (start + two positions) read, two positions written, packed text compared
+        pb = 1
+        while pb < st["key_bits"]:
+            pb *= 2
+        out["k_pairs"] = st["pairs"] * (8 + 4 * I) + st["cmp_symbols"][0] * pb / 8
     return out
 
 
