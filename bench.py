@@ -316,6 +316,7 @@ def run_genome(args, n: int, filtered: bool = False) -> dict:
 
         def step():
             dev.build_device_filtered(d_text, n, 1, d_keep, n_pos, idx_bytes=8, lp=lp, d_out=d_out, flags=flags,
+                                      gpu_buckets=args.genome_buckets,
                                       batch_records=args.batch_records)
             return caps_sa.last_stats()
     else:
@@ -325,6 +326,7 @@ def run_genome(args, n: int, filtered: bool = False) -> dict:
 
         def step():
             dev.build_device(d_text, n, 1, d_positions=d_pos, n_pos=n_pos, idx_bytes=8, lp=lp, d_out=d_out,
+                             gpu_buckets=args.genome_buckets,
                              flags=flags, batch_records=args.batch_records)
             return caps_sa.last_stats()
     print(f"[bench] genome: n={n} segments={n_seg} positions={n_pos} "
@@ -338,7 +340,7 @@ def run_genome(args, n: int, filtered: bool = False) -> dict:
     stats = []
     for i in range(args.steps):
         stats.append(step())
-        print(f"[bench] step {i}: {stats[-1]['ms_total']:.1f} ms", file=sys.stderr, flush=True)
+        print(f"[bench] step {i}: {stats[-1]['ms_total']:.1f} ms (scatter {stats[-1]['ms_scatter']:.1f}, giant {stats[-1]['ms_giant']:.1f})", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     ms_step = (time.perf_counter() - t0) * 1e3 / max(args.steps, 1)
     verified = None
@@ -362,7 +364,7 @@ def run_genome(args, n: int, filtered: bool = False) -> dict:
               "ms_refine", "ms_giant")
     last = stats[-1]
     return {"ms_step": ms_step, "n_pos": n_pos, "n": n, "segments": n_seg, "verified": verified,
-            "batches": last["batches"],
+            "batches": last["batches"], "runs": last["runs"], "classification_buckets": last["coarse_buckets"],
             "kernels": avg, "phases": {ph: round(sum(st[ph] for st in stats) / len(stats), 3) for ph in phases},
             "groups": {k: last[k] for k in ("pairs", "small_groups", "mid_groups", "giant_groups", "oversized",
                                              "buckets", "bucket_max")}, "key_k": last["key_k"]}
@@ -386,6 +388,8 @@ def main() -> int:
     ap.add_argument("--tile-sort", default="merge", choices=["merge", "radix"])
     ap.add_argument("--bucket-keys", type=int, default=0, help="mean records per bucket (0: library default)")
     ap.add_argument("--texts", type=int, default=4096, help="batch workload: number of texts")
+    ap.add_argument("--genome-buckets", type=int, default=0,
+                    help="genome workloads: classification buckets (0: library default)")
     ap.add_argument("--batch-records", type=int, default=0,
                     help="genome workloads: records per bucket batch (0: one batch while it fits in HBM)")
     ap.add_argument("--flags", type=int, default=0, help="extra CAPS_FLAG_* bits")
@@ -482,6 +486,7 @@ def main() -> int:
             "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
             "config": {"workload": args.workload, "n": r["n"], "segments": r["segments"], "positions": n_pos,
                        "text": desc, "index": "u64", "key_symbols": r["key_k"], "bucket_batches": r["batches"],
+                       "classification_buckets": r["classification_buckets"], "run_memo_runs": r["runs"],
                        "seed": args.seed},
             "verified_bit_exact": r["verified"], "phases_ms": r["phases"],
             "kernels_ms": {k: round(v, 3) for k, v in avg.items()}, "groups": r["groups"],

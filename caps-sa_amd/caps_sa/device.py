@@ -70,7 +70,7 @@ def build_device_noalloc(d_text: torch.Tensor, n: int, sym_bytes: int, d_positio
 
 def build_device_filtered(d_text: torch.Tensor, n: int, sym_bytes: int, d_keep: torch.Tensor, capacity: int,
                           idx_bytes=8, lp: LimitProvider = None, max_context=MAX_CONTEXT, router=0, d_out=None,
-                          d_lcp=None, want_lcp=False, flags=0, tile_keys=0, batch_records=0):
+                          d_lcp=None, want_lcp=False, flags=0, tile_keys=0, batch_records=0, gpu_buckets=0):
     """caps_build_device_filtered: text and kept-position bitmap ((n + 63) // 64
     words, as keep_bitmap builds) in HBM.  Returns (sa, lcp, kept count); the
     outputs hold `capacity` entries, of which the first `kept` are filled."""
@@ -80,7 +80,7 @@ def build_device_filtered(d_text: torch.Tensor, n: int, sym_bytes: int, d_keep: 
         d_lcp = torch.empty(max(capacity, 1), dtype=_IDX[idx_bytes], device=d_text.device)
     lim = (lp or PlainText(n))._c()
     o = _c_opts(max_context, router=router, want_lcp=int(d_lcp is not None), flags=flags, tile_keys=tile_keys,
-                batch_records=batch_records)
+                batch_records=batch_records, gpu_buckets=gpu_buckets)
     kept = ctypes.c_uint64(0)
     rc = lib().caps_build_device_filtered(d_text.data_ptr(), n, sym_bytes, d_keep.data_ptr(), idx_bytes,
                                           ctypes.byref(lim), ctypes.byref(o), d_out.data_ptr(), capacity,

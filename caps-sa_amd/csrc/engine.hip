@@ -657,13 +657,21 @@ static BatchPlan plan_batches(const Workspace &W, uint64_t n_pos, uint64_t reque
     if (g_arena) return {n_pos, false};  // no-alloc builds: one batch, sized by the scratch
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || free_b == 0) return {requested ? requested : n_pos, false};
+    // every buffer this build allocates after planning counts as free, so
+    // that a repeated build of the same input plans the same batches (a
+    // different plan would regrow multi-GB buffers in the middle of a build)
     size_t held = W.recs.cap + W.keys.cap + W.fid.cap + W.sinkbuf.cap + W.keys2.cap + W.idx2.cap + W.lcp.cap +
-                  W.lcp2.cap + W.tiles.cap;
+                  W.lcp2.cap + W.tiles.cap + W.big.cap + W.chunks.cap + W.jobs.cap + W.run_s.cap + W.run_e.cap +
+                  W.run_ss.cap + W.run_se.cap + W.run_dir.cap;
     for (int c = 0; c < kGroupClasses; ++c) held += W.glist[c].cap;
     const double avail = (double)(free_b + held) * 0.75;
     if (!requested && (double)n_pos * 112 <= avail) return {n_pos, false};  // one batch
     const double rest = avail - (double)n_pos * rec_bytes;  // beside all records
-    const uint64_t b_each = requested ? requested : std::max<uint64_t>(1ull << 24, (uint64_t)(avail / 112));
+    // batch sizes in whole quanta of 2^24 records: small changes in free
+    // memory between builds leave the plan as it was
+    constexpr uint64_t kQuantum = 1ull << 24;
+    const uint64_t b_each =
+        requested ? requested : std::max<uint64_t>(kQuantum, (uint64_t)(avail / 112) / kQuantum * kQuantum);
     if (!per_batch_pass1 && rest > 0) {
         if (requested) {
             if ((double)requested * 96 <= rest) return {requested, true};
@@ -671,7 +679,7 @@ static BatchPlan plan_batches(const Workspace &W, uint64_t n_pos, uint64_t reque
             // one pass 1 for all records, unless it costs more than one extra
             // batch: pass 2 runs one workgroup per coarse interval, so a batch
             // of few intervals leaves compute units idle
-            const uint64_t b = (uint64_t)(rest / 96);
+            const uint64_t b = (uint64_t)(rest / 96) / kQuantum * kQuantum;
             if (b && ceil_div(n_pos, b) <= ceil_div(n_pos, b_each) + 1) return {std::min<uint64_t>(b, n_pos), true};
         }
     }
@@ -826,9 +834,15 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     // `target` records each.
     if (o.gpu_buckets > kCoarseBucketsMax) throw CapsError(CAPS_E_INVALID, "gpu_buckets above 4096");
     const uint64_t want = std::max<uint64_t>(1, ceil_div(n_pos, target));
+    // 1,024 classification buckets where scatter pass 1 is staged in LDS
+    // (every suffix of one text, u32 index; it takes at most 1,024), else
+    // 2,048: smaller intervals keep scatter pass 2's LDS small enough for
+    // several workgroups per compute unit
+    const bool stageable = sizeof(I) == 4 && !idbits && !d_keep && d_pos == nullptr && n_pos == n;
+    const uint64_t c_default = stageable ? kStageIntervals : 2048;
     uint64_t C = o.gpu_buckets ? o.gpu_buckets
                                : std::min<uint64_t>(kCoarseBucketsMax,
-                                                    std::max<uint64_t>(std::min<uint64_t>(want, 1024),
+                                                    std::max<uint64_t>(std::min<uint64_t>(want, c_default),
                                                                        ceil_div(want, kFineMax)));
     C = std::max<uint64_t>(1, std::min<uint64_t>(C, n_pos));
     uint64_t stride = std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(want, C), n_pos / C));
