@@ -1,15 +1,17 @@
-// LDS radix sort of one tile of (key, idx) records, over only the key bits
-// that vary inside the tile, with tie-group emission for tiles that hold a
-// whole bucket.
+// Sorts of one tile of (key, idx) records in LDS, over only the key bits that
+// vary inside the tile, with tie-group emission for tiles that hold a whole
+// bucket.  Tiles are launched by size class (1024 / 2048 / 4096 records).
 //
-// Items are held blocked (thread t owns items t*ITEMS .. t*ITEMS+ITEMS-1,
-// consecutive in the current order).  Each 4-bit digit pass counts digits in
-// registers (sixteen 8-bit fields in two words per thread), writes the
-// counts to LDS columns cnt[digit][thread], scans them digit-major, and
-// scatters: thread order = item order, so the sort is stable without atomics
-// or ballots.  LDS rows are padded by one slot per
-// thread's run to keep the blocked reads conflict-light.  Tiles are launched
-// by size class (1024 / 2048 / 4096 records).
+// k_tile_radix (CAPS_FLAG_TILE_RADIX): 4-bit digit passes.  Items are held
+// blocked (thread t owns items t*ITEMS .. t*ITEMS+ITEMS-1, consecutive in the
+// current order).  Each pass counts digits in registers (sixteen 8-bit fields
+// in two words per thread), writes the counts to LDS columns
+// cnt[digit][thread], scans them digit-major, and scatters: thread order =
+// item order, so the sort is stable without atomics or ballots.  LDS rows are
+// padded by one slot per thread's run to keep the blocked reads
+// conflict-light.
+//
+// k_tile_merge (the default): a merge sort, described below.
 #pragma once
 #include "device.hpp"
 #include "refine.hpp"
@@ -180,24 +182,24 @@ __global__ __launch_bounds__(256) void k_tile_radix(uint64_t *__restrict__ keys,
 namespace caps {
 
 // ---------------------------------------------------------------------------
-// Merge-path block merge sort of one tile (alternative to the radix sort):
-// every thread sorts ITEMS = T/256 records in registers (odd-even
-// transposition network), then eight merge levels double the sorted runs
-// (8 -> 2048 for T = 2048): each thread finds where its ITEMS outputs start on
-// the merge path of its two input runs (binary search in LDS) and merges them
-// serially.  Records compare by (key, index in the tile), a strict total
-// order, so the sort is stable, pads (key ~0, index >= len) stay behind every
-// real record, and no two records tie.  LDS holds the keys and tile indices
-// (blocked, one pad slot per thread run) and the positions.
+// Merge sort of one tile (the default tile sort).  Records are unique u64
+// values: the key bits that vary inside the tile above the tile index (a
+// strict total order, so the sort is stable and no two records tie).  Every
+// thread sorts ITEMS = T/256 records in registers (Batcher's odd-even merge
+// network), then log2(256) merge levels double the sorted runs.  Runs are
+// kept in LDS in a bitonic layout: even runs ascending, odd runs descending.
+// The merge of an ascending A and a descending B then reads, past the end of
+// either run, the other run's largest records, which are never taken before
+// a smaller one: the serial merge needs no bounds tests and no branches, and
+// every read stays inside the two runs.  The merge path of each thread is
+// found by bit descent over its diagonal.
+//
+// The sort buffer is the only large LDS array (18.4 KB for 2,048 records):
+// eight workgroups per compute unit.  After the sort the same buffer stages
+// the tile's index values (original order, then sorted) and then the sorted
+// keys, so both outputs are written with coalesced stores.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool rec_less(uint64_t ka, uint32_t la, uint64_t kb, uint32_t lb) {
-    return ka < kb || (ka == kb && la < lb);
-}
 
-// Block merge sort of B*ITEMS records held blocked in registers (thread t
-// holds records t*ITEMS .. t*ITEMS+ITEMS-1), through the padded LDS arrays
-// sk/sl.  PACKED: the records are unique u64 values (key bits and tile index
-// packed together; li unused); otherwise (key, li) pairs compared as above.
 // Orders this wavefront's LDS accesses: every earlier LDS write of the
 // wavefront is complete and visible to its lanes before any later LDS access.
 __device__ __forceinline__ void wave_lds_sync() {
@@ -206,229 +208,266 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int B, int ITEMS, bool PACKED>
-__device__ __forceinline__ void block_merge_sort(uint64_t (&k)[ITEMS], uint32_t (&li)[ITEMS], uint64_t *sk,
-                                                 uint16_t *sl, uint32_t len) {
+// Batcher's odd-even merge sort network over N registers: cas(a, b) for each
+// comparator a < b (19 comparators for N = 8, 63 for N = 16).
+template <int N, typename F>
+__device__ __forceinline__ void batcher_network(F &&cas) {
+#pragma unroll
+    for (int p = 1; p < N; p <<= 1)
+#pragma unroll
+        for (int k = p; k >= 1; k >>= 1)
+#pragma unroll
+            for (int j = k % p; j + k < N; j += 2 * k)
+#pragma unroll
+                for (int i = 0; i < (k < N - j - k ? k : N - j - k); ++i)
+                    if ((i + j) / (2 * p) == (i + j + k) / (2 * p)) cas(i + j, i + j + k);
+}
+
+// Exclusive max-scan of one u32 per thread across a 256-thread block (0 for
+// thread 0).
+__device__ __forceinline__ uint32_t block_excl_max256(uint32_t v, uint32_t *wmax) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x = x > y ? x : y;
+    }
+    const uint32_t ex = __shfl_up(x, 1);
+    if (lane == 63) wmax[w] = x;
+    __syncthreads();
+    uint32_t before = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (q < w) before = before > wmax[q] ? before : wmax[q];
+    const uint32_t mine = lane ? ex : 0u;
+    return before > mine ? before : mine;
+}
+
+// Sorts the B*ITEMS unique records held blocked in registers (thread t holds
+// positions t*ITEMS .. t*ITEMS+ITEMS-1); positions >= len hold pads (~0),
+// above every real record.  sk: the padded LDS buffer (one pad slot per
+// ITEMS).  Returns with the sorted records in the same blocked registers.
+template <int B, int ITEMS>
+__device__ __forceinline__ void block_merge_sort(uint64_t (&k)[ITEMS], uint64_t *sk, uint32_t len) {
     constexpr uint32_t T = B * ITEMS;
     auto px = [](uint32_t i) { return i + i / ITEMS; };
-    auto less = [](uint64_t ka, uint32_t la, uint64_t kb, uint32_t lb) {
-        return PACKED ? ka < kb : rec_less(ka, la, kb, lb);
-    };
-    const int tid = threadIdx.x;
-    // Records with index >= len are pads, above every real record.  Merge
-    // sort keeps index ranges together (the merged run [a0, a0 + 2w) holds
-    // exactly the records of indices in that range), so a run's real records
-    // come first and their count is known: outputs past it are pads, and a
-    // second run made only of pads leaves the first one in place.
-    const bool all_pads = (uint32_t)tid * ITEMS >= len;
-    // register sort: odd-even transposition network over ITEMS records
-#pragma unroll
-    for (int round = 0; round < ITEMS && !all_pads; ++round) {
-#pragma unroll
-        for (int j = round & 1; j + 1 < ITEMS; j += 2) {
-            if (less(k[j + 1], li[j + 1], k[j], li[j])) {
-                uint64_t tk = k[j];
-                k[j] = k[j + 1];
-                k[j + 1] = tk;
-                if (!PACKED) {
-                    uint32_t tli = li[j];
-                    li[j] = li[j + 1];
-                    li[j + 1] = tli;
-                }
-            }
-        }
+    const uint32_t tid = threadIdx.x;
+    const uint32_t p0 = tid * ITEMS;
+    if (p0 < len) {
+        batcher_network<ITEMS>([&](int a, int b) {
+            const uint64_t ta = k[a], tb = k[b];
+            const bool sw = tb < ta;
+            k[a] = sw ? tb : ta;
+            k[b] = sw ? ta : tb;
+        });
     }
-    // merge levels: runs of w records -> runs of 2w.  While a merged run
-    // (2w records, 2w / ITEMS threads) lies inside one wavefront, only that
+    // While a merged run (2w records) lies inside one wavefront, only that
     // wavefront reads the LDS it wrote: a wavefront fence replaces the
-    // workgroup barrier, and the wavefronts of the block run independently.
+    // workgroup barrier.
     constexpr uint32_t kWaveSpan = 64 * ITEMS;
 #pragma unroll 1
     for (uint32_t w = ITEMS; w < T; w *= 2) {
         const bool in_wave = 2 * w <= kWaveSpan;
         if (in_wave) wave_lds_sync();
         else __syncthreads();
+        {
+            // this thread's ITEMS records of run r = p0 / w, at value ranks
+            // q0 .. q0+ITEMS-1 of the run: an aligned group of ITEMS slots,
+            // reversed in an odd (descending) run
+            const uint32_t r = p0 / w, q0 = p0 - r * w;
+            const bool desc = r & 1u;
+            const uint32_t g = px(desc ? (r + 1) * w - ITEMS - q0 : p0);
 #pragma unroll
-        for (int j = 0; j < ITEMS; ++j) {
-            sk[px(tid * ITEMS + j)] = k[j];
-            if (!PACKED) sl[px(tid * ITEMS + j)] = (uint16_t)li[j];
+            for (int j = 0; j < ITEMS; ++j) sk[g + (desc ? ITEMS - 1 - j : j)] = k[j];
         }
         if (in_wave) wave_lds_sync();
         else __syncthreads();
-        const uint32_t per = (2 * w) / ITEMS;  // threads per merged run
-        const uint32_t a0 = (tid / per) * 2 * w, b0 = a0 + w;
-        const uint32_t diag = (tid % per) * ITEMS;
+        const uint32_t lgper = __builtin_ctz(2 * w / ITEMS);  // threads per merged run: a power of two
+        const uint32_t q = tid & ((1u << lgper) - 1);
+        const uint32_t a0 = (tid >> lgper) * 2 * w, b0 = a0 + w;
+        const uint32_t diag = q * ITEMS;
         const uint32_t realc = len > a0 ? (len - a0 < 2 * w ? len - a0 : 2 * w) : 0;
-        if (diag >= realc || b0 >= len) {  // pads only, or the second run is pads
+        // Merge sort keeps index ranges together, so a run's real records come
+        // first and their count is known.  Outputs past it are pads; when B
+        // holds only pads the merged run is A (all of whose records are real
+        // whenever B has one) followed by pads.
+        if (diag >= realc || b0 >= len) {
 #pragma unroll
-            for (int j = 0; j < ITEMS; ++j) {
-                const uint32_t q = diag + j;
-                if (q < realc) {
-                    k[j] = sk[px(a0 + q)];
-                    if (!PACKED) li[j] = sl[px(a0 + q)];
-                } else {
-                    k[j] = ~0ull;
-                    if (!PACKED) li[j] = 0xFFFFu;
-                }
-            }
+            for (int j = 0; j < ITEMS; ++j) k[j] = diag + j < realc ? sk[px(a0 + diag + j)] : ~0ull;
             continue;
         }
-        // merge path: the first x in [lo, hi) with B[diag-1-x] < A[x] (hi if
-        // none), by bit descent over a range of at most w + 1 candidates: the
-        // loop runs the same log2(w) + 1 steps on every lane
+        // B's i-th smallest record is at slot bt - i.  Merge path: the largest
+        // x in [lo, hi] with A[x-1] <= B[diag-x] (bit descent, the same
+        // log2(w) + 1 steps on every lane).
+        const uint32_t bt = b0 + w - 1;
         const uint32_t lo = diag > w ? diag - w : 0, hi = diag < w ? diag : w;
         uint32_t x = lo;
         for (uint32_t st = w; st; st >>= 1) {
-            const uint32_t c = x + st;  // candidate count of A records before the split
-            if (c <= hi) {
-                const uint32_t bi = b0 + diag - c, ai = a0 + c - 1;
-                const bool bl = PACKED ? sk[px(bi)] < sk[px(ai)]
-                                       : rec_less(sk[px(bi)], sl[px(bi)], sk[px(ai)], sl[px(ai)]);
-                if (!bl) x = c;
-            }
+            const uint32_t c = x + st;
+            if (c <= hi && !(sk[px(bt - (diag - c))] < sk[px(a0 + c - 1)])) x = c;
         }
-        uint32_t a = a0 + x, b = b0 + (diag - x);
-        const uint32_t ae = a0 + w, be = b0 + w;
-        uint64_t ka = a < ae ? sk[px(a)] : ~0ull, kb = b < be ? sk[px(b)] : ~0ull;
-        uint32_t la = 0, lb = 0;
-        if (!PACKED) {
-            la = a < ae ? sl[px(a)] : 0xFFFFFFFFu;
-            lb = b < be ? sl[px(b)] : 0xFFFFFFFFu;
-        }
-        // serial merge without branches: each step refills only the side it
-        // took, with one LDS read
+        // Serial merge: pa walks A up, pb walks B's slots down.  A exhausted,
+        // pa reads B's largest record; B exhausted, pb reads A's largest: the
+        // other side is then always taken (a tie is the same slot, read by
+        // both, or pads, whose order does not matter).
+        uint32_t pa = a0 + x, pb = bt - (diag - x);
+        uint64_t ka = sk[px(pa)], kb = sk[px(pb)];
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
-            const bool take_a = b >= be || (a < ae && !less(kb, lb, ka, la));
-            k[j] = take_a ? ka : kb;
-            if (!PACKED) li[j] = take_a ? la : lb;
-            a += take_a ? 1u : 0u;
-            b += take_a ? 0u : 1u;
-            const uint32_t nx = take_a ? a : b;
-            const bool in = nx < (take_a ? ae : be);
-            const uint64_t nk = in ? sk[px(nx)] : ~0ull;
-            ka = take_a ? nk : ka;
-            kb = take_a ? kb : nk;
-            if (!PACKED) {
-                const uint32_t nl = in ? sl[px(nx)] : 0xFFFFFFFFu;
-                la = take_a ? nl : la;
-                lb = take_a ? lb : nl;
+            const bool ta = ka <= kb;
+            k[j] = ta ? ka : kb;
+            pa += ta ? 1u : 0u;
+            pb -= ta ? 0u : 1u;
+            if (j + 1 < ITEMS) {
+                const uint64_t nk = sk[px(ta ? pa : pb)];
+                ka = ta ? nk : ka;
+                kb = ta ? kb : nk;
             }
         }
     }
-    __syncthreads();
 }
 
+// One tile per workgroup: sort, write keys and indices in place, and (for a
+// tile that holds a whole bucket, when emit is set) append its tie groups to
+// the group lists.  Tiles whose keys vary in more than 64 - log2(T) bits sort
+// twice, stably: by the low bits, then by the high bits and the rank of the
+// first pass.
 template <typename I, int T>
-__global__ __launch_bounds__(256) void k_tile_merge(uint64_t *__restrict__ keys, I *__restrict__ idx,
-                                                    const Tile *__restrict__ tiles, GroupLists gl, int emit) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T <= 2048 ? 8 : 4))) void k_tile_merge(
+    uint64_t *__restrict__ keys, I *__restrict__ idx, const Tile *__restrict__ tiles, GroupLists gl, int emit) {
     constexpr int B = 256;
     constexpr int ITEMS = T / B;
     constexpr int PADT = T + B;
     constexpr int LB = __builtin_ctz(T);  // tile index bits
+    static_assert(sizeof(I) * T <= sizeof(uint64_t) * PADT, "index staging fits the sort buffer");
+    static_assert(3 * LB <= 64, "two-pass records: high bits, rank, tile index");
     __shared__ uint64_t sk[PADT];
-    __shared__ uint16_t sl[PADT];
-    __shared__ I si[T];
     __shared__ unsigned long long s_or;
-    __shared__ uint32_t gcnt[kGroupClasses], gbase[kGroupClasses];
+    __shared__ uint32_t gcnt[kGroupClasses], gbase[kGroupClasses], wmax[4];
+    __shared__ I wlast[4];
     auto px = [](uint32_t i) { return i + i / ITEMS; };
+    I *stage = (I *)sk;
 
     const Tile tl = tiles[blockIdx.x];
     const uint32_t len = tl.len;
-    const int tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x;
     if (len == 0) return;
     const uint64_t first = keys[tl.start];
     uint64_t my_or = 0;
     for (uint32_t i = tid; i < (uint32_t)T; i += B) {
+        uint64_t kk = ~0ull;
         if (i < len) {
-            const uint64_t kk = keys[tl.start + i];
-            sk[px(i)] = kk;
-            si[i] = idx[tl.start + i];
+            kk = keys[tl.start + i];
             my_or |= kk ^ first;
-        } else {
-            sk[px(i)] = ~0ull;
         }
-        sl[px(i)] = (uint16_t)i;
+        sk[px(i)] = kk;
     }
     if (tid == 0) s_or = 0;
+    if (tid < kGroupClasses) gcnt[tid] = 0;
     __syncthreads();
     if (my_or) atomicOr(&s_or, (unsigned long long)my_or);
     __syncthreads();
     const uint64_t diff = s_or;  // bits that vary inside the tile
     const int hi = diff ? 64 - __builtin_clzll(diff) : 0;
+    const bool one_pass = hi + LB <= 64;
+    // records: (varying key bits below `hi`, tile index); a pad (~0) is above
+    // every real record (a real record reaches ~0 only with index T-1, and
+    // then the tile is full)
+    const uint64_t mask = one_pass ? ((1ull << hi) - 1) : ((1ull << (64 - LB)) - 1);
+    const uint32_t p0 = tid * ITEMS;
     uint64_t k[ITEMS];
-    uint32_t li[ITEMS];
-    if (diff == 0) {
-        // one key: already in order
-    } else if (hi + LB <= 64) {
-        // varying bits below `hi`, tile index below them: unique u64 records
-        // (a pad, ~0, is above every real record: when hi + LB = 64 a real
-        // record reaches ~0 only with index T-1, and then the tile is full)
-        const uint64_t mask = hi == 64 ? ~0ull : ((1ull << hi) - 1);
 #pragma unroll
-        for (int j = 0; j < ITEMS; ++j) {
-            const uint32_t i = tid * ITEMS + j;
-            k[j] = i < len ? ((sk[px(i)] & mask) << LB) | i : ~0ull;
-        }
-        block_merge_sort<B, ITEMS, true>(k, li, sk, sl, len);
-        const uint64_t top = first & ~mask;
-#pragma unroll
-        for (int j = 0; j < ITEMS; ++j) {
-            sk[px(tid * ITEMS + j)] = top | (k[j] >> LB);  // pads (positions >= len) are never read
-            sl[px(tid * ITEMS + j)] = (uint16_t)(k[j] & (T - 1));
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < ITEMS; ++j) {
-            k[j] = sk[px(tid * ITEMS + j)];
-            li[j] = sl[px(tid * ITEMS + j)];
-        }
-        block_merge_sort<B, ITEMS, false>(k, li, sk, sl, len);
-#pragma unroll
-        for (int j = 0; j < ITEMS; ++j) {
-            sk[px(tid * ITEMS + j)] = k[j];
-            sl[px(tid * ITEMS + j)] = (uint16_t)li[j];
-        }
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t i = p0 + j;
+        k[j] = i < len ? ((sk[px(i)] & mask) << LB) | i : ~0ull;
     }
+    if (diff) block_merge_sort<B, ITEMS>(k, sk, len);
+    if (!one_pass) {
+        // second pass: (top LB key bits, rank after the first pass, tile index);
+        // its first level writes the buffer every wavefront read in the last
+        // level of the first pass
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const uint32_t p = p0 + j;
+            const uint32_t o1 = (uint32_t)(k[j] & (T - 1));
+            k[j] = p < len ? ((keys[tl.start + o1] >> (64 - LB)) << (2 * LB)) | ((uint64_t)p << LB) | o1 : ~0ull;
+        }
+        block_merge_sort<B, ITEMS>(k, sk, len);
+    }
+    __syncthreads();  // the sort buffer is free
+    // index values: original order into LDS, gathered in sorted order, back
+    // to LDS, written out coalesced
+    for (uint32_t i = tid; i < len; i += B) stage[i] = idx[tl.start + i];
     __syncthreads();
-    for (uint32_t i = tid; i < len; i += B) {
-        keys[tl.start + i] = sk[px(i)];
-        idx[tl.start + i] = si[sl[px(i)]];
-    }
+    I o[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) o[j] = p0 + j < len ? stage[(uint32_t)(k[j] & (T - 1))] : (I)0;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) stage[p0 + j] = o[j];
+    if ((tid & 63) == 63) wlast[tid >> 6] = o[ITEMS - 1];
+    __syncthreads();
+    uint64_t key[ITEMS];
+    const uint64_t top = first & ~mask;
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j)
+        key[j] = one_pass ? top | (k[j] >> LB)
+                          : (p0 + j < len ? keys[tl.start + (uint32_t)(k[j] & (T - 1))] : ~0ull);
+    for (uint32_t i = tid; i < len; i += B) idx[tl.start + i] = stage[i];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) sk[px(p0 + j)] = key[j];  // pads (positions >= len) are never read
+    __syncthreads();
+    for (uint32_t i = tid; i < len; i += B) keys[tl.start + i] = sk[px(i)];
     if (!emit || !tl.whole || len < 2) return;
-    // tie groups of the bucket, in one pass over the sorted tile: each group
-    // start takes a slot of its size class in LDS and keeps (size, class,
-    // slot) in a register (size <= T <= 4096: 13 bits; 8 classes: 3 bits;
-    // slot < T / 2: 11 bits); the block then reserves each class's slots in
-    // the global lists once and writes the groups
+    // Tie groups of the bucket: a run of equal keys starts where the key
+    // changes; an exclusive max-scan over the block carries the latest run
+    // start into each thread, and the thread holding a run's last record
+    // emits it: a slot of its size class in LDS, then one reservation per
+    // class in the global lists.  (size, class, slot) packed in a register:
+    // size <= T <= 4096: 13 bits; 8 classes: 3 bits; slot < T / 2: 11 bits.
     static_assert(T <= 4096 && kGroupClasses <= 8, "group record packing");
-    if (tid < kGroupClasses) gcnt[tid] = 0;
-    __syncthreads();
+    const uint64_t prev = p0 ? sk[px(p0 - 1)] : 0ull;
+    const uint64_t next = p0 + ITEMS < (uint32_t)T ? sk[px(p0 + ITEMS)] : 0ull;
+    I oprev = (I)__shfl_up((unsigned long long)o[ITEMS - 1], 1);  // the previous thread's last index
+    if ((tid & 63) == 0 && tid) oprev = wlast[(tid >> 6) - 1];
+    uint32_t st_mask = 0, end_mask = 0, last_start = 0;  // run starts as position + 1
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t p = p0 + j;
+        const bool st = p == 0 || (j ? key[j - 1] : prev) != key[j];
+        const bool en = p + 1 >= len || (j + 1 < ITEMS ? key[j + 1] : next) != key[j];
+        if (p < len) {
+            st_mask |= st ? 1u << j : 0u;
+            end_mask |= en ? 1u << j : 0u;
+            last_start = st ? p + 1 : last_start;
+        }
+    }
+    uint32_t rs = block_excl_max256(last_start, wmax);
     uint32_t info[ITEMS];
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-        const uint32_t r = tid + i * B;
-        info[i] = 0;
-        if (r < len) {
-            const uint64_t kr = sk[px(r)];
-            if ((r == 0 || sk[px(r - 1)] != kr) && r + 1 < len && sk[px(r + 1)] == kr) {
-                uint32_t e = r + 2;
-                while (e < len && sk[px(e)] == kr) ++e;
-                const uint32_t c = (uint32_t)group_class(e - r);
-                info[i] = (e - r) << 14 | c << 11 | atomicAdd(&gcnt[c], 1u);
-            }
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t p = p0 + j;
+        rs = (st_mask >> j) & 1u ? p + 1 : rs;
+        info[j] = 0;
+        const uint32_t size = p + 2 - rs;
+        if (((end_mask >> j) & 1u) && size >= 2) {
+            const uint32_t c = (uint32_t)group_class(size);
+            info[j] = size << 14 | c << 11 | atomicAdd(&gcnt[c], 1u);
         }
     }
     __syncthreads();
     if (tid < kGroupClasses) gbase[tid] = gcnt[tid] ? atomicAdd(&gl.counts[tid], gcnt[tid]) : 0u;
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-        if (info[i]) {
-            const uint32_t r = tid + i * B;
-            const uint32_t size = info[i] >> 14, c = (info[i] >> 11) & 7u;
-            const uint32_t slot = gbase[c] + (info[i] & 0x7FFu);
-            if (c == 0) ((PairRec<I> *)gl.list[0])[slot] = PairRec<I>{tl.start + r, si[sl[px(r)]], si[sl[px(r + 1)]]};
+    for (int j = 0; j < ITEMS; ++j) {
+        if (info[j]) {
+            const uint32_t size = info[j] >> 14, c = (info[j] >> 11) & 7u;
+            const uint32_t r = p0 + j + 1 - size;
+            const uint32_t slot = gbase[c] + (info[j] & 0x7FFu);
+            if (c == 0) ((PairRec<I> *)gl.list[0])[slot] = PairRec<I>{tl.start + r, j ? o[j - 1] : oprev, o[j]};
             else gl.list[c][slot] = Group{tl.start + r, size};
         }
     }
