@@ -641,10 +641,10 @@ struct BatchSink {
 };
 
 // Bucket-batch plan.  Per record a batch needs at most: 16-byte record,
-// 8-byte key, 2-byte bucket id, about 20 bytes of group lists (sized for the
-// worst case of every class), 8-byte output staging and 32 bytes of merge
-// scratch (keys, positions, two LCP arrays) when oversized buckets or giant
-// groups need them: 112 bytes with margin; 96 without the record when the
+// 8-byte key, about 20 bytes of group lists (sized for the worst case of
+// every class), 8-byte output staging and 32 bytes of merge scratch (keys,
+// positions, two LCP arrays) when oversized buckets or giant groups need
+// them: 112 bytes with margin; 96 without the record when the
 // records of all batches are written at once (pass 1 runs once instead of
 // once per batch).  75% of the free HBM is used; buffers the workspace
 // already holds count as free, since this build reuses them.
@@ -660,7 +660,7 @@ static BatchPlan plan_batches(const Workspace &W, uint64_t n_pos, uint64_t reque
     // every buffer this build allocates after planning counts as free, so
     // that a repeated build of the same input plans the same batches (a
     // different plan would regrow multi-GB buffers in the middle of a build)
-    size_t held = W.recs.cap + W.keys.cap + W.fid.cap + W.sinkbuf.cap + W.keys2.cap + W.idx2.cap + W.lcp.cap +
+    size_t held = W.recs.cap + W.keys.cap + W.sinkbuf.cap + W.keys2.cap + W.idx2.cap + W.lcp.cap +
                   W.lcp2.cap + W.tiles.cap + W.big.cap + W.chunks.cap + W.jobs.cap + W.run_s.cap + W.run_e.cap +
                   W.run_ss.cap + W.run_se.cap + W.run_dir.cap;
     for (int c = 0; c < kGroupClasses; ++c) held += W.glist[c].cap;
@@ -978,9 +978,6 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     // batch-sized buffers
     uint64_t *keys = (uint64_t *)W.keys.get(bcap * 8 + 8);
     KRec<I> *tmp = (KRec<I> *)W.recs.get((all_records ? n_pos : bcap) * sizeof(KRec<I>) + 16);
-    // the coarse ids of a single batch are consumed by pass 1, so their buffer
-    // then holds the per-record bucket within its interval; batches need theirs
-    uint16_t *fid = single ? cid : (uint16_t *)W.fid.get(bcap * 2 + 2);
     I *sink_buf = sink ? (I *)W.sinkbuf.get(bcap * sizeof(I) + 8) : nullptr;
     // the previous batch's last suffix (key, position), for the next batch's first LCP
     uint64_t prev_key = 0, prev_pos = 0;
@@ -1055,16 +1052,20 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         // ---- two-level scatter ----
         if (!all_records) pass1(c_lo, c_hi, rbase);
         kt.begin(&stats.k_scatter2);
-        // 1,024 threads when LDS allows one block per compute unit (wide
-        // intervals) or the batch has too few intervals to fill the device
-        const bool wide = SP.stride * 28 > 48 * 1024 || (c_hi - c_lo) < 2 * W.n_cu;
+        // 1,024 threads and 2,048-record tiles when the LDS of a wide interval
+        // leaves one block per compute unit, or the batch has too few
+        // intervals to fill the device; else 4,096-record tiles, with 512
+        // threads for a u32 index (57 KB + 28 bytes per bucket: two blocks per
+        // compute unit) and 1,024 for a u64 index (73 KB: one block)
+        constexpr int kNT = sizeof(I) == 4 ? 512 : 1024;
+        const bool wide = SP.stride * 28 > 22 * 1024 || (c_hi - c_lo) < 2 * W.n_cu;
         KRec<I> *trec = all_records ? tmp + rbase : tmp;
         if (wide)
-            k_scatter2<I, 1024><<<c_hi - c_lo, 1024, SP.stride * 28, st>>>(trec, fid, coff, SP, (uint32_t)p, off,
-                                                                          keys, out, c_lo, c_hi, rbase);
+            k_scatter2<I, 1024, 2048><<<c_hi - c_lo, 1024, SP.stride * 28, st>>>(trec, coff, SP, (uint32_t)p, off,
+                                                                                keys, out, c_lo, c_hi, rbase);
         else
-            k_scatter2<I, 256><<<c_hi - c_lo, 256, SP.stride * 28, st>>>(trec, fid, coff, SP, (uint32_t)p, off,
-                                                                        keys, out, c_lo, c_hi, rbase);
+            k_scatter2<I, kNT, 4096><<<c_hi - c_lo, kNT, SP.stride * 28, st>>>(trec, coff, SP, (uint32_t)p, off,
+                                                                              keys, out, c_lo, c_hi, rbase);
         LAUNCHCHK();
         kt.end();
         tm.mark(PH_SCATTER);

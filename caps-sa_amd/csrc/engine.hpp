@@ -46,7 +46,7 @@ struct Workspace {
     DevBuf bid, hist, hblk, off, cursor, keys, keys2, idx2;
     // bucket batches: per-record bucket ids, a batch of output; kept-position
     // bitmap, its count
-    DevBuf fid, sinkbuf, keep, keep_count, poscheck;
+    DevBuf sinkbuf, keep, keep_count, poscheck;
     // tiles
     DevBuf tcnt, toff, toff2, tiles, big, counts, recs;
     // refinement

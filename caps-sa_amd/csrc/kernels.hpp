@@ -812,8 +812,9 @@ __global__ __launch_bounds__(256) void k_scatter1_staged(const S *__restrict__ t
 }
 
 // Exclusive scan of cnt[0..nb) into out[0..nb) (+ base) across the block;
-// each thread scans a contiguous slice.  Returns the total in every thread.
-// part: blockDim.x u64 of LDS.
+// each thread scans a contiguous slice, the slice sums are scanned with
+// wavefront shuffles and one exchange of wavefront totals.  Returns the total
+// in every thread.  part: blockDim.x / 64 u64 of LDS.
 __device__ __forceinline__ unsigned long long block_scan_slices(const uint32_t *cnt, uint32_t nb,
                                                                 unsigned long long base,
                                                                 unsigned long long *out,
@@ -822,40 +823,69 @@ __device__ __forceinline__ unsigned long long block_scan_slices(const uint32_t *
     const uint32_t j0 = threadIdx.x * per, j1 = j0 + per < nb ? j0 + per : nb;
     unsigned long long sum = 0;
     for (uint32_t j = j0; j < j1; ++j) sum += cnt[j];
-    part[threadIdx.x] = sum;
-    __syncthreads();
-    for (uint32_t o = 1; o < blockDim.x; o <<= 1) {
-        unsigned long long v = threadIdx.x >= o ? part[threadIdx.x - o] : 0ull;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    unsigned long long x = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned long long y = __shfl_up(x, o);
+        if (lane >= o) x += y;
     }
-    unsigned long long run = base + (threadIdx.x ? part[threadIdx.x - 1] : 0ull);
+    if (lane == 63) part[w] = x;
+    __syncthreads();
+    unsigned long long before = 0, total = 0;
+    for (int q = 0; q < nw; ++q) {
+        const unsigned long long v = part[q];
+        before += q < w ? v : 0ull;
+        total += v;
+    }
+    unsigned long long run = base + before + x - sum;
     for (uint32_t j = j0; j < j1; ++j) {
         out[j] = run;
         run += cnt[j];
     }
-    unsigned long long total = part[blockDim.x - 1];
-    __syncthreads();
+    __syncthreads();  // part is reused by the next scan
     return total;
 }
 
-constexpr uint32_t kScatterTile = 2048;  // records staged per placing step of pass 2
+// Bucket of each of U keys inside interval c: the number of the interval's
+// inner splitters (sspl[0 .. ninner)) at or below the key, by binary lifting
+// in lockstep.
+template <int U>
+__device__ __forceinline__ void fine_buckets(const uint64_t *sspl, uint32_t ninner, uint32_t fsteps,
+                                             const uint64_t (&kk)[U], uint32_t (&f)[U]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) f[u] = 0;
+    for (uint32_t step = fsteps ? 1u << (fsteps - 1) : 0; step; step >>= 1) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t pr = f[u] + step;
+            if (pr <= ninner && sspl[pr - 1] <= kk[u]) f[u] = pr;
+        }
+    }
+}
 
-// NT threads per block: 256, or 1,024 when the LDS of a wide interval (or a
-// batch of few intervals) leaves one block per compute unit.
-template <typename I, int NT>
-__global__ __launch_bounds__(NT) void k_scatter2(const KRec<I> *__restrict__ tmp, uint16_t *__restrict__ fid,
-                                                  const uint64_t *__restrict__ coff, Splitters SP, uint32_t p,
-                                                  uint64_t *__restrict__ off, uint64_t *__restrict__ keys,
-                                                  I *__restrict__ idx, uint32_t c_lo, uint32_t c_hi, uint64_t rbase) {
+// NT threads and TILE records staged per placing step: 512 and 4,096 (two
+// workgroups per compute unit), or 1,024 and 2,048 when the LDS of a wide
+// interval (or a batch of few intervals) leaves one.  The bucket of a record
+// is found twice, in the counting pass and again in the placing pass, instead
+// of being stored between them: a binary search over the interval's inner
+// splitters in LDS costs less than writing and reading back two bytes per
+// record (3.18 against 3.48 ms on 268M records in 1,024 intervals; 4,096-record
+// tiles instead of 2,048 took 3.82 to 3.48 ms: longer bucket runs per write).
+template <typename I, int NT, int TILE>
+__global__ __launch_bounds__(NT) void k_scatter2(const KRec<I> *__restrict__ tmp, const uint64_t *__restrict__ coff,
+                                                  Splitters SP, uint32_t p, uint64_t *__restrict__ off,
+                                                  uint64_t *__restrict__ keys, I *__restrict__ idx, uint32_t c_lo,
+                                                  uint32_t c_hi, uint64_t rbase) {
+    constexpr int U = TILE / NT;
+    static_assert(TILE % NT == 0 && NT % 64 == 0, "tile of whole rows");
     // LDS sized by the launch for SP.stride buckets: inner splitters, global
     // cursors, tile offsets, counts
     extern __shared__ unsigned long long scatter2_lds[];
-    __shared__ unsigned long long part[NT];
-    __shared__ uint64_t tkey[kScatterTile];
-    __shared__ I tidx[kScatterTile];
-    __shared__ uint16_t tf[kScatterTile];
+    __shared__ unsigned long long part[NT / 64];
+    __shared__ uint64_t tkey[TILE];
+    __shared__ I tidx[TILE];
+    __shared__ uint16_t tf[TILE];
     uint64_t *sspl = (uint64_t *)scatter2_lds;
     unsigned long long *gcur = scatter2_lds + SP.stride;
     unsigned long long *lbase = scatter2_lds + 2 * SP.stride;
@@ -874,31 +904,19 @@ __global__ __launch_bounds__(NT) void k_scatter2(const KRec<I> *__restrict__ tmp
         if (j < ninner) sspl[j] = SP.spl[b0 + j];
     }
     __syncthreads();
-    // counting pass: bucket of every record (kept in fid) and bucket sizes
-    for (uint64_t i0 = s; i0 < e; i0 += 8 * blockDim.x) {
+    // counting pass: bucket sizes
+    for (uint64_t i0 = s; i0 < e; i0 += 8 * NT) {
         uint64_t kk[8];
         uint32_t f[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            uint64_t i = i0 + u * blockDim.x + threadIdx.x;
+            const uint64_t i = i0 + u * NT + threadIdx.x;
             kk[u] = i < e ? tmp[i].key : 0ull;
-            f[u] = 0;
         }
-        for (uint32_t step = fsteps ? 1u << (fsteps - 1) : 0; step; step >>= 1) {
+        fine_buckets<8>(sspl, ninner, fsteps, kk, f);
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                uint32_t pr = f[u] + step;
-                if (pr <= ninner && sspl[pr - 1] <= kk[u]) f[u] = pr;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            uint64_t i = i0 + u * blockDim.x + threadIdx.x;
-            if (i < e) {
-                fid[i] = (uint16_t)f[u];
-                atomicAdd(&cnt[f[u]], 1u);
-            }
-        }
+        for (int u = 0; u < 8; ++u)
+            if (i0 + u * NT + threadIdx.x < e) atomicAdd(&cnt[f[u]], 1u);
     }
     __syncthreads();
     block_scan_slices(cnt, nb, s, gcur, part);
@@ -908,31 +926,36 @@ __global__ __launch_bounds__(NT) void k_scatter2(const KRec<I> *__restrict__ tmp
     }
     if (c + 1 == c_hi && threadIdx.x == 0) off[b0 + nb] = e;  // the batch's end
     __syncthreads();
-    // placing pass, one tile of kScatterTile records at a time: count the
-    // tile's buckets, scan, stage the records in LDS grouped by bucket, then
-    // write each bucket's run of the tile with consecutive threads
-    for (uint64_t t0 = s; t0 < e; t0 += kScatterTile) {
-        const uint32_t tl = (uint32_t)(e - t0 < kScatterTile ? e - t0 : kScatterTile);
-        constexpr int U = kScatterTile / NT;
+    // placing pass, one tile of TILE records at a time: bucket of every
+    // record, count the tile's buckets, scan, stage the records in LDS
+    // grouped by bucket, then write each bucket's run of the tile with
+    // consecutive threads
+    for (uint64_t t0 = s; t0 < e; t0 += TILE) {
+        const uint32_t tl = (uint32_t)(e - t0 < TILE ? e - t0 : TILE);
+        uint64_t kk[U];
+        I ix[U];
         uint32_t f[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            uint32_t q = u * NT + threadIdx.x;
-            f[u] = q < tl ? fid[t0 + q] : 0xFFFFFFFFu;
-            if (q < tl) atomicAdd(&cnt[f[u]], 1u);
+            const uint32_t q = u * NT + threadIdx.x;
+            const KRec<I> r = q < tl ? tmp[t0 + q] : KRec<I>{0ull, (I)0};
+            kk[u] = r.key;
+            ix[u] = r.idx;
         }
+        fine_buckets<U>(sspl, ninner, fsteps, kk, f);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (u * NT + threadIdx.x < tl) atomicAdd(&cnt[f[u]], 1u);
         __syncthreads();
         block_scan_slices(cnt, nb, 0, lbase, part);
         for (uint32_t j = threadIdx.x; j < nb; j += blockDim.x) cnt[j] = 0;
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            uint32_t q = u * NT + threadIdx.x;
-            if (q < tl) {
-                const KRec<I> r = tmp[t0 + q];
+            if (u * NT + threadIdx.x < tl) {
                 const uint32_t slot = (uint32_t)lbase[f[u]] + atomicAdd(&cnt[f[u]], 1u);
-                tkey[slot] = r.key;
-                tidx[slot] = r.idx;
+                tkey[slot] = kk[u];
+                tidx[slot] = ix[u];
                 tf[slot] = (uint16_t)f[u];
             }
         }
