@@ -1055,17 +1055,18 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         // 1,024 threads and 2,048-record tiles when the LDS of a wide interval
         // leaves one block per compute unit, or the batch has too few
         // intervals to fill the device; else 4,096-record tiles, with 512
-        // threads for a u32 index (57 KB + 28 bytes per bucket: two blocks per
-        // compute unit) and 1,024 for a u64 index (73 KB: one block)
+        // threads for a u32 index (57 KB + about 28 bytes per bucket: two
+        // blocks per compute unit) and 1,024 for a u64 index (73 KB: one block)
         constexpr int kNT = sizeof(I) == 4 ? 512 : 1024;
-        const bool wide = SP.stride * 28 > 22 * 1024 || (c_hi - c_lo) < 2 * W.n_cu;
+        const uint32_t s2_lds = scatter2_lds_bytes(SP.stride);
+        const bool wide = s2_lds > 22 * 1024 || (c_hi - c_lo) < 2 * W.n_cu;
         KRec<I> *trec = all_records ? tmp + rbase : tmp;
         if (wide)
-            k_scatter2<I, 1024, 2048><<<c_hi - c_lo, 1024, SP.stride * 28, st>>>(trec, coff, SP, (uint32_t)p, off,
-                                                                                keys, out, c_lo, c_hi, rbase);
+            k_scatter2<I, 1024, 2048><<<c_hi - c_lo, 1024, s2_lds, st>>>(trec, coff, SP, (uint32_t)p, off, keys, out,
+                                                                        c_lo, c_hi, rbase);
         else
-            k_scatter2<I, kNT, 4096><<<c_hi - c_lo, kNT, SP.stride * 28, st>>>(trec, coff, SP, (uint32_t)p, off,
-                                                                              keys, out, c_lo, c_hi, rbase);
+            k_scatter2<I, kNT, 4096><<<c_hi - c_lo, kNT, s2_lds, st>>>(trec, coff, SP, (uint32_t)p, off, keys, out,
+                                                                      c_lo, c_hi, rbase);
         LAUNCHCHK();
         kt.end();
         tm.mark(PH_SCATTER);

@@ -179,10 +179,45 @@ __device__ __forceinline__ uint64_t coarse_splitter(const Splitters &SP, uint32_
     return SP.spl[(uint64_t)(c + 1) * SP.stride - 1];
 }
 
-// LDS of the classify kernels, sized by the launch: the coarse table, then
-// the per-interval histogram.
+// Splitter tables searched in LDS are kept in breadth-first (Eytzinger)
+// order: node i (1-based) of the complete binary tree of height L holds the
+// sorted table's entry in_order(i), entries past the table hold ~0.  The
+// nodes of one tree level are contiguous, so the first levels of a search,
+// where the lanes of a wavefront probe few distinct entries, read distinct
+// LDS banks; in sorted order those entries lie 2^k entries apart and all fall
+// in one bank (the classify pass stalled on bank conflicts for most of its
+// time).
+__host__ __device__ constexpr uint32_t pow2_at_least(uint32_t v) {
+    uint32_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+__device__ __forceinline__ uint32_t eyt_in_order(uint32_t i, uint32_t L) {
+    const uint32_t d = 31 - __builtin_clz(i);
+    return ((2 * (i - (1u << d)) + 1) << (L - 1 - d)) - 1;
+}
+// Number of the table's n entries at or below each key (tree of height L,
+// node i at t[i - 1]), for N keys in lockstep.
+template <int N>
+__device__ __forceinline__ void eyt_upper_rank(const uint64_t *t, uint32_t L, uint32_t n, const uint64_t (&key)[N],
+                                               uint32_t (&pos)[N]) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) pos[j] = 1;
+    for (uint32_t lvl = 0; lvl < L; ++lvl) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) pos[j] = 2 * pos[j] + (t[pos[j] - 1] <= key[j] ? 1u : 0u);
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const uint32_t r = pos[j] - (1u << L);
+        pos[j] = r < n ? r : n;
+    }
+}
+
+// LDS of the classify kernels, sized by the launch: the coarse table (room
+// for the padded breadth-first tree), then the per-interval histogram.
 __host__ __device__ constexpr uint32_t classify_lds_bytes(uint32_t n_coarse) {
-    return ((n_coarse + 1) & ~1u) * 8 + (n_coarse + 1) * 4;
+    return pow2_at_least(n_coarse + 1) * 8 + (n_coarse + 1) * 4;
 }
 
 // Packed code text: word w holds the codes of symbols w*F .. w*F+F-1
@@ -308,14 +343,21 @@ __device__ __forceinline__ uint32_t router_correct(uint32_t window, const uint64
 template <int N>
 __device__ __forceinline__ void coarse_bsearch(const Splitters &SP, const uint64_t *coarse, const uint64_t (&key)[N],
                                                uint32_t (&pos)[N]) {
-#pragma unroll
-    for (int j = 0; j < N; ++j) pos[j] = 0;
-    for (uint32_t step = SP.coarse_steps ? 1u << (SP.coarse_steps - 1) : 0; step; step >>= 1) {
-#pragma unroll
-        for (int j = 0; j < N; ++j) {
-            uint32_t pr = pos[j] + step;
-            if (pr <= SP.n_coarse && coarse[pr - 1] <= key[j]) pos[j] = pr;
+    eyt_upper_rank<N>(coarse, SP.coarse_steps, SP.n_coarse, key, pos);
+}
+
+// The classify kernels' coarse table: breadth-first for the binary search
+// (RK == 0), sorted for the routers' local correction.
+template <int RK>
+__device__ __forceinline__ void load_coarse_table(const Splitters &SP, uint64_t *coarse) {
+    if (RK == 0) {
+        const uint32_t L = SP.coarse_steps, m = (1u << L) - 1;
+        for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+            const uint32_t c = eyt_in_order(i + 1, L);
+            coarse[i] = c < SP.n_coarse ? coarse_splitter(SP, c) : ~0ull;
         }
+    } else {
+        for (uint32_t c = threadIdx.x; c < SP.n_coarse; c += blockDim.x) coarse[c] = coarse_splitter(SP, c);
     }
 }
 
@@ -355,7 +397,7 @@ __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeM
     constexpr uint32_t NS = kPwlSeg;
     extern __shared__ uint64_t classify_lds[];
     uint64_t *coarse = classify_lds;
-    uint32_t *chist = (uint32_t *)(classify_lds + ((SP.n_coarse + 1) & ~1u));
+    uint32_t *chist = (uint32_t *)(classify_lds + pow2_at_least(SP.n_coarse + 1));
     for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) chist[c] = 0;
     __shared__ uint64_t rknots[RK == 1 ? NS + 1 : 1];
     __shared__ float ra[RK == 1 ? NS : 1], rb[RK == 1 ? NS : 1], rinv[RK == 1 ? NS : 1];
@@ -363,7 +405,7 @@ __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeM
     __shared__ uint16_t rradix[RK == 1 ? (1u << kRouterRadixBits) + 1 : 1];
     __shared__ float nib[RK == 2 ? 16 * 16 * kMlpHid : 1], w2[RK == 2 ? kMlpHid + 1 : 1];
     __shared__ unsigned long long st_sum, st_max, st_fb, st_cnt;
-    for (uint32_t c = threadIdx.x; c < SP.n_coarse; c += blockDim.x) coarse[c] = coarse_splitter(SP, c);
+    load_coarse_table<RK>(SP, coarse);
     if (RK == 1) {
         for (uint32_t c = threadIdx.x; c < R.ns; c += blockDim.x) {
             rknots[c] = R.knots[c];
@@ -494,7 +536,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RK == 0 ? 4
     constexpr uint32_t NS = kPwlSeg;
     extern __shared__ uint64_t classify_lds[];
     uint64_t *coarse = classify_lds;
-    uint32_t *chist = (uint32_t *)(classify_lds + ((SP.n_coarse + 1) & ~1u));
+    uint32_t *chist = (uint32_t *)(classify_lds + pow2_at_least(SP.n_coarse + 1));
     for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) chist[c] = 0;
     __shared__ S win[kSeqChunk + 72];
     __shared__ uint64_t rknots[RK == 1 ? NS + 1 : 1];
@@ -503,7 +545,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RK == 0 ? 4
     __shared__ uint16_t rradix[RK == 1 ? (1u << kRouterRadixBits) + 1 : 1];
     __shared__ float nib[RK == 2 ? 16 * 16 * kMlpHid : 1], w2[RK == 2 ? kMlpHid + 1 : 1];
     __shared__ unsigned long long st_sum, st_max, st_fb, st_cnt;
-    for (uint32_t c = threadIdx.x; c < SP.n_coarse; c += blockDim.x) coarse[c] = coarse_splitter(SP, c);
+    load_coarse_table<RK>(SP, coarse);
     if (RK == 1) {
         for (uint32_t c = threadIdx.x; c < R.ns; c += blockDim.x) {
             rknots[c] = R.knots[c];
@@ -847,21 +889,11 @@ __device__ __forceinline__ unsigned long long block_scan_slices(const uint32_t *
     return total;
 }
 
-// Bucket of each of U keys inside interval c: the number of the interval's
-// inner splitters (sspl[0 .. ninner)) at or below the key, by binary lifting
-// in lockstep.
-template <int U>
-__device__ __forceinline__ void fine_buckets(const uint64_t *sspl, uint32_t ninner, uint32_t fsteps,
-                                             const uint64_t (&kk)[U], uint32_t (&f)[U]) {
-#pragma unroll
-    for (int u = 0; u < U; ++u) f[u] = 0;
-    for (uint32_t step = fsteps ? 1u << (fsteps - 1) : 0; step; step >>= 1) {
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t pr = f[u] + step;
-            if (pr <= ninner && sspl[pr - 1] <= kk[u]) f[u] = pr;
-        }
-    }
+// LDS of scatter pass 2, sized by the launch for `stride` buckets per
+// interval: the inner splitters as a breadth-first tree, global cursors, tile
+// offsets, counts.
+__host__ __device__ constexpr uint32_t scatter2_lds_bytes(uint32_t stride) {
+    return pow2_at_least(stride) * 8 + stride * (8 + 8 + 4);
 }
 
 // NT threads and TILE records staged per placing step: 512 and 4,096 (two
@@ -879,17 +911,17 @@ __global__ __launch_bounds__(NT) void k_scatter2(const KRec<I> *__restrict__ tmp
                                                   uint32_t c_hi, uint64_t rbase) {
     constexpr int U = TILE / NT;
     static_assert(TILE % NT == 0 && NT % 64 == 0, "tile of whole rows");
-    // LDS sized by the launch for SP.stride buckets: inner splitters, global
-    // cursors, tile offsets, counts
+    // LDS sized by the launch (scatter2_lds_bytes)
     extern __shared__ unsigned long long scatter2_lds[];
     __shared__ unsigned long long part[NT / 64];
     __shared__ uint64_t tkey[TILE];
     __shared__ I tidx[TILE];
     __shared__ uint16_t tf[TILE];
+    const uint32_t p2 = pow2_at_least(SP.stride);
     uint64_t *sspl = (uint64_t *)scatter2_lds;
-    unsigned long long *gcur = scatter2_lds + SP.stride;
-    unsigned long long *lbase = scatter2_lds + 2 * SP.stride;
-    uint32_t *cnt = (uint32_t *)(scatter2_lds + 3 * SP.stride);
+    unsigned long long *gcur = scatter2_lds + p2;
+    unsigned long long *lbase = gcur + SP.stride;
+    uint32_t *cnt = (uint32_t *)(lbase + SP.stride);
     // interval c of the batch [c_lo, c_hi); record offsets relative to rbase
     const uint32_t c = c_lo + blockIdx.x;
     const uint64_t s = coff[c] - rbase, e = coff[c + 1] - rbase;
@@ -899,9 +931,10 @@ __global__ __launch_bounds__(NT) void k_scatter2(const KRec<I> *__restrict__ tmp
     const uint32_t ninner = nb - 1;
     uint32_t fsteps = 0;
     while ((1u << fsteps) <= ninner) ++fsteps;
-    for (uint32_t j = threadIdx.x; j < nb; j += blockDim.x) {
-        cnt[j] = 0;
-        if (j < ninner) sspl[j] = SP.spl[b0 + j];
+    for (uint32_t j = threadIdx.x; j < nb; j += blockDim.x) cnt[j] = 0;
+    for (uint32_t i = threadIdx.x; i + 1 < (1u << fsteps); i += blockDim.x) {
+        const uint32_t j = eyt_in_order(i + 1, fsteps);
+        sspl[i] = j < ninner ? SP.spl[b0 + j] : ~0ull;
     }
     __syncthreads();
     // counting pass: bucket sizes
@@ -913,7 +946,7 @@ __global__ __launch_bounds__(NT) void k_scatter2(const KRec<I> *__restrict__ tmp
             const uint64_t i = i0 + u * NT + threadIdx.x;
             kk[u] = i < e ? tmp[i].key : 0ull;
         }
-        fine_buckets<8>(sspl, ninner, fsteps, kk, f);
+        eyt_upper_rank<8>(sspl, fsteps, ninner, kk, f);
 #pragma unroll
         for (int u = 0; u < 8; ++u)
             if (i0 + u * NT + threadIdx.x < e) atomicAdd(&cnt[f[u]], 1u);
@@ -942,7 +975,7 @@ __global__ __launch_bounds__(NT) void k_scatter2(const KRec<I> *__restrict__ tmp
             kk[u] = r.key;
             ix[u] = r.idx;
         }
-        fine_buckets<U>(sspl, ninner, fsteps, kk, f);
+        eyt_upper_rank<U>(sspl, fsteps, ninner, kk, f);
 #pragma unroll
         for (int u = 0; u < U; ++u)
             if (u * NT + threadIdx.x < tl) atomicAdd(&cnt[f[u]], 1u);
