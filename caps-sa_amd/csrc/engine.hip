@@ -939,7 +939,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         k_classify<S, I, 0><<<nblk, 256, cls_lds, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hblk, pc, chunk);
     LAUNCHCHK();
     kt.end();
-    k_block_offsets<<<(unsigned)ceil_div(pc, 256), 256, 0, st>>>(hblk, nblk, pc, hist);
+    k_block_offsets<<<(unsigned)ceil_div(pc, kBoCols), kBoCols * kBoRows, 0, st>>>(hblk, nblk, pc, hist);
     LAUNCHCHK();
     k_scan_u64<<<1, 1024, 0, st>>>(hist, pc, coff);
     LAUNCHCHK();
@@ -1073,7 +1073,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
 
         // ---- tiles: sort per tile (by size class); whole-bucket tiles emit their tie groups ----
         HIPCHK(hipMemsetAsync(counters, 0, 128, st));
-        k_tile_emit<<<(unsigned)ceil_div(pb, 256), 256, 0, st>>>(boff, pb, T, tiles, tcap, counters + 1, big,
+        k_tile_emit<<<(unsigned)ceil_div(pb, 256 * kEmitPer), 256, 0, st>>>(boff, pb, T, tiles, tcap, counters + 1, big,
                                                                   counters, counters + 4,
                                                                   (unsigned long long *)(counters + 24));
         LAUNCHCHK();

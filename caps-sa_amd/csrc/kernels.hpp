@@ -646,19 +646,58 @@ __global__ __launch_bounds__(1024) void k_scan_u64(const uint64_t *__restrict__ 
 // Per-block histograms (hblk[b * pc + c]) -> exclusive prefix over blocks in
 // place (block b's first slot inside interval c, relative to the interval) and
 // interval totals in hist[c], summed in 64 bits: an interval of 2^32 or more
-// records (which the host then refuses) cannot wrap its total.  One thread
-// per interval.
-__global__ void k_block_offsets(uint32_t *__restrict__ hblk, uint32_t nblocks, uint32_t pc,
-                                uint64_t *__restrict__ hist) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+// records (which the host then refuses) cannot wrap its total.  A workgroup
+// of 1,024 threads takes 16 intervals; its 64 rows of threads each sum a
+// contiguous slice of the blocks (independent loads, eight in flight), the
+// slice sums are scanned across the rows in LDS, and every row rewrites its
+// slice with its running prefix.
+constexpr uint32_t kBoCols = 16, kBoRows = 64;
+__global__ __launch_bounds__(1024) void k_block_offsets(uint32_t *__restrict__ hblk, uint32_t nblocks, uint32_t pc,
+                                                        uint64_t *__restrict__ hist) {
+    __shared__ unsigned long long part[kBoRows][kBoCols];
+    const uint32_t cl = threadIdx.x % kBoCols, r = threadIdx.x / kBoCols;
+    const uint32_t c = blockIdx.x * kBoCols + cl;
+    const uint32_t per = (nblocks + kBoRows - 1) / kBoRows;
+    const uint32_t b0 = r * per < nblocks ? r * per : nblocks;
+    const uint32_t b1 = b0 + per < nblocks ? b0 + per : nblocks;
+    unsigned long long sum = 0;
+    if (c < pc) {
+        uint32_t b = b0;
+        for (; b + 8 <= b1; b += 8) {
+            uint32_t v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = hblk[(uint64_t)(b + u) * pc + c];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) sum += v[u];
+        }
+        for (; b < b1; ++b) sum += hblk[(uint64_t)b * pc + c];
+    }
+    part[r][cl] = sum;
+    __syncthreads();
+    unsigned long long run = 0, total = 0;
+    for (uint32_t q = 0; q < kBoRows; ++q) {
+        const unsigned long long v = part[q][cl];
+        run += q < r ? v : 0ull;
+        total += v;
+    }
     if (c >= pc) return;
-    uint64_t run = 0;
-    for (uint32_t b = 0; b < nblocks; ++b) {
+    uint32_t b = b0;
+    for (; b + 8 <= b1; b += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = hblk[(uint64_t)(b + u) * pc + c];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            hblk[(uint64_t)(b + u) * pc + c] = (uint32_t)run;
+            run += v[u];
+        }
+    }
+    for (; b < b1; ++b) {
         const uint32_t v = hblk[(uint64_t)b * pc + c];
         hblk[(uint64_t)b * pc + c] = (uint32_t)run;
         run += v;
     }
-    hist[c] = run;
+    if (r == 0) hist[c] = total;
 }
 
 // ---------------------------------------------------------------------------
@@ -1020,54 +1059,101 @@ struct Range {
 // Cuts every bucket into tiles of at most T records and files each tile
 // under its size class (tiles[cls * cap + slot]); buckets larger than T are
 // also listed in `big` for the key merges.  trec[c] receives the records of
-// class c (the tile sort's work).
-__global__ void k_tile_emit(const uint64_t *__restrict__ off, uint32_t p, uint32_t T, Tile *__restrict__ tiles,
-                            uint64_t cap, unsigned int *__restrict__ tcount, Range *__restrict__ big,
-                            unsigned int *__restrict__ n_big, unsigned int *__restrict__ max_bucket,
-                            unsigned long long *__restrict__ trec) {
-    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t s = b < p ? off[b] : 0, sz = b < p ? off[b + 1] - s : 0;
-    // largest bucket: wavefront maximum first, one atomic per wavefront
-    uint32_t mx = (uint32_t)(sz < 0xFFFFFFFFull ? sz : 0xFFFFFFFFull);
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        uint32_t v = __shfl_xor(mx, o);
-        mx = v > mx ? v : mx;
-    }
-    if ((threadIdx.x & 63) == 0 && mx) atomicMax(max_bucket, mx);
-    // tiles: one list slot reservation per wavefront and class (ballot + popcount)
+// class c (the tile sort's work).  A workgroup takes kEmitPer * 256
+// consecutive buckets in two identical walks: the first counts each
+// wavefront's tiles per class (ballots) and reserves the wavefront's slots in
+// LDS, then one global reservation per class and workgroup; the second files
+// the tiles into those slots.
+constexpr uint32_t kEmitPer = 4;
+__global__ __launch_bounds__(256) void k_tile_emit(const uint64_t *__restrict__ off, uint32_t p, uint32_t T,
+                                                   Tile *__restrict__ tiles, uint64_t cap,
+                                                   unsigned int *__restrict__ tcount, Range *__restrict__ big,
+                                                   unsigned int *__restrict__ n_big,
+                                                   unsigned int *__restrict__ max_bucket,
+                                                   unsigned long long *__restrict__ trec) {
+    __shared__ unsigned int s_cnt[kTileClasses], s_base[kTileClasses], s_max;
+    __shared__ unsigned long long s_rec[kTileClasses];
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t below = (1ull << lane) - 1;
-    for (uint64_t x = 0; __any(x < sz); x += T) {
-        const bool have = x < sz;
-        Tile tl{};
-        int cls = -1;
-        if (have) {
-            tl.start = s + x;
-            tl.len = (uint32_t)(sz - x < T ? sz - x : T);
-            tl.whole = sz <= T ? 1u : 0u;
-            cls = tile_class(tl.len);
-        }
-        for (int c = 0; c < kTileClasses; ++c) {
-            const uint64_t m = __ballot(cls == c);
-            if (!m) continue;
-            const uint32_t first = (uint32_t)__ffsll((unsigned long long)m) - 1;
-            unsigned int base = 0;
-            uint32_t recs = cls == c ? tl.len : 0u;
+    if (threadIdx.x < kTileClasses) {
+        s_cnt[threadIdx.x] = 0;
+        s_rec[threadIdx.x] = 0;
+    }
+    if (threadIdx.x == 0) s_max = 0;
+    __syncthreads();
+    const uint64_t b_first = (uint64_t)blockIdx.x * 256 * kEmitPer;
+    uint32_t wcnt[kTileClasses] = {};
+    unsigned long long recs[kTileClasses] = {};
+    uint32_t mx = 0;
+    for (uint32_t k = 0; k < kEmitPer; ++k) {
+        const uint64_t b = b_first + k * 256 + threadIdx.x;
+        const uint64_t s = b < p ? off[b] : 0, sz = b < p ? off[b + 1] - s : 0;
+        const uint32_t szc = (uint32_t)(sz < 0xFFFFFFFFull ? sz : 0xFFFFFFFFull);
+        mx = szc > mx ? szc : mx;
+        for (uint64_t x = 0; __any(x < sz); x += T) {
+            const bool have = x < sz;
+            const uint32_t len = have ? (uint32_t)(sz - x < T ? sz - x : T) : 0u;
+            const int cls = have ? tile_class(len) : -1;
 #pragma unroll
-            for (int o = 32; o > 0; o >>= 1) recs += __shfl_xor(recs, o);
-            if (lane == first) {
-                base = atomicAdd(&tcount[c], (unsigned)__popcll(m));
-                atomicAdd(&trec[c], (unsigned long long)recs);
+            for (int c = 0; c < kTileClasses; ++c) {
+                wcnt[c] += (uint32_t)__popcll(__ballot(cls == c));
+                recs[c] += cls == c ? len : 0u;
             }
-            base = __shfl(base, (int)first);
-            if (cls == c) tiles[(uint64_t)c * cap + base + __popcll(m & below)] = tl;
         }
     }
-    if (b < p && sz > T) {
-        unsigned int q = atomicAdd(n_big, 1u);
-        big[q].start = s;
-        big[q].len = sz;
+    // wavefront totals: largest bucket and records per class
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t v = __shfl_xor(mx, o);
+        mx = v > mx ? v : mx;
+#pragma unroll
+        for (int c = 0; c < kTileClasses; ++c) recs[c] += __shfl_xor(recs[c], o);
+    }
+    uint32_t wbase[kTileClasses];
+#pragma unroll
+    for (int c = 0; c < kTileClasses; ++c) {
+        uint32_t v = 0;
+        if (lane == 0 && wcnt[c]) v = atomicAdd(&s_cnt[c], wcnt[c]);
+        wbase[c] = __shfl(v, 0);
+        if (lane == 0 && recs[c]) atomicAdd(&s_rec[c], recs[c]);
+    }
+    if (lane == 0 && mx) atomicMax(&s_max, mx);
+    __syncthreads();
+    if (threadIdx.x < kTileClasses) {
+        const uint32_t c = threadIdx.x;
+        s_base[c] = s_cnt[c] ? atomicAdd(&tcount[c], s_cnt[c]) : 0u;
+        if (s_rec[c]) atomicAdd(&trec[c], s_rec[c]);
+    }
+    if (threadIdx.x == 0 && s_max) atomicMax(max_bucket, s_max);
+    __syncthreads();
+    uint32_t cur[kTileClasses];
+#pragma unroll
+    for (int c = 0; c < kTileClasses; ++c) cur[c] = s_base[c] + wbase[c];
+    for (uint32_t k = 0; k < kEmitPer; ++k) {
+        const uint64_t b = b_first + k * 256 + threadIdx.x;
+        const uint64_t s = b < p ? off[b] : 0, sz = b < p ? off[b + 1] - s : 0;
+        for (uint64_t x = 0; __any(x < sz); x += T) {
+            const bool have = x < sz;
+            Tile tl{};
+            int cls = -1;
+            if (have) {
+                tl.start = s + x;
+                tl.len = (uint32_t)(sz - x < T ? sz - x : T);
+                tl.whole = sz <= T ? 1u : 0u;
+                cls = tile_class(tl.len);
+            }
+#pragma unroll
+            for (int c = 0; c < kTileClasses; ++c) {
+                const uint64_t m = __ballot(cls == c);
+                if (cls == c) tiles[(uint64_t)c * cap + cur[c] + __popcll(m & below)] = tl;
+                cur[c] += (uint32_t)__popcll(m);
+            }
+        }
+        if (b < p && sz > T) {
+            const unsigned int q = atomicAdd(n_big, 1u);
+            big[q].start = s;
+            big[q].len = sz;
+        }
     }
 }
 

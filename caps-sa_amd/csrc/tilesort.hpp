@@ -299,28 +299,38 @@ __device__ __forceinline__ void block_merge_sort(uint64_t (&k)[ITEMS], uint64_t 
         }
         // B's i-th smallest record is at slot bt - i.  Merge path: the largest
         // x in [lo, hi] with A[x-1] <= B[diag-x] (bit descent, the same
-        // log2(w) + 1 steps on every lane).
+        // log2(w) + 1 steps on every lane, no branch inside).  A lane with
+        // diag > 0 has lo < hi, so the clamped candidate min(c, hi) lies in
+        // [lo + 1, hi] and both of its reads stay inside the two runs; a lane
+        // with diag = 0 has x = 0 and skips the search.
         const uint32_t bt = b0 + w - 1;
         const uint32_t lo = diag > w ? diag - w : 0, hi = diag < w ? diag : w;
         uint32_t x = lo;
-        for (uint32_t st = w; st; st >>= 1) {
-            const uint32_t c = x + st;
-            if (c <= hi && !(sk[px(bt - (diag - c))] < sk[px(a0 + c - 1)])) x = c;
+        if (diag) {
+            const uint32_t bb = bt - diag;  // B slot of candidate c: bb + c
+#pragma unroll 1
+            for (uint32_t st = w; st; st >>= 1) {
+                const uint32_t c = x + st;
+                const uint32_t cc = c < hi ? c : hi;
+                const bool take = !(sk[px(bb + cc)] < sk[px(a0 + cc - 1)]);
+                x = (c <= hi && take) ? c : x;
+            }
         }
         // Serial merge: pa walks A up, pb walks B's slots down.  A exhausted,
         // pa reads B's largest record; B exhausted, pb reads A's largest: the
         // other side is then always taken (a tie is the same slot, read by
-        // both, or pads, whose order does not matter).
-        uint32_t pa = a0 + x, pb = bt - (diag - x);
-        uint64_t ka = sk[px(pa)], kb = sk[px(pb)];
+        // both, or pads, whose order does not matter).  After j outputs
+        // pb = pa + e - j, so only pa is carried.
+        const uint32_t e = bt - diag - a0;
+        uint32_t pa = a0 + x;
+        uint64_t ka = sk[px(pa)], kb = sk[px(pa + e)];
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
             const bool ta = ka <= kb;
             k[j] = ta ? ka : kb;
             pa += ta ? 1u : 0u;
-            pb -= ta ? 0u : 1u;
             if (j + 1 < ITEMS) {
-                const uint64_t nk = sk[px(ta ? pa : pb)];
+                const uint64_t nk = sk[px(ta ? pa : pa + e - (uint32_t)(j + 1))];
                 ka = ta ? nk : ka;
                 kb = ta ? kb : nk;
             }
