@@ -910,7 +910,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
                    : rkind == CAPS_ROUTER_MLP ? resident_blocks(k_classify<S, I, 2>, 256, cls_lds)
                                               : resident_blocks(k_classify<S, I, 0>, 256, cls_lds);
     const uint64_t chunk =
-        std::max<uint64_t>(kSeqChunk, ceil_div(ceil_div(n_iter, (uint64_t)resident), kSeqChunk) * kSeqChunk);
+        std::max<uint64_t>(kStageStep, ceil_div(ceil_div(n_iter, (uint64_t)resident), kStageStep) * kStageStep);
     const unsigned nblk = (unsigned)ceil_div(n_iter, chunk);
     uint32_t *hblk = (uint32_t *)W.hblk.get((uint64_t)nblk * pc * 4);
     if (seq) {
@@ -1020,10 +1020,10 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         if (kStageable && seq && !idbits && pc <= kStageIntervals && !(o.flags & CAPS_FLAG_SCATTER_DIRECT)) {
             KRec<uint32_t> *t32 = (KRec<uint32_t> *)tmp;
             if (d_keep)
-                k_scatter1_staged<S, true><<<nblk, 256, 0, st>>>(d_text, n_iter, cm, g, L, max_ctx, cid, pc, coff,
+                k_scatter1_staged<S, true><<<nblk, kStageNT, 0, st>>>(d_text, n_iter, cm, g, L, max_ctx, cid, pc, coff,
                                                                  hblk, t32, chunk, d_keep, c0, c1, rb);
             else
-                k_scatter1_staged<S, false><<<nblk, 256, 0, st>>>(d_text, n_iter, cm, g, L, max_ctx, cid, pc, coff,
+                k_scatter1_staged<S, false><<<nblk, kStageNT, 0, st>>>(d_text, n_iter, cm, g, L, max_ctx, cid, pc, coff,
                                                                   hblk, t32, chunk, nullptr, c0, c1, rb);
         } else if (seq && d_keep)
             k_scatter1<S, I, true, true><<<nblk, 256, (size_t)pc * 12, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_iter, cid, pc,
@@ -1054,19 +1054,23 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         kt.begin(&stats.k_scatter2);
         // 1,024 threads and 2,048-record tiles when the LDS of a wide interval
         // leaves one block per compute unit, or the batch has too few
-        // intervals to fill the device; else 4,096-record tiles, with 512
-        // threads for a u32 index (57 KB + about 28 bytes per bucket: two
-        // blocks per compute unit) and 1,024 for a u64 index (73 KB: one block)
-        constexpr int kNT = sizeof(I) == 4 ? 512 : 1024;
+        // intervals to fill the device; else 1,024 threads with 8,192-record
+        // tiles for a u32 index (112 KB + about 28 bytes per bucket: one block
+        // per compute unit; twice the bucket run per write of 4,096-record
+        // tiles at two blocks, 3.08 -> 2.72 ms on 268M records) and
+        // 4,096-record tiles for a u64 index (73 KB)
         const uint32_t s2_lds = scatter2_lds_bytes(SP.stride);
         const bool wide = s2_lds > 22 * 1024 || (c_hi - c_lo) < 2 * W.n_cu;
         KRec<I> *trec = all_records ? tmp + rbase : tmp;
         if (wide)
             k_scatter2<I, 1024, 2048><<<c_hi - c_lo, 1024, s2_lds, st>>>(trec, coff, SP, (uint32_t)p, off, keys, out,
                                                                         c_lo, c_hi, rbase);
+        else if constexpr (sizeof(I) == 4)
+            k_scatter2<I, 1024, 8192><<<c_hi - c_lo, 1024, s2_lds, st>>>(trec, coff, SP, (uint32_t)p, off, keys, out,
+                                                                        c_lo, c_hi, rbase);
         else
-            k_scatter2<I, kNT, 4096><<<c_hi - c_lo, kNT, s2_lds, st>>>(trec, coff, SP, (uint32_t)p, off, keys, out,
-                                                                      c_lo, c_hi, rbase);
+            k_scatter2<I, 1024, 4096><<<c_hi - c_lo, 1024, s2_lds, st>>>(trec, coff, SP, (uint32_t)p, off, keys, out,
+                                                                        c_lo, c_hi, rbase);
         LAUNCHCHK();
         kt.end();
         tm.mark(PH_SCATTER);

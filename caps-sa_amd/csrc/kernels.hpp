@@ -797,31 +797,58 @@ __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint6
 }
 
 // Scatter pass 1 for contiguous positions (every suffix, or the kept ones
-// of a bitmap) with coalesced writes: each 4,096-position step of a block
-// counting-sorts its records by interval in LDS, then consecutive threads
-// write each interval's run of the step, so a wavefront's store covers a few
-// contiguous runs instead of 64 scattered records.  For at most
+// of a bitmap) with coalesced writes: each kStageStep-position step of a
+// block counting-sorts its records by interval in LDS, then consecutive
+// threads write each interval's run of the step, so a wavefront's store
+// covers a few contiguous runs instead of 64 scattered records.  For at most
 // kStageIntervals intervals and 12-byte records (u32 index); the block owns
-// the same range of positions as in k_scatter1.
+// the same range of positions as in k_scatter1.  One block of 512 threads
+// per compute unit (137 KB of LDS with u8 text) stages 8,192 positions a
+// step: about eight records per interval and step instead of the four of
+// 4,096-position steps at two blocks of 256.
 constexpr uint32_t kStageIntervals = 1024;
+constexpr uint32_t kStageNT = 512;
+constexpr uint32_t kStageStep = kStageNT * kSeqItems;
+
+// Exclusive scan of one u32 per thread across a block of NT threads; wsum
+// holds NT / 64 entries.
+template <int NT>
+__device__ __forceinline__ uint32_t block_excl_scan_nt(uint32_t v, uint32_t *wsum) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t before = 0;
+#pragma unroll
+    for (int q = 0; q < NT / 64; ++q)
+        if (q < w) before += wsum[q];
+    __syncthreads();
+    return before + x - v;
+}
+
 template <typename S, bool KEEP>
-__global__ __launch_bounds__(256) void k_scatter1_staged(const S *__restrict__ t, uint64_t n, CodeMap<S> cm,
-                                                         KeyGeom g, DevLimits L, uint64_t max_ctx,
-                                                         const uint16_t *__restrict__ cid, uint32_t pc,
-                                                         const uint64_t *__restrict__ coff,
-                                                         const uint32_t *__restrict__ hblk,
-                                                         KRec<uint32_t> *__restrict__ tmp, uint64_t chunk,
-                                                         const uint64_t *__restrict__ keep, uint32_t c_lo,
-                                                         uint32_t c_hi, uint64_t rbase) {
-    constexpr uint32_t E = kStageIntervals / 256;  // intervals per thread in the scan
+__global__ __launch_bounds__(kStageNT) void k_scatter1_staged(const S *__restrict__ t, uint64_t n, CodeMap<S> cm,
+                                                              KeyGeom g, DevLimits L, uint64_t max_ctx,
+                                                              const uint16_t *__restrict__ cid, uint32_t pc,
+                                                              const uint64_t *__restrict__ coff,
+                                                              const uint32_t *__restrict__ hblk,
+                                                              KRec<uint32_t> *__restrict__ tmp, uint64_t chunk,
+                                                              const uint64_t *__restrict__ keep, uint32_t c_lo,
+                                                              uint32_t c_hi, uint64_t rbase) {
+    constexpr uint32_t E = kStageIntervals / kStageNT;  // intervals per thread in the scan
     constexpr uint32_t kSkip = 0xFFFFFFFFu;
-    __shared__ S win[kSeqChunk + 72];
-    __shared__ uint64_t s_key[kSeqChunk];
-    __shared__ uint32_t s_pos[kSeqChunk];
-    __shared__ uint16_t s_c[kSeqChunk];
+    __shared__ S win[kStageStep + 72];
+    __shared__ uint64_t s_key[kStageStep];
+    __shared__ uint32_t s_pos[kStageStep];
+    __shared__ uint16_t s_c[kStageStep];
     __shared__ unsigned long long cbase[kStageIntervals];
     __shared__ uint32_t icnt[kStageIntervals], ioff[kStageIntervals];
-    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t wsum[kStageNT / 64];
     const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
     if (c0 >= n) return;
     const uint64_t c1 = c0 + chunk < n ? c0 + chunk : n;
@@ -829,8 +856,8 @@ __global__ __launch_bounds__(256) void k_scatter1_staged(const S *__restrict__ t
         cbase[c] = c < pc ? coff[c] - rbase + hblk[(uint64_t)blockIdx.x * pc + c] : 0ull;
         icnt[c] = 0;
     }
-    for (uint64_t base = c0; base < c1; base += kSeqChunk) {
-        const uint32_t cnt = (uint32_t)(c1 - base < kSeqChunk ? c1 - base : kSeqChunk);
+    for (uint64_t base = c0; base < c1; base += kStageStep) {
+        const uint32_t cnt = (uint32_t)(c1 - base < kStageStep ? c1 - base : kStageStep);
         uint64_t key[kSeqItems];
         thread_keys_seq<S>(t, n, cm, g, L, max_ctx, base, cnt, win, key);  // begins with a barrier
         const uint32_t first = threadIdx.x * kSeqItems;
@@ -863,8 +890,10 @@ __global__ __launch_bounds__(256) void k_scatter1_staged(const S *__restrict__ t
             v[e] = icnt[threadIdx.x * E + e];
             sum += v[e];
         }
-        uint32_t pre = block_excl_scan256(sum, wsum);
-        const uint32_t total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        uint32_t pre = block_excl_scan_nt<kStageNT>(sum, wsum);
+        uint32_t total = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < kStageNT / 64; ++q) total += wsum[q];
 #pragma unroll
         for (uint32_t e = 0; e < E; ++e) {
             ioff[threadIdx.x * E + e] = pre;

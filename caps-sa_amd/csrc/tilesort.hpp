@@ -320,7 +320,9 @@ __device__ __forceinline__ void block_merge_sort(uint64_t (&k)[ITEMS], uint64_t 
         // pa reads B's largest record; B exhausted, pb reads A's largest: the
         // other side is then always taken (a tie is the same slot, read by
         // both, or pads, whose order does not matter).  After j outputs
-        // pb = pa + e - j, so only pa is carried.
+        // pb = pa + e - j, so only pa is carried, and both heads are read
+        // again each step (two independent loads instead of one load and
+        // the selects that would keep the untaken head).
         const uint32_t e = bt - diag - a0;
         uint32_t pa = a0 + x;
         uint64_t ka = sk[px(pa)], kb = sk[px(pa + e)];
@@ -330,9 +332,8 @@ __device__ __forceinline__ void block_merge_sort(uint64_t (&k)[ITEMS], uint64_t 
             k[j] = ta ? ka : kb;
             pa += ta ? 1u : 0u;
             if (j + 1 < ITEMS) {
-                const uint64_t nk = sk[px(ta ? pa : pa + e - (uint32_t)(j + 1))];
-                ka = ta ? nk : ka;
-                kb = ta ? kb : nk;
+                ka = sk[px(pa)];
+                kb = sk[px(pa + e - (uint32_t)(j + 1))];
             }
         }
     }
@@ -365,10 +366,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T <= 2048 ?
     if (len == 0) return;
     const uint64_t first = keys[tl.start];
     uint64_t my_or = 0;
-    for (uint32_t i = tid; i < (uint32_t)T; i += B) {
+    // the tile's index values are loaded now, with the keys, and staged in
+    // LDS after the sort: no global round trip between the sort and the output
+    I ix[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t i = tid + (uint32_t)j * B;
         uint64_t kk = ~0ull;
+        ix[j] = (I)0;
         if (i < len) {
             kk = keys[tl.start + i];
+            ix[j] = idx[tl.start + i];
             my_or |= kk ^ first;
         }
         sk[px(i)] = kk;
@@ -409,7 +417,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T <= 2048 ?
     __syncthreads();  // the sort buffer is free
     // index values: original order into LDS, gathered in sorted order, back
     // to LDS, written out coalesced
-    for (uint32_t i = tid; i < len; i += B) stage[i] = idx[tl.start + i];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j)
+        if (tid + (uint32_t)j * B < len) stage[tid + (uint32_t)j * B] = ix[j];
     __syncthreads();
     I o[ITEMS];
 #pragma unroll
