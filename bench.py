@@ -378,7 +378,7 @@ def main() -> int:
     ap.add_argument("--workload", default="repeat-rich-268M", choices=sorted(WORKLOADS))
     ap.add_argument("--n", type=int, default=0, help="override the text length")
     ap.add_argument("--seed", type=int, default=0xC1)
-    ap.add_argument("--router", default="pwl", choices=["bsearch", "pwl", "mlp"],
+    ap.add_argument("--router", default="spline", choices=["bsearch", "pwl", "mlp", "spline"],
                     help="the learned router timed for `value` (repeat-rich workload); binary search is the plain "
                          "build it is compared with")
     ap.add_argument("--buckets", type=int, default=4096,
@@ -516,7 +516,7 @@ def run_single(args, kind: str, desc: str, n_default: int) -> int:
     d_text = dev.text_to_device(text)
     d_out = torch.empty(n, dtype=torch.int32, device="cuda:0")
     d_other = torch.empty(n, dtype=torch.int32, device="cuda:0")
-    rk = {"bsearch": 0, "pwl": 1, "mlp": 2}[args.router]
+    rk = {"bsearch": 0, "pwl": 1, "mlp": 2, "spline": 3}[args.router]
     tile_keys = args.bucket_keys
     flags = ((caps_sa.FLAG_PERSISTENT_REFINE if args.refine == "queue" else 0)
              | (caps_sa.FLAG_TILE_RADIX if args.tile_sort == "radix" else 0) | args.flags)
@@ -564,10 +564,14 @@ def run_single(args, kind: str, desc: str, n_default: int) -> int:
                            "train_and_sample_ms": avg(st_l, "ms_sample"),
                            "mean_mispredict": round(ll["router_err_sum"] / max(ll["router_count"], 1), 4),
                            "max_mispredict": ll["router_err_max"],
-                           "fallback_frac": round(ll["router_fallbacks"] / max(ll["router_count"], 1), 6)},
+                           "fallback_frac": round(ll["router_fallbacks"] / max(ll["router_count"], 1), 6),
+                           "error_bound": ([ll["router_bound_lo"], ll["router_bound_hi"]]
+                                           if args.router == "spline" else None)},
                "plain": {"router": "bsearch", "ms_per_step": round(ms_p, 3),
                          "value": round(n / (ms_p / 1e3) / 1e6, 2), "k_classify_ms": avg(st_p, "k_classify"),
                          "sample_ms": avg(st_p, "ms_sample")},
+               "kernels_ms": {w: {k: round(sum(launch_ms(x)[k] for x in sts) / len(sts), 3) for k in launch_ms(sts[0])}
+                              for w, sts in (("learned", st_l), ("plain", st_p))},
                "speedup": round(ms_p / ms_l, 4),
                "imbalance": {"classification": round(lp["coarse_max"] * C / n, 4),
                              "bucket": round(lp["bucket_max"] * lp["buckets"] / n, 4)},

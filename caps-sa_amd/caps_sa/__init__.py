@@ -103,7 +103,8 @@ class Stats(ctypes.Structure):
          ("k_scatter1", ctypes.c_double), ("k_scatter2", ctypes.c_double), ("batches", ctypes.c_uint64),
          ("coarse_buckets", ctypes.c_uint64), ("coarse_max", ctypes.c_uint64), ("scratch_bytes", ctypes.c_uint64),
          ("k_tile", ctypes.c_double * 3), ("tile_records", ctypes.c_uint64 * 3), ("cmp_symbols", ctypes.c_uint64 * 3),
-         ("k_graph", ctypes.c_double * 3), ("runs", ctypes.c_uint64)]
+         ("k_graph", ctypes.c_double * 3), ("runs", ctypes.c_uint64), ("router_bound_lo", ctypes.c_int64),
+         ("router_bound_hi", ctypes.c_int64)]
 
     def as_dict(self) -> dict:
         out = {}
@@ -168,6 +169,8 @@ _SIGNATURES = {
                                                    ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                    ctypes.c_void_p]),
+    "caps_router_spline_fit_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                                      ctypes.c_void_p, ctypes.c_void_p]),
     "caps_build_device_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p,
                                                   ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(_Limits),
                                                   ctypes.POINTER(_Opts), ctypes.c_void_p, ctypes.c_void_p,
@@ -337,6 +340,7 @@ class Router(enum.IntEnum):
     BSEARCH = 0
     PWL = 1
     MLP = 2
+    SPLINE = 3
 
 
 @dataclass

@@ -150,6 +150,20 @@ def pwl_fit(keys: np.ndarray, labels: np.ndarray, ns: int, n_buckets: int, devic
     return out
 
 
+def spline_fit(splitters: np.ndarray, device="cuda:0"):
+    """caps_router_spline_fit_device over a sorted splitter table: (knots
+    (4,097 u16), (min, max) of predicted - true position over every key)."""
+    n = len(splitters)
+    k = torch.from_numpy(np.ascontiguousarray(splitters, dtype=np.uint64).view(np.int64)).to(device) if n else None
+    knots = np.zeros(SPLINE_CELLS + 1, dtype=np.uint16)
+    bound = np.zeros(2, dtype=np.int32)
+    _check(lib().caps_router_spline_fit_device(k.data_ptr() if n else None, n, knots.ctypes.data,
+                                               bound.ctypes.data, _stream_ptr()))
+    return knots, (int(bound[0]), int(bound[1]))
+
+
+SPLINE_CELLS = 4096
+
 MLP_IN, MLP_HID = 64, 32
 MLP_PARAMS = MLP_IN * MLP_HID + MLP_HID + MLP_HID + 1
 

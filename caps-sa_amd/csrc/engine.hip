@@ -513,6 +513,15 @@ static std::vector<float> mlp_init_params(uint64_t seed) {
     return p;
 }
 
+// Spline router: knots from the sorted splitter table ctab[0..nc), then its
+// exact error bound into bound[0..2) (zeroed by the caller).
+static void spline_fit(const uint64_t *ctab, uint32_t nc, uint16_t *kn, int *bound, hipStream_t st) {
+    k_spline_knots<<<(unsigned)ceil_div(kSplineCells + 1, 256), 256, 0, st>>>(ctab, nc, kn);
+    LAUNCHCHK();
+    k_spline_bound<<<(unsigned)ceil_div(2ull * nc + 2 * kSplineCells, 256), 256, 0, st>>>(ctab, nc, kn, bound);
+    LAUNCHCHK();
+}
+
 template <typename S, typename I>
 static RouterDev train_router(Workspace &W, uint32_t kind, uint32_t flags, const uint64_t *sorted, uint64_t s,
                               const Splitters &SP, const S *d_text, CodeMap<S> cm, KeyGeom g, DevLimits L,
@@ -584,6 +593,19 @@ static RouterDev train_router(Workspace &W, uint32_t kind, uint32_t flags, const
         LAUNCHCHK();
         R.nib = nib;
         R.w2 = params + kMlpIn * kMlpHid + kMlpHid;
+    } else if (kind == CAPS_ROUTER_SPLINE) {
+        uint16_t *kn = (uint16_t *)W.r_radix.get((kSplineCells + 1) * 2);
+        int *bound = (int *)(stats + 4);  // zeroed with the statistics
+        spline_fit(ctab, nc, kn, bound, st);
+        int hb[2];
+        HIPCHK(hipMemcpyAsync(hb, bound, 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        R.spline = kn;
+        R.el = -hb[0];
+        R.eh = hb[1];
+        uint32_t w = (uint32_t)(R.el + R.eh), bl = 0;
+        while (w >> bl) ++bl;
+        R.steps = bl;
     }
     return R;
 }
@@ -883,7 +905,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     uint64_t *off = (uint64_t *)W.off.get((p + 1) * 8);
     RouterDev R{};
     uint32_t rkind = p > 1 ? o.router : CAPS_ROUTER_BSEARCH;
-    if (rkind > CAPS_ROUTER_MLP) throw CapsError(CAPS_E_INVALID, "unknown router");
+    if (rkind > CAPS_ROUTER_SPLINE) throw CapsError(CAPS_E_INVALID, "unknown router");
     if (rkind != CAPS_ROUTER_BSEARCH)
         R = train_router<S, I>(W, rkind, o.flags, sorted_sample, s, SP, d_text, cm, g, L, max_ctx, d_pos, n_pos, st,
                                d_keep, n);
@@ -899,50 +921,54 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     // block's histogram is then exactly the scatter block's run sizes.  About
     // one resident wave of classify blocks; the coarse splitter table is
     // staged once per block.
-    unsigned resident = 0;
+    // The grid follows the binary-search kernel whichever router runs, so
+    // the block ranges (and scatter pass 1's grid, which derives from them)
+    // do not depend on the router.
     const size_t cls_lds = classify_lds_bytes(SP.n_coarse);
-    if (seq)
-        resident = rkind == CAPS_ROUTER_PWL   ? resident_blocks(k_classify_seq<S, I, 1, false>, 256, cls_lds)
-                   : rkind == CAPS_ROUTER_MLP ? resident_blocks(k_classify_seq<S, I, 2, false>, 256, cls_lds)
-                                              : resident_blocks(k_classify_seq<S, I, 0, false>, 256, cls_lds);
-    else
-        resident = rkind == CAPS_ROUTER_PWL   ? resident_blocks(k_classify<S, I, 1>, 256, cls_lds)
-                   : rkind == CAPS_ROUTER_MLP ? resident_blocks(k_classify<S, I, 2>, 256, cls_lds)
-                                              : resident_blocks(k_classify<S, I, 0>, 256, cls_lds);
+    const unsigned resident = seq ? resident_blocks(k_classify_seq<S, I, 0, false>, 256, cls_lds)
+                                  : resident_blocks(k_classify<S, I, 0>, 256, cls_lds);
     const uint64_t chunk =
         std::max<uint64_t>(kStageStep, ceil_div(ceil_div(n_iter, (uint64_t)resident), kStageStep) * kStageStep);
     const unsigned nblk = (unsigned)ceil_div(n_iter, chunk);
     uint32_t *hblk = (uint32_t *)W.hblk.get((uint64_t)nblk * pc * 4);
-    if (seq) {
-        auto classify_seq = [&](auto rk, auto kp) {
-            constexpr int RK = decltype(rk)::value;
-            constexpr bool KP = decltype(kp)::value;
-            k_classify_seq<S, I, RK, KP><<<nblk, 256, cls_lds, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, cid, hblk, pc,
-                                                               chunk, d_keep);
-        };
-        using kno = std::false_type;
-        using kyes = std::true_type;
-        if (rkind == CAPS_ROUTER_PWL)
-            d_keep ? classify_seq(std::integral_constant<int, 1>{}, kyes{})
-                   : classify_seq(std::integral_constant<int, 1>{}, kno{});
-        else if (rkind == CAPS_ROUTER_MLP)
-            d_keep ? classify_seq(std::integral_constant<int, 2>{}, kyes{})
-                   : classify_seq(std::integral_constant<int, 2>{}, kno{});
+    auto classify = [&](uint32_t kind) {
+        if (seq) {
+            auto classify_seq = [&](auto rk, auto kp) {
+                constexpr int RK = decltype(rk)::value;
+                constexpr bool KP = decltype(kp)::value;
+                k_classify_seq<S, I, RK, KP><<<nblk, 256, cls_lds, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, cid, hblk,
+                                                                   pc, chunk, d_keep);
+            };
+            using kno = std::false_type;
+            using kyes = std::true_type;
+            if (kind == CAPS_ROUTER_PWL)
+                d_keep ? classify_seq(std::integral_constant<int, 1>{}, kyes{})
+                       : classify_seq(std::integral_constant<int, 1>{}, kno{});
+            else if (kind == CAPS_ROUTER_MLP)
+                d_keep ? classify_seq(std::integral_constant<int, 2>{}, kyes{})
+                       : classify_seq(std::integral_constant<int, 2>{}, kno{});
+            else if (kind == CAPS_ROUTER_SPLINE)
+                d_keep ? classify_seq(std::integral_constant<int, 3>{}, kyes{})
+                       : classify_seq(std::integral_constant<int, 3>{}, kno{});
+            else
+                d_keep ? classify_seq(std::integral_constant<int, 0>{}, kyes{})
+                       : classify_seq(std::integral_constant<int, 0>{}, kno{});
+        } else if (kind == CAPS_ROUTER_PWL)
+            k_classify<S, I, 1><<<nblk, 256, cls_lds, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hblk, pc, chunk);
+        else if (kind == CAPS_ROUTER_MLP)
+            k_classify<S, I, 2><<<nblk, 256, cls_lds, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hblk, pc, chunk);
+        else if (kind == CAPS_ROUTER_SPLINE)
+            k_classify<S, I, 3><<<nblk, 256, cls_lds, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hblk, pc, chunk);
         else
-            d_keep ? classify_seq(std::integral_constant<int, 0>{}, kyes{})
-                   : classify_seq(std::integral_constant<int, 0>{}, kno{});
-    } else if (rkind == CAPS_ROUTER_PWL)
-        k_classify<S, I, 1><<<nblk, 256, cls_lds, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hblk, pc, chunk);
-    else if (rkind == CAPS_ROUTER_MLP)
-        k_classify<S, I, 2><<<nblk, 256, cls_lds, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hblk, pc, chunk);
-    else
-        k_classify<S, I, 0><<<nblk, 256, cls_lds, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hblk, pc, chunk);
-    LAUNCHCHK();
-    kt.end();
-    k_block_offsets<<<(unsigned)ceil_div(pc, kBoCols), kBoCols * kBoRows, 0, st>>>(hblk, nblk, pc, hist);
-    LAUNCHCHK();
-    k_scan_u64<<<1, 1024, 0, st>>>(hist, pc, coff);
-    LAUNCHCHK();
+            k_classify<S, I, 0><<<nblk, 256, cls_lds, st>>>(d_text, cm, g, L, max_ctx, d_pos, n_pos, SP, R, cid, hblk, pc, chunk);
+        LAUNCHCHK();
+        if (kind == rkind) kt.end();  // the build's own classify launch (opened above)
+        k_block_offsets<<<(unsigned)ceil_div(pc, kBoCols), kBoCols * kBoRows, 0, st>>>(hblk, nblk, pc, hist);
+        LAUNCHCHK();
+        k_scan_u64<<<1, 1024, 0, st>>>(hist, pc, coff);
+        LAUNCHCHK();
+    };
+    classify(rkind);
     tm.mark(PH_CLASSIFY);
 
     // ---- bucket batches: consecutive coarse intervals, at most bmax records each ----
@@ -950,7 +976,14 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     // key order, so the suffix array is their concatenation.
     std::vector<uint64_t> hcoff(pc + 1);
     HIPCHK(hipMemcpyAsync(hcoff.data(), coff, (pc + 1) * 8, hipMemcpyDeviceToHost, st));
+    unsigned long long spline_fb = 0;  // keys outside the spline's window (none while its bound holds)
+    if (rkind == CAPS_ROUTER_SPLINE) HIPCHK(hipMemcpyAsync(&spline_fb, R.stats + 2, 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    if (spline_fb) {  // binary search is the judge: classify every key again
+        classify(CAPS_ROUTER_BSEARCH);
+        HIPCHK(hipMemcpyAsync(hcoff.data(), coff, (pc + 1) * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
     // per-interval counts and in-interval offsets are 32-bit in the scatter
     // passes (block prefixes, LDS bucket counts)
     uint64_t cmax = 0;
@@ -1017,23 +1050,33 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         // builds: their keys are text-major, so a step's records fall in one
         // or two intervals and are written contiguously already.
         constexpr bool kStageable = sizeof(I) == 4;
-        if (kStageable && seq && !idbits && pc <= kStageIntervals && !(o.flags & CAPS_FLAG_SCATTER_DIRECT)) {
+        const bool staged = kStageable && seq && !idbits && pc <= kStageIntervals &&
+                            !(o.flags & CAPS_FLAG_SCATTER_DIRECT);
+        // A workgroup of pass 1 takes hstride consecutive classify blocks (their
+        // runs in every interval are consecutive): about kP1StagedPerCu (staged)
+        // or kP1DirectPerCu workgroups per compute unit, whatever the classify
+        // kernel's own occupancy, so each workgroup's run per interval is long.
+        const uint32_t target = (staged ? kP1StagedPerCu : kP1DirectPerCu) * W.n_cu;
+        const uint32_t hs = (uint32_t)std::max<uint64_t>(1, ceil_div(nblk, std::max<uint32_t>(target, 1)));
+        const unsigned nb1 = (unsigned)ceil_div(nblk, hs);
+        const uint64_t ch1 = chunk * hs;
+        if (staged) {
             KRec<uint32_t> *t32 = (KRec<uint32_t> *)tmp;
             if (d_keep)
-                k_scatter1_staged<S, true><<<nblk, kStageNT, 0, st>>>(d_text, n_iter, cm, g, L, max_ctx, cid, pc, coff,
-                                                                 hblk, t32, chunk, d_keep, c0, c1, rb);
+                k_scatter1_staged<S, true><<<nb1, kStageNT, 0, st>>>(d_text, n_iter, cm, g, L, max_ctx, cid, pc, coff,
+                                                                     hblk, t32, ch1, d_keep, c0, c1, rb, hs);
             else
-                k_scatter1_staged<S, false><<<nblk, kStageNT, 0, st>>>(d_text, n_iter, cm, g, L, max_ctx, cid, pc, coff,
-                                                                  hblk, t32, chunk, nullptr, c0, c1, rb);
+                k_scatter1_staged<S, false><<<nb1, kStageNT, 0, st>>>(d_text, n_iter, cm, g, L, max_ctx, cid, pc, coff,
+                                                                      hblk, t32, ch1, nullptr, c0, c1, rb, hs);
         } else if (seq && d_keep)
-            k_scatter1<S, I, true, true><<<nblk, 256, (size_t)pc * 12, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_iter, cid, pc,
-                                                               coff, hblk, tmp, chunk, d_keep, c0, c1, rb);
+            k_scatter1<S, I, true, true><<<nb1, 256, (size_t)pc * 12, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_iter, cid, pc,
+                                                               coff, hblk, tmp, ch1, d_keep, c0, c1, rb, hs);
         else if (seq)
-            k_scatter1<S, I, true, false><<<nblk, 256, (size_t)pc * 12, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_iter, cid, pc,
-                                                                coff, hblk, tmp, chunk, nullptr, c0, c1, rb);
+            k_scatter1<S, I, true, false><<<nb1, 256, (size_t)pc * 12, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_iter, cid, pc,
+                                                                coff, hblk, tmp, ch1, nullptr, c0, c1, rb, hs);
         else
-            k_scatter1<S, I, false, false><<<nblk, 256, (size_t)pc * 12, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, cid, pc,
-                                                                 coff, hblk, tmp, chunk, nullptr, c0, c1, rb);
+            k_scatter1<S, I, false, false><<<nb1, 256, (size_t)pc * 12, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_pos, cid, pc,
+                                                                 coff, hblk, tmp, ch1, nullptr, c0, c1, rb, hs);
         LAUNCHCHK();
         kt.end();
     };
@@ -1332,6 +1375,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         HIPCHK(hipStreamSynchronize(st));
         stats.router_err_sum = rs[0], stats.router_err_max = rs[1], stats.router_fallbacks = rs[2];
         stats.router_count = rs[3];
+        if (R.kind == CAPS_ROUTER_SPLINE) stats.router_bound_lo = -R.el, stats.router_bound_hi = R.eh;
     }
     HIPCHK(hipStreamSynchronize(st));
     tm.collect(stats);
@@ -1991,6 +2035,24 @@ int caps_router_pwl_fit_device(const uint64_t *d_keys, const uint32_t *d_labels,
         HIPCHK(hipMemcpyAsync(inv, di, ns * 4, hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(lo, dlo, ns * 4, hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(hi, dhi, ns * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
+int caps_router_spline_fit_device(const uint64_t *d_splitters, uint32_t n, uint16_t *knots, int32_t *bound,
+                                  void *stream) {
+    return guarded([&] {
+        if (n >= 65536) throw CapsError(CAPS_E_INVALID, "at most 65,535 splitters");
+        if (!knots || !bound) throw CapsError(CAPS_E_INVALID, "output pointer is NULL");
+        if (n && !d_splitters) throw CapsError(CAPS_E_INVALID, "splitter pointer is NULL");
+        hipStream_t st = (hipStream_t)stream;
+        Workspace &W = workspace();
+        uint16_t *dk = (uint16_t *)W.r_radix.get((kSplineCells + 1) * 2);
+        int *db = (int *)W.r_lo.get(8);
+        HIPCHK(hipMemsetAsync(db, 0, 8, st));
+        spline_fit(d_splitters, n, dk, db, st);
+        HIPCHK(hipMemcpyAsync(knots, dk, (kSplineCells + 1) * 2, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(bound, db, 8, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
     });
 }

@@ -389,6 +389,63 @@ __device__ __forceinline__ void coarse_routed(const Splitters &SP, const uint64_
 }
 
 
+// The spline router (RK == 3): prediction, then the exact position inside
+// the window [pred - eh, pred + el] that the spline's error bound guarantees,
+// by a fixed number of branch-free steps over the sorted table (N keys in
+// lockstep, each phase's loads issued for all keys before any is used).  The
+// search returns the largest window position whose splitter below is <= key,
+// so a key outside its window could only end at one of the window's ends:
+// a result there is checked against the splitter beyond that end.  A key
+// that fails is counted as a fallback, and the host, finding any,
+// classifies the whole input again by binary search.
+template <int N>
+__device__ __forceinline__ void coarse_spline(const uint64_t *tab, uint32_t n, const RouterDev &R, const uint16_t *kn,
+                                              const uint64_t (&key)[N], const bool (&valid)[N], uint32_t (&pos)[N],
+                                              unsigned long long &sum, unsigned long long &mx,
+                                              unsigned long long &fbs, unsigned long long &cnt) {
+    const uint32_t el = (uint32_t)R.el, eh = (uint32_t)R.eh;
+    uint32_t pred[N], x[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const uint32_t c = (uint32_t)(key[j] >> (64 - kSplineBits));
+        pred[j] = kn[c];
+        x[j] = kn[c + 1];
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const uint32_t f = (uint32_t)(key[j] >> (48 - kSplineBits)) & 0xFFFFu;
+        pred[j] += (f * (x[j] - pred[j])) >> 16;
+        x[j] = pred[j] > eh ? pred[j] - eh : 0u;
+    }
+    for (uint32_t st = R.steps ? 1u << (R.steps - 1) : 0u; st; st >>= 1) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            const uint32_t whi = pred[j] + el < n ? pred[j] + el : n;
+            const uint32_t c = x[j] + st;
+            const uint32_t cc = c < whi ? c : whi;
+            const bool take = (c <= whi) & (tab[cc ? cc - 1 : 0] <= key[j]);
+            x[j] = take ? c : x[j];
+        }
+    }
+    uint32_t esum = 0, emax = 0, nv = 0, nbad = 0;  // errors are below 2^16
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const uint32_t wlo = pred[j] > eh ? pred[j] - eh : 0u;
+        const uint32_t whi = pred[j] + el < n ? pred[j] + el : n;
+        if (x[j] == wlo && wlo > 0 && valid[j]) nbad += tab[wlo - 1] > key[j] ? 1u : 0u;
+        if (x[j] == whi && whi < n && valid[j]) nbad += key[j] >= tab[whi] ? 1u : 0u;
+        pos[j] = x[j];
+        const uint32_t e = x[j] > pred[j] ? x[j] - pred[j] : pred[j] - x[j];
+        esum += valid[j] ? e : 0u;
+        emax = valid[j] && e > emax ? e : emax;
+        nv += valid[j] ? 1u : 0u;
+    }
+    sum += esum;
+    mx = emax > mx ? emax : mx;
+    fbs += nbad;
+    cnt += nv;
+}
+
 template <typename S, typename I, int RK>
 __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeMap<S> cm, KeyGeom g, DevLimits L,
                                                   uint64_t max_ctx, const I *__restrict__ pos, uint64_t n_pos,
@@ -404,6 +461,7 @@ __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeM
     __shared__ uint32_t rlo[RK == 1 ? NS : 1], rhi[RK == 1 ? NS : 1];
     __shared__ uint16_t rradix[RK == 1 ? (1u << kRouterRadixBits) + 1 : 1];
     __shared__ float nib[RK == 2 ? 16 * 16 * kMlpHid : 1], w2[RK == 2 ? kMlpHid + 1 : 1];
+    __shared__ uint16_t rkn[RK == 3 ? kSplineCells + 1 : 1];
     __shared__ unsigned long long st_sum, st_max, st_fb, st_cnt;
     load_coarse_table<RK>(SP, coarse);
     if (RK == 1) {
@@ -421,6 +479,8 @@ __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeM
         for (uint32_t c = threadIdx.x; c < 16 * 16 * kMlpHid; c += blockDim.x) nib[c] = R.nib[c];
         for (uint32_t c = threadIdx.x; c <= (uint32_t)kMlpHid; c += blockDim.x) w2[c] = R.w2[c];
     }
+    if (RK == 3)
+        for (uint32_t c = threadIdx.x; c <= kSplineCells; c += blockDim.x) rkn[c] = R.spline[c];
     if (threadIdx.x == 0) st_sum = st_max = st_fb = st_cnt = 0;
     __syncthreads();
     unsigned long long my_sum = 0, my_max = 0, my_fb = 0, my_cnt = 0;
@@ -445,6 +505,8 @@ __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeM
             }
         }
         if (RK == 0) coarse_bsearch<NB>(SP, coarse, key, b);
+        else if (RK == 3)
+            coarse_spline<NB>(coarse, SP.n_coarse, R, rkn, key, valid, b, my_sum, my_max, my_fb, my_cnt);
         else
             coarse_routed<RK, NB>(SP, coarse, R, rknots, rradix, ra, rb, rinv, rlo, rhi, nib, w2, key, valid, b,
                                   my_sum, my_max, my_fb, my_cnt);
@@ -529,7 +591,7 @@ __device__ __forceinline__ void thread_keys_seq(const S *__restrict__ t, uint64_
 // instead of three; the allocator spills two dwords): 1.85 -> 1.60 ms on the
 // 268M text.  The router variants keep their own allocation.
 template <typename S, typename I, int RK, bool KEEP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RK == 0 ? 4 : 1))) void k_classify_seq(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, KeyGeom g,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RK == 0 || RK == 3 ? 4 : 1))) void k_classify_seq(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, KeyGeom g,
                                                       DevLimits L, uint64_t max_ctx, Splitters SP, RouterDev R,
                                                       uint16_t *__restrict__ cid, uint32_t *__restrict__ hblk,
                                                       uint32_t pc, uint64_t chunk, const uint64_t *__restrict__ keep) {
@@ -544,6 +606,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RK == 0 ? 4
     __shared__ uint32_t rlo[RK == 1 ? NS : 1], rhi[RK == 1 ? NS : 1];
     __shared__ uint16_t rradix[RK == 1 ? (1u << kRouterRadixBits) + 1 : 1];
     __shared__ float nib[RK == 2 ? 16 * 16 * kMlpHid : 1], w2[RK == 2 ? kMlpHid + 1 : 1];
+    __shared__ uint16_t rkn[RK == 3 ? kSplineCells + 1 : 1];
     __shared__ unsigned long long st_sum, st_max, st_fb, st_cnt;
     load_coarse_table<RK>(SP, coarse);
     if (RK == 1) {
@@ -561,6 +624,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RK == 0 ? 4
         for (uint32_t c = threadIdx.x; c < 16 * 16 * kMlpHid; c += blockDim.x) nib[c] = R.nib[c];
         for (uint32_t c = threadIdx.x; c <= (uint32_t)kMlpHid; c += blockDim.x) w2[c] = R.w2[c];
     }
+    if (RK == 3)
+        for (uint32_t c = threadIdx.x; c <= kSplineCells; c += blockDim.x) rkn[c] = R.spline[c];
     if (threadIdx.x == 0) st_sum = st_max = st_fb = st_cnt = 0;
     unsigned long long my_sum = 0, my_max = 0, my_fb = 0, my_cnt = 0;
     // this block's positions: [c0, c1), the same range the scatter's block owns
@@ -580,6 +645,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RK == 0 ? 4
 #pragma unroll
         for (uint32_t j = 0; j < kSeqItems; ++j) valid[j] = first + j < cnt && ((kb >> j) & 1u);
         if (RK == 0) coarse_bsearch<kSeqItems>(SP, coarse, key, b);
+        else if (RK == 3)
+            coarse_spline<kSeqItems>(coarse, SP.n_coarse, R, rkn, key, valid, b, my_sum, my_max, my_fb, my_cnt);
         else
             coarse_routed<RK, kSeqItems>(SP, coarse, R, rknots, rradix, ra, rb, rinv, rlo, rhi, nib, w2, key, valid,
                                          b, my_sum, my_max, my_fb, my_cnt);
@@ -733,7 +800,7 @@ __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint6
                                                   const uint64_t *__restrict__ coff,
                                                   const uint32_t *__restrict__ hblk, KRec<I> *__restrict__ tmp,
                                                   uint64_t chunk, const uint64_t *__restrict__ keep, uint32_t c_lo,
-                                                  uint32_t c_hi, uint64_t rbase) {
+                                                  uint32_t c_hi, uint64_t rbase, uint32_t hstride) {
     __shared__ S win[SEQ ? kSeqChunk + 72 : 1];
     // LDS sized by the launch (pc * 12 bytes): run cursors, then counts
     extern __shared__ unsigned long long scatter1_lds[];
@@ -743,11 +810,12 @@ __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint6
     const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
     if (c0 >= n_pos) return;
     const uint64_t c1 = c0 + chunk < n_pos ? c0 + chunk : n_pos;
-    // this block's runs: the classify pass counted its range per interval;
-    // only intervals [c_lo, c_hi) (this batch) are written, at offsets
-    // relative to the batch's first record rbase
+    // this block's runs: the classify pass counted its range per interval
+    // (hstride consecutive classify blocks, whose runs in each interval are
+    // consecutive); only intervals [c_lo, c_hi) (this batch) are written, at
+    // offsets relative to the batch's first record rbase
     for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) {
-        cbase[c] = coff[c] - rbase + hblk[(uint64_t)blockIdx.x * pc + c];
+        cbase[c] = coff[c] - rbase + hblk[(uint64_t)blockIdx.x * hstride * pc + c];
         lcnt[c] = 0;
     }
     __syncthreads();
@@ -807,6 +875,9 @@ __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint6
 // step: about eight records per interval and step instead of the four of
 // 4,096-position steps at two blocks of 256.
 constexpr uint32_t kStageIntervals = 1024;
+// scatter pass 1 workgroups per compute unit (staged / direct pass), each
+// taking consecutive classify blocks
+constexpr uint32_t kP1StagedPerCu = 4, kP1DirectPerCu = 2;
 constexpr uint32_t kStageNT = 512;
 constexpr uint32_t kStageStep = kStageNT * kSeqItems;
 
@@ -839,7 +910,7 @@ __global__ __launch_bounds__(kStageNT) void k_scatter1_staged(const S *__restric
                                                               const uint32_t *__restrict__ hblk,
                                                               KRec<uint32_t> *__restrict__ tmp, uint64_t chunk,
                                                               const uint64_t *__restrict__ keep, uint32_t c_lo,
-                                                              uint32_t c_hi, uint64_t rbase) {
+                                                              uint32_t c_hi, uint64_t rbase, uint32_t hstride) {
     constexpr uint32_t E = kStageIntervals / kStageNT;  // intervals per thread in the scan
     constexpr uint32_t kSkip = 0xFFFFFFFFu;
     __shared__ S win[kStageStep + 72];
@@ -853,7 +924,7 @@ __global__ __launch_bounds__(kStageNT) void k_scatter1_staged(const S *__restric
     if (c0 >= n) return;
     const uint64_t c1 = c0 + chunk < n ? c0 + chunk : n;
     for (uint32_t c = threadIdx.x; c < kStageIntervals; c += blockDim.x) {
-        cbase[c] = c < pc ? coff[c] - rbase + hblk[(uint64_t)blockIdx.x * pc + c] : 0ull;
+        cbase[c] = c < pc ? coff[c] - rbase + hblk[(uint64_t)blockIdx.x * hstride * pc + c] : 0ull;
         icnt[c] = 0;
     }
     for (uint64_t base = c0; base < c1; base += kStageStep) {

@@ -15,6 +15,12 @@
 //        bucket count.  Trained by SGD (Adam) with the kernels below on a
 //        seeded suffix sample; evaluated through nibble tables (16 nibble
 //        positions x 16 values x 32 hidden sums) held in LDS.
+//   Spline: a monotone piecewise-linear CDF with knots at 4,096 equal-width
+//        cells of the key space, each valued at the exact splitter rank of
+//        the cell's first key; in between, linear in the next 16 key bits
+//        (integer arithmetic).  Its error bound is exact (k_spline_bound),
+//        so classification corrects a prediction with a fixed number of
+//        branch-free steps over a window of the sorted splitter table.
 #pragma once
 #include "device.hpp"
 
@@ -27,8 +33,13 @@ constexpr int kMlpIn = 64, kMlpHid = 32;
 constexpr int kMlpParams = kMlpIn * kMlpHid + kMlpHid + kMlpHid + 1;
 
 struct RouterDev {
-    uint32_t kind;  // 0 none, 1 PWL, 2 MLP
+    uint32_t kind;  // 0 none, 1 PWL, 2 MLP, 3 spline
     uint32_t window;
+    // spline: knots (kSplineCells + 1), error bound pred - pos in [-el, eh],
+    // search steps (bit length of el + eh)
+    const uint16_t *spline;
+    int32_t el, eh;
+    uint32_t steps;
     // PWL
     const uint64_t *knots;  // ns + 1
     const float *a, *b, *inv;
@@ -70,6 +81,71 @@ __device__ __forceinline__ uint32_t pwl_predict(const RouterDev &R, const uint64
     if (u < lo[j]) u = lo[j];
     if (u > hi[j]) u = hi[j];
     return u;
+}
+
+// ---- spline -------------------------------------------------------------
+// Knot t (0 <= t <= 4096) is the number of splitters below t * 2^52; a key
+// of cell t = key >> 52 is placed at knot t plus the fraction f / 2^16 of the
+// cell's rank span, f = the next 16 key bits.  The prediction never
+// decreases as the key grows (inside a cell f does not; across cells it
+// stays below knot t+1, the next cell's first prediction), and the exact
+// position (the number of splitters <= key) is a non-decreasing step
+// function: on any key range where the position is constant and one cell
+// holds every key, the error pred - pos is extreme at the range's ends.  Those
+// ends are the splitters, the keys one below them, and the cells' first and
+// last keys, where k_spline_bound evaluates it.
+constexpr uint32_t kSplineBits = 12;
+constexpr uint32_t kSplineCells = 1u << kSplineBits;
+
+__device__ __forceinline__ uint32_t spline_predict(const uint16_t *kn, uint64_t key) {
+    const uint32_t t = (uint32_t)(key >> (64 - kSplineBits));
+    const uint32_t lo = kn[t], hi = kn[t + 1];
+    const uint32_t f = (uint32_t)(key >> (48 - kSplineBits)) & 0xFFFFu;
+    return lo + ((f * (hi - lo)) >> 16);
+}
+
+// Knots from the sorted splitter table ctab[0..n), n < 2^16.
+__global__ void k_spline_knots(const uint64_t *__restrict__ ctab, uint32_t n, uint16_t *__restrict__ kn) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > kSplineCells) return;
+    uint32_t lo = 0, hi = n;
+    if (t < kSplineCells) {
+        const uint64_t k0 = (uint64_t)t << (64 - kSplineBits);
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (ctab[mid] < k0) lo = mid + 1;
+            else hi = mid;
+        }
+    } else {
+        lo = n;
+    }
+    kn[t] = (uint16_t)lo;
+}
+
+// Error bound of the spline: bound[0] = min and bound[1] = max of
+// pred(q) - pos(q) over the 2n + 2 * 4096 range ends (bound[] starts at 0).
+__global__ void k_spline_bound(const uint64_t *__restrict__ ctab, uint32_t n, const uint16_t *__restrict__ kn,
+                               int *__restrict__ bound) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * n + 2 * kSplineCells) return;
+    uint64_t q;
+    if (i < 2 * n) {
+        const uint64_t sp = ctab[i >> 1];
+        q = (i & 1) ? sp : (sp ? sp - 1 : 0ull);
+    } else {
+        const uint32_t j = i - 2 * n, t = j >> 1;
+        const uint64_t c0 = (uint64_t)t << (64 - kSplineBits);
+        q = (j & 1) ? c0 + ((1ull << (64 - kSplineBits)) - 1) : c0;
+    }
+    uint32_t lo = 0, hi = n;  // exact position: splitters <= q
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (ctab[mid] <= q) lo = mid + 1;
+        else hi = mid;
+    }
+    const int e = (int)spline_predict(kn, q) - (int)lo;
+    atomicMin(&bound[0], e);
+    atomicMax(&bound[1], e);
 }
 
 // ---- MLP ----------------------------------------------------------------

@@ -59,6 +59,7 @@ typedef struct caps_limits {
 #define CAPS_ROUTER_BSEARCH 0u /* binary search over the splitter table   */
 #define CAPS_ROUTER_PWL 1u     /* learned piecewise-linear CDF + check     */
 #define CAPS_ROUTER_MLP 2u     /* learned 64-32-1 MLP + check              */
+#define CAPS_ROUTER_SPLINE 3u  /* spline CDF + bounded-window exact search */
 
 /* Packed key geometry. */
 #define CAPS_KEY_DENSE 0u     /* ceil(log2(sigma))-bit fields, weakly order preserving */
@@ -303,6 +304,8 @@ typedef struct caps_stats {
     double k_graph[3];             /* CAPS_FLAG_GRAPH builds: launch-to-end time of the
                                       tile-sort, refinement and giant-group graphs      */
     uint64_t runs;                 /* runs recorded in the run memo (0: not built)      */
+    int64_t router_bound_lo, router_bound_hi; /* spline router: the exact range of its
+                                      prediction error (predicted - true position)      */
 } caps_stats_t;
 int caps_last_stats(caps_stats_t *out);
 
@@ -416,6 +419,16 @@ int caps_router_mlp_forward_device(const float *d_params, const uint64_t *d_keys
 int caps_router_pwl_fit_device(const uint64_t *d_keys, const uint32_t *d_labels, uint64_t s, uint32_t ns,
                                uint32_t n_buckets, uint64_t *knots, float *a, float *b, float *inv, uint32_t *lo,
                                uint32_t *hi, void *stream);
+
+/* The spline router's fit, as the build runs it on its sorted splitter table
+ * d_splitters[0..n) (n < 65,536): knots (4,097 u16, host) = the number of
+ * splitters below t * 2^52 for t < 4,096, and n; bound (2 int32, host) =
+ * the minimum and maximum of predicted - true position over every key, where
+ * predicted = knots[t] + ((f * (knots[t+1] - knots[t])) >> 16) for
+ * t = key >> 52, f = (key >> 36) & 0xFFFF, and true = the number of
+ * splitters <= key. */
+int caps_router_spline_fit_device(const uint64_t *d_splitters, uint32_t n, uint16_t *knots, int32_t *bound,
+                                  void *stream);
 
 #ifdef __cplusplus
 }

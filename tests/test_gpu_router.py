@@ -54,13 +54,17 @@ def test_router_builds_are_bit_exact(cuda, buckets):
     text = gen.markov_repeats(1 << 20, seed=0xC3, mean_len=2000.0)
     ctx = O.Ctx(text)
     want = ctx.build_in_memory(threads=8)
-    for router in (caps_sa.Router.BSEARCH, caps_sa.Router.PWL, caps_sa.Router.MLP):
+    for router in (caps_sa.Router.BSEARCH, caps_sa.Router.PWL, caps_sa.Router.MLP, caps_sa.Router.SPLINE):
         got, st = _build(text, int(router), buckets)
         print(f"router={router.name} buckets={st['buckets']} err_sum={st['router_err_sum']} "
               f"err_max={st['router_err_max']} fallbacks={st['router_fallbacks']} count={st['router_count']}")
         assert np.array_equal(got, want), router
         if router != caps_sa.Router.BSEARCH:
             assert st["router_count"] == len(text)
+        if router == caps_sa.Router.SPLINE:
+            # every suffix's key fell inside the window the bound guarantees
+            assert st["router_fallbacks"] == 0
+            assert st["router_err_max"] <= max(-st["router_bound_lo"], st["router_bound_hi"])
 
 
 def test_training_reduces_misprediction(cuda):
@@ -111,3 +115,54 @@ def test_pwl_fit_matches_least_squares(cuda):
         assert abs(got["a"][j] - a) <= 2e-3 * scale + 1e-3 * abs(a), (j, got["a"][j], a)
         assert abs(got["b"][j] - b) <= 2e-3 * scale, (j, got["b"][j], b)
     assert got["knots"][ns] == np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def _spline_ref(spl):
+    """Knots by numpy.searchsorted, and the prediction of the spline."""
+    spl = np.asarray(spl, dtype=np.uint64)
+    knots = np.concatenate([np.searchsorted(spl, np.arange(4096, dtype=np.uint64) << np.uint64(52), side="left"),
+                            [len(spl)]]).astype(np.int64)
+
+    def predict(keys):
+        t = (keys >> np.uint64(52)).astype(np.int64)
+        f = ((keys >> np.uint64(36)) & np.uint64(0xFFFF)).astype(np.int64)
+        return knots[t] + ((f * (knots[t + 1] - knots[t])) >> 16)
+
+    return knots, predict
+
+
+@pytest.mark.parametrize("case", ["uniform", "clustered", "duplicates", "extremes", "one", "empty"])
+def test_spline_fit_and_bound(cuda, case):
+    """The spline's knots against numpy.searchsorted, and its error bound
+    against the errors of a million random keys plus every splitter, the
+    keys next to them and every cell edge: no key's error leaves the bound,
+    and some key reaches each end of it."""
+    rng = np.random.default_rng(21)
+    top = np.uint64(0xFFFFFFFFFFFFFFFF)
+    if case == "uniform":
+        spl = rng.integers(0, 2 ** 63, 4095, dtype=np.uint64) * np.uint64(2)
+    elif case == "clustered":  # most splitters inside a few cells (a skewed key distribution)
+        base = rng.integers(0, 4096, 6, dtype=np.uint64) << np.uint64(52)
+        spl = (base[rng.integers(0, 6, 4000)] + rng.integers(0, 1 << 52, 4000, dtype=np.uint64))
+        spl = np.concatenate([spl, rng.integers(0, 2 ** 63, 95, dtype=np.uint64) * np.uint64(2)])
+    elif case == "duplicates":
+        spl = np.repeat(rng.integers(0, 2 ** 63, 300, dtype=np.uint64) * np.uint64(2), 7)
+    elif case == "extremes":
+        spl = np.array([0, 0, 1, (1 << 52) - 1, 1 << 52, int(top) - 1, int(top), int(top)], dtype=np.uint64)
+    elif case == "one":
+        spl = np.array([12345 << 40], dtype=np.uint64)
+    else:
+        spl = np.zeros(0, dtype=np.uint64)
+    spl = np.sort(spl)
+    knots, (lo, hi) = dev.spline_fit(spl)
+    want_knots, predict = _spline_ref(spl)
+    assert np.array_equal(knots.astype(np.int64), want_knots)
+    probes = [rng.integers(0, 2 ** 63, 1 << 20, dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, 1 << 20).astype(np.uint64),
+              spl, spl - np.uint64(1), spl + np.uint64(1),
+              np.arange(4096, dtype=np.uint64) << np.uint64(52),
+              (np.arange(4096, dtype=np.uint64) << np.uint64(52)) + np.uint64((1 << 52) - 1)]
+    keys = np.concatenate(probes)  # wrap-around of spl - 1 / spl + 1 at the ends is still a valid key
+    err = predict(keys) - np.searchsorted(spl, keys, side="right").astype(np.int64)
+    assert lo <= 0 <= hi or len(spl) == 0
+    assert err.min() >= lo and err.max() <= hi
+    assert err.min() == lo and err.max() == hi
