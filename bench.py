@@ -73,9 +73,12 @@ def launch_bytes(st: dict, n: int, w: int, I: int) -> dict:
         "k_classify": n * (w + 2),            # text once + coarse interval id (u16) written
         "k_scatter1": n * (w + 2 + R),        # text once + interval id read + record written
         "k_scatter2": n * 2 * R,              # record read + record written to its bucket
-        "k_tile_1024": st["tile_records"][0] * 2 * R,  # records of the class read and written once
-        "k_tile_2048": st["tile_records"][1] * 2 * R,
-        "k_tile_4096": st["tile_records"][2] * 2 * R,
+        # records of the class read once, their positions written once (a
+        # build without an LCP array writes no sorted keys back for tiles
+        # that hold a whole bucket, and here every tile does)
+        "k_tile_1024": st["tile_records"][0] * (R + I),
+        "k_tile_2048": st["tile_records"][1] * (R + I),
+        "k_tile_4096": st["tile_records"][2] * (R + I),
     }
     if st.get("cmp_symbols") and any(st["cmp_symbols"]):
         # pair records (start + two positions) read, two positions written, packed text compared

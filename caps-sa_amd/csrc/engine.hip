@@ -473,7 +473,7 @@ static void sort_keys_single_range(Workspace &W, uint64_t *keys, uint32_t *ids, 
     Tile *dt = (Tile *)W.stiles.get(nt * sizeof(Tile));
     HIPCHK(hipMemcpyAsync(dt, W.h_tiles.data(), nt * sizeof(Tile), hipMemcpyHostToDevice, st));
     GroupLists none{};
-    k_tile_merge<uint32_t, T><<<(unsigned)nt, 256, 0, st>>>(keys, ids, dt, none, 0);
+    k_tile_merge<uint32_t, T><<<(unsigned)nt, 256, 0, st>>>(keys, ids, dt, none, 0, 1);
     LAUNCHCHK();
     Range r{0, count};
     W.h_range = r;
@@ -1149,8 +1149,9 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
                 constexpr uint32_t TT = decltype(ttag)::value;
                 if (!hc[1 + c]) return;
                 if (!gs.capturing) kt.begin(&stats.k_tile[c]);
-                if (radix) k_tile_radix<I, TT><<<hc[1 + c], 256, 0, gs.ls>>>(keys, out, tiles + c * tcap, gl, 1);
-                else k_tile_merge<I, TT><<<hc[1 + c], 256, 0, gs.ls>>>(keys, out, tiles + c * tcap, gl, 1);
+                const int keep = lcp ? 1 : 0;  // boundary LCPs read the sorted keys
+                if (radix) k_tile_radix<I, TT><<<hc[1 + c], 256, 0, gs.ls>>>(keys, out, tiles + c * tcap, gl, 1, keep);
+                else k_tile_merge<I, TT><<<hc[1 + c], 256, 0, gs.ls>>>(keys, out, tiles + c * tcap, gl, 1, keep);
                 LAUNCHCHK();
                 if (!gs.capturing) kt.end();
             };

@@ -48,7 +48,8 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t *wsu
 
 template <typename I, int T>
 __global__ __launch_bounds__(256) void k_tile_radix(uint64_t *__restrict__ keys, I *__restrict__ idx,
-                                                    const Tile *__restrict__ tiles, GroupLists gl, int emit) {
+                                                    const Tile *__restrict__ tiles, GroupLists gl, int emit,
+                                                    int keep_keys) {
     constexpr int B = 256;
     constexpr int ITEMS = T / B;
     constexpr int PADT = T + B;  // one pad slot per thread run
@@ -142,8 +143,9 @@ __global__ __launch_bounds__(256) void k_tile_radix(uint64_t *__restrict__ keys,
             __syncthreads();
         }
     }
+    const bool wk = keep_keys || !tl.whole;  // sorted keys read again: see k_tile_merge
     for (uint32_t i = tid; i < len; i += B) {
-        keys[tl.start + i] = sk[px(i)];
+        if (wk) keys[tl.start + i] = sk[px(i)];
         idx[tl.start + i] = si[sl[px(i)]];
     }
     if (!emit || !tl.whole || len < 2) return;
@@ -346,7 +348,8 @@ __device__ __forceinline__ void block_merge_sort(uint64_t (&k)[ITEMS], uint64_t 
 // first pass.
 template <typename I, int T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T <= 2048 ? 8 : 4))) void k_tile_merge(
-    uint64_t *__restrict__ keys, I *__restrict__ idx, const Tile *__restrict__ tiles, GroupLists gl, int emit) {
+    uint64_t *__restrict__ keys, I *__restrict__ idx, const Tile *__restrict__ tiles, GroupLists gl, int emit,
+    int keep_keys) {
     constexpr int B = 256;
     constexpr int ITEMS = T / B;
     constexpr int PADT = T + B;
@@ -440,7 +443,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T <= 2048 ?
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) sk[px(p0 + j)] = key[j];  // pads (positions >= len) are never read
     __syncthreads();
-    for (uint32_t i = tid; i < len; i += B) keys[tl.start + i] = sk[px(i)];
+    // sorted keys are read again only by the key merges of oversized buckets
+    // (tiles that are not whole) and by the LCP array's bucket boundaries
+    if (keep_keys || !tl.whole)
+        for (uint32_t i = tid; i < len; i += B) keys[tl.start + i] = sk[px(i)];
     if (!emit || !tl.whole || len < 2) return;
     // Tie groups of the bucket: a run of equal keys starts where the key
     // changes; an exclusive max-scan over the block carries the latest run
