@@ -764,14 +764,30 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         return;
     }
     if (d_pos) {  // every position must address the text: refuse the build before any key gather
+        // A strictly increasing list covering at least half of the text is
+        // the same set as a kept-position bitmap, whose builds walk the text
+        // in LDS-staged windows: about 21 ps per text position for classify
+        // and scatter pass 1 against 35 ps per listed position (820M genome
+        // share: 48.3 against 59.5 ms a build).  The suffix array is the
+        // same.  The check writes the bitmap on the way.
+        const bool try_keep = !d_keep && !idbits && !g_arena && n_pos < n && 2 * n_pos >= n;
+        unsigned long long *kb = nullptr;
+        if (try_keep) {
+            kb = (unsigned long long *)W.keep.get(ceil_div(n, 64) * 8);
+            HIPCHK(hipMemsetAsync(kb, 0, ceil_div(n, 64) * 8, st));
+        }
         unsigned int *bad = (unsigned int *)W.poscheck.get(4);
         unsigned int hbad = 0;
         HIPCHK(hipMemsetAsync(bad, 0, 4, st));
-        k_positions_check<I><<<grid_for(n_pos, 256, 8192), 256, 0, st>>>(d_pos, n_pos, n, bad);
+        k_positions_check<I><<<grid_for(n_pos, 256, 8192), 256, 0, st>>>(d_pos, n_pos, n, bad, kb);
         LAUNCHCHK();
         HIPCHK(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        if (hbad) throw CapsError(CAPS_E_INVALID, "position out of range");
+        if (hbad & 1u) throw CapsError(CAPS_E_INVALID, "position out of range");
+        if (try_keep && !(hbad & 2u)) {
+            d_keep = (const uint64_t *)kb;
+            d_pos = nullptr;
+        }
     }
     if (d_pos && (const void *)d_pos == (const void *)d_out) {
         I *cp = (I *)W.pos_copy.get(n_pos * sizeof(I));

@@ -127,16 +127,46 @@ __global__ __launch_bounds__(256) void k_random_keys(const S *__restrict__ t, Co
     keys[i] = with_id(g, key_at_seed(t, cm, g, p, lim < max_ctx ? lim : max_ctx), seg);
 }
 
-// Sets *bad when any of the positions is n or more (one atomic per
-// wavefront that finds one).
+// Sets bit 0 of *bad when any of the positions is n or more, bit 1 when the
+// list is not strictly increasing (one atomic per wavefront that finds one).
+// With `keep` (zeroed beforehand) it also writes the list's kept-position
+// bitmap, complete whenever the list turns out strictly increasing: each
+// wavefront takes 64 consecutive entries; lanes whose positions share a
+// bitmap word (a contiguous run of lanes) OR their bits together by a
+// segmented suffix scan of shuffles, and the run's first lane ORs the word
+// in.  A wavefront whose entries are out of range or out of order writes
+// nothing (its list takes the position-list path).
 template <typename I>
 __global__ __launch_bounds__(256) void k_positions_check(const I *__restrict__ pos, uint64_t n_pos, uint64_t n,
-                                                         unsigned int *__restrict__ bad) {
-    bool out = false;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pos;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        out |= (uint64_t)pos[i] >= n;
-    if (__ballot(out) && (threadIdx.x & 63) == 0) atomicOr(bad, 1u);
+                                                         unsigned int *__restrict__ bad,
+                                                         unsigned long long *__restrict__ keep) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    bool out = false, down = false;
+    for (uint64_t c = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; c * 64 < n_pos; c += nw) {
+        const uint64_t i = c * 64 + lane;
+        const bool have = i < n_pos;
+        const uint64_t p = have ? (uint64_t)pos[i] : ~0ull;
+        uint64_t prev = shfl64(p, lane ? lane - 1 : 0);
+        if (lane == 0 && i > 0) prev = (uint64_t)pos[i - 1];
+        const bool o = have && p >= n, d = have && i > 0 && prev >= p;
+        out |= o;
+        down |= d;
+        if (keep && __ballot(o || d) == 0) {
+            const uint64_t w = p >> 6;
+            unsigned long long bits = have ? 1ull << (p & 63) : 0ull;
+#pragma unroll
+            for (int s = 1; s < 64; s <<= 1) {
+                const uint64_t wn = shfl64(w, lane + s < 64 ? lane + s : lane);
+                const unsigned long long bn = shfl64(bits, lane + s < 64 ? lane + s : lane);
+                if (lane + s < 64 && wn == w) bits |= bn;
+            }
+            const uint64_t wp = shfl64(w, lane ? lane - 1 : 0);
+            if (have && (lane == 0 || wp != w)) atomicOr(&keep[w], bits);
+        }
+    }
+    const uint64_t bo = __ballot(out), bd = __ballot(down);
+    if (lane == 0 && (bo || bd)) atomicOr(bad, (bo ? 1u : 0u) | (bd ? 2u : 0u));
 }
 
 // splitter j-1 = sorted_sample[(j * s) / p], j = 1..p-1 (ext_mem.rs:1341).

@@ -110,6 +110,29 @@ def test_positions_subsets():
     check(t, positions=np.arange(0, 11, 2), label="even positions")
 
 
+@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("max_ctx", [MAX, 9])
+def test_dense_increasing_position_lists(order, max_ctx):
+    """A strictly increasing list covering at least half of the text is built
+    through a kept-position bitmap; the same set shuffled, and a sparser
+    list, stay on the position-list path.  Every form equals the oracle."""
+    rng = np.random.default_rng(0xD5 + order)
+    for n, keep in [(5000, 0.95), (20000, 0.6), (20000, 0.45), (64, 0.5)]:
+        lengths = rng.integers(5, 400, max(1, n // 150))
+        lengths = lengths[np.cumsum(lengths) <= n]
+        ends = np.concatenate([np.cumsum(lengths), [n]]).astype(np.uint64)
+        ends = np.unique(ends)
+        t = rng.integers(0, 3, n).astype(np.uint8)
+        pos = np.flatnonzero(rng.random(n) < keep).astype(np.uint64)
+        check(t, positions=pos, ends=ends, order=order, max_ctx=max_ctx, label=f"increasing {keep}")
+        shuffled = pos.copy()
+        rng.shuffle(shuffled)
+        check(t, positions=shuffled, ends=ends, order=order, max_ctx=max_ctx, label=f"shuffled {keep}")
+    t = gen.markov_repeats(1 << 16, seed=12, mean_len=800.0)
+    check(t, positions=np.arange(1, 1 << 16, dtype=np.uint64), label="all but the first")
+    check(t, positions=np.arange(0, 1 << 16, 2, dtype=np.uint64), label="even positions")
+
+
 def test_segmented_shorter_first():
     rng = np.random.default_rng(0x5E6)
     t = np.frombuffer(b"helloworldbananamississippi", dtype=np.uint8)
