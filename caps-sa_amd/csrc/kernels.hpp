@@ -429,24 +429,16 @@ __device__ __forceinline__ void coarse_routed(const Splitters &SP, const uint64_
 // that fails is counted as a fallback, and the host, finding any,
 // classifies the whole input again by binary search.
 template <int N>
-__device__ __forceinline__ void coarse_spline(const uint64_t *tab, uint32_t n, const RouterDev &R, const uint16_t *kn,
+__device__ __forceinline__ void coarse_spline(const uint64_t *tab, uint32_t n, const RouterDev &R, const uint32_t *kp,
                                               const uint64_t (&key)[N], const bool (&valid)[N], uint32_t (&pos)[N],
                                               unsigned long long &sum, unsigned long long &mx,
                                               unsigned long long &fbs, unsigned long long &cnt) {
     const uint32_t el = (uint32_t)R.el, eh = (uint32_t)R.eh;
     uint32_t pred[N], x[N];
 #pragma unroll
-    for (int j = 0; j < N; ++j) {
-        const uint32_t c = (uint32_t)(key[j] >> (64 - kSplineBits));
-        pred[j] = kn[c];
-        x[j] = kn[c + 1];
-    }
+    for (int j = 0; j < N; ++j) pred[j] = spline_predict_pair(kp, key[j]);
 #pragma unroll
-    for (int j = 0; j < N; ++j) {
-        const uint32_t f = (uint32_t)(key[j] >> (48 - kSplineBits)) & 0xFFFFu;
-        pred[j] += (f * (x[j] - pred[j])) >> 16;
-        x[j] = pred[j] > eh ? pred[j] - eh : 0u;
-    }
+    for (int j = 0; j < N; ++j) x[j] = pred[j] > eh ? pred[j] - eh : 0u;
     for (uint32_t st = R.steps ? 1u << (R.steps - 1) : 0u; st; st >>= 1) {
 #pragma unroll
         for (int j = 0; j < N; ++j) {
@@ -491,7 +483,7 @@ __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeM
     __shared__ uint32_t rlo[RK == 1 ? NS : 1], rhi[RK == 1 ? NS : 1];
     __shared__ uint16_t rradix[RK == 1 ? (1u << kRouterRadixBits) + 1 : 1];
     __shared__ float nib[RK == 2 ? 16 * 16 * kMlpHid : 1], w2[RK == 2 ? kMlpHid + 1 : 1];
-    __shared__ uint16_t rkn[RK == 3 ? kSplineCells + 1 : 1];
+    __shared__ uint32_t rkn[RK == 3 ? kSplineCells : 1];
     __shared__ unsigned long long st_sum, st_max, st_fb, st_cnt;
     load_coarse_table<RK>(SP, coarse);
     if (RK == 1) {
@@ -510,7 +502,8 @@ __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeM
         for (uint32_t c = threadIdx.x; c <= (uint32_t)kMlpHid; c += blockDim.x) w2[c] = R.w2[c];
     }
     if (RK == 3)
-        for (uint32_t c = threadIdx.x; c <= kSplineCells; c += blockDim.x) rkn[c] = R.spline[c];
+        for (uint32_t c = threadIdx.x; c < kSplineCells; c += blockDim.x)
+            rkn[c] = (uint32_t)R.spline[c] | ((uint32_t)R.spline[c + 1] << 16);
     if (threadIdx.x == 0) st_sum = st_max = st_fb = st_cnt = 0;
     __syncthreads();
     unsigned long long my_sum = 0, my_max = 0, my_fb = 0, my_cnt = 0;
@@ -636,7 +629,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RK == 0 || 
     __shared__ uint32_t rlo[RK == 1 ? NS : 1], rhi[RK == 1 ? NS : 1];
     __shared__ uint16_t rradix[RK == 1 ? (1u << kRouterRadixBits) + 1 : 1];
     __shared__ float nib[RK == 2 ? 16 * 16 * kMlpHid : 1], w2[RK == 2 ? kMlpHid + 1 : 1];
-    __shared__ uint16_t rkn[RK == 3 ? kSplineCells + 1 : 1];
+    __shared__ uint32_t rkn[RK == 3 ? kSplineCells : 1];
     __shared__ unsigned long long st_sum, st_max, st_fb, st_cnt;
     load_coarse_table<RK>(SP, coarse);
     if (RK == 1) {
@@ -655,7 +648,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RK == 0 || 
         for (uint32_t c = threadIdx.x; c <= (uint32_t)kMlpHid; c += blockDim.x) w2[c] = R.w2[c];
     }
     if (RK == 3)
-        for (uint32_t c = threadIdx.x; c <= kSplineCells; c += blockDim.x) rkn[c] = R.spline[c];
+        for (uint32_t c = threadIdx.x; c < kSplineCells; c += blockDim.x)
+            rkn[c] = (uint32_t)R.spline[c] | ((uint32_t)R.spline[c + 1] << 16);
     if (threadIdx.x == 0) st_sum = st_max = st_fb = st_cnt = 0;
     unsigned long long my_sum = 0, my_max = 0, my_fb = 0, my_cnt = 0;
     // this block's positions: [c0, c1), the same range the scatter's block owns

@@ -104,6 +104,16 @@ __device__ __forceinline__ uint32_t spline_predict(const uint16_t *kn, uint64_t 
     return lo + ((f * (hi - lo)) >> 16);
 }
 
+// The same prediction from knot pairs kp[t] = knot t | knot t+1 << 16 (one
+// LDS read per key in classification).
+__device__ __forceinline__ uint32_t spline_predict_pair(const uint32_t *kp, uint64_t key) {
+    const uint32_t t = (uint32_t)(key >> (64 - kSplineBits));
+    const uint32_t pr = kp[t];
+    const uint32_t lo = pr & 0xFFFFu, hi = pr >> 16;
+    const uint32_t f = (uint32_t)(key >> (48 - kSplineBits)) & 0xFFFFu;
+    return lo + ((f * (hi - lo)) >> 16);
+}
+
 // Knots from the sorted splitter table ctab[0..n), n < 2^16.
 __global__ void k_spline_knots(const uint64_t *__restrict__ ctab, uint32_t n, uint16_t *__restrict__ kn) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
