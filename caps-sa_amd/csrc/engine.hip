@@ -603,7 +603,10 @@ static RouterDev train_router(Workspace &W, uint32_t kind, uint32_t flags, const
         R.spline = kn;
         R.el = -hb[0];
         R.eh = hb[1];
-        uint32_t w = (uint32_t)(R.el + R.eh), bl = 0;
+        if (flags & CAPS_FLAG_ROUTER_UNTRAINED) R.el = R.eh = 0;  // no bound: the fallback's test
+        // steps over the window widened by one position on each side: its
+        // positions lie within el + eh + 2 of its first
+        uint32_t w = (uint32_t)(R.el + R.eh + 2), bl = 0;
         while (w >> bl) ++bl;
         R.steps = bl;
     }

@@ -421,41 +421,40 @@ __device__ __forceinline__ void coarse_routed(const Splitters &SP, const uint64_
 
 // The spline router (RK == 3): prediction, then the exact position inside
 // the window [pred - eh, pred + el] that the spline's error bound guarantees,
-// by a fixed number of branch-free steps over the sorted table (N keys in
-// lockstep, each phase's loads issued for all keys before any is used).  The
-// search returns the largest window position whose splitter below is <= key,
-// so a key outside its window could only end at one of the window's ends:
-// a result there is checked against the splitter beyond that end.  A key
-// that fails is counted as a fallback, and the host, finding any,
-// classifies the whole input again by binary search.
+// found by a fixed number of branch-free steps over the sorted table (N keys
+// in lockstep) across the window widened by one position on each side,
+// [lo, hi].  The steps return the largest c in [lo, hi] with c == lo or
+// tab[c-1] <= key: a result strictly inside (lo, hi) is exact (tab[x-1] <=
+// key < tab[x]), and a result on a widened end means the key's position lay
+// outside its window.  Such a key is counted as a fallback, and the host,
+// finding any, classifies the whole input again by binary search.
 template <int N>
 __device__ __forceinline__ void coarse_spline(const uint64_t *tab, uint32_t n, const RouterDev &R, const uint32_t *kp,
                                               const uint64_t (&key)[N], const bool (&valid)[N], uint32_t (&pos)[N],
                                               unsigned long long &sum, unsigned long long &mx,
                                               unsigned long long &fbs, unsigned long long &cnt) {
-    const uint32_t el = (uint32_t)R.el, eh = (uint32_t)R.eh;
+    const uint32_t el1 = (uint32_t)R.el + 1, eh1 = (uint32_t)R.eh + 1;
     uint32_t pred[N], x[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) pred[j] = spline_predict_pair(kp, key[j]);
 #pragma unroll
-    for (int j = 0; j < N; ++j) x[j] = pred[j] > eh ? pred[j] - eh : 0u;
+    for (int j = 0; j < N; ++j) x[j] = pred[j] > eh1 ? pred[j] - eh1 : 0u;
     for (uint32_t st = R.steps ? 1u << (R.steps - 1) : 0u; st; st >>= 1) {
 #pragma unroll
         for (int j = 0; j < N; ++j) {
-            const uint32_t whi = pred[j] + el < n ? pred[j] + el : n;
+            const uint32_t hi = pred[j] + el1 < n ? pred[j] + el1 : n;
             const uint32_t c = x[j] + st;
-            const uint32_t cc = c < whi ? c : whi;
-            const bool take = (c <= whi) & (tab[cc ? cc - 1 : 0] <= key[j]);
+            const uint32_t cc = c < hi ? c : hi;
+            const bool take = (c <= hi) & (tab[cc ? cc - 1 : 0] <= key[j]);
             x[j] = take ? c : x[j];
         }
     }
     uint32_t esum = 0, emax = 0, nv = 0, nbad = 0;  // errors are below 2^16
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        const uint32_t wlo = pred[j] > eh ? pred[j] - eh : 0u;
-        const uint32_t whi = pred[j] + el < n ? pred[j] + el : n;
-        if (x[j] == wlo && wlo > 0 && valid[j]) nbad += tab[wlo - 1] > key[j] ? 1u : 0u;
-        if (x[j] == whi && whi < n && valid[j]) nbad += key[j] >= tab[whi] ? 1u : 0u;
+        const bool low = pred[j] >= eh1 && x[j] == pred[j] - eh1;
+        const bool high = pred[j] + el1 <= n && x[j] == pred[j] + el1;
+        nbad += valid[j] && (low || high) ? 1u : 0u;
         pos[j] = x[j];
         const uint32_t e = x[j] > pred[j] ? x[j] - pred[j] : pred[j] - x[j];
         esum += valid[j] ? e : 0u;
@@ -669,8 +668,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RK == 0 || 
 #pragma unroll
         for (uint32_t j = 0; j < kSeqItems; ++j) valid[j] = first + j < cnt && ((kb >> j) & 1u);
         if (RK == 0) coarse_bsearch<kSeqItems>(SP, coarse, key, b);
-        else if (RK == 3)
-            coarse_spline<kSeqItems>(coarse, SP.n_coarse, R, rkn, key, valid, b, my_sum, my_max, my_fb, my_cnt);
+        else if (RK == 3) {
+            // two halves of eight keys: sixteen keys' windows would spill
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                uint64_t k8[8];
+                bool v8[8];
+                uint32_t b8[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) k8[j] = key[8 * h + j], v8[j] = valid[8 * h + j];
+                coarse_spline<8>(coarse, SP.n_coarse, R, rkn, k8, v8, b8, my_sum, my_max, my_fb, my_cnt);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) b[8 * h + j] = b8[j];
+            }
+        }
         else
             coarse_routed<RK, kSeqItems>(SP, coarse, R, rknots, rradix, ra, rb, rinv, rlo, rhi, nib, w2, key, valid,
                                          b, my_sum, my_max, my_fb, my_cnt);

@@ -99,7 +99,9 @@ typedef struct caps_opts {
 #define CAPS_SEED_POLICY_DENSE_ALPHABET_ONLY 1u
 #define CAPS_SEED_POLICY_REMAP 2u
 /* Skip router training (the learned router runs with its initial parameters;
- * used to show what training gains).  The output never changes. */
+ * used to show what training gains).  The spline router keeps its knots but
+ * skips its error bound (a window of one position), so keys outside it take
+ * the binary-search fallback.  The output never changes. */
 #define CAPS_FLAG_ROUTER_UNTRAINED 1u
 /* Refine groups of 2..64 members with persistent kernels fed from device work
  * queues (slots take the next group when theirs is done) instead of one grid

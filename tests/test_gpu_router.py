@@ -67,6 +67,21 @@ def test_router_builds_are_bit_exact(cuda, buckets):
             assert st["router_err_max"] <= max(-st["router_bound_lo"], st["router_bound_hi"])
 
 
+def test_spline_without_bound_falls_back_exactly(cuda):
+    """The spline router with a window of one position (no error bound):
+    keys it mispredicts land on the widened window's ends, are counted as
+    fallbacks, and the host classifies the input again by binary search;
+    the suffix array is unchanged."""
+    text = gen.markov_repeats(1 << 20, seed=0xC5, mean_len=2000.0)
+    want = O.Ctx(text).build_in_memory(threads=8)
+    for buckets in (0, 4096):
+        got, st = _build(text, int(caps_sa.Router.SPLINE), buckets, caps_sa.FLAG_ROUTER_UNTRAINED)
+        assert np.array_equal(got, want)
+        assert st["router_fallbacks"] > 0
+        got, st = _build(text, int(caps_sa.Router.SPLINE), buckets)
+        assert np.array_equal(got, want) and st["router_fallbacks"] == 0
+
+
 def test_training_reduces_misprediction(cuda):
     """What the trained network gains over the same network untrained: the
     mean distance between predicted and true bucket."""
