@@ -368,21 +368,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T <= 2048 ?
     const uint32_t tid = threadIdx.x;
     if (len == 0) return;
     const uint64_t first = keys[tl.start];
+    const uint32_t p0 = tid * ITEMS;
     uint64_t my_or = 0;
-    // the tile's index values are loaded now, with the keys, and staged in
-    // LDS after the sort: no global round trip between the sort and the output
+    // each thread loads its own ITEMS consecutive keys into registers, and
+    // their index values, which go to LDS after the sort: no LDS round trip
+    // before the sort, no global one between the sort and the output
+    uint64_t kk[ITEMS];
     I ix[ITEMS];
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
-        const uint32_t i = tid + (uint32_t)j * B;
-        uint64_t kk = ~0ull;
-        ix[j] = (I)0;
-        if (i < len) {
-            kk = keys[tl.start + i];
-            ix[j] = idx[tl.start + i];
-            my_or |= kk ^ first;
-        }
-        sk[px(i)] = kk;
+        const uint32_t i = p0 + j;
+        kk[j] = i < len ? keys[tl.start + i] : ~0ull;
+        ix[j] = i < len ? idx[tl.start + i] : (I)0;
+        my_or |= i < len ? kk[j] ^ first : 0ull;
     }
     if (tid == 0) s_or = 0;
     if (tid < kGroupClasses) gcnt[tid] = 0;
@@ -396,12 +394,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T <= 2048 ?
     // every real record (a real record reaches ~0 only with index T-1, and
     // then the tile is full)
     const uint64_t mask = one_pass ? ((1ull << hi) - 1) : ((1ull << (64 - LB)) - 1);
-    const uint32_t p0 = tid * ITEMS;
     uint64_t k[ITEMS];
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
         const uint32_t i = p0 + j;
-        k[j] = i < len ? ((sk[px(i)] & mask) << LB) | i : ~0ull;
+        k[j] = i < len ? ((kk[j] & mask) << LB) | i : ~0ull;
     }
     if (diff) block_merge_sort<B, ITEMS>(k, sk, len);
     if (!one_pass) {
@@ -418,28 +415,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T <= 2048 ?
         block_merge_sort<B, ITEMS>(k, sk, len);
     }
     __syncthreads();  // the sort buffer is free
-    // index values: original order into LDS, gathered in sorted order, back
-    // to LDS, written out coalesced
+    // index values: original order into LDS, gathered in sorted order, each
+    // thread's ITEMS consecutive ones written out directly
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j)
-        if (tid + (uint32_t)j * B < len) stage[tid + (uint32_t)j * B] = ix[j];
+        if (p0 + j < len) stage[p0 + j] = ix[j];
     __syncthreads();
     I o[ITEMS];
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) o[j] = p0 + j < len ? stage[(uint32_t)(k[j] & (T - 1))] : (I)0;
-    __syncthreads();
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j) stage[p0 + j] = o[j];
+    for (int j = 0; j < ITEMS; ++j)
+        if (p0 + j < len) idx[tl.start + p0 + j] = o[j];
     if ((tid & 63) == 63) wlast[tid >> 6] = o[ITEMS - 1];
-    __syncthreads();
     uint64_t key[ITEMS];
     const uint64_t top = first & ~mask;
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j)
         key[j] = one_pass ? top | (k[j] >> LB)
                           : (p0 + j < len ? keys[tl.start + (uint32_t)(k[j] & (T - 1))] : ~0ull);
-    for (uint32_t i = tid; i < len; i += B) idx[tl.start + i] = stage[i];
-    __syncthreads();
+    __syncthreads();  // every gather from the staged index values is done
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) sk[px(p0 + j)] = key[j];  // pads (positions >= len) are never read
     __syncthreads();
