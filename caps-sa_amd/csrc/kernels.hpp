@@ -1150,17 +1150,17 @@ __global__ __launch_bounds__(NT) void k_scatter2(const KRec<I> *__restrict__ tmp
             ix[u] = r.idx;
         }
         eyt_upper_rank<U>(sspl, fsteps, ninner, kk, f);
+        // the count's atomic returns each record's rank inside its bucket's
+        // run of the tile
+        uint32_t rk[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (u * NT + threadIdx.x < tl) atomicAdd(&cnt[f[u]], 1u);
+        for (int u = 0; u < U; ++u) rk[u] = u * NT + threadIdx.x < tl ? atomicAdd(&cnt[f[u]], 1u) : 0u;
         __syncthreads();
-        block_scan_slices(cnt, nb, 0, lbase, part);
-        for (uint32_t j = threadIdx.x; j < nb; j += blockDim.x) cnt[j] = 0;
-        __syncthreads();
+        block_scan_slices(cnt, nb, 0, lbase, part);  // ends with a barrier
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (u * NT + threadIdx.x < tl) {
-                const uint32_t slot = (uint32_t)lbase[f[u]] + atomicAdd(&cnt[f[u]], 1u);
+                const uint32_t slot = (uint32_t)lbase[f[u]] + rk[u];
                 tkey[slot] = kk[u];
                 tidx[slot] = ix[u];
                 tf[slot] = (uint16_t)f[u];
