@@ -701,11 +701,15 @@ static BatchPlan plan_batches(const Workspace &W, uint64_t n_pos, uint64_t reque
         if (requested) {
             if ((double)requested * 96 <= rest) return {requested, true};
         } else {
-            // one pass 1 for all records, unless it costs more than one extra
-            // batch: pass 2 runs one workgroup per coarse interval, so a batch
-            // of few intervals leaves compute units idle
+            // one pass 1 for all records (and smaller batches beside them)
+            // unless that takes many more batches: a pass 1 over every
+            // position costs about as much as nine batches' own overhead
+            // (whole genome: 47 ms a pass against about 5 ms a batch; 10
+            // batches with one pass 1 build in 464 ms, 5 with a pass 1 each
+            // in 580 ms)
             const uint64_t b = (uint64_t)(rest / 96) / kQuantum * kQuantum;
-            if (b && ceil_div(n_pos, b) <= ceil_div(n_pos, b_each) + 1) return {std::min<uint64_t>(b, n_pos), true};
+            const uint64_t nb_per = ceil_div(n_pos, b_each);
+            if (b && ceil_div(n_pos, b) <= nb_per + 9 * (nb_per - 1)) return {std::min<uint64_t>(b, n_pos), true};
         }
     }
     return {std::min<uint64_t>(b_each, n_pos), false};
