@@ -879,12 +879,12 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     // `target` records each.
     if (o.gpu_buckets > kCoarseBucketsMax) throw CapsError(CAPS_E_INVALID, "gpu_buckets above 4096");
     const uint64_t want = std::max<uint64_t>(1, ceil_div(n_pos, target));
-    // 1,024 classification buckets where scatter pass 1 is staged in LDS
-    // (every suffix of one text, u32 index; it takes at most 1,024), else
-    // 2,048: smaller intervals keep scatter pass 2's LDS small enough for
-    // several workgroups per compute unit
-    const bool stageable = sizeof(I) == 4 && !idbits && !d_keep && d_pos == nullptr && n_pos == n;
-    const uint64_t c_default = stageable ? kStageIntervals : 2048;
+    // 1,024 classification buckets for every suffix of one text with a u32
+    // index (longer runs per interval in scatter pass 1), else 2,048:
+    // smaller intervals keep scatter pass 2's LDS small enough for several
+    // workgroups per compute unit with 16-byte records
+    const bool every_suffix = sizeof(I) == 4 && !idbits && !d_keep && d_pos == nullptr && n_pos == n;
+    const uint64_t c_default = every_suffix ? 1024 : 2048;
     uint64_t C = o.gpu_buckets ? o.gpu_buckets
                                : std::min<uint64_t>(kCoarseBucketsMax,
                                                     std::max<uint64_t>(std::min<uint64_t>(want, c_default),
@@ -1068,13 +1068,12 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     // scatter pass 1 for coarse intervals [c0, c1) into tmp, at offsets relative to rb
     auto pass1 = [&](uint32_t c0, uint32_t c1, uint64_t rb) {
         kt.begin(&stats.k_scatter1);
-        // contiguous positions, u32 index, at most 1,024 intervals: the
-        // LDS-staged pass with coalesced record writes.  Not for batched
-        // builds: their keys are text-major, so a step's records fall in one
-        // or two intervals and are written contiguously already.
-        constexpr bool kStageable = sizeof(I) == 4;
-        const bool staged = kStageable && seq && !idbits && pc <= kStageIntervals &&
-                            !(o.flags & CAPS_FLAG_SCATTER_DIRECT);
+        // contiguous positions (every suffix, or a bitmap's kept ones), at
+        // most 2,048 intervals: the LDS-staged pass with coalesced record
+        // writes.  Not for batched builds: their keys are text-major, so a
+        // step's records fall in one or two intervals and are written
+        // contiguously already.
+        const bool staged = seq && !idbits && pc <= kStageIntervals && !(o.flags & CAPS_FLAG_SCATTER_DIRECT);
         // Over contiguous positions a workgroup of pass 1 takes hstride
         // consecutive classify blocks (their runs in every interval are
         // consecutive): about kP1StagedPerCu (staged) or kP1DirectPerCu
@@ -1088,13 +1087,12 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         const unsigned nb1 = (unsigned)ceil_div(nblk, hs);
         const uint64_t ch1 = chunk * hs;
         if (staged) {
-            KRec<uint32_t> *t32 = (KRec<uint32_t> *)tmp;
             if (d_keep)
-                k_scatter1_staged<S, true><<<nb1, kStageNT, 0, st>>>(d_text, n_iter, cm, g, L, max_ctx, cid, pc, coff,
-                                                                     hblk, t32, ch1, d_keep, c0, c1, rb, hs);
+                k_scatter1_staged<S, I, true><<<nb1, kStageNT, 0, st>>>(d_text, n_iter, cm, g, L, max_ctx, cid, pc,
+                                                                        coff, hblk, tmp, ch1, d_keep, c0, c1, rb, hs);
             else
-                k_scatter1_staged<S, false><<<nb1, kStageNT, 0, st>>>(d_text, n_iter, cm, g, L, max_ctx, cid, pc, coff,
-                                                                      hblk, t32, ch1, nullptr, c0, c1, rb, hs);
+                k_scatter1_staged<S, I, false><<<nb1, kStageNT, 0, st>>>(d_text, n_iter, cm, g, L, max_ctx, cid, pc,
+                                                                         coff, hblk, tmp, ch1, nullptr, c0, c1, rb, hs);
         } else if (seq && d_keep)
             k_scatter1<S, I, true, true><<<nb1, 256, (size_t)pc * 12, st>>>(d_text, n, cm, g, L, max_ctx, d_pos, n_iter, cid, pc,
                                                                coff, hblk, tmp, ch1, d_keep, c0, c1, rb, hs);

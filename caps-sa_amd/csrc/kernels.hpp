@@ -904,17 +904,18 @@ __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint6
 // block counting-sorts its records by interval in LDS, then consecutive
 // threads write each interval's run of the step, so a wavefront's store
 // covers a few contiguous runs instead of 64 scattered records.  For at most
-// kStageIntervals intervals and 12-byte records (u32 index); the block owns
-// the same range of positions as in k_scatter1.  One block of 512 threads
-// per compute unit (137 KB of LDS with u8 text) stages 8,192 positions a
-// step: about eight records per interval and step instead of the four of
-// 4,096-position steps at two blocks of 256.
-constexpr uint32_t kStageIntervals = 1024;
+// kStageIntervals intervals; the block owns the same range of positions as in
+// k_scatter1.  One block of 512 threads per compute unit (136 KB of LDS with
+// u8 text: keys, step-local positions, interval ids, interval tables) stages
+// 8,192 positions a step: about eight records per interval and step over
+// 1,024 intervals, instead of the four of 4,096-position steps at two blocks
+// of 256.
+constexpr uint32_t kStageIntervals = 2048;
+constexpr uint32_t kStageNT = 512;
+constexpr uint32_t kStageStep = kStageNT * kSeqItems;
 // scatter pass 1 workgroups per compute unit (staged / direct pass), each
 // taking consecutive classify blocks
 constexpr uint32_t kP1StagedPerCu = 4, kP1DirectPerCu = 2;
-constexpr uint32_t kStageNT = 512;
-constexpr uint32_t kStageStep = kStageNT * kSeqItems;
 
 // Exclusive scan of one u32 per thread across a block of NT threads; wsum
 // holds NT / 64 entries.
@@ -937,20 +938,20 @@ __device__ __forceinline__ uint32_t block_excl_scan_nt(uint32_t v, uint32_t *wsu
     return before + x - v;
 }
 
-template <typename S, bool KEEP>
+template <typename S, typename I, bool KEEP>
 __global__ __launch_bounds__(kStageNT) void k_scatter1_staged(const S *__restrict__ t, uint64_t n, CodeMap<S> cm,
                                                               KeyGeom g, DevLimits L, uint64_t max_ctx,
                                                               const uint16_t *__restrict__ cid, uint32_t pc,
                                                               const uint64_t *__restrict__ coff,
                                                               const uint32_t *__restrict__ hblk,
-                                                              KRec<uint32_t> *__restrict__ tmp, uint64_t chunk,
+                                                              KRec<I> *__restrict__ tmp, uint64_t chunk,
                                                               const uint64_t *__restrict__ keep, uint32_t c_lo,
                                                               uint32_t c_hi, uint64_t rbase, uint32_t hstride) {
     constexpr uint32_t E = kStageIntervals / kStageNT;  // intervals per thread in the scan
     constexpr uint32_t kSkip = 0xFFFFFFFFu;
     __shared__ S win[kStageStep + 72];
     __shared__ uint64_t s_key[kStageStep];
-    __shared__ uint32_t s_pos[kStageStep];
+    __shared__ uint16_t s_li[kStageStep];  // position - step base
     __shared__ uint16_t s_c[kStageStep];
     __shared__ unsigned long long cbase[kStageIntervals];
     __shared__ uint32_t icnt[kStageIntervals], ioff[kStageIntervals];
@@ -1011,13 +1012,13 @@ __global__ __launch_bounds__(kStageNT) void k_scatter1_staged(const S *__restric
             if (cc[j] != kSkip) {
                 const uint32_t q = ioff[cc[j]] + lr[j];
                 s_key[q] = key[j];
-                s_pos[q] = (uint32_t)(base + first + j);
+                s_li[q] = (uint16_t)(first + j);
                 s_c[q] = (uint16_t)cc[j];
             }
         __syncthreads();
         for (uint32_t q = threadIdx.x; q < total; q += blockDim.x) {
             const uint32_t c = s_c[q];
-            tmp[cbase[c] + (q - ioff[c])] = KRec<uint32_t>{s_key[q], s_pos[q]};
+            tmp[cbase[c] + (q - ioff[c])] = KRec<I>{s_key[q], (I)(base + s_li[q])};
         }
         __syncthreads();
         for (uint32_t c = threadIdx.x; c < kStageIntervals; c += blockDim.x) {
