@@ -1120,18 +1120,18 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         // ---- two-level scatter ----
         if (!all_records) pass1(c_lo, c_hi, rbase);
         kt.begin(&stats.k_scatter2);
-        // 1,024 threads and 2,048-record tiles when the LDS of a wide interval
-        // leaves one block per compute unit, or the batch has too few
-        // intervals to fill the device; else 1,024 threads with 8,192-record
-        // tiles for a u32 index (112 KB + about 28 bytes per bucket: one block
-        // per compute unit; twice the bucket run per write of 4,096-record
-        // tiles at two blocks, 3.08 -> 2.72 ms on 268M records) and
-        // 4,096-record tiles for a u64 index (73 KB)
+        // 1,024 threads and 4,096-record tiles when the LDS of a wide
+        // interval (up to 80 KB for 4,096 buckets) or a batch of few intervals
+        // calls for them; else 1,024 threads with 8,192-record tiles for a u32
+        // index (112 KB + about 20 bytes per bucket: one block per compute
+        // unit; twice the bucket run per write of 4,096-record tiles at two
+        // blocks, 3.08 -> 2.72 ms on 268M records) and 4,096-record tiles for a
+        // u64 index (72 KB)
         const uint32_t s2_lds = scatter2_lds_bytes(SP.stride);
         const bool wide = s2_lds > 22 * 1024 || (c_hi - c_lo) < 2 * W.n_cu;
         KRec<I> *trec = all_records ? tmp + rbase : tmp;
         if (wide)
-            k_scatter2<I, 1024, 2048><<<c_hi - c_lo, 1024, s2_lds, st>>>(trec, coff, SP, (uint32_t)p, off, keys, out,
+            k_scatter2<I, 1024, 4096><<<c_hi - c_lo, 1024, s2_lds, st>>>(trec, coff, SP, (uint32_t)p, off, keys, out,
                                                                         c_lo, c_hi, rbase);
         else if constexpr (sizeof(I) == 4)
             k_scatter2<I, 1024, 8192><<<c_hi - c_lo, 1024, s2_lds, st>>>(trec, coff, SP, (uint32_t)p, off, keys, out,

@@ -1032,9 +1032,9 @@ __global__ __launch_bounds__(kStageNT) void k_scatter1_staged(const S *__restric
 // each thread scans a contiguous slice, the slice sums are scanned with
 // wavefront shuffles and one exchange of wavefront totals.  Returns the total
 // in every thread.  part: blockDim.x / 64 u64 of LDS.
+template <typename O>
 __device__ __forceinline__ unsigned long long block_scan_slices(const uint32_t *cnt, uint32_t nb,
-                                                                unsigned long long base,
-                                                                unsigned long long *out,
+                                                                unsigned long long base, O *out,
                                                                 unsigned long long *part) {
     const uint32_t per = (nb + blockDim.x - 1) / blockDim.x;
     const uint32_t j0 = threadIdx.x * per, j1 = j0 + per < nb ? j0 + per : nb;
@@ -1057,7 +1057,7 @@ __device__ __forceinline__ unsigned long long block_scan_slices(const uint32_t *
     }
     unsigned long long run = base + before + x - sum;
     for (uint32_t j = j0; j < j1; ++j) {
-        out[j] = run;
+        out[j] = (O)run;
         run += cnt[j];
     }
     __syncthreads();  // part is reused by the next scan
@@ -1065,10 +1065,11 @@ __device__ __forceinline__ unsigned long long block_scan_slices(const uint32_t *
 }
 
 // LDS of scatter pass 2, sized by the launch for `stride` buckets per
-// interval: the inner splitters as a breadth-first tree, global cursors, tile
-// offsets, counts.
+// interval: the inner splitters as a breadth-first tree, then 32-bit bucket
+// cursors (relative to the interval's first record: an interval holds fewer
+// than 2^32 records), tile offsets and counts.
 __host__ __device__ constexpr uint32_t scatter2_lds_bytes(uint32_t stride) {
-    return pow2_at_least(stride) * 8 + stride * (8 + 8 + 4);
+    return pow2_at_least(stride) * 8 + stride * (4 + 4 + 4);
 }
 
 // NT threads and TILE records staged per placing step: 512 and 4,096 (two
@@ -1094,9 +1095,9 @@ __global__ __launch_bounds__(NT) void k_scatter2(const KRec<I> *__restrict__ tmp
     __shared__ uint16_t tf[TILE];
     const uint32_t p2 = pow2_at_least(SP.stride);
     uint64_t *sspl = (uint64_t *)scatter2_lds;
-    unsigned long long *gcur = scatter2_lds + p2;
-    unsigned long long *lbase = gcur + SP.stride;
-    uint32_t *cnt = (uint32_t *)(lbase + SP.stride);
+    uint32_t *gcur = (uint32_t *)(scatter2_lds + p2);
+    uint32_t *lbase = gcur + SP.stride;
+    uint32_t *cnt = lbase + SP.stride;
     // interval c of the batch [c_lo, c_hi); record offsets relative to rbase
     const uint32_t c = c_lo + blockIdx.x;
     const uint64_t s = coff[c] - rbase, e = coff[c + 1] - rbase;
@@ -1127,9 +1128,9 @@ __global__ __launch_bounds__(NT) void k_scatter2(const KRec<I> *__restrict__ tmp
             if (i0 + u * NT + threadIdx.x < e) atomicAdd(&cnt[f[u]], 1u);
     }
     __syncthreads();
-    block_scan_slices(cnt, nb, s, gcur, part);
+    block_scan_slices(cnt, nb, 0, gcur, part);
     for (uint32_t j = threadIdx.x; j < nb; j += blockDim.x) {
-        off[b0 + j] = gcur[j];
+        off[b0 + j] = s + gcur[j];
         cnt[j] = 0;
     }
     if (c + 1 == c_hi && threadIdx.x == 0) off[b0 + nb] = e;  // the batch's end
@@ -1170,7 +1171,7 @@ __global__ __launch_bounds__(NT) void k_scatter2(const KRec<I> *__restrict__ tmp
         __syncthreads();
         for (uint32_t q = threadIdx.x; q < tl; q += NT) {
             const uint32_t fb = tf[q];
-            const uint64_t dst = gcur[fb] + (q - lbase[fb]);
+            const uint64_t dst = s + gcur[fb] + (q - lbase[fb]);
             keys[dst] = tkey[q];
             idx[dst] = tidx[q];
         }
