@@ -478,6 +478,7 @@ FLAG_SCATTER_DIRECT = 256
 FLAG_GRAPH = 512
 FLAG_SEED_ESCAPE32 = 1024
 FLAG_NO_RUN_MEMO = 2048
+FLAG_MID_BY_SIZE = 4096
 TUNE_MAX = 16
 
 

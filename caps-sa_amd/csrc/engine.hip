@@ -689,6 +689,7 @@ static BatchPlan plan_batches(const Workspace &W, uint64_t n_pos, uint64_t reque
                   W.lcp2.cap + W.tiles.cap + W.big.cap + W.chunks.cap + W.jobs.cap + W.run_s.cap + W.run_e.cap +
                   W.run_ss.cap + W.run_se.cap + W.run_dir.cap;
     for (int c = 0; c < kGroupClasses; ++c) held += W.glist[c].cap;
+    held += W.mid.cap;
     const double avail = (double)(free_b + held) * 0.75;
     if (!requested && (double)n_pos * 112 <= avail) return {n_pos, false};  // one batch
     const double rest = avail - (double)n_pos * rec_bytes;  // beside all records
@@ -1041,12 +1042,16 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     Tile *tiles = (Tile *)W.tiles.get(kTileClasses * tcap * sizeof(Tile));
     Range *big = (Range *)W.big.get(p * sizeof(Range));
     // [0] oversized, [1..4) tile classes, [4] largest bucket, [8..16) group counts,
-    // [16..24) refinement work queues
-    unsigned int *counters = (unsigned int *)W.counts.get(128);
+    // [16..24) refinement work queues, [24..30) class records (u64),
+    // [32..35) mid-size groups per LDS class, [35..38) their claim counters
+    unsigned int *counters = (unsigned int *)W.counts.get(256);
     GroupLists gl{};
     for (int c = 0; c < kGroupClasses; ++c)
         gl.list[c] = (Group *)W.glist[c].get((bcap / group_class_min(c) + 1) *
                                              (c == 0 ? sizeof(PairRec<I>) : sizeof(Group)));
+    const uint64_t mid_cap = bcap / group_class_min(6) + 1;
+    Group *mid = (Group *)W.mid.get(kMidClasses * mid_cap * sizeof(Group));
+    unsigned int *mid_counts = counters + 32;
     gl.counts = counters + 8;
     I *const lcp_all = want_lcp ? d_lcp_out : nullptr;
     // symbols every tie group's members are known to share: the key's, or none
@@ -1144,7 +1149,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         tm.mark(PH_SCATTER);
 
         // ---- tiles: sort per tile (by size class); whole-bucket tiles emit their tie groups ----
-        HIPCHK(hipMemsetAsync(counters, 0, 128, st));
+        HIPCHK(hipMemsetAsync(counters, 0, 256, st));
         k_tile_emit<<<(unsigned)ceil_div(pb, 256 * kEmitPer), 256, 0, st>>>(boff, pb, T, tiles, tcap, counters + 1, big,
                                                                   counters, counters + 4,
                                                                   (unsigned long long *)(counters + 24));
@@ -1281,8 +1286,28 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
             if (timed) kt.end();
             if (cnt[6]) {
                 if (timed) kt.begin(&stats.k_groups_mid);
-                k_groups<S, I, 256, kMidMax><<<cnt[6], 256, 0, ls>>>(d_text, cm, gref, L, max_ctx, gl.list[6],
-                                                                      depth0, out, lcp, 0, 0);
+                if (cnt[6] >= 16u * W.n_cu || (o.flags & CAPS_FLAG_MID_BY_SIZE)) {
+                    // many groups: by LDS class, so groups of up to 512
+                    // members keep 4 to 14 workgroups per compute unit; each
+                    // launch claims the groups its split list holds, with a
+                    // grid of at most one resident wave of it
+                    k_split_mid<<<(unsigned)ceil_div(cnt[6], 256), 256, 0, ls>>>(gl.list[6], cnt[6], mid, mid_cap,
+                                                                                 mid_counts);
+                    auto mid_class = [&](auto kern, unsigned block, uint32_t c) {
+                        const unsigned grid = (unsigned)std::min<uint64_t>(cnt[6], resident_blocks(kern, block));
+                        kern<<<grid, block, 0, ls>>>(d_text, cm, gref, L, max_ctx, mid + c * mid_cap, mid_counts + c,
+                                                     0, mid_counts + kMidClasses + c, depth0, out, lcp, 0);
+                    };
+                    mid_class(k_groups<S, I, 128, 128>, 128, 0);
+                    mid_class(k_groups<S, I, 256, 512>, 256, 1);
+                    mid_class(k_groups<S, I, 256, kMidMax>, 256, 2);
+                } else {
+                    // few groups (often long ones): one launch, one workgroup
+                    // per group, so no launch waits for another's last groups
+                    k_groups<S, I, 256, kMidMax><<<cnt[6], 256, 0, ls>>>(d_text, cm, gref, L, max_ctx, gl.list[6],
+                                                                          nullptr, cnt[6], nullptr, depth0, out, lcp,
+                                                                          0);
+                }
                 LAUNCHCHK();
                 if (timed) kt.end();
             }
@@ -1337,8 +1362,8 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
             const hipStream_t gst = gsg.ls;
             for (uint64_t c0 = 0; c0 < hch.size(); c0 += 65535u * 16) {
                 const unsigned nbk = (unsigned)std::min<uint64_t>(hch.size() - c0, 65535u * 16);
-                k_groups<S, I, 256, kMidMax><<<nbk, 256, 0, gst>>>(d_text, cm, gref, L, max_ctx, dc + c0, depth0, out,
-                                                                  lbuf, 1, 0);
+                k_groups<S, I, 256, kMidMax><<<nbk, 256, 0, gst>>>(d_text, cm, gref, L, max_ctx, dc + c0, nullptr, nbk,
+                                                                  nullptr, depth0, out, lbuf, 1);
                 LAUNCHCHK();
             }
             I *is = out, *ls = lbuf, *id = i2, *ld = l2;
@@ -2136,10 +2161,10 @@ template <typename RUN>
 static void tune(caps_opts_t &o, uint32_t reps, caps_tune_report_t *rep, RUN &&run) {
     static const uint32_t kKeys[] = {1024, 1280, 1536};
     static const uint32_t kFlags[] = {0, CAPS_FLAG_TILE_RADIX, CAPS_FLAG_PERSISTENT_REFINE, CAPS_FLAG_MERGE_W16,
-                                      CAPS_FLAG_MERGE_W64};
+                                      CAPS_FLAG_MERGE_W64, CAPS_FLAG_MID_BY_SIZE};
     std::vector<TuneCand> cands;
     const uint32_t keep = o.flags & ~(CAPS_FLAG_TILE_RADIX | CAPS_FLAG_PERSISTENT_REFINE | CAPS_FLAG_MERGE_W16 |
-                                      CAPS_FLAG_MERGE_W64);
+                                      CAPS_FLAG_MERGE_W64 | CAPS_FLAG_MID_BY_SIZE);
     for (uint32_t k : kKeys) cands.push_back({k, keep});
     for (size_t f = 1; f < sizeof(kFlags) / sizeof(kFlags[0]); ++f) cands.push_back({0, keep | kFlags[f]});
     double best = 1e300;

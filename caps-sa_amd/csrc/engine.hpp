@@ -50,7 +50,7 @@ struct Workspace {
     // tiles
     DevBuf tcnt, toff, toff2, tiles, big, counts, recs;
     // refinement
-    DevBuf glist[kGroupClasses], lcp, lcp2, first, jobs, chunks;
+    DevBuf glist[kGroupClasses], mid, lcp, lcp2, first, jobs, chunks;
     // run memo: raw and sorted run bounds, directory, run count
     DevBuf run_s, run_e, run_ss, run_se, run_dir, run_count;
     std::vector<std::pair<uint64_t, uint64_t>> h_runs;

@@ -1452,9 +1452,12 @@ __global__ __launch_bounds__(256) void k_boundary_lcp(const uint64_t *__restrict
 }
 
 // ---------------------------------------------------------------------------
-// Group refinement: one workgroup per group of suffixes known to agree on
-// their first depth0 symbols (65..2048 members, or one chunk of a larger
-// group).  The first kChunkRounds rounds sort each subgroup by the next k
+// Group refinement: a workgroup refines groups of suffixes known to agree on
+// their first depth0 symbols (65..2048 members, or chunks of 2,048 members of
+// larger groups) one at a time, in LDS arrays sized for M members: groups of
+// up to 128 members go to 128-thread workgroups of 8.4 KB, up to 512 to
+// 33.8 KB, larger ones to 135 KB (k_split_mid files them by size), so small
+// groups keep many workgroups per compute unit.  The first kChunkRounds rounds sort each subgroup by the next k
 // symbols at its depth (chunk key, plus the boundary value of members whose
 // window ends inside the chunk), which splits short common prefixes into many
 // classes at once.  Later rounds compare every unresolved member with the first
@@ -1603,203 +1606,231 @@ __device__ __forceinline__ uint32_t code_at(const S *__restrict__ t, const CodeM
     return (uint32_t)(w >> (64 - (f + 1) * g.bits)) & ((1u << g.bits) - 1);
 }
 
+// Mid-size groups by LDS class: class c of k_split_mid holds sizes in
+// (kMidClassMax[c-1], kMidClassMax[c]].
+constexpr uint32_t kMidClasses = 3;
+__host__ __device__ __forceinline__ uint32_t mid_class(uint64_t sz) { return sz <= 128 ? 0u : sz <= 512 ? 1u : 2u; }
+
+// Files the mid-size groups (65..2048 members) into the lists of their LDS
+// class (out + c * cap, counts[c]; counts start at zero).  The order inside a
+// list does not matter: every group is refined on its own.
+__global__ __launch_bounds__(256) void k_split_mid(const Group *__restrict__ in, uint32_t n, Group *__restrict__ out,
+                                                   uint64_t cap, unsigned int *__restrict__ counts) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const Group gr = in[i];
+    const uint32_t c = mid_class(gr.size);
+    out[c * cap + atomicAdd(&counts[c], 1u)] = gr;
+}
+
+// groups[0 .. n), n = *n_dev when n_dev is set (a count written by an earlier
+// kernel), else n_host; every group has at most M members.  With `next` (a
+// counter that starts at zero) workgroups claim groups one at a time, so
+// groups of very different cost balance over the grid; else workgroup b takes
+// groups b, b + gridDim.x, ...
 template <typename S, typename I, int B, int M>
 __global__ __launch_bounds__(B) void k_groups(const S *__restrict__ t, CodeMap<S> cm, KeyGeom g, DevLimits L,
-                                              uint64_t max_ctx, const Group *__restrict__ groups, uint64_t depth0,
-                                              I *__restrict__ idx, I *__restrict__ lcp, int run_mode,
-                                              uint32_t chunk_len) {
+                                              uint64_t max_ctx, const Group *__restrict__ groups,
+                                              const unsigned int *__restrict__ n_dev, uint32_t n_host,
+                                              unsigned int *__restrict__ next, uint64_t depth0,
+                                              I *__restrict__ idx, I *__restrict__ lcp, int run_mode) {
+    static_assert((M & (M - 1)) == 0 && M <= 65536, "power-of-two member slots with u16 ranks");
     // member-indexed: position, window, class key, tie key, subgroup depth, l
     __shared__ uint64_t s_pos[M], s_cap[M], s_key[M], s_aux[M], s_dep[M], s_l[M];
     __shared__ uint64_t s_lcp[M];  // split depth of sorted neighbours r-1, r
     __shared__ uint32_t s_sub[M];  // subgroup (its first sorted position) | settled bit
     __shared__ uint16_t s_perm[M];
     __shared__ uint32_t s_scan[M];
-    __shared__ uint32_t s_unres;
-    Group gr = groups[blockIdx.x];
-    uint64_t gstart = gr.start, gsize = gr.size;
-    if (chunk_len) {  // giant group cut into runs of chunk_len members
-        gstart += (uint64_t)blockIdx.y * chunk_len;
-        if (gstart >= gr.start + gr.size) return;
-        gsize = gr.start + gr.size - gstart;
-        if (gsize > chunk_len) gsize = chunk_len;
-    }
-    const uint32_t m = (uint32_t)gsize;
-    uint32_t P = 1;
-    while (P < m) P <<= 1;
+    __shared__ uint32_t s_unres, s_gi;
+    const uint32_t ng = n_dev ? *n_dev : n_host;
     const bool star = L.order == 1;
-    for (uint32_t s = threadIdx.x; s < m; s += B) {
-        uint64_t p = idx[gstart + s];
-        uint64_t lim = lim_at(L, p);
-        s_pos[s] = p;
-        s_cap[s] = lim < max_ctx ? lim : max_ctx;
-        s_sub[s] = 0;
-        s_dep[s] = depth0;
-        s_lcp[s] = 0;
-    }
-    for (uint32_t r = threadIdx.x; r < P; r += B) s_perm[r] = (uint16_t)r;
-    __syncthreads();
-    const uint64_t cont = star ? 0 : kAuxCont;
     uint64_t nsym = 0;  // symbols read
-    for (uint32_t round = 0;; ++round) {
-        // the first rounds split by chunk keys (k symbols at the subgroup's
-        // depth: many classes at once when members part early); later rounds
-        // by the reference's LCP (one round for nested common prefixes)
-        const bool chunk_round = round < kChunkRounds;
-        // (1) sort key of every unresolved member
-        for (uint32_t s = threadIdx.x; s < m; s += B) {
-            if (s_sub[s] & 0x80000000u) continue;  // settled
-            const uint64_t D = s_dep[s];
-            if (chunk_round) {
-                const uint64_t cap = s_cap[s];
-                s_key[s] = key_at(t, cm, g, s_pos[s], D, cap);
-                const uint64_t rem = cap > D ? cap - D : 0;
-                nsym += rem < g.k ? rem : g.k;
-                uint64_t aux = cont;
-                if (rem < g.k) {
-                    const uint64_t lm = lim_at(L, s_pos[s]);
-                    aux = star ? (1 + (kAuxCont - 1 - lm)) : lm;
-                }
-                s_aux[s] = aux;
-                continue;
-            }
-            const uint32_t ref = s_perm[s_sub[s]];
-            const uint64_t pa = s_pos[s], pr = s_pos[ref];
-            const uint64_t cs = s_cap[s], cref = s_cap[ref];
-            const uint64_t cmin = cs < cref ? cs : cref;
-            const uint64_t lim2 = cmin > D ? cmin - D : 0;  // common window from depth D
-            const uint64_t l = s == ref ? lim2 : lcp_codes(t, cm, g, pa + D, pr + D, lim2);
-            if (s != ref) nsym += 2 * (l < lim2 ? l + 1 : l);
-            // A: absolute depth where the member parts from the reference (or
-            // its window ends, possibly above D); classes order by it
-            uint64_t key, aux = 0;
-            if (l < lim2) {  // codes differ
-                const uint64_t A = D + l;
-                const uint32_t sa = code_at(t, cm, g, pa + A), sr = code_at(t, cm, g, pr + A);
-                key = sa < sr ? ((A << 17) | (sa + 1)) : ((2ull << 62) | ((kLMax - A) << 17) | (sa + 1));
-            } else if (cs < cref) {  // the member's window ends first: settled
-                const uint64_t A = cs;
-                key = star ? ((2ull << 62) | ((kLMax - A) << 17) | 0x1FFFFull) : ((A << 17) | 0u);
-                const uint64_t lm = lim_at(L, pa);
-                aux = star ? ~lm : lm;
-            } else if (cs == cref) {  // both end here: the boundary order decides; settled
-                key = 1ull << 62;
-                const uint64_t lm = lim_at(L, pa);
-                aux = star ? ~lm : lm;
-            } else {  // the reference's window ends first: these continue together
-                const uint64_t A = D + l;
-                key = star ? ((A << 17) | 0x1FFFFull) : ((2ull << 62) | ((kLMax - A) << 17) | 0u);
-            }
-            s_key[s] = key;
-            s_aux[s] = aux;
-            s_l[s] = l;
+    for (uint32_t gi = blockIdx.x;; gi += gridDim.x) {
+        __syncthreads();  // the previous group's last reads of the LDS arrays and of s_gi are done
+        if (next) {
+            if (threadIdx.x == 0) s_gi = atomicAdd(next, 1u);
+            __syncthreads();
+            gi = s_gi;
         }
+        if (gi >= ng) break;
+        const Group gr = groups[gi];
+        const uint64_t gstart = gr.start;
+        const uint32_t m = (uint32_t)gr.size;
+        uint32_t P = 1;
+        while (P < m) P <<= 1;
+        for (uint32_t s = threadIdx.x; s < m; s += B) {
+            uint64_t p = idx[gstart + s];
+            uint64_t lim = lim_at(L, p);
+            s_pos[s] = p;
+            s_cap[s] = lim < max_ctx ? lim : max_ctx;
+            s_sub[s] = 0;
+            s_dep[s] = depth0;
+            s_lcp[s] = 0;
+        }
+        for (uint32_t r = threadIdx.x; r < P; r += B) s_perm[r] = (uint16_t)r;
         __syncthreads();
-        // (2) bitonic sort of the permutation by (sub, key, aux, pos)
-        for (uint32_t size = 2; size <= P; size <<= 1) {
-            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-                for (uint32_t q = threadIdx.x; q < P / 2; q += B) {
-                    uint32_t i = 2 * q - (q & (stride - 1));
-                    uint32_t j = i + stride;
-                    bool up = (i & size) == 0;
-                    uint32_t a = s_perm[i], b = s_perm[j];
-                    bool gt;
-                    if (a >= m) gt = true;
-                    else if (b >= m) gt = false;
-                    else {
-                        uint32_t sa = s_sub[a] & 0x7FFFFFFFu, sb = s_sub[b] & 0x7FFFFFFFu;
-                        if (sa != sb) gt = sa > sb;
+        const uint64_t cont = star ? 0 : kAuxCont;
+        for (uint32_t round = 0;; ++round) {
+            // the first rounds split by chunk keys (k symbols at the subgroup's
+            // depth: many classes at once when members part early); later rounds
+            // by the reference's LCP (one round for nested common prefixes)
+            const bool chunk_round = round < kChunkRounds;
+            // (1) sort key of every unresolved member
+            for (uint32_t s = threadIdx.x; s < m; s += B) {
+                if (s_sub[s] & 0x80000000u) continue;  // settled
+                const uint64_t D = s_dep[s];
+                if (chunk_round) {
+                    const uint64_t cap = s_cap[s];
+                    s_key[s] = key_at(t, cm, g, s_pos[s], D, cap);
+                    const uint64_t rem = cap > D ? cap - D : 0;
+                    nsym += rem < g.k ? rem : g.k;
+                    uint64_t aux = cont;
+                    if (rem < g.k) {
+                        const uint64_t lm = lim_at(L, s_pos[s]);
+                        aux = star ? (1 + (kAuxCont - 1 - lm)) : lm;
+                    }
+                    s_aux[s] = aux;
+                    continue;
+                }
+                const uint32_t ref = s_perm[s_sub[s]];
+                const uint64_t pa = s_pos[s], pr = s_pos[ref];
+                const uint64_t cs = s_cap[s], cref = s_cap[ref];
+                const uint64_t cmin = cs < cref ? cs : cref;
+                const uint64_t lim2 = cmin > D ? cmin - D : 0;  // common window from depth D
+                const uint64_t l = s == ref ? lim2 : lcp_codes(t, cm, g, pa + D, pr + D, lim2);
+                if (s != ref) nsym += 2 * (l < lim2 ? l + 1 : l);
+                // A: absolute depth where the member parts from the reference (or
+                // its window ends, possibly above D); classes order by it
+                uint64_t key, aux = 0;
+                if (l < lim2) {  // codes differ
+                    const uint64_t A = D + l;
+                    const uint32_t sa = code_at(t, cm, g, pa + A), sr = code_at(t, cm, g, pr + A);
+                    key = sa < sr ? ((A << 17) | (sa + 1)) : ((2ull << 62) | ((kLMax - A) << 17) | (sa + 1));
+                } else if (cs < cref) {  // the member's window ends first: settled
+                    const uint64_t A = cs;
+                    key = star ? ((2ull << 62) | ((kLMax - A) << 17) | 0x1FFFFull) : ((A << 17) | 0u);
+                    const uint64_t lm = lim_at(L, pa);
+                    aux = star ? ~lm : lm;
+                } else if (cs == cref) {  // both end here: the boundary order decides; settled
+                    key = 1ull << 62;
+                    const uint64_t lm = lim_at(L, pa);
+                    aux = star ? ~lm : lm;
+                } else {  // the reference's window ends first: these continue together
+                    const uint64_t A = D + l;
+                    key = star ? ((A << 17) | 0x1FFFFull) : ((2ull << 62) | ((kLMax - A) << 17) | 0u);
+                }
+                s_key[s] = key;
+                s_aux[s] = aux;
+                s_l[s] = l;
+            }
+            __syncthreads();
+            // (2) bitonic sort of the permutation by (sub, key, aux, pos)
+            for (uint32_t size = 2; size <= P; size <<= 1) {
+                for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                    for (uint32_t q = threadIdx.x; q < P / 2; q += B) {
+                        uint32_t i = 2 * q - (q & (stride - 1));
+                        uint32_t j = i + stride;
+                        bool up = (i & size) == 0;
+                        uint32_t a = s_perm[i], b = s_perm[j];
+                        bool gt;
+                        if (a >= m) gt = true;
+                        else if (b >= m) gt = false;
                         else {
-                            // settled members keep their place: they are alone in their subgroup
-                            uint64_t ka = (s_sub[a] & 0x80000000u) ? 0 : s_key[a];
-                            uint64_t kb = (s_sub[b] & 0x80000000u) ? 0 : s_key[b];
-                            uint64_t xa = (s_sub[a] & 0x80000000u) ? 0 : s_aux[a];
-                            uint64_t xb = (s_sub[b] & 0x80000000u) ? 0 : s_aux[b];
-                            if (ka != kb) gt = ka > kb;
-                            else if (xa != xb) gt = xa > xb;
-                            else gt = s_pos[a] > s_pos[b];
+                            uint32_t sa = s_sub[a] & 0x7FFFFFFFu, sb = s_sub[b] & 0x7FFFFFFFu;
+                            if (sa != sb) gt = sa > sb;
+                            else {
+                                // settled members keep their place: they are alone in their subgroup
+                                uint64_t ka = (s_sub[a] & 0x80000000u) ? 0 : s_key[a];
+                                uint64_t kb = (s_sub[b] & 0x80000000u) ? 0 : s_key[b];
+                                uint64_t xa = (s_sub[a] & 0x80000000u) ? 0 : s_aux[a];
+                                uint64_t xb = (s_sub[b] & 0x80000000u) ? 0 : s_aux[b];
+                                if (ka != kb) gt = ka > kb;
+                                else if (xa != xb) gt = xa > xb;
+                                else gt = s_pos[a] > s_pos[b];
+                            }
+                        }
+                        if (gt == up) {
+                            s_perm[i] = (uint16_t)b;
+                            s_perm[j] = (uint16_t)a;
                         }
                     }
-                    if (gt == up) {
-                        s_perm[i] = (uint16_t)b;
-                        s_perm[j] = (uint16_t)a;
+                    __syncthreads();
+                }
+            }
+            // (3) new subgroups: a member starts one unless it continues a tie
+            // (same old subgroup, same class key, a class that can tie)
+            for (uint32_t r = threadIdx.x; r < m; r += B) {
+                uint32_t s = s_perm[r];
+                bool f = true;
+                if (r > 0 && !(s_sub[s] & 0x80000000u)) {
+                    uint32_t q = s_perm[r - 1];
+                    bool same_old = (s_sub[s] & 0x7FFFFFFFu) == (s_sub[q] & 0x7FFFFFFFu);
+                    if (same_old && chunk_round) {
+                        s_lcp[r] = s_dep[s] + shared_fields(g, s_key[q], s_key[s]);
+                        f = !(s_key[s] == s_key[q] && s_aux[s] == s_aux[q]);
+                    } else if (same_old) {
+                        const uint64_t ls = s_l[s], lq = s_l[q];
+                        s_lcp[r] = s_dep[s] + (ls < lq ? ls : lq);
+                        const uint64_t k = s_key[s];
+                        f = !(group_key_continues(k, star) && k == s_key[q]);
                     }
                 }
+                s_scan[r] = f ? r : 0;
+            }
+            __syncthreads();
+            for (uint32_t off = 1; off < m; off <<= 1) {
+                uint32_t v[(M + B - 1) / B];
+                uint32_t c = 0;
+                for (uint32_t r = threadIdx.x; r < m; r += B, ++c)
+                    v[c] = (r >= off && s_scan[r - off] > s_scan[r]) ? s_scan[r - off] : s_scan[r];
+                __syncthreads();
+                c = 0;
+                for (uint32_t r = threadIdx.x; r < m; r += B, ++c) s_scan[r] = v[c];
                 __syncthreads();
             }
-        }
-        // (3) new subgroups: a member starts one unless it continues a tie
-        // (same old subgroup, same class key, a class that can tie)
-        for (uint32_t r = threadIdx.x; r < m; r += B) {
-            uint32_t s = s_perm[r];
-            bool f = true;
-            if (r > 0 && !(s_sub[s] & 0x80000000u)) {
-                uint32_t q = s_perm[r - 1];
-                bool same_old = (s_sub[s] & 0x7FFFFFFFu) == (s_sub[q] & 0x7FFFFFFFu);
-                if (same_old && chunk_round) {
-                    s_lcp[r] = s_dep[s] + shared_fields(g, s_key[q], s_key[s]);
-                    f = !(s_key[s] == s_key[q] && s_aux[s] == s_aux[q]);
-                } else if (same_old) {
-                    const uint64_t ls = s_l[s], lq = s_l[q];
-                    s_lcp[r] = s_dep[s] + (ls < lq ? ls : lq);
-                    const uint64_t k = s_key[s];
-                    f = !(group_key_continues(k, star) && k == s_key[q]);
-                }
-            }
-            s_scan[r] = f ? r : 0;
-        }
-        __syncthreads();
-        for (uint32_t off = 1; off < m; off <<= 1) {
-            uint32_t v[(M + B - 1) / B];
-            uint32_t c = 0;
-            for (uint32_t r = threadIdx.x; r < m; r += B, ++c)
-                v[c] = (r >= off && s_scan[r - off] > s_scan[r]) ? s_scan[r - off] : s_scan[r];
+            // (4) settle singletons and ended members; ties move to their new depth
+            if (threadIdx.x == 0) s_unres = 0;
             __syncthreads();
-            c = 0;
-            for (uint32_t r = threadIdx.x; r < m; r += B, ++c) s_scan[r] = v[c];
+            uint32_t unres = 0;
+            for (uint32_t r = threadIdx.x; r < m; r += B) {
+                uint32_t s = s_perm[r];
+                uint32_t ns = s_scan[r];
+                bool has_nb = (r + 1 < m && s_scan[r + 1] == ns) || (r > 0 && s_scan[r - 1] == ns);
+                bool settled = (s_sub[s] & 0x80000000u) || !has_nb;
+                // chunk rounds: members whose window ended inside the chunk are
+                // comparator-Equal (position order) with their tie
+                if (chunk_round && !settled && s_aux[s] != cont) settled = true;
+                if (!settled) {
+                    if (chunk_round) {
+                        s_dep[s] = s_dep[s] + g.k;
+                    } else {  // a tie: past the differing code, or past the reference's end
+                        const uint64_t k = s_key[s];
+                        const uint64_t sym = k & 0x1FFFFull;
+                        s_dep[s] = s_dep[s] + s_l[s] + (sym != 0 && sym != 0x1FFFFull ? 1 : 0);
+                    }
+                }
+                s_sub[s] = ns | (settled ? 0x80000000u : 0u);
+                unres += settled ? 0 : 1;
+            }
+            if (unres) atomicAdd(&s_unres, unres);
+            __syncthreads();
+            if (s_unres == 0) break;
             __syncthreads();
         }
-        // (4) settle singletons and ended members; ties move to their new depth
-        if (threadIdx.x == 0) s_unres = 0;
-        __syncthreads();
-        uint32_t unres = 0;
         for (uint32_t r = threadIdx.x; r < m; r += B) {
             uint32_t s = s_perm[r];
-            uint32_t ns = s_scan[r];
-            bool has_nb = (r + 1 < m && s_scan[r + 1] == ns) || (r > 0 && s_scan[r - 1] == ns);
-            bool settled = (s_sub[s] & 0x80000000u) || !has_nb;
-            // chunk rounds: members whose window ended inside the chunk are
-            // comparator-Equal (position order) with their tie
-            if (chunk_round && !settled && s_aux[s] != cont) settled = true;
-            if (!settled) {
-                if (chunk_round) {
-                    s_dep[s] = s_dep[s] + g.k;
-                } else {  // a tie: past the differing code, or past the reference's end
-                    const uint64_t k = s_key[s];
-                    const uint64_t sym = k & 0x1FFFFull;
-                    s_dep[s] = s_dep[s] + s_l[s] + (sym != 0 && sym != 0x1FFFFull ? 1 : 0);
+            idx[gstart + r] = (I)s_pos[s];
+            if (r > 0) {
+                if (lcp) {
+                    uint32_t q = s_perm[r - 1];
+                    uint64_t v = s_lcp[r];
+                    v = v < s_cap[s] ? v : s_cap[s];
+                    v = v < s_cap[q] ? v : s_cap[q];
+                    lcp[gstart + r] = (I)v;
                 }
+            } else if (run_mode && lcp) {
+                lcp[gstart] = 0;
             }
-            s_sub[s] = ns | (settled ? 0x80000000u : 0u);
-            unres += settled ? 0 : 1;
-        }
-        if (unres) atomicAdd(&s_unres, unres);
-        __syncthreads();
-        if (s_unres == 0) break;
-        __syncthreads();
-    }
-    for (uint32_t r = threadIdx.x; r < m; r += B) {
-        uint32_t s = s_perm[r];
-        idx[gstart + r] = (I)s_pos[s];
-        if (r > 0) {
-            if (lcp) {
-                uint32_t q = s_perm[r - 1];
-                uint64_t v = s_lcp[r];
-                v = v < s_cap[s] ? v : s_cap[s];
-                v = v < s_cap[q] ? v : s_cap[q];
-                lcp[gstart + r] = (I)v;
-            }
-        } else if (run_mode && lcp) {
-            lcp[gstart] = 0;
         }
     }
     count_symbols(2, nsym);

@@ -375,12 +375,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T <= 2048 ?
     // before the sort, no global one between the sort and the output
     uint64_t kk[ITEMS];
     I ix[ITEMS];
+    // a thread whose records all lie inside the tile loads (and later
+    // stores) them unconditionally: adjacent accesses merge into 16-byte
+    // vector loads and stores (2,048-record class 2.54 -> 2.46 ms)
+    if (p0 + ITEMS <= len) {
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-        const uint32_t i = p0 + j;
-        kk[j] = i < len ? keys[tl.start + i] : ~0ull;
-        ix[j] = i < len ? idx[tl.start + i] : (I)0;
-        my_or |= i < len ? kk[j] ^ first : 0ull;
+        for (int j = 0; j < ITEMS; ++j) {
+            kk[j] = keys[tl.start + p0 + j];
+            ix[j] = idx[tl.start + p0 + j];
+            my_or |= kk[j] ^ first;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const uint32_t i = p0 + j;
+            kk[j] = i < len ? keys[tl.start + i] : ~0ull;
+            ix[j] = i < len ? idx[tl.start + i] : (I)0;
+            my_or |= i < len ? kk[j] ^ first : 0ull;
+        }
     }
     if (tid == 0) s_or = 0;
     if (tid < kGroupClasses) gcnt[tid] = 0;
@@ -424,9 +436,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T <= 2048 ?
     I o[ITEMS];
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) o[j] = p0 + j < len ? stage[(uint32_t)(k[j] & (T - 1))] : (I)0;
+    if (p0 + ITEMS <= len) {
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j)
-        if (p0 + j < len) idx[tl.start + p0 + j] = o[j];
+        for (int j = 0; j < ITEMS; ++j) idx[tl.start + p0 + j] = o[j];
+    } else {
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j)
+            if (p0 + j < len) idx[tl.start + p0 + j] = o[j];
+    }
     if ((tid & 63) == 63) wlast[tid >> 6] = o[ITEMS - 1];
     uint64_t key[ITEMS];
     const uint64_t top = first & ~mask;

@@ -314,6 +314,42 @@ def test_shared_stretches_persistent():
         assert np.array_equal(lcp.cpu().numpy().view(np.uint32).astype(np.uint64), want_lcp)
 
 
+def test_mid_groups_every_lds_class():
+    """Tie groups of 65..2048 members in each LDS class of the mid-size
+    refinement (up to 128, 129..512 and 513..2048 members): a unit copied c
+    times gives groups of exactly c members at its first offsets and smaller
+    ones further in, with windows (max_context, segment ends) ending inside
+    the shared stretches."""
+    from caps_sa import device as dev
+
+    counts = [65, 100, 128, 129, 300, 512, 513, 1000, 2000]
+    t = _copies_text(counts, 160, seed=81)
+    t16 = _copies_text(counts, 160, seed=82, sigma=300, dtype=np.uint16)
+    check(t, label="mid copies u8")
+    check(t, max_ctx=90, label="mid copies max_ctx=90")
+    rng = np.random.default_rng(83)
+    ends = np.unique(np.concatenate([np.sort(rng.integers(1, len(t), 300)), [len(t)]]))
+    check(t, ends=ends, label="mid copies segmented", with_lcp=False)
+    check(t, ends=ends, order=1, label="mid copies star", with_lcp=False)
+    check(t16, label="mid copies u16")
+    # the same groups through the three launches by size (a build with
+    # thousands of such groups takes them by itself)
+    cases = [(t, None, 0, MAX), (t, None, 0, 90), (t16, None, 0, MAX), (t, ends, 1, MAX), (t, ends, 0, MAX)]
+    for text, e, order, mx in cases:
+        print(f"case by-size n={len(text)} {text.dtype} ends={e is not None} order={order} max_ctx={mx}", flush=True)
+        d_text = dev.text_to_device(text)
+        sa, lcp = dev.build_device(d_text, len(text), text.dtype.itemsize, lp=_lp(len(text), e, order),
+                                   max_context=mx, want_lcp=True, flags=caps_sa.FLAG_MID_BY_SIZE)
+        ctx = O.Ctx(text, e, order)
+        want = ctx.build_in_memory(max_ctx=mx, threads=4)
+        if e is not None and order == 0:
+            want = ctx.canonicalize(want, mx)
+        assert np.array_equal(sa.cpu().numpy().view(np.uint32).astype(np.uint64), want)
+        if e is None:
+            want_lcp = ctx.lcp_array(want, mx)
+            assert np.array_equal(lcp.cpu().numpy().view(np.uint32).astype(np.uint64), want_lcp)
+
+
 def test_tight_and_pow2_seed_fields_agree():
     """Seed keys with ceil(log2(sigma))-bit fields (narrowed from the packed
     text's power-of-two fields) and with power-of-two fields give the same
