@@ -319,7 +319,7 @@ def run_genome(args, n: int, filtered: bool = False) -> dict:
 
         def step():
             dev.build_device_filtered(d_text, n, 1, d_keep, n_pos, idx_bytes=8, lp=lp, d_out=d_out, flags=flags,
-                                      gpu_buckets=args.genome_buckets,
+                                      gpu_buckets=args.genome_buckets, tile_keys=args.bucket_keys,
                                       batch_records=args.batch_records)
             return caps_sa.last_stats()
     else:
@@ -329,7 +329,7 @@ def run_genome(args, n: int, filtered: bool = False) -> dict:
 
         def step():
             dev.build_device(d_text, n, 1, d_positions=d_pos, n_pos=n_pos, idx_bytes=8, lp=lp, d_out=d_out,
-                             gpu_buckets=args.genome_buckets,
+                             gpu_buckets=args.genome_buckets, tile_keys=args.bucket_keys,
                              flags=flags, batch_records=args.batch_records)
             return caps_sa.last_stats()
     print(f"[bench] genome: n={n} segments={n_seg} positions={n_pos} "

@@ -568,10 +568,10 @@ __global__ __launch_bounds__(256) void k_classify(const S *__restrict__ t, CodeM
 constexpr uint32_t kSeqItems = 16;
 constexpr uint32_t kSeqChunk = 256 * kSeqItems;
 
-template <typename S>
+template <typename S, int NI = kSeqItems>
 __device__ __forceinline__ void thread_keys_seq(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, const KeyGeom &g,
                                                 const DevLimits &L, uint64_t max_ctx, uint64_t base, uint32_t cnt,
-                                                S *win, uint64_t (&out)[kSeqItems]) {
+                                                S *win, uint64_t (&out)[NI]) {
     // window: positions [base, base + cnt + k - 1) clipped to the text
     __syncthreads();  // the previous chunk's readers of win are done
     uint64_t wlen = (uint64_t)cnt + g.k - 1;
@@ -586,11 +586,11 @@ __device__ __forceinline__ void thread_keys_seq(const S *__restrict__ t, uint64_
         for (uint32_t i = threadIdx.x; i < wl; i += blockDim.x) win[i] = t[base + i];
     }
     __syncthreads();
-    const uint32_t first = threadIdx.x * kSeqItems;
-    uint64_t caps[kSeqItems], keys[kSeqItems], segs[kSeqItems];
+    const uint32_t first = threadIdx.x * NI;
+    uint64_t caps[NI], keys[NI], segs[NI];
     uint64_t seg_end = 0, seg = 0;
 #pragma unroll
-    for (uint32_t i = 0; i < kSeqItems; ++i) {
+    for (uint32_t i = 0; i < NI; ++i) {
         uint64_t q = base + first + i;
         uint64_t lim;
         if (q >= n) {
@@ -604,9 +604,9 @@ __device__ __forceinline__ void thread_keys_seq(const S *__restrict__ t, uint64_
         caps[i] = lim < max_ctx ? lim : max_ctx;
         segs[i] = seg;
     }
-    if (first < cnt) keys_from_window<S, kSeqItems>(win, cm, g, first, caps, keys);
+    if (first < cnt) keys_from_window<S, NI>(win, cm, g, first, caps, keys);
 #pragma unroll
-    for (uint32_t i = 0; i < kSeqItems; ++i) out[i] = first + i < cnt ? with_id(g, keys[i], segs[i]) : 0ull;
+    for (uint32_t i = 0; i < NI; ++i) out[i] = first + i < cnt ? with_id(g, keys[i], segs[i]) : 0ull;
 }
 
 // The binary-search variant is held to 128 VGPRs (four wavefronts per SIMD
@@ -905,14 +905,18 @@ __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint6
 // threads write each interval's run of the step, so a wavefront's store
 // covers a few contiguous runs instead of 64 scattered records.  For at most
 // kStageIntervals intervals; the block owns the same range of positions as in
-// k_scatter1.  One block of 512 threads per compute unit (136 KB of LDS with
-// u8 text: keys, step-local positions, interval ids, interval tables) stages
-// 8,192 positions a step: about eight records per interval and step over
-// 1,024 intervals, instead of the four of 4,096-position steps at two blocks
-// of 256.
+// k_scatter1.  One block of 1,024 threads per compute unit (136 KB of LDS
+// with u8 text: keys, step-local positions, interval ids, interval tables)
+// stages 8,192 positions a step, eight per thread: about eight records per
+// interval and step over 1,024 intervals, instead of the four of
+// 4,096-position steps at two blocks of 256 (and 1,024 threads hide the
+// step's latencies better than 512 with sixteen positions each: 2.30 ->
+// 2.11 ms on 268M positions).
 constexpr uint32_t kStageIntervals = 2048;
-constexpr uint32_t kStageNT = 512;
-constexpr uint32_t kStageStep = kStageNT * kSeqItems;
+constexpr uint32_t kStageStep = 8192;
+constexpr uint32_t kStageNT = 1024;
+constexpr uint32_t kStageItems = kStageStep / kStageNT;  // consecutive positions per thread
+static_assert(kStageItems % 8 == 0 && kStageItems <= 32, "whole 16-byte interval-id loads, one keep word");
 // scatter pass 1 workgroups per compute unit (staged / direct pass), each
 // taking consecutive classify blocks
 constexpr uint32_t kP1StagedPerCu = 4, kP1DirectPerCu = 2;
@@ -965,25 +969,31 @@ __global__ __launch_bounds__(kStageNT) void k_scatter1_staged(const S *__restric
     }
     for (uint64_t base = c0; base < c1; base += kStageStep) {
         const uint32_t cnt = (uint32_t)(c1 - base < kStageStep ? c1 - base : kStageStep);
-        uint64_t key[kSeqItems];
-        thread_keys_seq<S>(t, n, cm, g, L, max_ctx, base, cnt, win, key);  // begins with a barrier
-        const uint32_t first = threadIdx.x * kSeqItems;
-        uint32_t cc[kSeqItems], lr[kSeqItems];
-        if (first + kSeqItems <= cnt) {
+        constexpr uint32_t NI = kStageItems;
+        uint64_t key[NI];
+        thread_keys_seq<S, NI>(t, n, cm, g, L, max_ctx, base, cnt, win, key);  // begins with a barrier
+        const uint32_t first = threadIdx.x * NI;
+        uint32_t cc[NI], lr[NI];
+        if (first + NI <= cnt) {
             const uint4 *src = (const uint4 *)(cid + base + first);
-            const uint4 v0 = src[0], v1 = src[1];
-            const uint32_t w[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+            uint32_t w[NI / 2];
 #pragma unroll
-            for (uint32_t j = 0; j < kSeqItems; ++j) cc[j] = (w[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
+            for (uint32_t q = 0; q < NI / 8; ++q) {
+                const uint4 v = src[q];
+                w[4 * q] = v.x, w[4 * q + 1] = v.y, w[4 * q + 2] = v.z, w[4 * q + 3] = v.w;
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < NI; ++j) cc[j] = (w[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
         } else {
 #pragma unroll
-            for (uint32_t j = 0; j < kSeqItems; ++j) cc[j] = first + j < cnt ? (uint32_t)cid[base + first + j] : kSkip;
+            for (uint32_t j = 0; j < NI; ++j) cc[j] = first + j < cnt ? (uint32_t)cid[base + first + j] : kSkip;
         }
-        uint32_t kb = 0xFFFFu;
-        if (KEEP && first < cnt) kb = (uint32_t)(keep[(base + first) >> 6] >> ((base + first) & 63)) & 0xFFFFu;
+        constexpr uint32_t kAll = (uint32_t)((1ull << NI) - 1);
+        uint32_t kb = kAll;
+        if (KEEP && first < cnt) kb = (uint32_t)(keep[(base + first) >> 6] >> ((base + first) & 63)) & kAll;
         // local rank of each record inside its interval's run of this step
 #pragma unroll
-        for (uint32_t j = 0; j < kSeqItems; ++j) {
+        for (uint32_t j = 0; j < NI; ++j) {
             const uint32_t c = cc[j];
             const bool take = ((kb >> j) & 1u) && c != kSkip && c - c_lo < c_hi - c_lo;
             cc[j] = take ? c : kSkip;
@@ -1008,7 +1018,7 @@ __global__ __launch_bounds__(kStageNT) void k_scatter1_staged(const S *__restric
         }
         __syncthreads();
 #pragma unroll
-        for (uint32_t j = 0; j < kSeqItems; ++j)
+        for (uint32_t j = 0; j < NI; ++j)
             if (cc[j] != kSkip) {
                 const uint32_t q = ioff[cc[j]] + lr[j];
                 s_key[q] = key[j];
