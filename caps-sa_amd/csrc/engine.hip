@@ -685,7 +685,7 @@ static BatchPlan plan_batches(const Workspace &W, uint64_t n_pos, uint64_t reque
     // every buffer this build allocates after planning counts as free, so
     // that a repeated build of the same input plans the same batches (a
     // different plan would regrow multi-GB buffers in the middle of a build)
-    size_t held = W.recs.cap + W.keys.cap + W.sinkbuf.cap + W.keys2.cap + W.idx2.cap + W.lcp.cap +
+    size_t held = W.recs.cap + W.keys.cap + W.brec.cap + W.sinkbuf.cap + W.keys2.cap + W.idx2.cap + W.lcp.cap +
                   W.lcp2.cap + W.tiles.cap + W.big.cap + W.chunks.cap + W.jobs.cap + W.run_s.cap + W.run_e.cap +
                   W.run_ss.cap + W.run_se.cap + W.run_dir.cap;
     for (int c = 0; c < kGroupClasses; ++c) held += W.glist[c].cap;
@@ -1035,6 +1035,12 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     // batch-sized buffers
     uint64_t *keys = (uint64_t *)W.keys.get(bcap * 8 + 8);
     KRec<I> *tmp = (KRec<I> *)W.recs.get((all_records ? n_pos : bcap) * sizeof(KRec<I>) + 16);
+    // one batch, merge tile sort: scatter pass 2 writes bucketed records (key
+    // and index side by side, one write stream per bucket run instead of two)
+    // and the tile sort reads them, writing the sorted index values to the
+    // output and sorted keys where they are read again
+    const bool aos = single && !(o.flags & CAPS_FLAG_TILE_RADIX);
+    KRec<I> *brec = aos ? (KRec<I> *)W.brec.get(bcap * sizeof(KRec<I>) + 16) : nullptr;
     I *sink_buf = sink ? (I *)W.sinkbuf.get(bcap * sizeof(I) + 8) : nullptr;
     // the previous batch's last suffix (key, position), for the next batch's first LCP
     uint64_t prev_key = 0, prev_pos = 0;
@@ -1135,15 +1141,20 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         const uint32_t s2_lds = scatter2_lds_bytes(SP.stride);
         const bool wide = s2_lds > 22 * 1024 || (c_hi - c_lo) < 2 * W.n_cu;
         KRec<I> *trec = all_records ? tmp + rbase : tmp;
-        if (wide)
-            k_scatter2<I, 1024, 4096><<<c_hi - c_lo, 1024, s2_lds, st>>>(trec, coff, SP, (uint32_t)p, off, keys, out,
-                                                                        c_lo, c_hi, rbase);
-        else if constexpr (sizeof(I) == 4)
-            k_scatter2<I, 1024, 8192><<<c_hi - c_lo, 1024, s2_lds, st>>>(trec, coff, SP, (uint32_t)p, off, keys, out,
-                                                                        c_lo, c_hi, rbase);
-        else
-            k_scatter2<I, 1024, 4096><<<c_hi - c_lo, 1024, s2_lds, st>>>(trec, coff, SP, (uint32_t)p, off, keys, out,
-                                                                        c_lo, c_hi, rbase);
+        auto scatter2 = [&](auto kern) {
+            kern<<<c_hi - c_lo, 1024, s2_lds, st>>>(trec, coff, SP, (uint32_t)p, off, keys, out, c_lo, c_hi, rbase,
+                                                    brec);
+        };
+        if (wide) {
+            if (aos) scatter2(k_scatter2<I, 1024, 4096, true>);
+            else scatter2(k_scatter2<I, 1024, 4096, false>);
+        } else if constexpr (sizeof(I) == 4) {
+            if (aos) scatter2(k_scatter2<I, 1024, 8192, true>);
+            else scatter2(k_scatter2<I, 1024, 8192, false>);
+        } else {
+            if (aos) scatter2(k_scatter2<I, 1024, 4096, true>);
+            else scatter2(k_scatter2<I, 1024, 4096, false>);
+        }
         LAUNCHCHK();
         kt.end();
         tm.mark(PH_SCATTER);
@@ -1177,6 +1188,9 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
                 if (!gs.capturing) kt.begin(&stats.k_tile[c]);
                 const int keep = lcp ? 1 : 0;  // boundary LCPs read the sorted keys
                 if (radix) k_tile_radix<I, TT><<<hc[1 + c], 256, 0, gs.ls>>>(keys, out, tiles + c * tcap, gl, 1, keep);
+                else if (aos)
+                    k_tile_merge<I, TT, true><<<hc[1 + c], 256, 0, gs.ls>>>(keys, out, tiles + c * tcap, gl, 1, keep,
+                                                                           brec);
                 else k_tile_merge<I, TT><<<hc[1 + c], 256, 0, gs.ls>>>(keys, out, tiles + c * tcap, gl, 1, keep);
                 LAUNCHCHK();
                 if (!gs.capturing) kt.end();

@@ -43,7 +43,7 @@ struct Workspace {
     std::vector<Tile> h_tiles;
     Range h_range;
     // classification / scatter
-    DevBuf bid, hist, hblk, off, cursor, keys, keys2, idx2;
+    DevBuf bid, hist, hblk, off, cursor, keys, keys2, idx2, brec;
     // bucket batches: per-record bucket ids, a batch of output; kept-position
     // bitmap, its count
     DevBuf sinkbuf, keep, keep_count, poscheck;

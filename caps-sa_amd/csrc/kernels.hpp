@@ -812,21 +812,6 @@ __global__ __launch_bounds__(1024) void k_block_offsets(uint32_t *__restrict__ h
 // the placing pass), histograms and scans the buckets, writes the bucket
 // offsets, and places every record with LDS cursors.
 // ---------------------------------------------------------------------------
-// A record between the two scatter passes: key and index, 16 bytes with a
-// u64 index; with a u32 index packed to 12 bytes (4-byte aligned), so both
-// passes move a quarter fewer bytes.
-template <typename I>
-struct KRec {
-    uint64_t key;
-    I idx;
-};
-template <>
-struct __attribute__((packed, aligned(4))) KRec<uint32_t> {
-    uint64_t key;
-    uint32_t idx;
-};
-static_assert(sizeof(KRec<uint32_t>) == 12 && alignof(KRec<uint32_t>) == 4, "packed u32 record");
-static_assert(sizeof(KRec<uint64_t>) == 16, "u64 record");
 
 template <typename S, typename I, bool SEQ, bool KEEP>
 __global__ __launch_bounds__(256) void k_scatter1(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, KeyGeom g,
@@ -1090,11 +1075,11 @@ __host__ __device__ constexpr uint32_t scatter2_lds_bytes(uint32_t stride) {
 // splitters in LDS costs less than writing and reading back two bytes per
 // record (3.18 against 3.48 ms on 268M records in 1,024 intervals; 4,096-record
 // tiles instead of 2,048 took 3.82 to 3.48 ms: longer bucket runs per write).
-template <typename I, int NT, int TILE>
+template <typename I, int NT, int TILE, bool AOS = false>
 __global__ __launch_bounds__(NT) void k_scatter2(const KRec<I> *__restrict__ tmp, const uint64_t *__restrict__ coff,
                                                   Splitters SP, uint32_t p, uint64_t *__restrict__ off,
                                                   uint64_t *__restrict__ keys, I *__restrict__ idx, uint32_t c_lo,
-                                                  uint32_t c_hi, uint64_t rbase) {
+                                                  uint32_t c_hi, uint64_t rbase, KRec<I> *__restrict__ brec = nullptr) {
     constexpr int U = TILE / NT;
     static_assert(TILE % NT == 0 && NT % 64 == 0, "tile of whole rows");
     // LDS sized by the launch (scatter2_lds_bytes)
@@ -1182,8 +1167,12 @@ __global__ __launch_bounds__(NT) void k_scatter2(const KRec<I> *__restrict__ tmp
         for (uint32_t q = threadIdx.x; q < tl; q += NT) {
             const uint32_t fb = tf[q];
             const uint64_t dst = s + gcur[fb] + (q - lbase[fb]);
-            keys[dst] = tkey[q];
-            idx[dst] = tidx[q];
+            if constexpr (AOS) {
+                brec[dst] = KRec<I>{tkey[q], tidx[q]};
+            } else {
+                keys[dst] = tkey[q];
+                idx[dst] = tidx[q];
+            }
         }
         __syncthreads();
         for (uint32_t j = threadIdx.x; j < nb; j += blockDim.x) {

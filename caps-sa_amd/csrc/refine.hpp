@@ -15,6 +15,23 @@ struct Group {
     uint64_t size;
 };
 
+// A record between the two scatter passes (and, from scatter pass 2 to the
+// tile sort, a bucketed record): key and index, 16 bytes with a u64 index;
+// with a u32 index packed to 12 bytes (4-byte aligned), so both passes move
+// a quarter fewer bytes.
+template <typename I>
+struct KRec {
+    uint64_t key;
+    I idx;
+};
+template <>
+struct __attribute__((packed, aligned(4))) KRec<uint32_t> {
+    uint64_t key;
+    uint32_t idx;
+};
+static_assert(sizeof(KRec<uint32_t>) == 12 && alignof(KRec<uint32_t>) == 4, "packed u32 record");
+static_assert(sizeof(KRec<uint64_t>) == 16, "u64 record");
+
 constexpr int kGroupClasses = 8;
 constexpr uint32_t kMidMax = 2048;
 

@@ -346,10 +346,10 @@ __device__ __forceinline__ void block_merge_sort(uint64_t (&k)[ITEMS], uint64_t 
 // the group lists.  Tiles whose keys vary in more than 64 - log2(T) bits sort
 // twice, stably: by the low bits, then by the high bits and the rank of the
 // first pass.
-template <typename I, int T>
+template <typename I, int T, bool AOS = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T <= 2048 ? 8 : 4))) void k_tile_merge(
     uint64_t *__restrict__ keys, I *__restrict__ idx, const Tile *__restrict__ tiles, GroupLists gl, int emit,
-    int keep_keys) {
+    int keep_keys, const KRec<I> *__restrict__ rec = nullptr) {
     constexpr int B = 256;
     constexpr int ITEMS = T / B;
     constexpr int PADT = T + B;
@@ -367,7 +367,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T <= 2048 ?
     const uint32_t len = tl.len;
     const uint32_t tid = threadIdx.x;
     if (len == 0) return;
-    const uint64_t first = keys[tl.start];
+    // AOS: the records come from rec (key, index) and the sorted index
+    // values and keys go to idx and keys; else keys and idx hold both
+    const uint64_t first = AOS ? rec[tl.start].key : keys[tl.start];
     const uint32_t p0 = tid * ITEMS;
     uint64_t my_or = 0;
     // each thread loads its own ITEMS consecutive keys into registers, and
@@ -381,16 +383,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T <= 2048 ?
     if (p0 + ITEMS <= len) {
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
-            kk[j] = keys[tl.start + p0 + j];
-            ix[j] = idx[tl.start + p0 + j];
+            if constexpr (AOS) {
+                const KRec<I> r = rec[tl.start + p0 + j];
+                kk[j] = r.key;
+                ix[j] = r.idx;
+            } else {
+                kk[j] = keys[tl.start + p0 + j];
+                ix[j] = idx[tl.start + p0 + j];
+            }
             my_or |= kk[j] ^ first;
         }
     } else {
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
             const uint32_t i = p0 + j;
-            kk[j] = i < len ? keys[tl.start + i] : ~0ull;
-            ix[j] = i < len ? idx[tl.start + i] : (I)0;
+            if constexpr (AOS) {
+                const KRec<I> r = i < len ? rec[tl.start + i] : KRec<I>{~0ull, (I)0};
+                kk[j] = r.key;
+                ix[j] = r.idx;
+            } else {
+                kk[j] = i < len ? keys[tl.start + i] : ~0ull;
+                ix[j] = i < len ? idx[tl.start + i] : (I)0;
+            }
             my_or |= i < len ? kk[j] ^ first : 0ull;
         }
     }
@@ -422,7 +436,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T <= 2048 ?
         for (int j = 0; j < ITEMS; ++j) {
             const uint32_t p = p0 + j;
             const uint32_t o1 = (uint32_t)(k[j] & (T - 1));
-            k[j] = p < len ? ((keys[tl.start + o1] >> (64 - LB)) << (2 * LB)) | ((uint64_t)p << LB) | o1 : ~0ull;
+            const uint64_t ko = p < len ? (AOS ? rec[tl.start + o1].key : keys[tl.start + o1]) : 0ull;
+            k[j] = p < len ? ((ko >> (64 - LB)) << (2 * LB)) | ((uint64_t)p << LB) | o1 : ~0ull;
         }
         block_merge_sort<B, ITEMS>(k, sk, len);
     }
@@ -449,8 +464,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T <= 2048 ?
     const uint64_t top = first & ~mask;
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j)
-        key[j] = one_pass ? top | (k[j] >> LB)
-                          : (p0 + j < len ? keys[tl.start + (uint32_t)(k[j] & (T - 1))] : ~0ull);
+        key[j] = one_pass     ? top | (k[j] >> LB)
+                 : p0 + j >= len ? ~0ull
+                 : AOS         ? rec[tl.start + (uint32_t)(k[j] & (T - 1))].key
+                               : keys[tl.start + (uint32_t)(k[j] & (T - 1))];
     __syncthreads();  // every gather from the staged index values is done
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) sk[px(p0 + j)] = key[j];  // pads (positions >= len) are never read
