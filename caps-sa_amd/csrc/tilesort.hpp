@@ -381,9 +381,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T <= 2048 ?
     // stores) them unconditionally: adjacent accesses merge into 16-byte
     // vector loads and stores (2,048-record class 2.54 -> 2.46 ms)
     if (p0 + ITEMS <= len) {
+        // packed 12-byte records: the thread's 3 * ITEMS dwords as 16-byte loads
+        uint32_t wd[sizeof(KRec<I>) == 12 ? 3 * ITEMS : 1];
+        if constexpr (AOS && sizeof(KRec<I>) == 12) {
+            typedef uint32_t u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
+            const u32x4_a4 *src = (const u32x4_a4 *)(rec + tl.start + p0);
+#pragma unroll
+            for (int q = 0; q < 3 * ITEMS / 4; ++q) {
+                const u32x4_a4 v = src[q];
+                wd[4 * q] = v.x, wd[4 * q + 1] = v.y, wd[4 * q + 2] = v.z, wd[4 * q + 3] = v.w;
+            }
+        }
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
-            if constexpr (AOS) {
+            if constexpr (AOS && sizeof(KRec<I>) == 12) {
+                kk[j] = (uint64_t)wd[3 * j] | ((uint64_t)wd[3 * j + 1] << 32);
+                ix[j] = (I)wd[3 * j + 2];
+            } else if constexpr (AOS) {
                 const KRec<I> r = rec[tl.start + p0 + j];
                 kk[j] = r.key;
                 ix[j] = r.idx;
