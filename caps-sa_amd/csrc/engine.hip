@@ -1035,11 +1035,12 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     // batch-sized buffers
     uint64_t *keys = (uint64_t *)W.keys.get(bcap * 8 + 8);
     KRec<I> *tmp = (KRec<I> *)W.recs.get((all_records ? n_pos : bcap) * sizeof(KRec<I>) + 16);
-    // one batch, merge tile sort: scatter pass 2 writes bucketed records (key
-    // and index side by side, one write stream per bucket run instead of two)
-    // and the tile sort reads them, writing the sorted index values to the
-    // output and sorted keys where they are read again
-    const bool aos = single && !(o.flags & CAPS_FLAG_TILE_RADIX);
+    // merge tile sort: scatter pass 2 writes bucketed records (key and index
+    // side by side, one write stream per bucket run instead of two) and the
+    // tile sort reads them, writing the sorted index values to the output and
+    // sorted keys where they are read again (oversized buckets, LCP builds:
+    // the boundary LCPs and the next batch's first one)
+    const bool aos = !(o.flags & CAPS_FLAG_TILE_RADIX);
     KRec<I> *brec = aos ? (KRec<I> *)W.brec.get(bcap * sizeof(KRec<I>) + 16) : nullptr;
     I *sink_buf = sink ? (I *)W.sinkbuf.get(bcap * sizeof(I) + 8) : nullptr;
     // the previous batch's last suffix (key, position), for the next batch's first LCP
