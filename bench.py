@@ -96,7 +96,7 @@ def launch_ms(st: dict) -> dict:
 
 
 # rocprofv3 names of those kernels (u8 text, u32 index, merge-path tile sort)
-ROCPROF_NAMES = {"k_classify": "k_classify_seq<u8, u32, {rk}, false>", "k_scatter1": "k_scatter1_staged<u8, u32, false>",
+ROCPROF_NAMES = {"k_classify": "k_classify_seq<u8, u32, {rk}, false, 8>", "k_scatter1": "k_scatter1_staged<u8, u32, false>",
                  "k_scatter2": "k_scatter2<u32, 1024, 8192, true>", "k_tile_1024": "k_tile_merge<u32, 1024, true>",
                  "k_tile_2048": "k_tile_merge<u32, 2048, true>", "k_tile_4096": "k_tile_merge<u32, 4096, true>",
                  "k_pairs": "k_pairs_packed<u8, u32>"}

@@ -949,8 +949,10 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     // the block ranges (and scatter pass 1's grid, which derives from them)
     // do not depend on the router.
     const size_t cls_lds = classify_lds_bytes(SP.n_coarse);
-    const unsigned resident = seq ? resident_blocks(k_classify_seq<S, I, 0, false>, 256, cls_lds)
-                                  : resident_blocks(k_classify<S, I, 0>, 256, cls_lds);
+    const bool cls16 = pc > 2048;  // positions per classify thread: 16, else 8
+    const unsigned resident = !seq    ? resident_blocks(k_classify<S, I, 0>, 256, cls_lds)
+                              : cls16 ? resident_blocks(k_classify_seq<S, I, 0, false, 16>, 256, cls_lds)
+                                      : resident_blocks(k_classify_seq<S, I, 0, false, 8>, 256, cls_lds);
     const uint64_t chunk =
         std::max<uint64_t>(kStageStep, ceil_div(ceil_div(n_iter, (uint64_t)resident), kStageStep) * kStageStep);
     const unsigned nblk = (unsigned)ceil_div(n_iter, chunk);
@@ -960,8 +962,15 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
             auto classify_seq = [&](auto rk, auto kp) {
                 constexpr int RK = decltype(rk)::value;
                 constexpr bool KP = decltype(kp)::value;
-                k_classify_seq<S, I, RK, KP><<<nblk, 256, cls_lds, st>>>(d_text, n, cm, g, L, max_ctx, SP, R, cid, hblk,
-                                                                   pc, chunk, d_keep);
+                if constexpr (RK == 1 || RK == 2)
+                    k_classify_seq<S, I, RK, KP, 16><<<nblk, 256, cls_lds, st>>>(d_text, n, cm, g, L, max_ctx, SP, R,
+                                                                               cid, hblk, pc, chunk, d_keep);
+                else if (cls16)
+                    k_classify_seq<S, I, RK, KP, 16><<<nblk, 256, cls_lds, st>>>(d_text, n, cm, g, L, max_ctx, SP, R,
+                                                                               cid, hblk, pc, chunk, d_keep);
+                else
+                    k_classify_seq<S, I, RK, KP, 8><<<nblk, 256, cls_lds, st>>>(d_text, n, cm, g, L, max_ctx, SP, R,
+                                                                              cid, hblk, pc, chunk, d_keep);
             };
             using kno = std::false_type;
             using kyes = std::true_type;

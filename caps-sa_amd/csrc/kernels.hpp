@@ -609,11 +609,17 @@ __device__ __forceinline__ void thread_keys_seq(const S *__restrict__ t, uint64_
     for (uint32_t i = 0; i < NI; ++i) out[i] = first + i < cnt ? with_id(g, keys[i], segs[i]) : 0ull;
 }
 
-// The binary-search variant is held to 128 VGPRs (four wavefronts per SIMD
-// instead of three; the allocator spills two dwords): 1.85 -> 1.60 ms on the
-// 268M text.  The router variants keep their own allocation.
-template <typename S, typename I, int RK, bool KEEP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RK == 0 || RK == 3 ? 4 : 1))) void k_classify_seq(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, KeyGeom g,
+// Each thread classifies CI consecutive positions.  Up to 2,048 intervals
+// (a coarse table of at most 16 KB) CI = 8 and the binary-search variant is
+// held to 64 VGPRs, eight wavefronts per SIMD (the allocator spills a few
+// dwords): 1.08 -> 0.99 ms on the 268M text against sixteen positions at four
+// wavefronts.  With 4,096 intervals the LDS allows three workgroups per
+// compute unit whatever the registers, and sixteen positions per thread keep
+// more searches in flight (1.42 against 1.66 ms).  The spline variant keeps
+// four wavefronts (at eight it would spill 32 dwords), the other routers
+// their own allocation.
+template <typename S, typename I, int RK, bool KEEP, int CI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RK == 0 ? (CI == 8 ? 8 : 4) : RK == 3 ? 4 : 1))) void k_classify_seq(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, KeyGeom g,
                                                       DevLimits L, uint64_t max_ctx, Splitters SP, RouterDev R,
                                                       uint16_t *__restrict__ cid, uint32_t *__restrict__ hblk,
                                                       uint32_t pc, uint64_t chunk, const uint64_t *__restrict__ keep) {
@@ -622,7 +628,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RK == 0 || 
     uint64_t *coarse = classify_lds;
     uint32_t *chist = (uint32_t *)(classify_lds + pow2_at_least(SP.n_coarse + 1));
     for (uint32_t c = threadIdx.x; c < pc; c += blockDim.x) chist[c] = 0;
-    __shared__ S win[kSeqChunk + 72];
+    __shared__ S win[(256 * CI) + 72];
     __shared__ uint64_t rknots[RK == 1 ? NS + 1 : 1];
     __shared__ float ra[RK == 1 ? NS : 1], rb[RK == 1 ? NS : 1], rinv[RK == 1 ? NS : 1];
     __shared__ uint32_t rlo[RK == 1 ? NS : 1], rhi[RK == 1 ? NS : 1];
@@ -654,24 +660,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RK == 0 || 
     // this block's positions: [c0, c1), the same range the scatter's block owns
     const uint64_t c0 = (uint64_t)blockIdx.x * chunk;
     const uint64_t c1 = c0 + chunk < n ? c0 + chunk : n;
-    for (uint64_t base = c0; base < c1; base += kSeqChunk) {
-        uint32_t cnt = (uint32_t)(c1 - base < kSeqChunk ? c1 - base : kSeqChunk);
-        uint64_t key[kSeqItems];
-        bool valid[kSeqItems];
-        uint32_t b[kSeqItems];
-        thread_keys_seq<S>(t, n, cm, g, L, max_ctx, base, cnt, win, key);
-        const uint32_t first = threadIdx.x * kSeqItems;
+    for (uint64_t base = c0; base < c1; base += (256 * CI)) {
+        uint32_t cnt = (uint32_t)(c1 - base < (256 * CI) ? c1 - base : (256 * CI));
+        uint64_t key[CI];
+        bool valid[CI];
+        uint32_t b[CI];
+        thread_keys_seq<S, CI>(t, n, cm, g, L, max_ctx, base, cnt, win, key);
+        const uint32_t first = threadIdx.x * CI;
         // kept positions (bit q of the bitmap; all when keep is null): the
-        // thread's 16 positions start at a multiple of 16, inside one word
-        uint32_t kb = 0xFFFFu;
-        if (KEEP && first < cnt) kb = (uint32_t)(keep[(base + first) >> 6] >> ((base + first) & 63)) & 0xFFFFu;
+        // thread's positions start at a multiple of their count, inside one word
+        constexpr uint32_t kAll = (uint32_t)((1ull << CI) - 1);
+        uint32_t kb = kAll;
+        if (KEEP && first < cnt) kb = (uint32_t)(keep[(base + first) >> 6] >> ((base + first) & 63)) & kAll;
 #pragma unroll
-        for (uint32_t j = 0; j < kSeqItems; ++j) valid[j] = first + j < cnt && ((kb >> j) & 1u);
-        if (RK == 0) coarse_bsearch<kSeqItems>(SP, coarse, key, b);
+        for (uint32_t j = 0; j < CI; ++j) valid[j] = first + j < cnt && ((kb >> j) & 1u);
+        if (RK == 0) coarse_bsearch<CI>(SP, coarse, key, b);
         else if (RK == 3) {
-            // two halves of eight keys: sixteen keys' windows would spill
+            // groups of eight keys: sixteen keys' windows would spill
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
+            for (int h = 0; h < (int)CI / 8; ++h) {
                 uint64_t k8[8];
                 bool v8[8];
                 uint32_t b8[8];
@@ -683,21 +690,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RK == 0 || 
             }
         }
         else
-            coarse_routed<RK, kSeqItems>(SP, coarse, R, rknots, rradix, ra, rb, rinv, rlo, rhi, nib, w2, key, valid,
+            coarse_routed<RK, CI>(SP, coarse, R, rknots, rradix, ra, rb, rinv, rlo, rhi, nib, w2, key, valid,
                                          b, my_sum, my_max, my_fb, my_cnt);
-        if (first + kSeqItems <= cnt) {  // 32 contiguous, 32-byte aligned bytes per thread
+        if (first + CI <= cnt) {  // 2 * CI contiguous, aligned bytes per thread
             uint4 *dst = (uint4 *)(cid + base + first);
 #pragma unroll
-            for (uint32_t q = 0; q < kSeqItems / 8; ++q)
+            for (uint32_t q = 0; q < CI / 8; ++q)
                 dst[q] = make_uint4(b[8 * q] | (b[8 * q + 1] << 16), b[8 * q + 2] | (b[8 * q + 3] << 16),
                                     b[8 * q + 4] | (b[8 * q + 5] << 16), b[8 * q + 6] | (b[8 * q + 7] << 16));
         } else {
 #pragma unroll
-            for (uint32_t j = 0; j < kSeqItems; ++j)
+            for (uint32_t j = 0; j < CI; ++j)
                 if (valid[j]) cid[base + first + j] = (uint16_t)b[j];
         }
 #pragma unroll
-        for (uint32_t j = 0; j < kSeqItems; ++j)
+        for (uint32_t j = 0; j < CI; ++j)
             if (valid[j]) atomicAdd(&chist[b[j]], 1u);
         __syncthreads();
     }
