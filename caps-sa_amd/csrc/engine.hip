@@ -1007,30 +1007,44 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     // ---- bucket batches: consecutive coarse intervals, at most bmax records each ----
     // (an interval larger than bmax is a batch of its own).  Batches follow
     // key order, so the suffix array is their concatenation.
-    std::vector<uint64_t> hcoff(pc + 1);
-    HIPCHK(hipMemcpyAsync(hcoff.data(), coff, (pc + 1) * 8, hipMemcpyDeviceToHost, st));
-    unsigned long long spline_fb = 0;  // keys outside the spline's window (none while its bound holds)
-    if (rkind == CAPS_ROUTER_SPLINE) HIPCHK(hipMemcpyAsync(&spline_fb, R.stats + 2, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    if (spline_fb) {  // binary search is the judge: classify every key again
-        classify(CAPS_ROUTER_BSEARCH);
-        HIPCHK(hipMemcpyAsync(hcoff.data(), coff, (pc + 1) * 8, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-    }
-    // per-interval counts and in-interval offsets are 32-bit in the scatter
-    // passes (block prefixes, LDS bucket counts)
-    uint64_t cmax = 0;
-    for (uint32_t c = 0; c < pc; ++c) cmax = std::max<uint64_t>(cmax, hcoff[c + 1] - hcoff[c]);
-    if (cmax >= (1ull << 32))
-        throw CapsError(CAPS_E_UNSUPPORTED, "a classification bucket holds 2^32 or more suffixes");
-    stats.coarse_buckets = pc;
-    stats.coarse_max = cmax;
     const BatchPlan plan_b = plan_batches(W, n_pos, o.batch_records, sizeof(KRec<I>),
                                           (o.flags & CAPS_FLAG_SCATTER_PER_BATCH) != 0);
     const uint64_t bmax = plan_b.bmax;
+    // One batch of every record, known without the interval totals: binary
+    // search (no router fallback to check), all records within one batch and
+    // fewer than 2^32 of them (so is every interval).  The batch then starts
+    // without waiting for classification; the totals are read at the end,
+    // for the statistics.
+    const bool one_batch = rkind == CAPS_ROUTER_BSEARCH && bmax >= n_pos && n_pos < (1ull << 32);
+    std::vector<uint64_t> hcoff(pc + 1);
+    if (!one_batch) {
+        HIPCHK(hipMemcpyAsync(hcoff.data(), coff, (pc + 1) * 8, hipMemcpyDeviceToHost, st));
+        unsigned long long spline_fb = 0;  // keys outside the spline's window (none while its bound holds)
+        if (rkind == CAPS_ROUTER_SPLINE)
+            HIPCHK(hipMemcpyAsync(&spline_fb, R.stats + 2, 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (spline_fb) {  // binary search is the judge: classify every key again
+            classify(CAPS_ROUTER_BSEARCH);
+            HIPCHK(hipMemcpyAsync(hcoff.data(), coff, (pc + 1) * 8, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+        }
+    } else {
+        hcoff[pc] = n_pos;  // the batch: records [0, n_pos)
+    }
+    // per-interval counts and in-interval offsets are 32-bit in the scatter
+    // passes (block prefixes, LDS bucket counts)
+    auto interval_max = [&]() {
+        uint64_t m = 0;
+        for (uint32_t c = 0; c < pc; ++c) m = std::max<uint64_t>(m, hcoff[c + 1] - hcoff[c]);
+        return m;
+    };
+    if (!one_batch && interval_max() >= (1ull << 32))
+        throw CapsError(CAPS_E_UNSUPPORTED, "a classification bucket holds 2^32 or more suffixes");
+    stats.coarse_buckets = pc;
     std::vector<uint32_t> cuts{0};
-    for (uint32_t c = 0; c < pc; ++c)
-        if (c > cuts.back() && hcoff[c + 1] - hcoff[cuts.back()] > bmax) cuts.push_back(c);
+    if (!one_batch)
+        for (uint32_t c = 0; c < pc; ++c)
+            if (c > cuts.back() && hcoff[c + 1] - hcoff[cuts.back()] > bmax) cuts.push_back(c);
     cuts.push_back(pc);
     uint64_t bcap = 0;
     for (size_t bi = 0; bi + 1 < cuts.size(); ++bi) bcap = std::max(bcap, hcoff[cuts[bi + 1]] - hcoff[cuts[bi]]);
@@ -1444,6 +1458,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         tm.mark(PH_GIANT);
         if (sink) sink->flush(out, nb, st);
     }
+    if (one_batch) HIPCHK(hipMemcpyAsync(hcoff.data(), coff, (pc + 1) * 8, hipMemcpyDeviceToHost, st));
     if (R.stats) {
         unsigned long long rs[4];
         HIPCHK(hipMemcpyAsync(rs, R.stats, 32, hipMemcpyDeviceToHost, st));
@@ -1453,6 +1468,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         if (R.kind == CAPS_ROUTER_SPLINE) stats.router_bound_lo = -R.el, stats.router_bound_hi = R.eh;
     }
     HIPCHK(hipStreamSynchronize(st));
+    stats.coarse_max = interval_max();
     tm.collect(stats);
     kt.collect();
     stats.k_scatter = stats.k_scatter1 + stats.k_scatter2;
