@@ -41,6 +41,20 @@ __global__ __launch_bounds__(256) void k_alphabet(const S *__restrict__ t, uint6
                 for (int b = 0; b < 4; ++b) note((w[q] >> (8 * b)) & 0xFFu);
         }
         for (uint64_t i = (n16 << 4) + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) note(t[i]);
+    } else if (sizeof(S) == 2 && ((uintptr_t)t & 15) == 0) {
+        // eight u16 symbols per 16-byte load, then the tail
+        const uint64_t n8 = n >> 3;
+        const uint4 *t8 = (const uint4 *)t;
+        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n8; i += stride) {
+            const uint4 v = t8[i];
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                note(w[q] & 0xFFFFu);
+                note(w[q] >> 16);
+            }
+        }
+        for (uint64_t i = (n8 << 3) + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) note(t[i]);
     } else {
         for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) note(t[i]);
     }
@@ -256,21 +270,23 @@ template <typename S>
 __global__ void k_pack_codes(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, uint32_t bits, uint32_t lgf,
                              uint64_t *__restrict__ out, uint64_t nwords) {
     const uint32_t F = 1u << lgf;
-    // u8 text at a 16-byte aligned address, 8+ codes a word: the word's F
-    // bytes come in 8-byte loads (aligned: j0 is a multiple of F)
-    const bool vec = sizeof(S) == 1 && F >= 8 && ((uintptr_t)t & 15) == 0;
+    // text at a 16-byte aligned address, a word's symbols filling whole
+    // 8-byte loads (u8: 8+ codes a word, u16: 4+; aligned: j0 is a multiple
+    // of F)
+    constexpr uint32_t PER = 8 / sizeof(S);  // symbols per 8-byte load
+    const bool vec = F >= PER && ((uintptr_t)t & 15) == 0;
     for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords;
          w += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t j0 = w << lgf;
         uint64_t v = 0;
         if (vec && j0 + F <= n) {
             const uint64_t *src = (const uint64_t *)(t + j0);
-            for (uint32_t q = 0; q < F / 8; ++q) {
+            for (uint32_t q = 0; q < F / PER; ++q) {
                 const uint64_t x = src[q];
 #pragma unroll
-                for (uint32_t b = 0; b < 8; ++b) {
-                    const uint32_t c = cm((S)((x >> (8 * b)) & 0xFFu));
-                    v |= (uint64_t)c << (64 - (q * 8 + b + 1) * bits);
+                for (uint32_t b = 0; b < PER; ++b) {
+                    const uint32_t c = cm((S)((x >> (8 * sizeof(S) * b)) & ((1ull << (8 * sizeof(S))) - 1)));
+                    v |= (uint64_t)c << (64 - (q * PER + b + 1) * bits);
                 }
             }
         } else {
@@ -289,10 +305,32 @@ __global__ void k_pack_codes(const S *__restrict__ t, uint64_t n, CodeMap<S> cm,
 template <typename S>
 __global__ void k_pack_seed(const S *__restrict__ t, uint64_t n, CodeMap<S> cm, uint32_t bits, uint32_t K,
                             uint64_t *__restrict__ out, uint64_t nwords) {
+    // u8 text at a 4-byte aligned address, at most 21 codes a word (fields
+    // of 3+ bits): the word's bytes from seven aligned dword loads, shifted
+    // into place by byte alignment
+    const bool vec = sizeof(S) == 1 && K <= 21 && ((uintptr_t)t & 3) == 0;
     for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords;
          w += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t j0 = w * K;
         uint64_t v = 0;
+        if (vec && j0 + 28 <= n) {
+            const uint32_t *src = (const uint32_t *)(t + (j0 & ~3ull));
+            const uint32_t sh = (uint32_t)(j0 & 3);
+            uint32_t d[7], e[6];
+#pragma unroll
+            for (int q = 0; q < 7; ++q) d[q] = src[q];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) e[q] = __builtin_amdgcn_alignbyte(d[q + 1], d[q], sh);
+#pragma unroll
+            for (uint32_t f = 0; f < 21; ++f) {
+                if (f < K) {
+                    const uint32_t c = cm((S)((e[f >> 2] >> (8 * (f & 3))) & 0xFFu));
+                    v |= (uint64_t)c << (64 - (f + 1) * bits);
+                }
+            }
+            out[w] = v;
+            continue;
+        }
         for (uint32_t f = 0; f < K; ++f) {
             const uint64_t j = j0 + f;
             const uint32_t c = j < n ? cm(t[j]) : 0u;
