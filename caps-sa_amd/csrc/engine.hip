@@ -139,15 +139,11 @@ DevLimits upload_limits(Workspace &W, const caps_limits_t *lim, uint64_t n, hipS
         while (bs < minb) bs <<= 1;
         uint32_t shift = (uint32_t)__builtin_ctzll(bs);
         uint64_t nb = ceil_div(n, bs);
-        W.h_dir.resize(nb + 1);
-        uint64_t e = 0;
-        for (uint64_t b = 0; b <= nb; ++b) {
-            uint64_t s = std::min(b * bs, n);
-            while (e < ne && ends[e] <= s) ++e;
-            W.h_dir[b] = (uint32_t)e;
-        }
+        // built on the device from the uploaded ends (one binary search per
+        // block) instead of a host walk and a second upload
         uint32_t *d_dir = (uint32_t *)W.dir.get((nb + 1) * 4);
-        HIPCHK(hipMemcpyAsync(d_dir, W.h_dir.data(), (nb + 1) * 4, hipMemcpyHostToDevice, st));
+        k_seg_dir<<<(unsigned)ceil_div(nb + 1, 256), 256, 0, st>>>(d_ends, ne, n, shift, nb, d_dir);
+        LAUNCHCHK();
         L.dir = d_dir;
         L.block_shift = shift;
     }

@@ -33,7 +33,6 @@ struct Workspace {
     // limits
     DevBuf ends, dir;
     std::vector<uint64_t> h_ends;
-    std::vector<uint32_t> h_dir;
     // key geometry
     DevBuf bitmap, code, packed, seedpk;
     std::vector<uint8_t> h_code8;

@@ -1614,6 +1614,22 @@ __global__ void k_find_runs(const uint64_t *__restrict__ pk, uint64_t nwords, ui
     }
 }
 
+// Segment directory: dir[b] = the number of segment ends at or below
+// min(b << shift, n) (the first end past block b's start), for b in [0, nb].
+__global__ void k_seg_dir(const uint64_t *__restrict__ ends, uint64_t ne, uint64_t n, uint32_t shift, uint64_t nb,
+                          uint32_t *__restrict__ dir) {
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > nb) return;
+    const uint64_t x = (b << shift) < n ? (b << shift) : n;
+    uint64_t lo = 0, hi = ne;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (ends[mid] <= x) lo = mid + 1;
+        else hi = mid;
+    }
+    dir[b] = (uint32_t)lo;
+}
+
 // Run memo construction, pass 2: dir[b] = the first run (sorted) whose end
 // lies past block b's start, for b in [0, n_dir].
 __global__ void k_run_dir(const uint64_t *__restrict__ re, uint64_t n_runs, uint64_t n_dir,
