@@ -487,10 +487,16 @@ constexpr uint32_t kFFFirst = 4;
 constexpr int kFFWords = 4;
 constexpr int kFFProbes = 16;
 
+// After a probe that matched in full, with a run memo: when every open
+// member's last matched symbol lies in a recorded run of one code, each
+// member agrees with its subgroup's first member (whose last matched symbol
+// is the same code) on the shorter rest of their runs, so the slot skips the
+// least rest over its open members (whole words, inside every open window)
+// at once.
 template <int W>
 __device__ __forceinline__ uint64_t ff_advance(const uint64_t *__restrict__ pk, uint32_t lgf, uint32_t bits,
-                                               uint64_t pos, uint64_t D, uint64_t cap, bool open, uint32_t sub,
-                                               uint64_t &nsym) {
+                                               const RunTable &runs, uint64_t pos, uint64_t D, uint64_t cap,
+                                               bool open, uint32_t sub, uint64_t &nsym) {
     const int lane = threadIdx.x & 63;
     const uint64_t smask = (W == 64) ? ~0ull : (((1ull << W) - 1) << (lane & ~(W - 1)));
     if ((__ballot(open) & smask) == 0) return 0;  // a settled slot
@@ -501,6 +507,23 @@ __device__ __forceinline__ uint64_t ff_advance(const uint64_t *__restrict__ pk, 
         adv += sk;
         if (sk != ((uint64_t)lim << lgf)) break;
         lim = kFFWords;
+        if (runs.n_runs) {
+            uint64_t r = ~0ull;  // symbols this member may skip (settled lanes do not bound the slot)
+            if (open) {
+                const uint64_t x = pos + D + adv;        // first symbol past the matched words (adv > 0)
+                const uint64_t rr = run_remaining(runs, x - 1);  // x - 1 inside a run: rr >= 1
+                const uint64_t rem = cap > D + adv ? cap - (D + adv) : 0;
+                r = rr ? rr - 1 : 0;
+                r = r < rem ? r : rem;
+                r = (r >> lgf) << lgf;
+            }
+#pragma unroll
+            for (int o = 1; o < W; o <<= 1) {
+                const uint64_t v = seg_xor64<W>(r, o);
+                r = v < r ? v : r;
+            }
+            if (r != ~0ull && r > 0) adv += r;
+        }
     }
     return adv;
 }
@@ -536,7 +559,7 @@ __global__ __launch_bounds__(256) void k_wgroups_grid(const S *__restrict__ t, C
     uint64_t nsym = 0;  // symbols read
     for (uint32_t round = 0;; ++round) {
         if (cm.packed && round >= kFFAfter && __any(!settled))
-            D += ff_advance<W>(cm.packed, cm.lgf, g.bits, pos, D, cap, !settled, sub, nsym);
+            D += ff_advance<W>(cm.packed, cm.lgf, g.bits, cm.runs, pos, D, cap, !settled, sub, nsym);
         if (!settled) {
             chunk = key_at(t, cm, g, pos, D, cap);
             uint64_t rem = cap > D ? cap - D : 0;
@@ -727,7 +750,7 @@ __global__ __launch_bounds__(256) void k_wgroups_wq(const S *__restrict__ t, Cod
         if (!__any(busy || v1 || v2)) break;
         const bool act = busy && r < m;
         if (cm.packed && busy && rounds >= kFFAfter)
-            D += ff_advance<W>(cm.packed, cm.lgf, g.bits, pos, D, cap, !settled, sub, nsym);
+            D += ff_advance<W>(cm.packed, cm.lgf, g.bits, cm.runs, pos, D, cap, !settled, sub, nsym);
         if (!settled) {
             chunk = key_at(t, cm, g, pos, D, cap);
             uint64_t rem = cap > D ? cap - D : 0;
