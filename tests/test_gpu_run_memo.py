@@ -71,3 +71,40 @@ def test_run_memo_all_suffixes_plain(cuda):
         torch.cuda.synchronize()
         assert np.array_equal(sa.cpu().numpy().view(np.uint32).astype(np.uint64), want), flags
         assert np.array_equal(lcp.cpu().numpy().view(np.uint32).astype(np.uint64), want_lcp), flags
+
+
+def test_run_memo_small_groups(cuda):
+    """Groups of 3..64 tied suffixes whose shared stretch is a long run of
+    one code, of nearly equal lengths (the small-group fast-forward's run
+    skip), beside the groups of suffixes inside the runs (65+ members, so the
+    memo is built): memo on, memo off and the oracle agree, every suffix, in
+    both boundary orders and under max_context."""
+    rng = np.random.default_rng(77)
+    parts = []
+    for c in [3, 4, 5, 7, 8, 9, 16, 17, 32, 33, 64]:
+        prefix = rng.integers(0, 4, 6).astype(np.uint8)
+        ln = int(rng.choice([300, 700, 1500]))
+        sym = int(rng.integers(0, 6))
+        for _ in range(c):
+            parts.append(rng.integers(0, 4, int(rng.integers(1, 9))).astype(np.uint8))
+            parts.append(prefix)
+            parts.append(np.full(ln + int(rng.integers(0, 3)), sym, dtype=np.uint8))
+            parts.append(rng.integers(0, 4, 5).astype(np.uint8))
+    text = np.concatenate(parts)
+    n = len(text)
+    d_text = dev.text_to_device(text)
+    ends = np.unique(np.append(rng.integers(1, n, 40), n)).astype(np.uint64)
+    for order, max_ctx, segmented in ((0, caps_sa.MAX_CONTEXT, False), (0, 1000, False), (1, caps_sa.MAX_CONTEXT, True)):
+        lp = (caps_sa.StarSegmented(n, ends) if order else caps_sa.SegmentedText(n, ends)) if segmented else None
+        ctx = O.Ctx(text, ends if segmented else None, order)
+        want, want_lcp = ctx.build_in_memory(max_ctx=max_ctx, threads=8, want_lcp=True)
+        for flags in (0, NO):
+            print(f"case small-group runs n={n} order={order} max_ctx={max_ctx} flags={flags}", flush=True)
+            sa, lcp = dev.build_device(d_text, n, 1, idx_bytes=4, lp=lp, max_context=max_ctx, want_lcp=True,
+                                       flags=flags)
+            torch.cuda.synchronize()
+            st = caps_sa.last_stats()
+            if flags == 0:
+                assert st["runs"] > 0 and st["small_groups"] > 0
+            assert np.array_equal(sa.cpu().numpy().view(np.uint32).astype(np.uint64), want)
+            assert np.array_equal(lcp.cpu().numpy().view(np.uint32).astype(np.uint64), want_lcp)
