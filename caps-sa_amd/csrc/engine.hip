@@ -1366,15 +1366,15 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
             uint64_t maxsize = 0;
             for (const Group &gr : hg) {
                 maxsize = std::max<uint64_t>(maxsize, gr.size);
-                for (uint64_t c = 0; c < gr.size; c += kMidMax)
-                    hch.push_back(Group{gr.start + c, std::min<uint64_t>(kMidMax, gr.size - c)});
+                for (uint64_t c = 0; c < gr.size; c += kGiantChunk)
+                    hch.push_back(Group{gr.start + c, std::min<uint64_t>(kGiantChunk, gr.size - c)});
             }
             int levels = 0;
-            for (uint64_t run = kMidMax; run < maxsize; run *= 2) ++levels;
+            for (uint64_t run = kGiantChunk; run < maxsize; run *= 2) ++levels;
             std::vector<uint64_t> level_off(levels + 1, 0);
             W.h_jobs.clear();
             for (int lv = 0; lv < levels; ++lv) {
-                const uint64_t run = (uint64_t)kMidMax << lv;
+                const uint64_t run = (uint64_t)kGiantChunk << lv;
                 level_off[lv] = W.h_jobs.size();
                 for (const Group &gr : hg)
                     for (uint64_t b = 0; b < gr.size; b += 2 * run) {
@@ -1396,7 +1396,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
             const hipStream_t gst = gsg.ls;
             for (uint64_t c0 = 0; c0 < hch.size(); c0 += 65535u * 16) {
                 const unsigned nbk = (unsigned)std::min<uint64_t>(hch.size() - c0, 65535u * 16);
-                k_groups<S, I, 256, kMidMax><<<nbk, 256, 0, gst>>>(d_text, cm, gref, L, max_ctx, dc + c0, nullptr, nbk,
+                k_groups<S, I, 256, kGiantChunk><<<nbk, 256, 0, gst>>>(d_text, cm, gref, L, max_ctx, dc + c0, nullptr, nbk,
                                                                   nullptr, depth0, out, lbuf, 1);
                 LAUNCHCHK();
             }
@@ -1414,7 +1414,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
                 k_lcp_merge<S, I, MW><<<grid, 128, 0, gst>>>(d_text, cm, L, max_ctx, depth0, jobs, is, ls, id, ld);
             };
             for (int lv = 0; lv < levels; ++lv) {
-                const uint64_t run = (uint64_t)kMidMax << lv;
+                const uint64_t run = (uint64_t)kGiantChunk << lv;
                 const uint64_t span = std::min<uint64_t>(2 * run, maxsize);
                 for (uint64_t j0 = level_off[lv]; j0 < level_off[lv + 1]; j0 += 65535) {
                     const unsigned nj = (unsigned)std::min<uint64_t>(level_off[lv + 1] - j0, 65535);
