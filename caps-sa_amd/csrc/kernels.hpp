@@ -1583,33 +1583,62 @@ __global__ void k_find_runs(const uint64_t *__restrict__ pk, uint64_t nwords, ui
     const uint32_t bits = 64u >> lgf;
     const uint64_t F = 1ull << lgf;
     auto uniform = [&](uint64_t v) { return v == ((v << bits) | (v >> (64 - bits))); };
-    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords;
-         w += (uint64_t)gridDim.x * blockDim.x) {
-        if ((w << lgf) >= n) break;
-        const uint64_t v = pk[w];
-        if (!uniform(v)) continue;
-        if (w > 0 && pk[w - 1] == v) continue;  // not the stretch's first word
-        uint64_t m = 1;
-        while (((w + m) << lgf) < n && pk[w + m] == v) ++m;
-        if (m < kRunMinWords) continue;
-        const uint64_t code = v >> (64 - bits);
-        // codes of the same value just before and just after the stretch
-        uint64_t back = 0, fwd = 0;
-        if (w > 0) {
-            const uint64_t pv = pk[w - 1];
-            while (back < F && ((pv >> (back * bits)) & ((1ull << bits) - 1)) == code) ++back;
+    // four consecutive words per thread and step, in two 16-byte loads
+    for (uint64_t w0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4; w0 < nwords;
+         w0 += (uint64_t)gridDim.x * blockDim.x * 4) {
+        uint64_t vw[4];
+        if (w0 + 4 <= nwords) {
+            const uint4 *p4 = (const uint4 *)(pk + w0);
+            const uint4 a = p4[0], b = p4[1];
+            vw[0] = (uint64_t)a.x | ((uint64_t)a.y << 32), vw[1] = (uint64_t)a.z | ((uint64_t)a.w << 32);
+            vw[2] = (uint64_t)b.x | ((uint64_t)b.y << 32), vw[3] = (uint64_t)b.z | ((uint64_t)b.w << 32);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) vw[k] = w0 + k < nwords ? pk[w0 + k] : 0ull;
         }
-        if (((w + m) << lgf) < n) {
-            const uint64_t nv = pk[w + m];
-            while (fwd < F && ((nv >> (64 - (fwd + 1) * bits)) & ((1ull << bits) - 1)) == code) ++fwd;
-        }
-        const uint64_t s = (w << lgf) - back;
-        uint64_t e = ((w + m) << lgf) + fwd;
-        if (e > n) e = n;
-        const unsigned long long slot = atomicAdd(count, 1ull);
-        if (slot < cap) {
-            rs[slot] = s;
-            re[slot] = e;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint64_t w = w0 + (uint64_t)k;
+            if (w >= nwords || (w << lgf) >= n) break;
+            const uint64_t v = vw[k];
+            if (!uniform(v)) continue;
+            if (w > 0 && (k ? vw[k - 1] : pk[w - 1]) == v) continue;  // not the stretch's first word
+            // the stretch's length, eight independent word loads a step (a
+            // 10,000-symbol run is about 2,500 words)
+            uint64_t m = 1;
+            for (;;) {
+                uint32_t eq = 0;
+                bool on = true;
+    #pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const uint64_t x = w + m + (uint64_t)q;
+                    const bool same = ((x << lgf) < n) && pk[x] == v;
+                    on = on && same;
+                    eq += on ? 1u : 0u;
+                }
+                m += eq;
+                if (eq < 8) break;
+            }
+            if (m < kRunMinWords) continue;
+            const uint64_t code = v >> (64 - bits);
+            // codes of the same value just before and just after the stretch
+            uint64_t back = 0, fwd = 0;
+            if (w > 0) {
+                const uint64_t pv = pk[w - 1];
+                while (back < F && ((pv >> (back * bits)) & ((1ull << bits) - 1)) == code) ++back;
+            }
+            if (((w + m) << lgf) < n) {
+                const uint64_t nv = pk[w + m];
+                while (fwd < F && ((nv >> (64 - (fwd + 1) * bits)) & ((1ull << bits) - 1)) == code) ++fwd;
+            }
+            const uint64_t s = (w << lgf) - back;
+            uint64_t e = ((w + m) << lgf) + fwd;
+            if (e > n) e = n;
+            const unsigned long long slot = atomicAdd(count, 1ull);
+            if (slot < cap) {
+                rs[slot] = s;
+                re[slot] = e;
+            }
         }
     }
 }
