@@ -148,8 +148,9 @@ __global__ __launch_bounds__(256) void k_random_keys(const S *__restrict__ t, Co
 // wavefront takes 64 consecutive entries; lanes whose positions share a
 // bitmap word (a contiguous run of lanes) OR their bits together by a
 // segmented suffix scan of shuffles, and the run's first lane ORs the word
-// in.  A wavefront whose entries are out of range or out of order writes
-// nothing (its list takes the position-list path).
+// in (64 consecutive positions skip the scan: their bits are one range over
+// at most two words).  A wavefront whose entries are out of range or out of
+// order writes nothing (its list takes the position-list path).
 template <typename I>
 __global__ __launch_bounds__(256) void k_positions_check(const I *__restrict__ pos, uint64_t n_pos, uint64_t n,
                                                          unsigned int *__restrict__ bad,
@@ -167,6 +168,15 @@ __global__ __launch_bounds__(256) void k_positions_check(const I *__restrict__ p
         out |= o;
         down |= d;
         if (keep && __ballot(o || d) == 0) {
+            // 64 consecutive positions (a dense list away from its gaps): two
+            // words at most, whose bits are a contiguous range
+            const uint64_t p0 = shfl64(p, 0);
+            if (__ballot(have && p == p0 + (uint64_t)lane) == ~0ull) {
+                const uint32_t sh = (uint32_t)(p0 & 63);
+                if (lane == 0) atomicOr(&keep[p0 >> 6], ~0ull << sh);
+                if (lane == 1 && sh) atomicOr(&keep[(p0 >> 6) + 1], (1ull << sh) - 1);
+                continue;
+            }
             const uint64_t w = p >> 6;
             unsigned long long bits = have ? 1ull << (p & 63) : 0ull;
 #pragma unroll
