@@ -1644,7 +1644,14 @@ __global__ void k_find_runs(const uint64_t *__restrict__ pk, uint64_t nwords, ui
             const uint64_t s = (w << lgf) - back;
             uint64_t e = ((w + m) << lgf) + fwd;
             if (e > n) e = n;
-            const unsigned long long slot = atomicAdd(count, 1ull);
+            // one reservation per wavefront for the lanes that found a run
+            // here (one counter for every run would serialize on it)
+            const uint64_t here = __ballot(true);
+            const int lane = threadIdx.x & 63, first = __ffsll((unsigned long long)here) - 1;
+            uint64_t base = 0;
+            if (lane == first) base = atomicAdd(count, (unsigned long long)__popcll(here));
+            base = shfl64(base, first);
+            const uint64_t slot = base + (uint64_t)__popcll(here & ((1ull << lane) - 1));
             if (slot < cap) {
                 rs[slot] = s;
                 re[slot] = e;
@@ -1716,10 +1723,22 @@ __host__ __device__ __forceinline__ uint32_t mid_class(uint64_t sz) { return sz 
 __global__ __launch_bounds__(256) void k_split_mid(const Group *__restrict__ in, uint32_t n, Group *__restrict__ out,
                                                    uint64_t cap, unsigned int *__restrict__ counts) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const Group gr = in[i];
-    const uint32_t c = mid_class(gr.size);
-    out[c * cap + atomicAdd(&counts[c], 1u)] = gr;
+    const int lane = threadIdx.x & 63;
+    const bool have = i < n;
+    const Group gr = have ? in[i] : Group{0, 0};
+    const uint32_t c = have ? mid_class(gr.size) : kMidClasses;
+    // one reservation per class and wavefront (three counters for every
+    // group would serialize on them)
+#pragma unroll
+    for (uint32_t k = 0; k < kMidClasses; ++k) {
+        const uint64_t m = __ballot(c == k);
+        if (!m) continue;
+        const int first = __ffsll((unsigned long long)m) - 1;
+        unsigned int base = 0;
+        if (lane == first) base = atomicAdd(&counts[k], (unsigned int)__popcll(m));
+        base = __shfl(base, first);
+        if (c == k) out[k * cap + base + (unsigned int)__popcll(m & ((1ull << lane) - 1))] = gr;
+    }
 }
 
 // groups[0 .. n), n = *n_dev when n_dev is set (a count written by an earlier
