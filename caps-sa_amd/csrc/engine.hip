@@ -1069,7 +1069,7 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     Range *big = (Range *)W.big.get(p * sizeof(Range));
     // [0] oversized, [1..4) tile classes, [4] largest bucket, [8..16) group counts,
     // [16..24) refinement work queues, [24..30) class records (u64),
-    // [32..35) mid-size groups per LDS class, [35..38) their claim counters
+    // [32..36) mid-size groups per LDS class, [36..40) their claim counters
     unsigned int *counters = (unsigned int *)W.counts.get(256);
     GroupLists gl{};
     for (int c = 0; c < kGroupClasses; ++c)
@@ -1334,7 +1334,8 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
                     };
                     mid_class(k_groups<S, I, 128, 128>, 128, 0);
                     mid_class(k_groups<S, I, 256, 512>, 256, 1);
-                    mid_class(k_groups<S, I, 256, kMidMax>, 256, 2);
+                    mid_class(k_groups<S, I, 256, 1024>, 256, 2);
+                    mid_class(k_groups<S, I, 256, kMidMax>, 256, 3);
                 } else {
                     // few groups (often long ones): one launch, one workgroup
                     // per group, so no launch waits for another's last groups

@@ -1507,11 +1507,12 @@ __global__ __launch_bounds__(256) void k_boundary_lcp(const uint64_t *__restrict
 
 // ---------------------------------------------------------------------------
 // Group refinement: a workgroup refines groups of suffixes known to agree on
-// their first depth0 symbols (65..2048 members, or chunks of 2,048 members of
-// larger groups) one at a time, in LDS arrays sized for M members: groups of
-// up to 128 members go to 128-thread workgroups of 8.4 KB, up to 512 to
-// 33.8 KB, larger ones to 135 KB (k_split_mid files them by size), so small
-// groups keep many workgroups per compute unit.  The first kChunkRounds rounds sort each subgroup by the next k
+// their first depth0 symbols (65..2048 members, or chunks of kGiantChunk
+// members of larger groups) one at a time, in LDS arrays sized for M
+// members: groups of up to 128 members go to 128-thread workgroups of 8.4 KB,
+// up to 512 to 33.8 KB, up to 1,024 to 67.6 KB, larger ones to 135 KB
+// (k_split_mid files them by size), so small groups keep many workgroups per
+// compute unit.  The first kChunkRounds rounds sort each subgroup by the next k
 // symbols at its depth (chunk key, plus the boundary value of members whose
 // window ends inside the chunk), which splits short common prefixes into many
 // classes at once.  Later rounds compare every unresolved member with the first
@@ -1714,8 +1715,10 @@ __device__ __forceinline__ uint32_t code_at(const S *__restrict__ t, const CodeM
 
 // Mid-size groups by LDS class: class c of k_split_mid holds sizes in
 // (kMidClassMax[c-1], kMidClassMax[c]].
-constexpr uint32_t kMidClasses = 3;
-__host__ __device__ __forceinline__ uint32_t mid_class(uint64_t sz) { return sz <= 128 ? 0u : sz <= 512 ? 1u : 2u; }
+constexpr uint32_t kMidClasses = 4;
+__host__ __device__ __forceinline__ uint32_t mid_class(uint64_t sz) {
+    return sz <= 128 ? 0u : sz <= 512 ? 1u : sz <= 1024 ? 2u : 3u;
+}
 
 // Files the mid-size groups (65..2048 members) into the lists of their LDS
 // class (out + c * cap, counts[c]; counts start at zero).  The order inside a
