@@ -154,8 +154,8 @@ typedef struct caps_opts {
  * common part of two runs of the same code at once.  The output never
  * changes. */
 #define CAPS_FLAG_NO_RUN_MEMO 2048u
-/* Refine the groups of 65..2048 tied suffixes in three launches by size (up
- * to 128, up to 512, up to 2,048 members: smaller LDS arrays, so more
+/* Refine the groups of 65..2048 tied suffixes in four launches by size (up
+ * to 128, 512, 1,024 and 2,048 members: smaller LDS arrays, so more
  * workgroups per compute unit) whatever their number; by default this
  * happens from 16 such groups per compute unit up, and fewer groups run in
  * one launch of one workgroup per group.  The output never changes. */
