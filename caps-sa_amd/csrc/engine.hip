@@ -639,7 +639,7 @@ static CountAddr count_symbols_addr() {
 // idbits > 0 builds many texts at once: `lim` delimits the texts and the top
 // idbits bits of every packed key hold the text index, so suffixes sort
 // text-major and no comparison crosses a text end.
-constexpr uint32_t kBucketTarget = 1280, kBucketTargetSeq = 1536;
+constexpr uint32_t kBucketTarget = 1536;
 
 // Launch style per refinement size class (pairs, 3-4, 5-8, 9-16, 17-32,
 // 33-64 members).  CAPS_FLAG_PERSISTENT_REFINE queues every class; the
@@ -866,14 +866,13 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
     // ---- buckets and splitters ----
     constexpr uint32_t T = 4096;
     const bool every_suffix = sizeof(I) == 4 && !idbits && !d_keep && d_pos == nullptr && n_pos == n;
-    // mean bucket size the splitters aim at: small enough that most buckets
-    // fit the 2048-record tile class (sample-based splitters spread bucket
-    // sizes by about a quarter around the mean); every suffix of one text
-    // with a u32 index aims higher, fewer buckets for scatter pass 2 against
-    // more 4,096-record tiles (268M text: 14.56-14.59 against 14.69-14.76 ms)
-    const uint64_t target = o.tile_keys ? std::max<uint32_t>(64, std::min<uint32_t>(o.tile_keys, T))
-                            : every_suffix ? kBucketTargetSeq
-                                           : kBucketTarget;
+    // mean bucket size the splitters aim at: most buckets fit the
+    // 2048-record tile class (sample-based splitters spread bucket sizes by
+    // about a quarter around the mean); 1,536 rather than 1,280 records:
+    // fewer buckets for scatter pass 2 against more 4,096-record tiles
+    // (268M text: 14.50-14.59 against 14.69-14.76 ms; whole genome 343.5
+    // against 346.8 ms)
+    const uint64_t target = o.tile_keys ? std::max<uint32_t>(64, std::min<uint32_t>(o.tile_keys, T)) : kBucketTarget;
     // p buckets in C coarse intervals of `stride` buckets.  C is the number of
     // classification buckets (the classify pass decides among them, by binary
     // search or the router): gpu_buckets when set, else 1,024 (more when the

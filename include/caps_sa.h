@@ -79,8 +79,7 @@ typedef struct caps_opts {
                                   split into LDS-sized buckets by the second scatter
                                   pass.  0 = auto (1,024, more for huge inputs)         */
     uint32_t tile_keys;        /* mean records per bucket the splitters aim at
-                                  (0 = 1536 for every suffix of one text with a u32
-                                  index, else 1280; 64..4096)                          */
+                                  (0 = 1536; 64..4096)                                 */
     uint32_t want_lcp;         /* fill the LCP array where an entry point takes one     */
     uint32_t flags;            /* CAPS_FLAG_* bits, 0 by default                        */
     uint64_t batch_records;    /* records per bucket batch: the build sorts consecutive
