@@ -2199,7 +2199,7 @@ struct TuneCand {
 };
 template <typename RUN>
 static void tune(caps_opts_t &o, uint32_t reps, caps_tune_report_t *rep, RUN &&run) {
-    static const uint32_t kKeys[] = {1024, 1280, 1536};
+    static const uint32_t kKeys[] = {1280, 1536, 1792};
     static const uint32_t kFlags[] = {0, CAPS_FLAG_TILE_RADIX, CAPS_FLAG_PERSISTENT_REFINE, CAPS_FLAG_MERGE_W16,
                                       CAPS_FLAG_MERGE_W64, CAPS_FLAG_MID_BY_SIZE};
     std::vector<TuneCand> cands;
