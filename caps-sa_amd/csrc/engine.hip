@@ -1171,7 +1171,9 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         if (wide) {
             if (aos) scatter2(k_scatter2<I, 1024, 4096, true>);
             else scatter2(k_scatter2<I, 1024, 4096, false>);
-        } else if constexpr (sizeof(I) == 4) {
+        } else if (sizeof(I) == 4 || s2_lds <= 12 * 1024) {
+            // u64 staging of 8,192 records takes 147.6 KB: room for at most
+            // 12 KB of splitters and cursors beside it
             if (aos) scatter2(k_scatter2<I, 1024, 8192, true>);
             else scatter2(k_scatter2<I, 1024, 8192, false>);
         } else {
