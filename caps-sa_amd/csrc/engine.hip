@@ -1159,8 +1159,9 @@ void build_impl(const S *d_text, uint64_t n, const I *d_pos, uint64_t n_pos, con
         // calls for them; else 1,024 threads with 8,192-record tiles for a u32
         // index (112 KB + about 20 bytes per bucket: one block per compute
         // unit; twice the bucket run per write of 4,096-record tiles at two
-        // blocks, 3.08 -> 2.72 ms on 268M records) and 4,096-record tiles for a
-        // u64 index (72 KB)
+        // blocks, 3.08 -> 2.72 ms on 268M records); a u64 index takes
+        // 8,192-record tiles (144 KB) when the interval's tables fit in the
+        // 12 KB left beside them, else 4,096-record tiles (72 KB)
         const uint32_t s2_lds = scatter2_lds_bytes(SP.stride);
         const bool wide = s2_lds > 22 * 1024 || (c_hi - c_lo) < 2 * W.n_cu;
         KRec<I> *trec = all_records ? tmp + rbase : tmp;
