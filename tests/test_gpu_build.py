@@ -246,6 +246,27 @@ def test_radix_tile_sort_matches_merge():
             assert np.array_equal(lcp.cpu().numpy().view(np.uint32).astype(np.uint64), want_lcp), flags
 
 
+def test_full_tiles_both_index_widths_both_tile_sorts():
+    """1.5M suffixes in buckets of about 1,536 records: 977 classification
+    intervals of one bucket each (at least 512 intervals, 20 bytes of
+    splitter and cursor tables), so scatter pass 2 stages 8,192-record tiles
+    for the u32 and the u64 index alike, as (key, index) records for the
+    merge tile sort and as key and index arrays for the radix one.  Every
+    build gives the oracle's SA and LCP."""
+    import torch
+    from caps_sa import device as dev
+
+    text = gen.markov_repeats(1500000, seed=43, mean_len=500.0)
+    d_text = dev.text_to_device(text)
+    want, want_lcp = O.Ctx(text).build_in_memory(threads=4, want_lcp=True)
+    for idx_bytes, view in ((4, np.uint32), (8, np.uint64)):
+        for flags in (0, caps_sa.FLAG_TILE_RADIX):
+            print(f"idx_bytes={idx_bytes} flags={flags}", flush=True)
+            sa, lcp = dev.build_device(d_text, len(text), 1, idx_bytes=idx_bytes, want_lcp=True, flags=flags)
+            assert np.array_equal(sa.cpu().numpy().view(view).astype(np.uint64), want), (idx_bytes, flags)
+            assert np.array_equal(lcp.cpu().numpy().view(view).astype(np.uint64), want_lcp), (idx_bytes, flags)
+
+
 def test_autotune_then_build_is_exact():
     """The autotuner times the candidate knobs on the input and returns one of
     them; builds with the tuned knobs (and with every candidate it timed) give
