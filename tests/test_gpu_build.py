@@ -6,6 +6,8 @@ and prefix doubling in test_oracle.py.  Under SegmentedText with the default
 boundary order, comparator-Equal suffixes are compared in ascending position
 order (the oracle's order inside those classes is canonicalized first).
 """
+import itertools
+
 import numpy as np
 import pytest
 
@@ -229,8 +231,9 @@ def test_persistent_refinement_matches_grid():
 
 
 def test_radix_tile_sort_matches_merge():
-    """Both tile sorts (merge path, 4-bit radix) give the oracle's SA and LCP,
-    including tiles with all-ones keys next to the pads."""
+    """Both tile sorts (merge path, 4-bit radix), with a u32 and a u64 index,
+    give the oracle's SA and LCP, including tiles with all-ones keys next to
+    the pads."""
     import torch
     from caps_sa import device as dev
 
@@ -240,10 +243,11 @@ def test_radix_tile_sort_matches_merge():
                  gen.uniform_u8(70000, 256, seed=4)):
         d_text = dev.text_to_device(text)
         want, want_lcp = O.Ctx(text).build_in_memory(threads=4, want_lcp=True)
-        for flags in (0, caps_sa.FLAG_TILE_RADIX):
-            sa, lcp = dev.build_device(d_text, len(text), 1, want_lcp=True, flags=flags)
-            assert np.array_equal(sa.cpu().numpy().view(np.uint32).astype(np.uint64), want), flags
-            assert np.array_equal(lcp.cpu().numpy().view(np.uint32).astype(np.uint64), want_lcp), flags
+        for (idx_bytes, view), flags in itertools.product(((4, np.uint32), (8, np.uint64)),
+                                                          (0, caps_sa.FLAG_TILE_RADIX)):
+            sa, lcp = dev.build_device(d_text, len(text), 1, idx_bytes=idx_bytes, want_lcp=True, flags=flags)
+            assert np.array_equal(sa.cpu().numpy().view(view).astype(np.uint64), want), (idx_bytes, flags)
+            assert np.array_equal(lcp.cpu().numpy().view(view).astype(np.uint64), want_lcp), (idx_bytes, flags)
 
 
 def test_full_tiles_both_index_widths_both_tile_sorts():
